@@ -1,0 +1,321 @@
+"""hyteg_amd -- MI355X-native matrix-free P1 geometric multigrid with HyTeG's operator/solver API.
+
+Python mirror of the reference's C++ interface for this path (classes and methods carry the reference's
+names and argument meaning); every call goes through the C-ABI of libhymi.so (include/hymi.h), whose
+kernels run on the GPU.  Nothing here computes on the CPU.
+
+    storage = PrimitiveStorage(MeshInfo.meshCuboid((0, 0, 0), (1, 1, 1), 2, 2, 2))
+    u = P1Function("u", storage, 0, 7); f = P1Function("f", storage, 0, 7)
+    A = P1ConstantLaplaceOperator(storage, 0, 7)
+    gmg = GeometricMultigridSolver(storage, SORSmoother(1.0), CGSolver(storage, 0, 0),
+                                   P1toP1LinearRestriction(), P1toP1LinearProlongation(), 0, 7, 3, 3)
+    gmg.solve(A, u, f, 7)
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import (ADD, ALL, CELL, DIRICHLET, EDGE, F32, F64, FACE, FREESLIP, INNER, NEUMANN, REPLACE,
+                   VERTEX, HymiError)
+
+__all__ = ["Context", "context", "MeshInfo", "PrimitiveStorage", "P1Function", "P1ConstantLaplaceOperator",
+           "P1ConstantMassOperator", "P1toP1LinearRestriction", "P1toP1LinearProlongation", "SORSmoother",
+           "CGSolver", "GeometricMultigridSolver", "HymiError", "INNER", "DIRICHLET", "NEUMANN", "FREESLIP", "ALL",
+           "REPLACE", "ADD", "F64", "F32", "VERTEX", "EDGE", "FACE", "CELL"]
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+class Context:
+    """A device + HIP stream (device=-1: host-only context for topology and stencil queries)."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        _lib.call("hymi_ctx_create", device, ctypes.byref(h))
+        self.h = h
+        self.device = device
+
+    def synchronize(self):
+        _lib.call("hymi_ctx_synchronize", self.h)
+
+    def bytes_in_use(self):
+        v = ctypes.c_int64()
+        _lib.call("hymi_ctx_bytes_in_use", self.h, ctypes.byref(v))
+        return v.value
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib._lib is not None:
+            _lib._lib.hymi_ctx_destroy(self.h)
+            self.h = None
+
+
+_contexts = {}
+
+
+def context(device=0):
+    if device not in _contexts:
+        _contexts[device] = Context(device)
+    return _contexts[device]
+
+
+class MeshInfo:
+    """Macro mesh description (MeshInfo::meshCuboid, MeshInfo::fromGmshFile)."""
+
+    def __init__(self, kind, args):
+        self.kind, self.args = kind, args
+
+    @staticmethod
+    def meshCuboid(lowerLeftFront, upperRightBack, nx, ny, nz):
+        return MeshInfo("cuboid", (tuple(lowerLeftFront), tuple(upperRightBack), nx, ny, nz))
+
+    @staticmethod
+    def fromGmshFile(path):
+        return MeshInfo("gmsh", (str(path),))
+
+    @staticmethod
+    def fromArrays(coords, cells):
+        return MeshInfo("arrays", (np.ascontiguousarray(coords, np.float64), np.ascontiguousarray(cells, np.int32)))
+
+
+class PrimitiveStorage:
+    """Macro primitives + boundary flags (SetupPrimitiveStorage + setMeshBoundaryFlagsOnBoundary(1, 0, true))."""
+
+    def __init__(self, meshInfo, ctx=None):
+        self.ctx = ctx if ctx is not None else context(0)
+        h = ctypes.c_void_p()
+        if meshInfo.kind == "cuboid":
+            lo, hi, nx, ny, nz = meshInfo.args
+            lo = np.array(lo, np.float64)
+            hi = np.array(hi, np.float64)
+            _lib.call("hymi_mesh_create_cuboid", self.ctx.h, _dp(lo), _dp(hi), nx, ny, nz, ctypes.byref(h))
+        elif meshInfo.kind == "gmsh":
+            _lib.call("hymi_mesh_from_gmsh", self.ctx.h, meshInfo.args[0].encode(), ctypes.byref(h))
+        else:
+            X, C = meshInfo.args
+            _lib.call("hymi_mesh_create", self.ctx.h, len(X), _dp(X), len(C), _ip(C), ctypes.byref(h))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib._lib is not None:
+            _lib._lib.hymi_mesh_destroy(self.h)
+            self.h = None
+
+    def counts(self):
+        out = np.zeros(4, np.int32)
+        _lib.call("hymi_mesh_counts", self.h, _ip(out))
+        return tuple(int(v) for v in out)
+
+    def primitive_type(self, cls, idx):
+        v = ctypes.c_int()
+        _lib.call("hymi_mesh_primitive_type", self.h, cls, idx, ctypes.byref(v))
+        return v.value
+
+    def owned_count(self, level):
+        v = ctypes.c_int64()
+        _lib.call("hymi_mesh_owned_count", self.h, level, ctypes.byref(v))
+        return v.value
+
+    def owned_coords(self, level):
+        out = np.zeros((self.owned_count(level), 3))
+        _lib.call("hymi_mesh_owned_coords", self.h, level, _dp(out))
+        return out
+
+    def owned_types(self, level):
+        out = np.zeros(self.owned_count(level), np.int32)
+        _lib.call("hymi_mesh_owned_types", self.h, level, _ip(out))
+        return out
+
+
+class P1Function:
+    """P1Function<real_t>(name, storage, minLevel, maxLevel); precision F64/F32, nrhs interleaved RHS."""
+
+    def __init__(self, name, storage, minLevel, maxLevel, precision=F64, nrhs=1):
+        self.name, self.storage = name, storage
+        self.minLevel, self.maxLevel = minLevel, maxLevel
+        self.precision, self.nrhs = precision, nrhs
+        h = ctypes.c_void_p()
+        _lib.call("hymi_function_create", storage.h, minLevel, maxLevel, precision, nrhs, ctypes.byref(h))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib._lib is not None:
+            _lib._lib.hymi_function_destroy(self.h)
+            self.h = None
+
+    # host transfers (owned order, see PrimitiveStorage.owned_coords)
+    def download(self, level):
+        out = np.zeros(self.storage.owned_count(level) * self.nrhs)
+        _lib.call("hymi_function_download_owned", self.h, level, _dp(out))
+        return out if self.nrhs == 1 else out.reshape(-1, self.nrhs)
+
+    def upload(self, level, values):
+        v = np.ascontiguousarray(values, np.float64).reshape(-1)
+        if v.size != self.storage.owned_count(level) * self.nrhs:
+            raise ValueError("wrong number of values")
+        _lib.call("hymi_function_upload_owned", self.h, level, _dp(v))
+
+    def download_primitive(self, level, cls, idx):
+        n = ctypes.c_int64()
+        _lib.call("hymi_function_primitive_size", self.h, level, cls, idx, ctypes.byref(n))
+        out = np.zeros(n.value)
+        _lib.call("hymi_function_download_primitive", self.h, level, cls, idx, _dp(out))
+        return out
+
+    def upload_primitive(self, level, cls, idx, values):
+        v = np.ascontiguousarray(values, np.float64)
+        _lib.call("hymi_function_upload_primitive", self.h, level, cls, idx, _dp(v))
+
+    def interpolate(self, value, level, flag=ALL):
+        """interpolate(constant | callable(xyz)->values, level, flag)."""
+        if callable(value):
+            X = self.storage.owned_coords(level)
+            vals = np.asarray(value(X), np.float64)
+            types = self.storage.owned_types(level)
+            cur = self.download(level)
+            sel = (types & flag) != 0
+            if self.nrhs == 1:
+                cur[sel] = vals[sel]
+            else:
+                cur[sel] = vals.reshape(len(X), -1)[sel]
+            self.upload(level, cur)
+        else:
+            _lib.call("hymi_interpolate_const", self.h, level, flag, float(value))
+
+    def _lincomb(self, name, scalars, functions, level, flag):
+        c = np.ascontiguousarray(scalars, np.float64)
+        arr = (ctypes.c_void_p * max(1, len(functions)))(*[f.h for f in functions])
+        _lib.call(name, self.h, level, flag, len(functions), _dp(c), arr)
+
+    def assign(self, scalars, functions, level, flag=ALL):
+        self._lincomb("hymi_assign", scalars, functions, level, flag)
+
+    def add(self, scalars, functions, level, flag=ALL):
+        self._lincomb("hymi_add", scalars, functions, level, flag)
+
+    def dotGlobal(self, other, level, flag=ALL):
+        out = np.zeros(self.nrhs)
+        _lib.call("hymi_dot", self.h, other.h, level, flag, _dp(out))
+        return float(out[0]) if self.nrhs == 1 else out
+
+    def copyFrom(self, other, level, flag=ALL):
+        """Copy another function of any precision on one level (mixed-precision refinement)."""
+        _lib.call("hymi_convert", self.h, other.h, level, flag)
+
+
+class _P1ConstantOperator:
+    _create = None
+
+    def __init__(self, storage, minLevel, maxLevel):
+        self.storage = storage
+        self.minLevel, self.maxLevel = minLevel, maxLevel
+        h = ctypes.c_void_p()
+        _lib.call(self._create, storage.h, minLevel, maxLevel, ctypes.byref(h))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib._lib is not None:
+            _lib._lib.hymi_operator_destroy(self.h)
+            self.h = None
+
+    def stencil(self, level, cls, idx):
+        n = ctypes.c_int()
+        _lib.call("hymi_operator_stencil", self.h, level, cls, idx, None, ctypes.byref(n))
+        out = np.zeros(n.value)
+        _lib.call("hymi_operator_stencil", self.h, level, cls, idx, _dp(out), ctypes.byref(n))
+        return out
+
+    def apply(self, src, dst, level, flag, updateType=REPLACE):
+        _lib.call("hymi_apply", self.h, src.h, dst.h, level, flag, updateType)
+
+    def residual(self, x, b, r, level, flag):
+        """r = b - A x on flag DoFs."""
+        _lib.call("hymi_residual", self.h, x.h, b.h, r.h, level, flag)
+
+    def smooth_sor(self, dst, rhs, relax, level, flag):
+        _lib.call("hymi_smooth_sor", self.h, dst.h, rhs.h, float(relax), level, flag)
+
+    def smooth_gs(self, dst, rhs, level, flag):
+        self.smooth_sor(dst, rhs, 1.0, level, flag)
+
+
+class P1ConstantLaplaceOperator(_P1ConstantOperator):
+    _create = "hymi_laplace_create"
+
+
+class P1ConstantMassOperator(_P1ConstantOperator):
+    _create = "hymi_mass_create"
+
+
+class P1toP1LinearRestriction:
+    def restrict(self, function, sourceLevel, flag):
+        _lib.call("hymi_restrict", function.h, sourceLevel, flag)
+
+
+class P1toP1LinearProlongation:
+    def prolongateAndAdd(self, function, sourceLevel, flag):
+        _lib.call("hymi_prolongate_add", function.h, sourceLevel, flag)
+
+
+class SORSmoother:
+    def __init__(self, relax=1.0):
+        self.relax = relax
+
+    def solve(self, A, x, b, level):
+        A.smooth_sor(x, b, self.relax, level, INNER | NEUMANN)
+
+
+class CGSolver:
+    """CGSolver(storage, minLevel, maxLevel, maxIter, tolerance) run on the device."""
+
+    def __init__(self, storage, minLevel, maxLevel, maxIter=0, tolerance=1e-16):
+        self.maxIter, self.tolerance = maxIter, tolerance
+        self.minLevel = minLevel
+
+    def solve(self, A, x, b, level):
+        it = self.maxIter if self.maxIter > 0 else max(1, x.storage.owned_count(level))
+        _lib.call("hymi_cg_solve", A.h, x.h, b.h, level, it, float(self.tolerance))
+
+
+class GeometricMultigridSolver:
+    """GeometricMultigridSolver(storage, smoother, coarseSolver, restriction, prolongation, minLevel, maxLevel,
+    preSmoothSteps, postSmoothSteps) -- V-cycles with SOR smoothing and a device CG coarse solver."""
+
+    def __init__(self, storage, smoother, coarseSolver, restrictionOperator, prolongationOperator, minLevel,
+                 maxLevel, preSmoothSteps=3, postSmoothSteps=3):
+        self.storage = storage
+        self.smoother, self.coarse = smoother, coarseSolver
+        self.minLevel, self.maxLevel = minLevel, maxLevel
+        self.nu1, self.nu2 = preSmoothSteps, postSmoothSteps
+        self._h = {}
+
+    def _handle(self, A, x):
+        key = (id(A), x.precision, x.nrhs)
+        if key not in self._h:
+            h = ctypes.c_void_p()
+            cg_it = getattr(self.coarse, "maxIter", 0)
+            cg_tol = getattr(self.coarse, "tolerance", 1e-16)
+            _lib.call("hymi_gmg_create", A.h, x.h, self.minLevel, self.maxLevel, self.nu1, self.nu2,
+                      float(self.smoother.relax), int(cg_it), float(cg_tol), ctypes.byref(h))
+            self._h[key] = (h, A)
+        return self._h[key][0]
+
+    def solve(self, A, x, b, level):
+        _lib.call("hymi_gmg_vcycle", self._handle(A, x), x.h, b.h, level)
+
+    def solve_graph(self, A, x, b, level, count=1):
+        """Replay a hipGraph-captured V-cycle `count` times."""
+        _lib.call("hymi_gmg_vcycle_graph", self._handle(A, x), x.h, b.h, level, count)
+
+    def __del__(self):
+        if _lib._lib is None:
+            return
+        for h, _ in getattr(self, "_h", {}).values():
+            _lib._lib.hymi_gmg_destroy(h)
+        self._h = {}

@@ -1,0 +1,1053 @@
+// C-ABI of libhymi: handles, error handling, host <-> device transfers, and the multigrid orchestration
+// (CG coarse solver on device scalars, V-cycle recursion, hipGraph capture of a V-cycle).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/hymi.h"
+#include "internal.hpp"
+#include "kernels.hpp"
+
+using namespace hymi;
+
+struct hymi_ctx_s
+{
+   Context c;
+};
+struct hymi_mesh_s
+{
+   Mesh m;
+};
+struct hymi_function_s
+{
+   Function f;
+};
+struct hymi_operator_s
+{
+   Operator op;
+};
+
+namespace {
+
+thread_local std::string g_lastError;
+
+int fail( int code, const std::string& msg )
+{
+   g_lastError = msg;
+   return code;
+}
+
+template < typename Fn >
+int guarded( Fn&& fn )
+{
+   try
+   {
+      fn();
+      return HYMI_OK;
+   } catch ( const HipError& e )
+   {
+      return fail( e.code == hipErrorOutOfMemory ? HYMI_ERR_OOM : HYMI_ERR_HIP,
+                   std::string( "HIP error " ) + hipGetErrorString( e.code ) + " in " + e.where );
+   } catch ( const std::bad_alloc& )
+   {
+      return fail( HYMI_ERR_OOM, "host allocation failed" );
+   } catch ( const std::invalid_argument& e )
+   {
+      return fail( HYMI_ERR_INVALID, e.what() );
+   } catch ( const std::out_of_range& e )
+   {
+      return fail( HYMI_ERR_LEVEL, e.what() );
+   } catch ( const std::runtime_error& e )
+   {
+      const std::string w = e.what();
+      if ( w.find( "unsupported" ) != std::string::npos ) return fail( HYMI_ERR_UNSUPPORTED, w );
+      return fail( HYMI_ERR_INTERNAL, w );
+   } catch ( ... )
+   {
+      return fail( HYMI_ERR_INTERNAL, "unknown error" );
+   }
+}
+
+void require( bool ok, const char* msg )
+{
+   if ( !ok ) throw std::invalid_argument( msg );
+}
+
+void requireLevel( const Function& f, int level )
+{
+   if ( level < f.minL || level > f.maxL ) throw std::out_of_range( "level not allocated for this function" );
+}
+
+// element index of every owned DoF in owned order
+std::vector< int64_t > ownedIndex( Mesh& M, int level )
+{
+   LevelPlan&             P = M.plan( level );
+   const Topology&        T = M.T;
+   const int              w = P.w;
+   std::vector< int64_t > idx;
+   for ( int v = 0; v < T.nV(); v++ )
+      idx.push_back( v );
+   for ( int e = 0; e < T.nE(); e++ )
+      for ( int x = 1; x <= w - 2; x++ )
+         idx.push_back( P.edgeOff( e ) + x );
+   for ( int f = 0; f < T.nF(); f++ )
+      for ( int y = 1; y <= w - 3; y++ )
+         for ( int x = 1; x + y <= w - 2; x++ )
+            idx.push_back( P.faceOff( f ) + faceIndex( w, x, y ) );
+   for ( int c = 0; c < T.nC(); c++ )
+      for ( int z = 1; z <= w - 4; z++ )
+         for ( int y = 1; y + z <= w - 3; y++ )
+            for ( int x = 1; x + y + z <= w - 2; x++ )
+               idx.push_back( P.cellOff( c ) + cellIndex( w, x, y, z ) );
+   return idx;
+}
+
+std::vector< double > downloadLevel( Function& f, int level )
+{
+   Mesh&                 M = *f.mesh;
+   LevelPlan&            P = M.plan( level );
+   const int64_t         n = P.total * f.nrhs;
+   std::vector< double > out( (size_t) n );
+   HYMI_HIP( hipStreamSynchronize( M.ctx->stream ) );
+   if ( f.prec == kF64 )
+      HYMI_HIP( hipMemcpy( out.data(), f.ptr( level ), (size_t) n * 8, hipMemcpyDeviceToHost ) );
+   else
+   {
+      std::vector< float > tmp( (size_t) n );
+      HYMI_HIP( hipMemcpy( tmp.data(), f.ptr( level ), (size_t) n * 4, hipMemcpyDeviceToHost ) );
+      std::copy( tmp.begin(), tmp.end(), out.begin() );
+   }
+   return out;
+}
+
+void uploadLevel( Function& f, int level, const std::vector< double >& in )
+{
+   Mesh&         M = *f.mesh;
+   LevelPlan&    P = M.plan( level );
+   const int64_t n = P.total * f.nrhs;
+   HYMI_HIP( hipStreamSynchronize( M.ctx->stream ) );
+   if ( f.prec == kF64 )
+      HYMI_HIP( hipMemcpy( f.ptr( level ), in.data(), (size_t) n * 8, hipMemcpyHostToDevice ) );
+   else
+   {
+      std::vector< float > tmp( in.begin(), in.end() );
+      HYMI_HIP( hipMemcpy( f.ptr( level ), tmp.data(), (size_t) n * 4, hipMemcpyHostToDevice ) );
+   }
+}
+
+int64_t numCoarseUnknowns( Mesh& M, int level, int flag )
+{
+   const Topology& T = M.T;
+   const int64_t   w = widthOf( level );
+   int64_t         n = 0;
+   for ( int v = 0; v < T.nV(); v++ )
+      n += ( T.vType[v] & flag ) ? 1 : 0;
+   for ( int e = 0; e < T.nE(); e++ )
+      n += ( T.eType[e] & flag ) ? std::max< int64_t >( 0, w - 2 ) : 0;
+   for ( int f = 0; f < T.nF(); f++ )
+      n += ( T.fType[f] & flag ) ? std::max< int64_t >( 0, ( w - 3 ) * ( w - 2 ) / 2 ) : 0;
+   for ( int c = 0; c < T.nC(); c++ )
+      n += ( T.cType[c] & flag ) ? std::max< int64_t >( 0, ( w - 4 ) * ( w - 3 ) * ( w - 2 ) / 6 ) : 0;
+   return n;
+}
+
+// scalar slots used by the device CG
+enum
+{
+   sRsOld = 0,
+   sPAp   = 1,
+   sAlpha = 2,
+   sRsNew = 3,
+   sBeta  = 4,
+   sUser  = 8,
+   kNumSlots = 16
+};
+
+void ensureScalars( Context& ctx, int nrhs )
+{
+   if ( (int64_t) ctx.scalars.n < (int64_t) kNumSlots * nrhs )
+   {
+      ctx.scalars.alloc( (size_t) kNumSlots * 32 );
+      HYMI_HIP( hipMemsetAsync( ctx.scalars.p, 0, ctx.scalars.n * sizeof( double ), ctx.stream ) );
+   }
+}
+
+struct CgWork
+{
+   std::unique_ptr< Function > p, r, ap;
+};
+
+std::unique_ptr< Function > makeFunction( Mesh& M, int minL, int maxL, int prec, int nrhs )
+{
+   if ( nrhs != 1 && nrhs != 32 ) throw std::runtime_error( "unsupported nrhs (1 or 32)" );
+   if ( prec != kF64 && prec != kF32 ) throw std::invalid_argument( "precision must be HYMI_F64 or HYMI_F32" );
+   if ( minL < 0 || maxL < minL || maxL > 12 ) throw std::invalid_argument( "invalid level range" );
+   auto F  = std::make_unique< Function >();
+   F->mesh = &M;
+   F->minL = minL;
+   F->maxL = maxL;
+   F->nrhs = nrhs;
+   F->prec = prec;
+   for ( int L = minL; L <= maxL; L++ )
+   {
+      LevelPlan&   P     = M.plan( L );
+      const size_t bytes = (size_t) P.total * nrhs * F->elemBytes();
+      void*        d     = nullptr;
+      hipError_t   e     = hipMalloc( &d, bytes );
+      if ( e != hipSuccess )
+      {
+         (void) hipGetLastError();
+         throw HipError{ hipErrorOutOfMemory, "hipMalloc of function level " + std::to_string( L ) };
+      }
+      HYMI_HIP( hipMemsetAsync( d, 0, bytes, M.ctx->stream ) );
+      F->data.push_back( d );
+      F->count.push_back( P.total * nrhs );
+      M.ctx->bytesInUse += bytes;
+   }
+   HYMI_HIP( hipStreamSynchronize( M.ctx->stream ) );
+   return F;
+}
+
+void cgSolve( Operator& A, Function& x, Function& b, int level, int maxIter, double tol, CgWork& W )
+{
+   Context&  ctx  = *A.mesh->ctx;
+   const int flag = kInner | kNeumann | kFreeslip;
+   const int nr   = x.nrhs;
+   ensureScalars( ctx, nr );
+   if ( maxIter <= 0 ) return;
+   const double g = tol * tol;
+   launchApply( A, x, *W.r, &b, level, flag, kResidual );
+   const double one[1] = { 1.0 };
+   Function*    rr[1]  = { W.r.get() };
+   launchLincomb( *W.p, level, flag, 1, one, rr, false, 0.0 );
+   launchDot( *W.r, *W.r, level, flag, sRsOld );
+   for ( int it = 0; it < maxIter; it++ )
+   {
+      launchApply( A, *W.p, *W.ap, nullptr, level, flag, kReplace );
+      launchDot( *W.p, *W.ap, level, flag, sPAp );
+      launchScalarRatio( ctx, nr, sAlpha, sRsOld, sPAp, sRsOld, g );
+      launchAxpyScalar( x, *W.p, level, flag, sAlpha, 1.0 );
+      launchAxpyScalar( *W.r, *W.ap, level, flag, sAlpha, -1.0 );
+      launchDot( *W.r, *W.r, level, flag, sRsNew );
+      launchScalarRatio( ctx, nr, sBeta, sRsNew, sRsOld, sRsNew, g );
+      launchScalarCopy( ctx, nr, sRsOld, sRsNew );
+      launchXpbyScalar( *W.p, *W.r, level, flag, sBeta );
+   }
+}
+
+} // namespace
+
+struct hymi_gmg_s
+{
+   Operator*                   A = nullptr;
+   int                         minL = 0, maxL = 0, nu1 = 3, nu2 = 3, prec = kF64, nrhs = 1;
+   double                      omega = 1.0, cgTol = 1e-16;
+   int                         cgIters = 0;
+   std::unique_ptr< Function > tmp;
+   CgWork                      cg;
+   std::map< std::tuple< void*, void*, int >, hipGraphExec_t > graphs;
+   ~hymi_gmg_s()
+   {
+      for ( auto& kv : graphs )
+         (void) hipGraphExecDestroy( kv.second );
+   }
+
+   void recurse( Function& x, Function& b, int L )
+   {
+      const int flag = kInner | kNeumann;
+      if ( L == minL )
+      {
+         cgSolve( *A, x, b, L, cgIters, cgTol, cg );
+         return;
+      }
+      for ( int i = 0; i < nu1; i++ )
+         launchSor( *A, x, b, omega, L, flag );
+      launchApply( *A, x, *tmp, &b, L, flag, kResidual );
+      // restriction written straight into b on the coarse level (== restrict + b.assign on flag DoFs)
+      launchRestrict( *tmp, b, L, flag );
+      launchLincomb( x, L - 1, kAll, 0, nullptr, nullptr, false, 0.0 );
+      recurse( x, b, L - 1 );
+      launchProlongateAdd( x, L - 1, flag );
+      for ( int i = 0; i < nu2; i++ )
+         launchSor( *A, x, b, omega, L, flag );
+   }
+};
+
+// ------------------------------------------------------------------------------------------------------
+extern "C" {
+
+const char* hymi_last_error_string( void ) { return g_lastError.c_str(); }
+int         hymi_version( void ) { return 1; }
+
+int hymi_ctx_create( int device, hymi_ctx* out )
+{
+   return guarded( [&] {
+      require( out != nullptr, "out is null" );
+      if ( device < 0 )
+      {
+         // host-only context: meshes, plans and stencil weights can be built and queried, nothing runs
+         auto c        = std::make_unique< hymi_ctx_s >();
+         c->c.device   = -1;
+         c->c.hostOnly = true;
+         *out          = c.release();
+         return;
+      }
+      int n = 0;
+      HYMI_HIP( hipGetDeviceCount( &n ) );
+      require( device >= 0 && device < n, "device index out of range" );
+      HYMI_HIP( hipSetDevice( device ) );
+      auto c      = std::make_unique< hymi_ctx_s >();
+      c->c.device = device;
+      HYMI_HIP( hipStreamCreateWithFlags( &c->c.stream, hipStreamNonBlocking ) );
+      ensureScalars( c->c, 32 );
+      c->c.partials.alloc( 1 << 16 );
+      HYMI_HIP( hipStreamSynchronize( c->c.stream ) );
+      *out = c.release();
+   } );
+}
+
+int hymi_ctx_destroy( hymi_ctx ctx )
+{
+   return guarded( [&] {
+      if ( !ctx ) return;
+      if ( !ctx->c.hostOnly )
+      {
+         (void) hipStreamSynchronize( ctx->c.stream );
+         ctx->c.partials.release();
+         ctx->c.scalars.release();
+         (void) hipStreamDestroy( ctx->c.stream );
+      }
+      delete ctx;
+   } );
+}
+
+int hymi_ctx_synchronize( hymi_ctx ctx )
+{
+   return guarded( [&] {
+      require( ctx != nullptr, "null context" );
+      HYMI_HIP( hipStreamSynchronize( ctx->c.stream ) );
+   } );
+}
+
+int hymi_ctx_bytes_in_use( hymi_ctx ctx, int64_t* out )
+{
+   return guarded( [&] {
+      require( ctx && out, "null argument" );
+      *out = (int64_t) ctx->c.bytesInUse;
+   } );
+}
+
+static int makeMesh( hymi_ctx ctx, Topology&& T, hymi_mesh* out )
+{
+   return guarded( [&] {
+      require( ctx && out, "null argument" );
+      if ( !ctx->c.hostOnly ) HYMI_HIP( hipSetDevice( ctx->c.device ) );
+      auto m   = std::make_unique< hymi_mesh_s >();
+      m->m.ctx = &ctx->c;
+      m->m.T   = std::move( T );
+      *out     = m.release();
+   } );
+}
+
+int hymi_mesh_create_cuboid( hymi_ctx ctx, const double lo[3], const double hi[3], int nx, int ny, int nz,
+                             hymi_mesh* out )
+{
+   Topology T;
+   int      rc = guarded( [&] {
+      require( lo && hi, "null bounds" );
+      T = Topology::cuboid( lo, hi, nx, ny, nz );
+   } );
+   if ( rc != HYMI_OK ) return rc;
+   return makeMesh( ctx, std::move( T ), out );
+}
+
+int hymi_mesh_create( hymi_ctx ctx, int nv, const double* coords, int nc, const int* cells, hymi_mesh* out )
+{
+   Topology T;
+   int      rc = guarded( [&] {
+      require( nv > 0 && nc > 0 && coords && cells, "empty mesh" );
+      std::vector< std::array< double, 3 > > X( nv );
+      std::vector< std::array< int, 4 > >    C( nc );
+      for ( int i = 0; i < nv; i++ )
+         X[i] = { coords[3 * i], coords[3 * i + 1], coords[3 * i + 2] };
+      for ( int i = 0; i < nc; i++ )
+         C[i] = { cells[4 * i], cells[4 * i + 1], cells[4 * i + 2], cells[4 * i + 3] };
+      T = Topology::fromCells( X, C );
+   } );
+   if ( rc != HYMI_OK ) return rc;
+   return makeMesh( ctx, std::move( T ), out );
+}
+
+int hymi_mesh_from_gmsh( hymi_ctx ctx, const char* path, hymi_mesh* out )
+{
+   Topology T;
+   int      rc = guarded( [&] {
+      require( path != nullptr, "null path" );
+      T = Topology::fromGmsh( path );
+   } );
+   if ( rc != HYMI_OK ) return rc;
+   return makeMesh( ctx, std::move( T ), out );
+}
+
+int hymi_mesh_destroy( hymi_mesh mesh )
+{
+   return guarded( [&] {
+      if ( !mesh ) return;
+      if ( !mesh->m.ctx->hostOnly ) (void) hipStreamSynchronize( mesh->m.ctx->stream );
+      delete mesh;
+   } );
+}
+
+int hymi_mesh_counts( hymi_mesh mesh, int out[4] )
+{
+   return guarded( [&] {
+      require( mesh && out, "null argument" );
+      out[0] = mesh->m.T.nV();
+      out[1] = mesh->m.T.nE();
+      out[2] = mesh->m.T.nF();
+      out[3] = mesh->m.T.nC();
+   } );
+}
+
+int hymi_mesh_primitive_type( hymi_mesh mesh, int cls, int idx, int* out )
+{
+   return guarded( [&] {
+      require( mesh && out, "null argument" );
+      const Topology& T = mesh->m.T;
+      switch ( cls )
+      {
+      case HYMI_VERTEX: *out = T.vType.at( idx ); break;
+      case HYMI_EDGE: *out = T.eType.at( idx ); break;
+      case HYMI_FACE: *out = T.fType.at( idx ); break;
+      case HYMI_CELL: *out = T.cType.at( idx ); break;
+      default: throw std::invalid_argument( "bad primitive class" );
+      }
+   } );
+}
+
+int hymi_mesh_owned_count( hymi_mesh mesh, int level, int64_t* out )
+{
+   return guarded( [&] {
+      require( mesh && out && level >= 0, "bad argument" );
+      *out = numCoarseUnknowns( mesh->m, level, kAll );
+   } );
+}
+
+int hymi_mesh_owned_coords( hymi_mesh mesh, int level, double* xyz )
+{
+   return guarded( [&] {
+      require( mesh && xyz && level >= 0, "bad argument" );
+      const Topology& T = mesh->m.T;
+      const int       w = widthOf( level );
+      const double    h = 1.0 / ( w - 1 );
+      int64_t         n = 0;
+      auto            put = [&]( double x, double y, double z ) {
+         xyz[3 * n] = x;
+         xyz[3 * n + 1] = y;
+         xyz[3 * n + 2] = z;
+         n++;
+      };
+      for ( int v = 0; v < T.nV(); v++ )
+         put( T.vx[v][0], T.vx[v][1], T.vx[v][2] );
+      for ( int e = 0; e < T.nE(); e++ )
+      {
+         const auto &a = T.vx[T.ev[e][0]], &b = T.vx[T.ev[e][1]];
+         for ( int x = 1; x <= w - 2; x++ )
+            put( a[0] + x * h * ( b[0] - a[0] ), a[1] + x * h * ( b[1] - a[1] ), a[2] + x * h * ( b[2] - a[2] ) );
+      }
+      for ( int f = 0; f < T.nF(); f++ )
+      {
+         const auto &a = T.vx[T.fv[f][0]], &b = T.vx[T.fv[f][1]], &c = T.vx[T.fv[f][2]];
+         for ( int y = 1; y <= w - 3; y++ )
+            for ( int x = 1; x + y <= w - 2; x++ )
+               put( a[0] + h * ( x * ( b[0] - a[0] ) + y * ( c[0] - a[0] ) ),
+                    a[1] + h * ( x * ( b[1] - a[1] ) + y * ( c[1] - a[1] ) ),
+                    a[2] + h * ( x * ( b[2] - a[2] ) + y * ( c[2] - a[2] ) ) );
+      }
+      for ( int c = 0; c < T.nC(); c++ )
+      {
+         const auto &p0 = T.vx[T.cv[c][0]], &p1 = T.vx[T.cv[c][1]], &p2 = T.vx[T.cv[c][2]], &p3 = T.vx[T.cv[c][3]];
+         for ( int z = 1; z <= w - 4; z++ )
+            for ( int y = 1; y + z <= w - 3; y++ )
+               for ( int x = 1; x + y + z <= w - 2; x++ )
+               {
+                  double q[3];
+                  for ( int d = 0; d < 3; d++ )
+                     q[d] = p0[d] + h * ( x * ( p1[d] - p0[d] ) + y * ( p2[d] - p0[d] ) + z * ( p3[d] - p0[d] ) );
+                  put( q[0], q[1], q[2] );
+               }
+      }
+   } );
+}
+
+int hymi_mesh_owned_types( hymi_mesh mesh, int level, int* types )
+{
+   return guarded( [&] {
+      require( mesh && types && level >= 0, "bad argument" );
+      const Topology& T = mesh->m.T;
+      const int64_t   w = widthOf( level );
+      int64_t         n = 0;
+      for ( int v = 0; v < T.nV(); v++ )
+         types[n++] = T.vType[v];
+      for ( int e = 0; e < T.nE(); e++ )
+         for ( int64_t k = 0; k < w - 2; k++ )
+            types[n++] = T.eType[e];
+      for ( int f = 0; f < T.nF(); f++ )
+         for ( int64_t k = 0; k < std::max< int64_t >( 0, ( w - 3 ) * ( w - 2 ) / 2 ); k++ )
+            types[n++] = T.fType[f];
+      for ( int c = 0; c < T.nC(); c++ )
+         for ( int64_t k = 0; k < std::max< int64_t >( 0, ( w - 4 ) * ( w - 3 ) * ( w - 2 ) / 6 ); k++ )
+            types[n++] = T.cType[c];
+   } );
+}
+
+int hymi_function_create( hymi_mesh mesh, int minL, int maxL, int precision, int nrhs, hymi_function* out )
+{
+   return guarded( [&] {
+      require( mesh && out, "null argument" );
+      require( !mesh->m.ctx->hostOnly, "host-only context: functions need a device" );
+      HYMI_HIP( hipSetDevice( mesh->m.ctx->device ) );
+      auto F  = makeFunction( mesh->m, minL, maxL, precision, nrhs );
+      auto h  = std::make_unique< hymi_function_s >();
+      h->f.mesh  = F->mesh;
+      h->f.minL  = F->minL;
+      h->f.maxL  = F->maxL;
+      h->f.nrhs  = F->nrhs;
+      h->f.prec  = F->prec;
+      h->f.data  = std::move( F->data );
+      h->f.count = std::move( F->count );
+      F->data.clear();
+      *out = h.release();
+   } );
+}
+
+Function::~Function()
+{
+   for ( size_t k = 0; k < data.size(); k++ )
+   {
+      if ( data[k] )
+      {
+         (void) hipFree( data[k] );
+         if ( mesh && mesh->ctx ) mesh->ctx->bytesInUse -= (size_t) count[k] * elemBytes();
+      }
+   }
+}
+
+int hymi_function_destroy( hymi_function f )
+{
+   return guarded( [&] {
+      if ( !f ) return;
+      (void) hipStreamSynchronize( f->f.mesh->ctx->stream );
+      delete f;
+   } );
+}
+
+int hymi_function_upload_owned( hymi_function fh, int level, const double* host )
+{
+   return guarded( [&] {
+      require( fh && host, "null argument" );
+      Function& f = fh->f;
+      requireLevel( f, level );
+      auto idx = ownedIndex( *f.mesh, level );
+      auto buf = downloadLevel( f, level );
+      for ( size_t k = 0; k < idx.size(); k++ )
+         for ( int r = 0; r < f.nrhs; r++ )
+            buf[(size_t) idx[k] * f.nrhs + r] = host[k * f.nrhs + r];
+      uploadLevel( f, level, buf );
+   } );
+}
+
+int hymi_function_download_owned( hymi_function fh, int level, double* host )
+{
+   return guarded( [&] {
+      require( fh && host, "null argument" );
+      Function& f = fh->f;
+      requireLevel( f, level );
+      auto idx = ownedIndex( *f.mesh, level );
+      auto buf = downloadLevel( f, level );
+      for ( size_t k = 0; k < idx.size(); k++ )
+         for ( int r = 0; r < f.nrhs; r++ )
+            host[k * f.nrhs + r] = buf[(size_t) idx[k] * f.nrhs + r];
+   } );
+}
+
+// reference per-primitive layout (VertexDoFMemory.hpp:44-69, VertexDoFIndexing.cpp:36-96,205-225)
+static int64_t primSize( const Topology& T, int level, int cls, int idx )
+{
+   const int64_t w = widthOf( level );
+   switch ( cls )
+   {
+   case HYMI_VERTEX: return 1 + (int64_t) T.vEdges.at( idx ).size();
+   case HYMI_EDGE: return w + (int64_t) T.eFaces.at( idx ).size() * ( w - 1 ) + (int64_t) T.eCells.at( idx ).size() * ( w - 2 );
+   case HYMI_FACE: return triSize( w ) + (int64_t) T.fCells.at( idx ).size() * triSize( w - 1 );
+   case HYMI_CELL: T.cv.at( idx ); return tetSize( w );
+   default: throw std::invalid_argument( "bad primitive class" );
+   }
+}
+
+int hymi_function_primitive_size( hymi_function fh, int level, int cls, int idx, int64_t* out )
+{
+   return guarded( [&] {
+      require( fh && out, "null argument" );
+      requireLevel( fh->f, level );
+      *out = primSize( fh->f.mesh->T, level, cls, idx );
+   } );
+}
+
+int hymi_function_download_primitive( hymi_function fh, int level, int cls, int idx, double* host )
+{
+   return guarded( [&] {
+      require( fh && host, "null argument" );
+      Function& f = fh->f;
+      require( f.nrhs == 1, "per-primitive transfer needs nrhs == 1" );
+      requireLevel( f, level );
+      launchRefresh( f, level, 7 );
+      Mesh&           M   = *f.mesh;
+      const Topology& T   = M.T;
+      LevelPlan&      P   = M.plan( level );
+      const int       w   = P.w;
+      auto            buf = downloadLevel( f, level );
+      switch ( cls )
+      {
+      case HYMI_VERTEX:
+      {
+         host[0] = buf[idx];
+         const VertexGeom& g = P.hVertexGeom.at( idx );
+         for ( int k = 0; k < g.nTerms; k++ )
+            host[1 + k] = buf[P.hVertexTerms[g.termOff + k]];
+         break;
+      }
+      case HYMI_EDGE:
+      {
+         for ( int t = 0; t < w; t++ )
+            host[t] = buf[P.edgeOff( idx ) + t];
+         const EdgeGeom& g  = P.hEdgeGeom.at( idx );
+         const int       nF = (int) T.eFaces[idx].size();
+         // face layers: reference index t = edge point of the layer point (0..w-2); W term of edge point
+         // x reads t = x-1, E term reads t = x
+         for ( int fi = 0; fi < nF; fi++ )
+         {
+            const EdgeTerm& e = P.hEdgeTerms[g.termOff + 2 * fi + 1];
+            for ( int t = 0; t <= w - 2; t++ )
+               host[w + fi * ( w - 1 ) + t] =
+                   buf[P.faceOff( e.prim ) + faceIndex( w, e.base[0] + t * e.inc[0], e.base[1] + t * e.inc[1] )];
+         }
+         for ( int ci = 0; ci < (int) T.eCells[idx].size(); ci++ )
+         {
+            const EdgeTerm& e = P.hEdgeTerms[g.termOff + 2 * nF + ci];
+            for ( int t = 0; t <= w - 3; t++ )
+            {
+               const int x = t + 1;
+               host[w + nF * ( w - 1 ) + ci * ( w - 2 ) + t] =
+                   buf[P.cellOff( e.prim ) +
+                       cellIndex( w, e.base[0] + x * e.inc[0], e.base[1] + x * e.inc[1], e.base[2] + x * e.inc[2] )];
+            }
+         }
+         break;
+      }
+      case HYMI_FACE:
+      {
+         for ( int j = 0; j < w; j++ )
+            for ( int i = 0; i + j < w; i++ )
+               host[faceIndex( w, i, j )] = buf[P.faceOff( idx ) + faceIndex( w, i, j )];
+         const FaceGeom& g = P.hFaceGeom.at( idx );
+         for ( int n = 0; n < g.nSides; n++ )
+         {
+            const FaceCellMap& m  = g.side[n];
+            const int64_t      gl = triSize( w ) + n * triSize( w - 1 );
+            for ( int j = 0; j < w - 1; j++ )
+               for ( int i = 0; i + j < w - 1; i++ )
+               {
+                  const int x = m.base[0] + i * m.a[0] + j * m.b[0] + m.o[0];
+                  const int y = m.base[1] + i * m.a[1] + j * m.b[1] + m.o[1];
+                  const int z = m.base[2] + i * m.a[2] + j * m.b[2] + m.o[2];
+                  host[gl + faceIndex( w - 1, i, j )] = buf[P.cellOff( m.cell ) + cellIndex( w, x, y, z )];
+               }
+         }
+         break;
+      }
+      case HYMI_CELL:
+         for ( int64_t k = 0; k < tetSize( w ); k++ )
+            host[k] = buf[P.cellOff( idx ) + k];
+         break;
+      default: throw std::invalid_argument( "bad primitive class" );
+      }
+   } );
+}
+
+int hymi_function_upload_primitive( hymi_function fh, int level, int cls, int idx, const double* host )
+{
+   return guarded( [&] {
+      require( fh && host, "null argument" );
+      Function& f = fh->f;
+      require( f.nrhs == 1, "per-primitive transfer needs nrhs == 1" );
+      requireLevel( f, level );
+      Mesh&      M   = *f.mesh;
+      LevelPlan& P   = M.plan( level );
+      const int  w   = P.w;
+      auto       buf = downloadLevel( f, level );
+      switch ( cls )
+      {
+      case HYMI_VERTEX: buf[idx] = host[0]; break;
+      case HYMI_EDGE:
+         M.T.ev.at( idx );
+         for ( int x = 1; x <= w - 2; x++ )
+            buf[P.edgeOff( idx ) + x] = host[x];
+         break;
+      case HYMI_FACE:
+         M.T.fv.at( idx );
+         for ( int j = 1; j <= w - 3; j++ )
+            for ( int i = 1; i + j <= w - 2; i++ )
+               buf[P.faceOff( idx ) + faceIndex( w, i, j )] = host[faceIndex( w, i, j )];
+         break;
+      case HYMI_CELL:
+         M.T.cv.at( idx );
+         for ( int z = 1; z <= w - 4; z++ )
+            for ( int y = 1; y + z <= w - 3; y++ )
+               for ( int x = 1; x + y + z <= w - 2; x++ )
+                  buf[P.cellOff( idx ) + cellIndex( w, x, y, z )] = host[cellIndex( w, x, y, z )];
+         break;
+      default: throw std::invalid_argument( "bad primitive class" );
+      }
+      uploadLevel( f, level, buf );
+   } );
+}
+
+int hymi_interpolate_const( hymi_function f, int level, int flag, double value )
+{
+   return guarded( [&] {
+      require( f != nullptr, "null function" );
+      requireLevel( f->f, level );
+      launchLincomb( f->f, level, flag, 0, nullptr, nullptr, false, value );
+   } );
+}
+
+static void lincombApi( hymi_function dst, int level, int flag, int n, const double* coeffs, const hymi_function* srcs,
+                        bool acc )
+{
+   require( dst != nullptr, "null function" );
+   require( n >= 0 && n <= 3, "at most three source functions" );
+   requireLevel( dst->f, level );
+   Function* s[3] = { nullptr, nullptr, nullptr };
+   for ( int k = 0; k < n; k++ )
+   {
+      require( srcs && srcs[k], "null source function" );
+      require( srcs[k]->f.mesh == dst->f.mesh && srcs[k]->f.prec == dst->f.prec && srcs[k]->f.nrhs == dst->f.nrhs,
+               "source functions must share mesh, precision and nrhs" );
+      requireLevel( srcs[k]->f, level );
+      s[k] = &srcs[k]->f;
+   }
+   launchLincomb( dst->f, level, flag, n, coeffs, s, acc, 0.0 );
+}
+
+int hymi_assign( hymi_function dst, int level, int flag, int n, const double* coeffs, const hymi_function* srcs )
+{
+   return guarded( [&] { lincombApi( dst, level, flag, n, coeffs, srcs, false ); } );
+}
+
+int hymi_add( hymi_function dst, int level, int flag, int n, const double* coeffs, const hymi_function* srcs )
+{
+   return guarded( [&] { lincombApi( dst, level, flag, n, coeffs, srcs, true ); } );
+}
+
+int hymi_dot( hymi_function a, hymi_function b, int level, int flag, double* out )
+{
+   return guarded( [&] {
+      require( a && b && out, "null argument" );
+      require( a->f.mesh == b->f.mesh && a->f.prec == b->f.prec && a->f.nrhs == b->f.nrhs, "incompatible functions" );
+      requireLevel( a->f, level );
+      requireLevel( b->f, level );
+      Context& ctx = *a->f.mesh->ctx;
+      launchDot( a->f, b->f, level, flag, sUser );
+      HYMI_HIP( hipMemcpyAsync( out, ctx.scalars.p + sUser * a->f.nrhs, sizeof( double ) * a->f.nrhs,
+                                hipMemcpyDeviceToHost, ctx.stream ) );
+      HYMI_HIP( hipStreamSynchronize( ctx.stream ) );
+   } );
+}
+
+int hymi_convert( hymi_function dst, hymi_function src, int level, int flag )
+{
+   return guarded( [&] {
+      require( dst && src, "null argument" );
+      requireLevel( dst->f, level );
+      requireLevel( src->f, level );
+      launchConvert( dst->f, level, src->f, level, flag );
+   } );
+}
+
+static int makeOperator( hymi_mesh mesh, int minL, int maxL, int form, hymi_operator* out )
+{
+   return guarded( [&] {
+      require( mesh && out, "null argument" );
+      require( minL >= 0 && maxL >= minL && maxL <= 12, "invalid level range" );
+      auto h     = std::make_unique< hymi_operator_s >();
+      h->op.mesh = &mesh->m;
+      h->op.minL = minL;
+      h->op.maxL = maxL;
+      h->op.form = form;
+      for ( int L = minL; L <= maxL; L++ )
+      {
+         h->op.lw.push_back( std::make_unique< Operator::LevelWeights >() );
+         buildOperatorWeights( h->op, L );
+      }
+      *out = h.release();
+   } );
+}
+
+int hymi_laplace_create( hymi_mesh mesh, int minL, int maxL, hymi_operator* out )
+{
+   return makeOperator( mesh, minL, maxL, 0, out );
+}
+
+int hymi_mass_create( hymi_mesh mesh, int minL, int maxL, hymi_operator* out )
+{
+   return makeOperator( mesh, minL, maxL, 1, out );
+}
+
+int hymi_operator_destroy( hymi_operator op )
+{
+   return guarded( [&] {
+      if ( !op ) return;
+      if ( !op->op.mesh->ctx->hostOnly ) (void) hipStreamSynchronize( op->op.mesh->ctx->stream );
+      delete op;
+   } );
+}
+
+int hymi_operator_stencil( hymi_operator op, int level, int cls, int idx, double* out, int* count )
+{
+   return guarded( [&] {
+      require( op && count, "null argument" );
+      if ( level < op->op.minL || level > op->op.maxL ) throw std::out_of_range( "level not allocated for operator" );
+      auto&       W = op->op.weights( level );
+      LevelPlan&  P = op->op.mesh->plan( level );
+      const double* src = nullptr;
+      int           n   = 0;
+      switch ( cls )
+      {
+      case HYMI_CELL:
+         op->op.mesh->T.cv.at( idx );
+         src = &W.hCell[(size_t) idx * kCellW];
+         n   = kCellW;
+         break;
+      case HYMI_FACE:
+         op->op.mesh->T.fv.at( idx );
+         src = &W.hFace[(size_t) idx * kFaceW];
+         n   = kFaceW;
+         break;
+      case HYMI_EDGE:
+      {
+         const EdgeGeom& g = P.hEdgeGeom.at( idx );
+         src               = &W.hEdge[3 * (size_t) idx + g.termOff];
+         n                 = 3 + g.nTerms;
+         break;
+      }
+      case HYMI_VERTEX:
+      {
+         const VertexGeom& g = P.hVertexGeom.at( idx );
+         src                 = &W.hVertex[(size_t) idx + g.termOff];
+         n                   = 1 + g.nTerms;
+         break;
+      }
+      default: throw std::invalid_argument( "bad primitive class" );
+      }
+      if ( out ) std::memcpy( out, src, sizeof( double ) * n );
+      *count = n;
+   } );
+}
+
+static void checkOpFunc( hymi_operator op, hymi_function f, int level )
+{
+   require( op && f, "null argument" );
+   require( op->op.mesh == f->f.mesh, "operator and function live on different meshes" );
+   if ( level < op->op.minL || level > op->op.maxL ) throw std::out_of_range( "level not allocated for operator" );
+   requireLevel( f->f, level );
+}
+
+int hymi_apply( hymi_operator op, hymi_function src, hymi_function dst, int level, int flag, int update )
+{
+   return guarded( [&] {
+      checkOpFunc( op, src, level );
+      checkOpFunc( op, dst, level );
+      require( src != dst, "src and dst must differ" );
+      require( src->f.prec == dst->f.prec && src->f.nrhs == dst->f.nrhs, "src and dst must share precision and nrhs" );
+      require( update == HYMI_REPLACE || update == HYMI_ADD, "bad update type" );
+      launchApply( op->op, src->f, dst->f, nullptr, level, flag, update == HYMI_ADD ? kAdd : kReplace );
+   } );
+}
+
+int hymi_residual( hymi_operator op, hymi_function x, hymi_function b, hymi_function r, int level, int flag )
+{
+   return guarded( [&] {
+      checkOpFunc( op, x, level );
+      checkOpFunc( op, b, level );
+      checkOpFunc( op, r, level );
+      require( x != r, "x and r must differ" );
+      require( x->f.prec == r->f.prec && x->f.prec == b->f.prec && x->f.nrhs == r->f.nrhs && x->f.nrhs == b->f.nrhs,
+               "functions must share precision and nrhs" );
+      launchApply( op->op, x->f, r->f, &b->f, level, flag, kResidual );
+   } );
+}
+
+int hymi_smooth_sor( hymi_operator op, hymi_function x, hymi_function b, double omega, int level, int flag )
+{
+   return guarded( [&] {
+      checkOpFunc( op, x, level );
+      checkOpFunc( op, b, level );
+      require( x != b, "x and b must differ" );
+      require( x->f.prec == b->f.prec && x->f.nrhs == b->f.nrhs, "functions must share precision and nrhs" );
+      launchSor( op->op, x->f, b->f, omega, level, flag );
+   } );
+}
+
+int hymi_restrict( hymi_function f, int srcLevel, int flag )
+{
+   return guarded( [&] {
+      require( f != nullptr, "null function" );
+      requireLevel( f->f, srcLevel );
+      requireLevel( f->f, srcLevel - 1 );
+      launchRestrict( f->f, f->f, srcLevel, flag );
+   } );
+}
+
+int hymi_prolongate_add( hymi_function f, int srcLevel, int flag )
+{
+   return guarded( [&] {
+      require( f != nullptr, "null function" );
+      requireLevel( f->f, srcLevel );
+      requireLevel( f->f, srcLevel + 1 );
+      launchProlongateAdd( f->f, srcLevel, flag );
+   } );
+}
+
+int hymi_cg_solve( hymi_operator op, hymi_function x, hymi_function b, int level, int maxIter, double tol )
+{
+   return guarded( [&] {
+      checkOpFunc( op, x, level );
+      checkOpFunc( op, b, level );
+      Function& xf = x->f;
+      CgWork    W;
+      W.p  = makeFunction( *xf.mesh, level, level, xf.prec, xf.nrhs );
+      W.r  = makeFunction( *xf.mesh, level, level, xf.prec, xf.nrhs );
+      W.ap = makeFunction( *xf.mesh, level, level, xf.prec, xf.nrhs );
+      cgSolve( op->op, xf, b->f, level, maxIter, tol, W );
+      HYMI_HIP( hipStreamSynchronize( xf.mesh->ctx->stream ) );
+   } );
+}
+
+int hymi_gmg_create( hymi_operator op, hymi_function xt, int minL, int maxL, int nu1, int nu2, double omega,
+                     int cgMaxIter, double cgTol, hymi_gmg* out )
+{
+   return guarded( [&] {
+      require( op && xt && out, "null argument" );
+      require( minL >= op->op.minL && maxL <= op->op.maxL && minL <= maxL, "levels outside the operator's range" );
+      require( nu1 >= 0 && nu2 >= 0, "negative smoothing steps" );
+      Mesh& M   = *op->op.mesh;
+      auto  g   = std::make_unique< hymi_gmg_s >();
+      g->A      = &op->op;
+      g->minL   = minL;
+      g->maxL   = maxL;
+      g->nu1    = nu1;
+      g->nu2    = nu2;
+      g->omega  = omega;
+      g->cgTol  = cgTol;
+      g->prec   = xt->f.prec;
+      g->nrhs   = xt->f.nrhs;
+      g->cgIters = cgMaxIter > 0 ? cgMaxIter : (int) numCoarseUnknowns( M, minL, kInner | kNeumann | kFreeslip );
+      g->tmp    = makeFunction( M, minL, maxL, g->prec, g->nrhs );
+      g->cg.p   = makeFunction( M, minL, minL, g->prec, g->nrhs );
+      g->cg.r   = makeFunction( M, minL, minL, g->prec, g->nrhs );
+      g->cg.ap  = makeFunction( M, minL, minL, g->prec, g->nrhs );
+      // build every table a V-cycle uses now, so that a V-cycle never allocates (graph capture)
+      const int segLen = 64 / g->nrhs;
+      int64_t   maxBlocks = 0;
+      for ( int L = minL; L <= maxL; L++ )
+      {
+         LevelPlan& P = M.plan( L );
+         int64_t    n = 0;
+         M.segments( P, segLen, &n );
+         maxBlocks = std::max( maxBlocks, ( n + 3 ) / 4 + 64 );
+         M.closureRanges( P, kInner | kNeumann, &n );
+         M.closureRanges( P, kInner | kNeumann | kFreeslip, &n );
+         M.closureRanges( P, kAll, &n );
+      }
+      Context& ctx = *M.ctx;
+      ensureScalars( ctx, g->nrhs );
+      if ( (int64_t) ctx.partials.n < maxBlocks * g->nrhs ) ctx.partials.alloc( (size_t) ( maxBlocks * g->nrhs ) );
+      HYMI_HIP( hipStreamSynchronize( ctx.stream ) );
+      *out = g.release();
+   } );
+}
+
+int hymi_gmg_destroy( hymi_gmg g )
+{
+   return guarded( [&] {
+      if ( !g ) return;
+      (void) hipStreamSynchronize( g->A->mesh->ctx->stream );
+      delete g;
+   } );
+}
+
+static void checkGmg( hymi_gmg g, hymi_function x, hymi_function b, int level )
+{
+   require( g && x && b, "null argument" );
+   require( x->f.mesh == g->A->mesh && b->f.mesh == g->A->mesh, "functions live on another mesh" );
+   require( x->f.prec == g->prec && b->f.prec == g->prec && x->f.nrhs == g->nrhs && b->f.nrhs == g->nrhs,
+            "functions must match the solver's precision and nrhs" );
+   require( level >= g->minL && level <= g->maxL, "level outside the solver's range" );
+   requireLevel( x->f, g->minL );
+   requireLevel( x->f, level );
+   requireLevel( b->f, g->minL );
+   requireLevel( b->f, level );
+}
+
+int hymi_gmg_vcycle( hymi_gmg g, hymi_function x, hymi_function b, int level )
+{
+   return guarded( [&] {
+      checkGmg( g, x, b, level );
+      g->recurse( x->f, b->f, level );
+   } );
+}
+
+int hymi_gmg_vcycle_graph( hymi_gmg g, hymi_function x, hymi_function b, int level, int count )
+{
+   return guarded( [&] {
+      checkGmg( g, x, b, level );
+      require( count >= 0, "negative count" );
+      hipStream_t s   = g->A->mesh->ctx->stream;
+      auto        key = std::make_tuple( (void*) x, (void*) b, level );
+      auto        it  = g->graphs.find( key );
+      if ( it == g->graphs.end() )
+      {
+         hipGraph_t graph = nullptr;
+         HYMI_HIP( hipStreamBeginCapture( s, hipStreamCaptureModeThreadLocal ) );
+         try
+         {
+            g->recurse( x->f, b->f, level );
+         } catch ( ... )
+         {
+            hipGraph_t dummy = nullptr;
+            (void) hipStreamEndCapture( s, &dummy );
+            if ( dummy ) (void) hipGraphDestroy( dummy );
+            throw;
+         }
+         HYMI_HIP( hipStreamEndCapture( s, &graph ) );
+         hipGraphExec_t exec = nullptr;
+         HYMI_HIP( hipGraphInstantiate( &exec, graph, nullptr, nullptr, 0 ) );
+         (void) hipGraphDestroy( graph );
+         it = g->graphs.emplace( key, exec ).first;
+      }
+      for ( int k = 0; k < count; k++ )
+         HYMI_HIP( hipGraphLaunch( it->second, s ) );
+   } );
+}
+
+} // extern "C"

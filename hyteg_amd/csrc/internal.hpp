@@ -1,0 +1,156 @@
+// Host-side objects behind the C-ABI handles: context (device + stream), mesh (topology + per-level
+// plans resident in HBM), function (per-level device buffers), operator (per-level stencil weights).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "plan.hpp"
+#include "topology.hpp"
+
+namespace hymi {
+
+struct HipError
+{
+   hipError_t  code;
+   std::string where;
+};
+
+#define HYMI_HIP( call )                                                                  \
+   do                                                                                     \
+   {                                                                                      \
+      hipError_t e_ = ( call );                                                           \
+      if ( e_ != hipSuccess ) throw ::hymi::HipError{ e_, std::string( #call ) };        \
+   } while ( 0 )
+
+// device buffer owned by RAII
+template < typename T >
+struct DevBuf
+{
+   T*     p = nullptr;
+   size_t n = 0;
+   DevBuf()  = default;
+   DevBuf( const DevBuf& ) = delete;
+   DevBuf& operator=( const DevBuf& ) = delete;
+   ~DevBuf() { release(); }
+   void release()
+   {
+      if ( p ) (void) hipFree( p );
+      p = nullptr;
+      n = 0;
+   }
+   void alloc( size_t count )
+   {
+      release();
+      if ( count == 0 ) count = 1;
+      HYMI_HIP( hipMalloc( (void**) &p, count * sizeof( T ) ) );
+      n = count;
+   }
+   void upload( const std::vector< T >& h, hipStream_t s )
+   {
+      alloc( h.size() );
+      if ( !h.empty() ) HYMI_HIP( hipMemcpyAsync( p, h.data(), h.size() * sizeof( T ), hipMemcpyHostToDevice, s ) );
+   }
+};
+
+struct Context
+{
+   int         device   = 0;
+   bool        hostOnly = false; // topology and stencil queries only, no device buffers
+   hipStream_t stream = nullptr;
+   // reduction scratch: block partials and scalar slots
+   DevBuf< double > partials;
+   DevBuf< double > scalars;
+   size_t           bytesInUse = 0;
+};
+
+// Per-level layout and geometry tables of a mesh (shared by all functions and operators on it).
+struct LevelPlan
+{
+   int     L = 0, w = 0;
+   int64_t offE = 0, offF = 0, offC = 0;
+   int64_t eStride = 0, fStride = 0, cStride = 0;
+   int64_t total = 0; // elements per right-hand side
+
+   // host copies (kept for building operator weights and for host-side checks)
+   std::vector< CellSeg >    hSeg64;
+   std::vector< FaceGeom >   hFaceGeom;
+   std::vector< EdgeGeom >   hEdgeGeom;
+   std::vector< EdgeTerm >   hEdgeTerms;
+   std::vector< VertexGeom > hVertexGeom;
+   std::vector< int64_t >    hVertexTerms; // neighbour element index per vertex term
+
+   DevBuf< V2E >        dV2E;
+   DevBuf< E2F >        dE2F;
+   DevBuf< F2C >        dF2C;
+   DevBuf< FaceGeom >   dFaceGeom;
+   DevBuf< EdgeGeom >   dEdgeGeom;
+   DevBuf< EdgeTerm >   dEdgeTerms;
+   DevBuf< VertexGeom > dVertexGeom;
+   DevBuf< int64_t >    dVertexTerms;
+   DevBuf< int32_t >    dCellType;
+
+   // cell row segments, keyed by segment length (in points)
+   std::map< int, std::unique_ptr< DevBuf< CellSeg > > > segs;
+   std::map< int, int64_t >                              segCount;
+   // closure ranges of primitives matching a flag, keyed by flag
+   std::map< int, std::unique_ptr< DevBuf< Range > > > ranges;
+   std::map< int, int64_t >                            rangeCount;
+
+   int64_t vertexOff( int v ) const { return v; }
+   int64_t edgeOff( int e ) const { return offE + e * eStride; }
+   int64_t faceOff( int f ) const { return offF + f * fStride; }
+   int64_t cellOff( int c ) const { return offC + c * cStride; }
+};
+
+struct Mesh
+{
+   Context*                                  ctx = nullptr;
+   Topology                                  T;
+   std::map< int, std::unique_ptr< LevelPlan > > plans;
+   LevelPlan& plan( int level );
+   const CellSeg* segments( LevelPlan& P, int len, int64_t* count );
+   const Range*   closureRanges( LevelPlan& P, int flag, int64_t* count );
+};
+
+enum Precision : int
+{
+   kF64 = 0,
+   kF32 = 1
+};
+
+struct Function
+{
+   Mesh*                    mesh = nullptr;
+   int                      minL = 0, maxL = 0, nrhs = 1, prec = kF64;
+   std::vector< void* >     data; // per level device pointers
+   std::vector< int64_t >   count;
+   ~Function();
+   void*  ptr( int level ) const { return data.at( level - minL ); }
+   size_t elemBytes() const { return prec == kF64 ? 8 : 4; }
+};
+
+struct Operator
+{
+   Mesh* mesh = nullptr;
+   int   minL = 0, maxL = 0, form = 0;
+   struct LevelWeights
+   {
+      DevBuf< double > cell64, face64, edge64, vertex64;
+      DevBuf< float >  cell32, face32, edge32, vertex32;
+      std::vector< double > hCell, hFace, hEdge, hVertex;
+   };
+   std::vector< std::unique_ptr< LevelWeights > > lw;
+   LevelWeights& weights( int level ) { return *lw.at( level - minL ); }
+};
+
+// setup (setup.cpp)
+void buildPlan( Mesh& M, LevelPlan& P, int level );
+void buildOperatorWeights( Operator& op, int level );
+
+} // namespace hymi

@@ -1,0 +1,1251 @@
+// HIP kernels for the matrix-free P1 multigrid path on CDNA4 (gfx950, wave64).
+//
+// Every kernel is memory-bound (29 flop per 16 B for the FP64 15-point apply), so the design goals are
+// coalesced row access, no atomics, and no redundant passes:
+//   * macro-cell kernels walk row segments of the tetrahedral layout: one wave per segment, lanes along x
+//     (or along x * rhs for interleaved multi-RHS data), the seven neighbour rows' start offsets computed
+//     once per wave; blocks are remapped so that each XCD processes a contiguous range of rows (its L2
+//     then holds the neighbouring rows);
+//   * face / edge / vertex kernels read their upward neighbours straight from the neighbour primitive's
+//     closure array through affine index maps (no ghost layers);
+//   * restriction and prolongation are owner-gathers (each output point reads the points it depends on),
+//     so they need no atomics and no additive communication pass.
+// Reference kernels restated: p1functionspace/generatedKernels/apply_3D_macrocell_vertexdof_to_vertexdof_*,
+// apply_3D_macroface_one_sided_vertexdof_to_vertexdof_*, sor_3D_macrocell_P1, sor_3D_macroface_P1,
+// P1Operator.hpp:1040-1104,1348-1417 (edges), VertexDoFMacroVertex.hpp:159-228 (vertices),
+// gridtransferoperators/generatedKernels/restrict_3D_macrocell_P1_pull_additive and
+// prolongate_3D_macrocell_P1_push_additive, VertexDoFPackInfo.hpp:118-460 (ghost copies).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <stdexcept>
+#include <type_traits>
+
+#include "kernels.hpp"
+
+namespace hymi {
+
+namespace {
+
+struct LV
+{
+   int     w;
+   int64_t offE, offF, offC, eS, fS, cS;
+};
+
+LV viewOf( const LevelPlan& P ) { return LV{ P.w, P.offE, P.offF, P.offC, P.eStride, P.fStride, P.cStride }; }
+
+__device__ __forceinline__ int64_t tet( int64_t w ) { return w * ( w + 1 ) * ( w + 2 ) / 6; }
+__device__ __forceinline__ int64_t fidx( int64_t w, int64_t x, int64_t y ) { return y * ( w + 1 ) - y * ( y + 1 ) / 2 + x; }
+__device__ __forceinline__ int64_t cidx( int64_t w, int64_t x, int64_t y, int64_t z )
+{
+   const int64_t wz = w - z;
+   return tet( w ) - tet( wz ) + y * ( wz + 1 ) - y * ( y + 1 ) / 2 + x;
+}
+// row start of (y, z) relative to the cell base
+__device__ __forceinline__ int64_t crow( int64_t w, int64_t t0, int64_t y, int64_t z )
+{
+   const int64_t wz = w - z;
+   return t0 - tet( wz ) + y * ( wz + 1 ) - y * ( y + 1 ) / 2;
+}
+// (a, b) of flat index k in the triangle of width m (row-major, row b has m - b points)
+__device__ __forceinline__ void triPos( int64_t m, int64_t k, int& a, int& b )
+{
+   const double  q  = double( 2 * m + 1 );
+   int64_t       bb = (int64_t) ( ( q - sqrt( q * q - 8.0 * double( k ) ) ) * 0.5 );
+   if ( bb < 0 ) bb = 0;
+   while ( bb > 0 && fidx( m, 0, bb ) > k )
+      bb--;
+   while ( fidx( m, 0, bb + 1 ) <= k )
+      bb++;
+   b = (int) bb;
+   a = (int) ( k - fidx( m, 0, bb ) );
+}
+
+// XCD-aware block order: XCD k (blocks b with b % 8 == k) gets a contiguous range of logical blocks
+__device__ __forceinline__ int64_t xcdBlock( int64_t nLogical )
+{
+   const int64_t per = ( nLogical + 7 ) / 8;
+   const int64_t b   = blockIdx.x;
+   return ( b % 8 ) * per + b / 8;
+}
+int64_t xcdGrid( int64_t nLogical ) { return ( ( nLogical + 7 ) / 8 ) * 8; }
+
+enum
+{
+   dC,
+   dW,
+   dE,
+   dN,
+   dS,
+   dNW,
+   dSE,
+   dTC,
+   dTW,
+   dTS,
+   dTSE,
+   dBC,
+   dBN,
+   dBE,
+   dBNW
+};
+
+// --------------------------------------------------------------------------------------------------------
+// ghost copies (owned value -> closure copies of higher-dimensional primitives)
+
+template < typename T, int NR >
+__global__ void kV2E( LV g, const V2E* __restrict__ tab, int n, T* __restrict__ u )
+{
+   const int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+   const int     r = (int) ( t % NR );
+   const int64_t k = t / NR;
+   if ( k >= 2 * (int64_t) n ) return;
+   const V2E     e   = tab[k >> 1];
+   const int     end = (int) ( k & 1 );
+   const int64_t dst = g.offE + (int64_t) e.edge * g.eS + ( end ? g.w - 1 : 0 );
+   const int64_t src = end ? e.v1 : e.v0;
+   u[dst * NR + r]   = u[src * NR + r];
+}
+
+template < typename T, int NR >
+__global__ void kE2F( LV g, const E2F* __restrict__ tab, int n, T* __restrict__ u )
+{
+   const int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+   const int     r = (int) ( t % NR );
+   const int64_t k = t / NR;
+   if ( k >= (int64_t) n * g.w ) return;
+   const E2F     m   = tab[k / g.w];
+   const int     p   = (int) ( k % g.w );
+   const int     col = m.base[0] + p * m.inc[0], row = m.base[1] + p * m.inc[1];
+   const int64_t dst = g.offF + (int64_t) m.face * g.fS + fidx( g.w, col, row );
+   const int64_t src = g.offE + (int64_t) m.edge * g.eS + p;
+   u[dst * NR + r]   = u[src * NR + r];
+}
+
+template < typename T, int NR >
+__global__ void kF2C( LV g, const F2C* __restrict__ tab, T* __restrict__ u )
+{
+   const F2C     m  = tab[blockIdx.y];
+   const int64_t np = (int64_t) g.w * ( g.w + 1 ) / 2;
+   const int64_t t  = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+   const int     r  = (int) ( t % NR );
+   const int64_t k  = t / NR;
+   if ( k >= np ) return;
+   int i, j;
+   triPos( g.w, k, i, j );
+   const int x = m.m.base[0] + i * m.m.a[0] + j * m.m.b[0];
+   const int y = m.m.base[1] + i * m.m.a[1] + j * m.m.b[1];
+   const int z = m.m.base[2] + i * m.m.a[2] + j * m.m.b[2];
+   const int64_t dst = g.offC + (int64_t) m.m.cell * g.cS + cidx( g.w, x, y, z );
+   const int64_t src = g.offF + (int64_t) m.face * g.fS + k;
+   u[dst * NR + r]   = u[src * NR + r];
+}
+
+// --------------------------------------------------------------------------------------------------------
+// macro-cell interior: apply / residual, multicolour SOR
+
+template < typename T, int NR, int MODE >
+__global__ __launch_bounds__( 256 ) void kCellApply( LV g, const CellSeg* __restrict__ segs, int64_t nseg,
+                                                     const T* __restrict__ W, const int* __restrict__ ctype, int flag,
+                                                     const T* __restrict__ u, T* __restrict__ y,
+                                                     const T* __restrict__ b )
+{
+   const int64_t s = xcdBlock( ( nseg + 3 ) / 4 ) * 4 + ( threadIdx.x >> 6 );
+   if ( s >= nseg ) return;
+   const CellSeg sg = segs[s];
+   if ( !( ctype[sg.cell] & flag ) ) return;
+   const int lane = threadIdx.x & 63;
+   const int x    = sg.x0 + lane / NR;
+   const int r    = lane % NR;
+   const int w    = g.w;
+   if ( x > w - 2 - sg.y - sg.z ) return;
+   const int64_t base = g.offC + (int64_t) sg.cell * g.cS;
+   const int64_t t0   = tet( w );
+   const int64_t r0   = base + crow( w, t0, sg.y, sg.z ) + x;
+   const int64_t rN   = base + crow( w, t0, sg.y + 1, sg.z ) + x;
+   const int64_t rS   = base + crow( w, t0, sg.y - 1, sg.z ) + x;
+   const int64_t rT   = base + crow( w, t0, sg.y, sg.z + 1 ) + x;
+   const int64_t rTS  = base + crow( w, t0, sg.y - 1, sg.z + 1 ) + x;
+   const int64_t rB   = base + crow( w, t0, sg.y, sg.z - 1 ) + x;
+   const int64_t rBN  = base + crow( w, t0, sg.y + 1, sg.z - 1 ) + x;
+   const T*      c    = W + (int64_t) sg.cell * 15;
+#define U( i ) u[( i ) *NR + r]
+   T acc = c[dC] * U( r0 );
+   acc += c[dW] * U( r0 - 1 ) + c[dE] * U( r0 + 1 );
+   acc += c[dN] * U( rN ) + c[dNW] * U( rN - 1 );
+   acc += c[dS] * U( rS ) + c[dSE] * U( rS + 1 );
+   acc += c[dTC] * U( rT ) + c[dTW] * U( rT - 1 );
+   acc += c[dTS] * U( rTS ) + c[dTSE] * U( rTS + 1 );
+   acc += c[dBC] * U( rB ) + c[dBE] * U( rB + 1 );
+   acc += c[dBN] * U( rBN ) + c[dBNW] * U( rBN - 1 );
+#undef U
+   const int64_t o = r0 * NR + r;
+   if ( MODE == kReplace )
+      y[o] = acc;
+   else if ( MODE == kAdd )
+      y[o] += acc;
+   else
+      y[o] = b[o] - acc;
+}
+
+template < typename T, int NR >
+__global__ __launch_bounds__( 256 ) void kCellSor( LV g, const CellSeg* __restrict__ segs, int64_t nseg,
+                                                   const T* __restrict__ W, const int* __restrict__ ctype, int flag,
+                                                   T* __restrict__ u, const T* __restrict__ b, T omega, int colour )
+{
+   const int64_t s = xcdBlock( ( nseg + 3 ) / 4 ) * 4 + ( threadIdx.x >> 6 );
+   if ( s >= nseg ) return;
+   const CellSeg sg = segs[s];
+   if ( !( ctype[sg.cell] & flag ) ) return;
+   const int lane = threadIdx.x & 63;
+   const int x    = sg.x0 + lane / NR;
+   const int r    = lane % NR;
+   const int w    = g.w;
+   if ( x > w - 2 - sg.y - sg.z ) return;
+   if ( ( ( x + 2 * sg.y + 3 * sg.z ) & 3 ) != colour ) return;
+   const int64_t base = g.offC + (int64_t) sg.cell * g.cS;
+   const int64_t t0   = tet( w );
+   const int64_t r0   = base + crow( w, t0, sg.y, sg.z ) + x;
+   const int64_t rN   = base + crow( w, t0, sg.y + 1, sg.z ) + x;
+   const int64_t rS   = base + crow( w, t0, sg.y - 1, sg.z ) + x;
+   const int64_t rT   = base + crow( w, t0, sg.y, sg.z + 1 ) + x;
+   const int64_t rTS  = base + crow( w, t0, sg.y - 1, sg.z + 1 ) + x;
+   const int64_t rB   = base + crow( w, t0, sg.y, sg.z - 1 ) + x;
+   const int64_t rBN  = base + crow( w, t0, sg.y + 1, sg.z - 1 ) + x;
+   const T*      c    = W + (int64_t) sg.cell * 15;
+#define U( i ) u[( i ) *NR + r]
+   T acc = c[dW] * U( r0 - 1 ) + c[dE] * U( r0 + 1 );
+   acc += c[dN] * U( rN ) + c[dNW] * U( rN - 1 );
+   acc += c[dS] * U( rS ) + c[dSE] * U( rS + 1 );
+   acc += c[dTC] * U( rT ) + c[dTW] * U( rT - 1 );
+   acc += c[dTS] * U( rTS ) + c[dTSE] * U( rTS + 1 );
+   acc += c[dBC] * U( rB ) + c[dBE] * U( rB + 1 );
+   acc += c[dBN] * U( rBN ) + c[dBNW] * U( rBN - 1 );
+#undef U
+   const int64_t o = r0 * NR + r;
+   u[o]            = ( T( 1 ) - omega ) * u[o] + omega / c[dC] * ( b[o] - acc );
+}
+
+// --------------------------------------------------------------------------------------------------------
+// macro-face interior
+
+template < typename T, int NR >
+__device__ __forceinline__ T faceStencilSum( const LV& g, const FaceGeom& G, const T* __restrict__ fw,
+                                             const T* __restrict__ u, int64_t fo, int i, int j, int r, bool withCentre )
+{
+   const int w = g.w;
+#define UF( ii, jj ) u[( fo + fidx( w, ii, jj ) ) * NR + r]
+   T acc = fw[1] * UF( i - 1, j ) + fw[2] * UF( i + 1, j );
+   acc += fw[3] * UF( i, j + 1 ) + fw[4] * UF( i, j - 1 );
+   acc += fw[5] * UF( i - 1, j + 1 ) + fw[6] * UF( i + 1, j - 1 );
+   if ( withCentre ) acc += fw[0] * UF( i, j );
+#undef UF
+   for ( int s = 0; s < G.nSides; s++ )
+   {
+      const FaceCellMap& M  = G.side[s];
+      const int64_t      cb = g.offC + (int64_t) M.cell * g.cS;
+#pragma unroll
+      for ( int k = 0; k < 4; k++ )
+      {
+         const int col = i + G.offCol[s][k], row = j + G.offRow[s][k];
+         const int x   = M.base[0] + col * M.a[0] + row * M.b[0] + M.o[0];
+         const int y   = M.base[1] + col * M.a[1] + row * M.b[1] + M.o[1];
+         const int z   = M.base[2] + col * M.a[2] + row * M.b[2] + M.o[2];
+         acc += fw[7 + 4 * s + k] * u[( cb + cidx( w, x, y, z ) ) * NR + r];
+      }
+   }
+   return acc;
+}
+
+template < typename T, int NR, int MODE >
+__global__ void kFaceApply( LV g, const FaceGeom* __restrict__ fg, const T* __restrict__ FW, int flag,
+                            const T* __restrict__ u, T* __restrict__ y, const T* __restrict__ b )
+{
+   const FaceGeom& G = fg[blockIdx.y];
+   if ( !( G.type & flag ) ) return;
+   const int     m  = g.w - 3;
+   const int64_t np = (int64_t) m * ( m + 1 ) / 2;
+   const int64_t t  = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+   const int     r  = (int) ( t % NR );
+   const int64_t k  = t / NR;
+   if ( k >= np ) return;
+   int a, bb;
+   triPos( m, k, a, bb );
+   const int     i   = a + 1, j = bb + 1;
+   const int64_t fo  = g.offF + (int64_t) G.face * g.fS;
+   const T       acc = faceStencilSum< T, NR >( g, G, FW + (int64_t) G.face * kFaceW, u, fo, i, j, r, true );
+   const int64_t o   = ( fo + fidx( g.w, i, j ) ) * NR + r;
+   if ( MODE == kReplace )
+      y[o] = acc;
+   else if ( MODE == kAdd )
+      y[o] += acc;
+   else
+      y[o] = b[o] - acc;
+}
+
+template < typename T, int NR >
+__global__ void kFaceSor( LV g, const FaceGeom* __restrict__ fg, const T* __restrict__ FW, int flag, T* __restrict__ u,
+                          const T* __restrict__ b, T omega, int colour )
+{
+   const FaceGeom& G = fg[blockIdx.y];
+   if ( !( G.type & flag ) ) return;
+   const int     m  = g.w - 3;
+   const int64_t np = (int64_t) m * ( m + 1 ) / 2;
+   const int64_t t  = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+   const int     r  = (int) ( t % NR );
+   const int64_t k  = t / NR;
+   if ( k >= np ) return;
+   int a, bb;
+   triPos( m, k, a, bb );
+   const int i = a + 1, j = bb + 1;
+   if ( ( i + 2 * j ) % 3 != colour ) return;
+   const int64_t fo  = g.offF + (int64_t) G.face * g.fS;
+   const T*      fw  = FW + (int64_t) G.face * kFaceW;
+   const T       acc = faceStencilSum< T, NR >( g, G, fw, u, fo, i, j, r, false );
+   const int64_t o   = ( fo + fidx( g.w, i, j ) ) * NR + r;
+   u[o]              = ( T( 1 ) - omega ) * u[o] + omega / fw[0] * ( b[o] - acc );
+}
+
+// --------------------------------------------------------------------------------------------------------
+// macro-edge interior
+
+template < typename T, int NR >
+__device__ __forceinline__ T edgeTermsSum( const LV& g, const EdgeGeom& G, const EdgeTerm* __restrict__ terms,
+                                           const T* __restrict__ tw, const T* __restrict__ u, int x, int r )
+{
+   T acc = T( 0 );
+   for ( int k = 0; k < G.nTerms; k++ )
+   {
+      const EdgeTerm& e = terms[G.termOff + k];
+      int64_t         idx;
+      if ( e.kind == 0 )
+         idx = g.offF + (int64_t) e.prim * g.fS + fidx( g.w, e.base[0] + x * e.inc[0], e.base[1] + x * e.inc[1] );
+      else
+         idx = g.offC + (int64_t) e.prim * g.cS +
+               cidx( g.w, e.base[0] + x * e.inc[0], e.base[1] + x * e.inc[1], e.base[2] + x * e.inc[2] );
+      acc += tw[k] * u[idx * NR + r];
+   }
+   return acc;
+}
+
+template < typename T, int NR, int MODE >
+__global__ void kEdgeApply( LV g, const EdgeGeom* __restrict__ eg, const EdgeTerm* __restrict__ terms,
+                            const T* __restrict__ EW, int flag, const T* __restrict__ u, T* __restrict__ y,
+                            const T* __restrict__ b )
+{
+   const EdgeGeom& G = eg[blockIdx.y];
+   if ( !( G.type & flag ) ) return;
+   const int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+   const int     r = (int) ( t % NR );
+   const int     x = (int) ( t / NR ) + 1;
+   if ( x > g.w - 2 ) return;
+   const T*      ew  = EW + 3 * (int64_t) G.edge + G.termOff;
+   const int64_t eo  = g.offE + (int64_t) G.edge * g.eS;
+   T             acc = ew[0] * u[( eo + x - 1 ) * NR + r] + ew[1] * u[( eo + x ) * NR + r] + ew[2] * u[( eo + x + 1 ) * NR + r];
+   acc += edgeTermsSum< T, NR >( g, G, terms, ew + 3, u, x, r );
+   const int64_t o = ( eo + x ) * NR + r;
+   if ( MODE == kReplace )
+      y[o] = acc;
+   else if ( MODE == kAdd )
+      y[o] += acc;
+   else
+      y[o] = b[o] - acc;
+}
+
+template < typename T, int NR >
+__global__ void kEdgeSor( LV g, const EdgeGeom* __restrict__ eg, const EdgeTerm* __restrict__ terms,
+                          const T* __restrict__ EW, int flag, T* __restrict__ u, const T* __restrict__ b, T omega,
+                          int colour )
+{
+   const EdgeGeom& G = eg[blockIdx.y];
+   if ( !( G.type & flag ) ) return;
+   const int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+   const int     r = (int) ( t % NR );
+   const int     x = (int) ( t / NR ) + 1;
+   if ( x > g.w - 2 || ( x & 1 ) != colour ) return;
+   const T*      ew  = EW + 3 * (int64_t) G.edge + G.termOff;
+   const int64_t eo  = g.offE + (int64_t) G.edge * g.eS;
+   T             acc = ew[0] * u[( eo + x - 1 ) * NR + r] + ew[2] * u[( eo + x + 1 ) * NR + r];
+   acc += edgeTermsSum< T, NR >( g, G, terms, ew + 3, u, x, r );
+   const int64_t o = ( eo + x ) * NR + r;
+   u[o]            = ( T( 1 ) - omega ) * u[o] + omega / ew[1] * ( b[o] - acc );
+}
+
+// --------------------------------------------------------------------------------------------------------
+// macro-vertices
+
+template < typename T, int NR, int MODE >
+__global__ void kVertexApply( LV g, const VertexGeom* __restrict__ vg, const int64_t* __restrict__ terms, int nV,
+                              const T* __restrict__ VW, int flag, const T* __restrict__ u, T* __restrict__ y,
+                              const T* __restrict__ b )
+{
+   const int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+   const int     r = (int) ( t % NR );
+   const int64_t v = t / NR;
+   if ( v >= nV ) return;
+   const VertexGeom G = vg[v];
+   if ( !( G.type & flag ) ) return;
+   const T* vw  = VW + G.vertex + G.termOff;
+   T        acc = vw[0] * u[(int64_t) G.vertex * NR + r];
+   for ( int k = 0; k < G.nTerms; k++ )
+      acc += vw[1 + k] * u[terms[G.termOff + k] * NR + r];
+   const int64_t o = (int64_t) G.vertex * NR + r;
+   if ( MODE == kReplace )
+      y[o] = acc;
+   else if ( MODE == kAdd )
+      y[o] += acc;
+   else
+      y[o] = b[o] - acc;
+}
+
+template < typename T, int NR >
+__global__ void kVertexSor( LV g, const VertexGeom* __restrict__ vg, const int64_t* __restrict__ terms, int nV,
+                            const T* __restrict__ VW, int flag, T* __restrict__ u, const T* __restrict__ b, T omega )
+{
+   const int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+   const int     r = (int) ( t % NR );
+   const int64_t v = t / NR;
+   if ( v >= nV ) return;
+   const VertexGeom G = vg[v];
+   if ( !( G.type & flag ) ) return;
+   const T* vw  = VW + G.vertex + G.termOff;
+   T        acc = T( 0 );
+   for ( int k = 0; k < G.nTerms; k++ )
+      acc += vw[1 + k] * u[terms[G.termOff + k] * NR + r];
+   const int64_t o = (int64_t) G.vertex * NR + r;
+   u[o]            = ( T( 1 ) - omega ) * u[o] + omega / vw[0] * ( b[o] - acc );
+}
+
+// --------------------------------------------------------------------------------------------------------
+// restriction (coarse owned point <- fine neighbourhood of 2p, weights 1 and 1/2)
+
+template < typename T, int NR >
+__global__ __launch_bounds__( 256 ) void kRestrictCell( LV gc, LV gf, const CellSeg* __restrict__ segs, int64_t nseg,
+                                                        const int* __restrict__ ctype, int flag,
+                                                        const T* __restrict__ uf, T* __restrict__ uc )
+{
+   const int64_t s = xcdBlock( ( nseg + 3 ) / 4 ) * 4 + ( threadIdx.x >> 6 );
+   if ( s >= nseg ) return;
+   const CellSeg sg = segs[s];
+   if ( !( ctype[sg.cell] & flag ) ) return;
+   const int lane = threadIdx.x & 63;
+   const int X    = sg.x0 + lane / NR;
+   const int r    = lane % NR;
+   if ( X > gc.w - 2 - sg.y - sg.z ) return;
+   const int     w    = gf.w;
+   const int     x    = 2 * X, y = 2 * sg.y, z = 2 * sg.z;
+   const int64_t base = gf.offC + (int64_t) sg.cell * gf.cS;
+   const int64_t t0   = tet( w );
+   const int64_t r0   = base + crow( w, t0, y, z ) + x;
+   const int64_t rN   = base + crow( w, t0, y + 1, z ) + x;
+   const int64_t rS   = base + crow( w, t0, y - 1, z ) + x;
+   const int64_t rT   = base + crow( w, t0, y, z + 1 ) + x;
+   const int64_t rTS  = base + crow( w, t0, y - 1, z + 1 ) + x;
+   const int64_t rB   = base + crow( w, t0, y, z - 1 ) + x;
+   const int64_t rBN  = base + crow( w, t0, y + 1, z - 1 ) + x;
+#define U( i ) uf[( i ) *NR + r]
+   T acc = U( r0 - 1 ) + U( r0 + 1 ) + U( rN ) + U( rN - 1 ) + U( rS ) + U( rS + 1 ) + U( rT ) + U( rT - 1 ) +
+           U( rTS ) + U( rTS + 1 ) + U( rB ) + U( rB + 1 ) + U( rBN ) + U( rBN - 1 );
+   acc = U( r0 ) + T( 0.5 ) * acc;
+#undef U
+   const int64_t cbase = gc.offC + (int64_t) sg.cell * gc.cS;
+   uc[( cbase + cidx( gc.w, X, sg.y, sg.z ) ) * NR + r] = acc;
+}
+
+template < typename T, int NR >
+__global__ void kRestrictFace( LV gc, LV gf, const FaceGeom* __restrict__ fgFine, int flag, const T* __restrict__ uf,
+                               T* __restrict__ uc )
+{
+   const FaceGeom& G = fgFine[blockIdx.y];
+   if ( !( G.type & flag ) ) return;
+   const int     m  = gc.w - 3;
+   const int64_t np = (int64_t) m * ( m + 1 ) / 2;
+   const int64_t t  = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+   const int     r  = (int) ( t % NR );
+   const int64_t k  = t / NR;
+   if ( k >= np ) return;
+   int a, bb;
+   triPos( m, k, a, bb );
+   const int I = a + 1, J = bb + 1;
+   const int i = 2 * I, j = 2 * J, w = gf.w;
+   const int64_t fo = gf.offF + (int64_t) G.face * gf.fS;
+#define UF( ii, jj ) uf[( fo + fidx( w, ii, jj ) ) * NR + r]
+   T acc = UF( i - 1, j ) + UF( i + 1, j ) + UF( i, j + 1 ) + UF( i, j - 1 ) + UF( i - 1, j + 1 ) + UF( i + 1, j - 1 );
+   const T centre = UF( i, j );
+#undef UF
+   for ( int s = 0; s < G.nSides; s++ )
+   {
+      const FaceCellMap& M  = G.side[s];
+      const int64_t      cb = gf.offC + (int64_t) M.cell * gf.cS;
+#pragma unroll
+      for ( int q = 0; q < 4; q++ )
+      {
+         const int col = i + G.offCol[s][q], row = j + G.offRow[s][q];
+         const int x   = M.base[0] + col * M.a[0] + row * M.b[0] + M.o[0];
+         const int y   = M.base[1] + col * M.a[1] + row * M.b[1] + M.o[1];
+         const int z   = M.base[2] + col * M.a[2] + row * M.b[2] + M.o[2];
+         acc += uf[( cb + cidx( w, x, y, z ) ) * NR + r];
+      }
+   }
+   const int64_t co = gc.offF + (int64_t) G.face * gc.fS;
+   uc[( co + fidx( gc.w, I, J ) ) * NR + r] = centre + T( 0.5 ) * acc;
+}
+
+template < typename T, int NR >
+__global__ void kRestrictEdge( LV gc, LV gf, const EdgeGeom* __restrict__ egFine, const EdgeTerm* __restrict__ terms,
+                               int flag, const T* __restrict__ uf, T* __restrict__ uc )
+{
+   const EdgeGeom& G = egFine[blockIdx.y];
+   if ( !( G.type & flag ) ) return;
+   const int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+   const int     r = (int) ( t % NR );
+   const int     X = (int) ( t / NR ) + 1;
+   if ( X > gc.w - 2 ) return;
+   const int     x   = 2 * X;
+   const int64_t eo  = gf.offE + (int64_t) G.edge * gf.eS;
+   T             acc = uf[( eo + x - 1 ) * NR + r] * T( 0.5 ) + uf[( eo + x + 1 ) * NR + r] * T( 0.5 );
+   acc += uf[( eo + x ) * NR + r];
+   for ( int k = 0; k < G.nTerms; k++ )
+   {
+      const EdgeTerm& e = terms[G.termOff + k];
+      if ( e.wr == 0.0f ) continue;
+      int64_t idx;
+      if ( e.kind == 0 )
+         idx = gf.offF + (int64_t) e.prim * gf.fS + fidx( gf.w, e.base[0] + x * e.inc[0], e.base[1] + x * e.inc[1] );
+      else
+         idx = gf.offC + (int64_t) e.prim * gf.cS +
+               cidx( gf.w, e.base[0] + x * e.inc[0], e.base[1] + x * e.inc[1], e.base[2] + x * e.inc[2] );
+      acc += T( e.wr ) * uf[idx * NR + r];
+   }
+   uc[( gc.offE + (int64_t) G.edge * gc.eS + X ) * NR + r] = acc;
+}
+
+template < typename T, int NR >
+__global__ void kRestrictVertex( const VertexGeom* __restrict__ vgFine, const int64_t* __restrict__ termsFine, int nV,
+                                 int flag, const T* __restrict__ uf, T* __restrict__ uc )
+{
+   const int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+   const int     r = (int) ( t % NR );
+   const int64_t v = t / NR;
+   if ( v >= nV ) return;
+   const VertexGeom G = vgFine[v];
+   if ( !( G.type & flag ) ) return;
+   T acc = T( 0 );
+   for ( int k = 0; k < G.nTerms; k++ )
+      acc += uf[termsFine[G.termOff + k] * NR + r];
+   uc[(int64_t) G.vertex * NR + r] = uf[(int64_t) G.vertex * NR + r] + T( 0.5 ) * acc;
+}
+
+// --------------------------------------------------------------------------------------------------------
+// prolongation-and-add (fine owned point += linear interpolation of the coarse closure)
+
+template < typename T, int NR >
+__global__ __launch_bounds__( 256 ) void kProlongCell( LV gc, LV gf, const CellSeg* __restrict__ segs, int64_t nseg,
+                                                       const int* __restrict__ ctype, int flag,
+                                                       const T* __restrict__ uc, T* __restrict__ uf )
+{
+   const int64_t s = xcdBlock( ( nseg + 3 ) / 4 ) * 4 + ( threadIdx.x >> 6 );
+   if ( s >= nseg ) return;
+   const CellSeg sg = segs[s];
+   if ( !( ctype[sg.cell] & flag ) ) return;
+   const int lane = threadIdx.x & 63;
+   const int x    = sg.x0 + lane / NR;
+   const int r    = lane % NR;
+   if ( x > gf.w - 2 - sg.y - sg.z ) return;
+   const int y = sg.y, z = sg.z;
+   // micro-edge direction with the parity of (x, y, z): E, N, SE, TC, TW, TS, TSE
+   const int px = x & 1, py = y & 1, pz = z & 1;
+   int       dx = px, dy = py, dz = pz;
+   if ( px && py ) dy = -1;
+   if ( pz && px && !py ) dx = -1;
+   if ( pz && py && !px ) dy = -1;
+   if ( px && py && pz )
+   {
+      dx = 1;
+      dy = -1;
+      dz = 1;
+   }
+   const int64_t cb = gc.offC + (int64_t) sg.cell * gc.cS;
+   T             v;
+   if ( !px && !py && !pz )
+      v = uc[( cb + cidx( gc.w, x / 2, y / 2, z / 2 ) ) * NR + r];
+   else
+      v = T( 0.5 ) * ( uc[( cb + cidx( gc.w, ( x - dx ) / 2, ( y - dy ) / 2, ( z - dz ) / 2 ) ) * NR + r] +
+                       uc[( cb + cidx( gc.w, ( x + dx ) / 2, ( y + dy ) / 2, ( z + dz ) / 2 ) ) * NR + r] );
+   const int64_t fb = gf.offC + (int64_t) sg.cell * gf.cS;
+   uf[( fb + cidx( gf.w, x, y, z ) ) * NR + r] += v;
+}
+
+template < typename T, int NR >
+__global__ void kProlongFace( LV gc, LV gf, const FaceGeom* __restrict__ fg, int flag, const T* __restrict__ uc,
+                              T* __restrict__ uf )
+{
+   const FaceGeom& G = fg[blockIdx.y];
+   if ( !( G.type & flag ) ) return;
+   const int     m  = gf.w - 3;
+   const int64_t np = (int64_t) m * ( m + 1 ) / 2;
+   const int64_t t  = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+   const int     r  = (int) ( t % NR );
+   const int64_t k  = t / NR;
+   if ( k >= np ) return;
+   int a, bb;
+   triPos( m, k, a, bb );
+   const int     i = a + 1, j = bb + 1;
+   const int64_t co = gc.offF + (int64_t) G.face * gc.fS;
+   const int     pi = i & 1, pj = j & 1;
+   T             v;
+   if ( !pi && !pj )
+      v = uc[( co + fidx( gc.w, i / 2, j / 2 ) ) * NR + r];
+   else
+   {
+      // in-plane micro-edges: (1,0), (0,1), (-1,1)
+      const int di = ( pi && pj ) ? -1 : pi, dj = pj;
+      v = T( 0.5 ) * ( uc[( co + fidx( gc.w, ( i - di ) / 2, ( j - dj ) / 2 ) ) * NR + r] +
+                       uc[( co + fidx( gc.w, ( i + di ) / 2, ( j + dj ) / 2 ) ) * NR + r] );
+   }
+   uf[( gf.offF + (int64_t) G.face * gf.fS + fidx( gf.w, i, j ) ) * NR + r] += v;
+}
+
+template < typename T, int NR >
+__global__ void kProlongEdge( LV gc, LV gf, const EdgeGeom* __restrict__ eg, int flag, const T* __restrict__ uc,
+                              T* __restrict__ uf )
+{
+   const EdgeGeom& G = eg[blockIdx.y];
+   if ( !( G.type & flag ) ) return;
+   const int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+   const int     r = (int) ( t % NR );
+   const int     x = (int) ( t / NR ) + 1;
+   if ( x > gf.w - 2 ) return;
+   const int64_t co = gc.offE + (int64_t) G.edge * gc.eS;
+   const T       v  = ( x & 1 ) ? T( 0.5 ) * ( uc[( co + x / 2 ) * NR + r] + uc[( co + x / 2 + 1 ) * NR + r] )
+                                : uc[( co + x / 2 ) * NR + r];
+   uf[( gf.offE + (int64_t) G.edge * gf.eS + x ) * NR + r] += v;
+}
+
+template < typename T, int NR >
+__global__ void kProlongVertex( const VertexGeom* __restrict__ vg, int nV, int flag, const T* __restrict__ uc,
+                                T* __restrict__ uf )
+{
+   const int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+   const int     r = (int) ( t % NR );
+   const int64_t v = t / NR;
+   if ( v >= nV ) return;
+   if ( !( vg[v].type & flag ) ) return;
+   uf[v * NR + r] += uc[v * NR + r];
+}
+
+// --------------------------------------------------------------------------------------------------------
+// vector operations over closure ranges
+
+template < typename T, typename S, int NR >
+__global__ void kLincomb( const Range* __restrict__ ranges, T* __restrict__ dst, int n, const S* __restrict__ s0,
+                          const S* __restrict__ s1, const S* __restrict__ s2, double c0, double c1, double c2,
+                          int accumulate, double constant )
+{
+   const Range   R   = ranges[blockIdx.x];
+   const int64_t len = (int64_t) R.len * NR;
+   const int64_t o   = R.off * NR;
+   for ( int64_t k = threadIdx.x; k < len; k += blockDim.x )
+   {
+      double v = constant;
+      if ( n > 0 ) v += c0 * (double) s0[o + k];
+      if ( n > 1 ) v += c1 * (double) s1[o + k];
+      if ( n > 2 ) v += c2 * (double) s2[o + k];
+      if ( accumulate ) v += (double) dst[o + k];
+      dst[o + k] = (T) v;
+   }
+}
+
+template < typename T, int NR >
+__global__ void kAxpyScalar( const Range* __restrict__ ranges, T* __restrict__ y, const T* __restrict__ x,
+                             const double* __restrict__ sc, double sign )
+{
+   const Range   R   = ranges[blockIdx.x];
+   const int64_t len = (int64_t) R.len * NR;
+   const int64_t o   = R.off * NR;
+   for ( int64_t k = threadIdx.x; k < len; k += blockDim.x )
+   {
+      const T a = (T) ( sign * sc[( o + k ) % NR] );
+      y[o + k] += a * x[o + k];
+   }
+}
+
+template < typename T, int NR >
+__global__ void kXpbyScalar( const Range* __restrict__ ranges, T* __restrict__ p, const T* __restrict__ x,
+                             const double* __restrict__ sc )
+{
+   const Range   R   = ranges[blockIdx.x];
+   const int64_t len = (int64_t) R.len * NR;
+   const int64_t o   = R.off * NR;
+   for ( int64_t k = threadIdx.x; k < len; k += blockDim.x )
+   {
+      const T bta = (T) sc[( o + k ) % NR];
+      p[o + k]    = x[o + k] + bta * p[o + k];
+   }
+}
+
+// --------------------------------------------------------------------------------------------------------
+// dot products: block partials (per rhs) then one fixed-order reduction
+
+template < int NR >
+__device__ __forceinline__ void blockReduceStore( double v, double* __restrict__ out )
+{
+   __shared__ double sh[256];
+   sh[threadIdx.x] = v;
+   __syncthreads();
+   for ( int stride = blockDim.x / 2; stride >= NR; stride >>= 1 )
+   {
+      if ( (int) threadIdx.x < stride ) sh[threadIdx.x] += sh[threadIdx.x + stride];
+      __syncthreads();
+   }
+   if ( (int) threadIdx.x < NR ) out[threadIdx.x] = sh[threadIdx.x];
+}
+
+template < typename T, int NR >
+__global__ __launch_bounds__( 256 ) void kDotCell( LV g, const CellSeg* __restrict__ segs, int64_t nseg,
+                                                   const int* __restrict__ ctype, int flag, const T* __restrict__ a,
+                                                   const T* __restrict__ b, double* __restrict__ partials )
+{
+   const int64_t s = (int64_t) blockIdx.x * 4 + ( threadIdx.x >> 6 );
+   double        v = 0.0;
+   if ( s < nseg )
+   {
+      const CellSeg sg   = segs[s];
+      const int     lane = threadIdx.x & 63;
+      const int     x    = sg.x0 + lane / NR;
+      const int     r    = lane % NR;
+      if ( ( ctype[sg.cell] & flag ) && x <= g.w - 2 - sg.y - sg.z )
+      {
+         const int64_t o = ( g.offC + (int64_t) sg.cell * g.cS + cidx( g.w, x, sg.y, sg.z ) ) * NR + r;
+         v               = (double) a[o] * (double) b[o];
+      }
+   }
+   blockReduceStore< NR >( v, partials + (int64_t) blockIdx.x * NR );
+}
+
+template < typename T, int NR >
+__global__ __launch_bounds__( 256 ) void kDotLow( LV g, const FaceGeom* __restrict__ fg, int nF,
+                                                  const EdgeGeom* __restrict__ eg, int nE,
+                                                  const VertexGeom* __restrict__ vg, int nV, int flag,
+                                                  const T* __restrict__ a, const T* __restrict__ b,
+                                                  double* __restrict__ partials )
+{
+   // block-stride walk over all owned face, edge and vertex points: index space
+   // [faces: nF * mF][edges: nE * mE][vertices: nV], times NR
+   const int64_t mF = (int64_t) ( g.w - 3 ) * ( g.w - 2 ) / 2;
+   const int64_t mE = g.w - 2;
+   const int64_t nTot = ( (int64_t) nF * mF + (int64_t) nE * mE + nV ) * NR;
+   double        v    = 0.0;
+   for ( int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; t < nTot; t += (int64_t) gridDim.x * blockDim.x )
+   {
+      const int     r = (int) ( t % NR );
+      int64_t       k = t / NR;
+      int64_t       o = -1;
+      if ( k < (int64_t) nF * mF )
+      {
+         const int f = (int) ( k / mF );
+         if ( fg[f].type & flag )
+         {
+            int aa, bb;
+            triPos( g.w - 3, k % mF, aa, bb );
+            o = g.offF + (int64_t) f * g.fS + fidx( g.w, aa + 1, bb + 1 );
+         }
+      }
+      else if ( ( k -= (int64_t) nF * mF ) < (int64_t) nE * mE )
+      {
+         const int e = (int) ( k / mE );
+         if ( eg[e].type & flag ) o = g.offE + (int64_t) e * g.eS + 1 + k % mE;
+      }
+      else
+      {
+         k -= (int64_t) nE * mE;
+         if ( vg[k].type & flag ) o = k;
+      }
+      if ( o >= 0 ) v += (double) a[o * NR + r] * (double) b[o * NR + r];
+   }
+   blockReduceStore< NR >( v, partials + (int64_t) blockIdx.x * NR );
+}
+
+template < int NR >
+__global__ void kFinishDot( const double* __restrict__ partials, int64_t nBlocks, double* __restrict__ out )
+{
+   // one block; thread t sums partial blocks t, t+256/NR*.. for rhs t % NR in fixed order
+   __shared__ double sh[256];
+   const int         r   = threadIdx.x % NR;
+   const int         lane = threadIdx.x / NR;
+   const int         nl   = blockDim.x / NR;
+   double            v    = 0.0;
+   for ( int64_t k = lane; k < nBlocks; k += nl )
+      v += partials[k * NR + r];
+   sh[threadIdx.x] = v;
+   __syncthreads();
+   for ( int stride = blockDim.x / 2; stride >= NR; stride >>= 1 )
+   {
+      if ( (int) threadIdx.x < stride ) sh[threadIdx.x] += sh[threadIdx.x + stride];
+      __syncthreads();
+   }
+   if ( (int) threadIdx.x < NR ) out[threadIdx.x] = sh[threadIdx.x];
+}
+
+__global__ void kScalarRatio( double* sc, int nrhs, int dst, int num, int den, int gate, double gateMin )
+{
+   const int r = threadIdx.x;
+   if ( r >= nrhs ) return;
+   const double d = sc[den * nrhs + r];
+   const double n = sc[num * nrhs + r];
+   const double g = gate >= 0 ? sc[gate * nrhs + r] : 1.0;
+   sc[dst * nrhs + r] = ( d != 0.0 && g > gateMin ) ? n / d : 0.0;
+}
+
+__global__ void kScalarCopy( double* sc, int nrhs, int dst, int src )
+{
+   const int r = threadIdx.x;
+   if ( r < nrhs ) sc[dst * nrhs + r] = sc[src * nrhs + r];
+}
+
+// --------------------------------------------------------------------------------------------------------
+// host-side dispatch
+
+inline unsigned blocksFor( int64_t n, int bs ) { return (unsigned) std::max< int64_t >( 1, ( n + bs - 1 ) / bs ); }
+
+template < int NR >
+int segLen()
+{
+   return 64 / NR;
+}
+
+template < typename T >
+const T* wsel( const DevBuf< double >& d64, const DevBuf< float >& d32 );
+template <>
+const double* wsel< double >( const DevBuf< double >& d64, const DevBuf< float >& ) { return d64.p; }
+template <>
+const float* wsel< float >( const DevBuf< double >&, const DevBuf< float >& d32 ) { return d32.p; }
+
+template < typename T, int NR >
+void refreshT( Function& f, int level, int mask )
+{
+   Mesh&       M = *f.mesh;
+   LevelPlan&  P = M.plan( level );
+   LV          g = viewOf( P );
+   hipStream_t s = M.ctx->stream;
+   T*          u = (T*) f.ptr( level );
+   if ( mask & 1 )
+      hipLaunchKernelGGL( ( kV2E< T, NR > ), dim3( blocksFor( 2 * (int64_t) M.T.nE() * NR, 256 ) ), dim3( 256 ), 0, s, g,
+                          P.dV2E.p, M.T.nE(), u );
+   if ( mask & 2 )
+      hipLaunchKernelGGL( ( kE2F< T, NR > ), dim3( blocksFor( 3 * (int64_t) M.T.nF() * P.w * NR, 256 ) ), dim3( 256 ), 0,
+                          s, g, P.dE2F.p, 3 * M.T.nF(), u );
+   if ( ( mask & 4 ) && M.T.nC() > 0 )
+      hipLaunchKernelGGL( ( kF2C< T, NR > ), dim3( blocksFor( triSize( P.w ) * NR, 256 ), 4 * M.T.nC() ), dim3( 256 ), 0, s,
+                          g, P.dF2C.p, u );
+}
+
+template < typename T, int NR >
+void applyT( Operator& op, Function& src, Function& dst, const Function* rhs, int level, int flag, int mode )
+{
+   Mesh&       M  = *op.mesh;
+   LevelPlan&  P  = M.plan( level );
+   LV          g  = viewOf( P );
+   hipStream_t s  = M.ctx->stream;
+   auto&       Wt = op.weights( level );
+   const T*    u  = (const T*) src.ptr( level );
+   T*          y  = (T*) dst.ptr( level );
+   const T*    b  = rhs ? (const T*) rhs->ptr( level ) : nullptr;
+   refreshT< T, NR >( src, level, 7 );
+   const T* VW = wsel< T >( Wt.vertex64, Wt.vertex32 );
+   const T* EW = wsel< T >( Wt.edge64, Wt.edge32 );
+   const T* FW = wsel< T >( Wt.face64, Wt.face32 );
+   const T* CW = wsel< T >( Wt.cell64, Wt.cell32 );
+   const int nV = M.T.nV(), nE = M.T.nE(), nF = M.T.nF();
+#define LAUNCH_MODES( KERNEL, GRID, BLOCK, ... )                                                                     \
+   do                                                                                                                \
+   {                                                                                                                 \
+      if ( mode == kReplace )                                                                                        \
+         hipLaunchKernelGGL( ( KERNEL< T, NR, kReplace > ), GRID, BLOCK, 0, s, __VA_ARGS__ );                        \
+      else if ( mode == kAdd )                                                                                       \
+         hipLaunchKernelGGL( ( KERNEL< T, NR, kAdd > ), GRID, BLOCK, 0, s, __VA_ARGS__ );                            \
+      else                                                                                                           \
+         hipLaunchKernelGGL( ( KERNEL< T, NR, kResidual > ), GRID, BLOCK, 0, s, __VA_ARGS__ );                       \
+   } while ( 0 )
+   LAUNCH_MODES( kVertexApply, dim3( blocksFor( (int64_t) nV * NR, 256 ) ), dim3( 256 ), g, P.dVertexGeom.p,
+                 P.dVertexTerms.p, nV, VW, flag, u, y, b );
+   if ( level >= 1 )
+      LAUNCH_MODES( kEdgeApply, dim3( blocksFor( (int64_t) ( P.w - 2 ) * NR, 64 ), nE ), dim3( 64 ), g, P.dEdgeGeom.p,
+                    P.dEdgeTerms.p, EW, flag, u, y, b );
+   if ( level >= 2 )
+   {
+      const int64_t m = P.w - 3;
+      if ( m > 0 && nF > 0 )
+         LAUNCH_MODES( kFaceApply, dim3( blocksFor( m * ( m + 1 ) / 2 * NR, 256 ), nF ), dim3( 256 ), g, P.dFaceGeom.p,
+                       FW, flag, u, y, b );
+      int64_t        nseg = 0;
+      const CellSeg* sg   = M.segments( P, segLen< NR >(), &nseg );
+      if ( nseg > 0 )
+         LAUNCH_MODES( kCellApply, dim3( (unsigned) xcdGrid( ( nseg + 3 ) / 4 ) ), dim3( 256 ), g, sg, nseg, CW,
+                       P.dCellType.p, flag, u, y, b );
+   }
+#undef LAUNCH_MODES
+}
+
+template < typename T, int NR >
+void sorT( Operator& op, Function& xf, const Function& bf, double omega, int level, int flag )
+{
+   Mesh&       M  = *op.mesh;
+   LevelPlan&  P  = M.plan( level );
+   LV          g  = viewOf( P );
+   hipStream_t s  = M.ctx->stream;
+   auto&       Wt = op.weights( level );
+   T*          u  = (T*) xf.ptr( level );
+   const T*    b  = (const T*) bf.ptr( level );
+   const T     om = (T) omega;
+   const int   nV = M.T.nV(), nE = M.T.nE(), nF = M.T.nF();
+   refreshT< T, NR >( xf, level, 7 );
+   hipLaunchKernelGGL( ( kVertexSor< T, NR > ), dim3( blocksFor( (int64_t) nV * NR, 256 ) ), dim3( 256 ), 0, s, g,
+                       P.dVertexGeom.p, P.dVertexTerms.p, nV, wsel< T >( Wt.vertex64, Wt.vertex32 ), flag, u, b, om );
+   refreshT< T, NR >( xf, level, 1 );
+   if ( level >= 1 )
+      for ( int c = 0; c < 2; c++ )
+         hipLaunchKernelGGL( ( kEdgeSor< T, NR > ), dim3( blocksFor( (int64_t) ( P.w - 2 ) * NR, 64 ), nE ), dim3( 64 ), 0,
+                             s, g, P.dEdgeGeom.p, P.dEdgeTerms.p, wsel< T >( Wt.edge64, Wt.edge32 ), flag, u, b, om, c );
+   refreshT< T, NR >( xf, level, 2 );
+   if ( level >= 2 )
+   {
+      const int64_t m = P.w - 3;
+      if ( m > 0 && nF > 0 )
+         for ( int c = 0; c < 3; c++ )
+            hipLaunchKernelGGL( ( kFaceSor< T, NR > ), dim3( blocksFor( m * ( m + 1 ) / 2 * NR, 256 ), nF ), dim3( 256 ), 0,
+                                s, g, P.dFaceGeom.p, wsel< T >( Wt.face64, Wt.face32 ), flag, u, b, om, c );
+      refreshT< T, NR >( xf, level, 4 );
+      int64_t        nseg = 0;
+      const CellSeg* sg   = M.segments( P, segLen< NR >(), &nseg );
+      if ( nseg > 0 )
+         for ( int c = 0; c < 4; c++ )
+            hipLaunchKernelGGL( ( kCellSor< T, NR > ), dim3( (unsigned) xcdGrid( ( nseg + 3 ) / 4 ) ), dim3( 256 ), 0, s,
+                                g, sg, nseg, wsel< T >( Wt.cell64, Wt.cell32 ), P.dCellType.p, flag, u, b, om, c );
+   }
+}
+
+template < typename T, int NR >
+void restrictT( Function& f, Function& dst, int srcLevel, int flag )
+{
+   Mesh&       M  = *f.mesh;
+   LevelPlan&  Pf = M.plan( srcLevel );
+   LevelPlan&  Pc = M.plan( srcLevel - 1 );
+   LV          gf = viewOf( Pf ), gc = viewOf( Pc );
+   hipStream_t s  = M.ctx->stream;
+   const T*    uf = (const T*) f.ptr( srcLevel );
+   T*          uc = (T*) dst.ptr( srcLevel - 1 );
+   refreshT< T, NR >( f, srcLevel, 7 );
+   const int nV = M.T.nV(), nE = M.T.nE(), nF = M.T.nF();
+   hipLaunchKernelGGL( ( kRestrictVertex< T, NR > ), dim3( blocksFor( (int64_t) nV * NR, 256 ) ), dim3( 256 ), 0, s,
+                       Pf.dVertexGeom.p, Pf.dVertexTerms.p, nV, flag, uf, uc );
+   if ( Pc.w > 2 )
+      hipLaunchKernelGGL( ( kRestrictEdge< T, NR > ), dim3( blocksFor( (int64_t) ( Pc.w - 2 ) * NR, 64 ), nE ), dim3( 64 ),
+                          0, s, gc, gf, Pf.dEdgeGeom.p, Pf.dEdgeTerms.p, flag, uf, uc );
+   const int64_t m = Pc.w - 3;
+   if ( m > 0 && nF > 0 )
+      hipLaunchKernelGGL( ( kRestrictFace< T, NR > ), dim3( blocksFor( m * ( m + 1 ) / 2 * NR, 256 ), nF ), dim3( 256 ), 0,
+                          s, gc, gf, Pf.dFaceGeom.p, flag, uf, uc );
+   int64_t        nseg = 0;
+   const CellSeg* sg   = M.segments( Pc, segLen< NR >(), &nseg );
+   if ( nseg > 0 )
+      hipLaunchKernelGGL( ( kRestrictCell< T, NR > ), dim3( (unsigned) xcdGrid( ( nseg + 3 ) / 4 ) ), dim3( 256 ), 0, s, gc,
+                          gf, sg, nseg, Pc.dCellType.p, flag, uf, uc );
+}
+
+template < typename T, int NR >
+void prolongT( Function& f, int srcLevel, int flag )
+{
+   Mesh&       M  = *f.mesh;
+   LevelPlan&  Pc = M.plan( srcLevel );
+   LevelPlan&  Pf = M.plan( srcLevel + 1 );
+   LV          gf = viewOf( Pf ), gc = viewOf( Pc );
+   hipStream_t s  = M.ctx->stream;
+   const T*    uc = (const T*) f.ptr( srcLevel );
+   T*          uf = (T*) f.ptr( srcLevel + 1 );
+   refreshT< T, NR >( f, srcLevel, 7 );
+   const int nV = M.T.nV(), nE = M.T.nE(), nF = M.T.nF();
+   hipLaunchKernelGGL( ( kProlongVertex< T, NR > ), dim3( blocksFor( (int64_t) nV * NR, 256 ) ), dim3( 256 ), 0, s,
+                       Pf.dVertexGeom.p, nV, flag, uc, uf );
+   hipLaunchKernelGGL( ( kProlongEdge< T, NR > ), dim3( blocksFor( (int64_t) ( Pf.w - 2 ) * NR, 64 ), nE ), dim3( 64 ), 0,
+                       s, gc, gf, Pf.dEdgeGeom.p, flag, uc, uf );
+   const int64_t m = Pf.w - 3;
+   if ( m > 0 && nF > 0 )
+      hipLaunchKernelGGL( ( kProlongFace< T, NR > ), dim3( blocksFor( m * ( m + 1 ) / 2 * NR, 256 ), nF ), dim3( 256 ), 0,
+                          s, gc, gf, Pf.dFaceGeom.p, flag, uc, uf );
+   int64_t        nseg = 0;
+   const CellSeg* sg   = M.segments( Pf, segLen< NR >(), &nseg );
+   if ( nseg > 0 )
+      hipLaunchKernelGGL( ( kProlongCell< T, NR > ), dim3( (unsigned) xcdGrid( ( nseg + 3 ) / 4 ) ), dim3( 256 ), 0, s, gc,
+                          gf, sg, nseg, Pf.dCellType.p, flag, uc, uf );
+}
+
+template < typename T, int NR >
+void dotT( Function& a, Function& b, int level, int flag, int slot )
+{
+   Mesh&       M   = *a.mesh;
+   Context&    ctx = *M.ctx;
+   LevelPlan&  P   = M.plan( level );
+   LV          g   = viewOf( P );
+   hipStream_t s   = ctx.stream;
+   int64_t     nseg = 0;
+   const CellSeg* sg = M.segments( P, segLen< NR >(), &nseg );
+   const int64_t nbCell = ( nseg + 3 ) / 4;
+   const int64_t nbLow  = 64;
+   const int64_t need   = ( nbCell + nbLow ) * NR;
+   if ( (int64_t) ctx.partials.n < need ) ctx.partials.alloc( (size_t) need );
+   if ( nbCell > 0 && level >= 2 )
+      hipLaunchKernelGGL( ( kDotCell< T, NR > ), dim3( (unsigned) nbCell ), dim3( 256 ), 0, s, g, sg, nseg, P.dCellType.p,
+                          flag, (const T*) a.ptr( level ), (const T*) b.ptr( level ), ctx.partials.p );
+   const int64_t nbC = ( level >= 2 ) ? nbCell : 0;
+   hipLaunchKernelGGL( ( kDotLow< T, NR > ), dim3( (unsigned) nbLow ), dim3( 256 ), 0, s, g, P.dFaceGeom.p, M.T.nF(),
+                       P.dEdgeGeom.p, M.T.nE(), P.dVertexGeom.p, M.T.nV(), flag, (const T*) a.ptr( level ),
+                       (const T*) b.ptr( level ), ctx.partials.p + nbC * NR );
+   hipLaunchKernelGGL( ( kFinishDot< NR > ), dim3( 1 ), dim3( 256 ), 0, s, ctx.partials.p, nbC + nbLow,
+                       ctx.scalars.p + (int64_t) slot * NR );
+}
+
+template < typename T, typename S, int NR >
+void lincombT( Function& dst, int level, int flag, int n, const double* c, Function* const* src, bool acc, double k )
+{
+   Mesh&        M  = *dst.mesh;
+   LevelPlan&   P  = M.plan( level );
+   int64_t      nr = 0;
+   const Range* R  = M.closureRanges( P, flag, &nr );
+   if ( nr == 0 ) return;
+   const S* s0 = n > 0 ? (const S*) src[0]->ptr( level ) : nullptr;
+   const S* s1 = n > 1 ? (const S*) src[1]->ptr( level ) : nullptr;
+   const S* s2 = n > 2 ? (const S*) src[2]->ptr( level ) : nullptr;
+   hipLaunchKernelGGL( ( kLincomb< T, S, NR > ), dim3( (unsigned) nr ), dim3( 256 ), 0, M.ctx->stream, R,
+                       (T*) dst.ptr( level ), n, s0, s1, s2, n > 0 ? c[0] : 0.0, n > 1 ? c[1] : 0.0, n > 2 ? c[2] : 0.0,
+                       acc ? 1 : 0, k );
+}
+
+template < typename T, int NR >
+void axpyScalarT( Function& y, Function& x, int level, int flag, int slot, double sign )
+{
+   Mesh&        M  = *y.mesh;
+   LevelPlan&   P  = M.plan( level );
+   int64_t      nr = 0;
+   const Range* R  = M.closureRanges( P, flag, &nr );
+   if ( nr == 0 ) return;
+   hipLaunchKernelGGL( ( kAxpyScalar< T, NR > ), dim3( (unsigned) nr ), dim3( 256 ), 0, M.ctx->stream, R,
+                       (T*) y.ptr( level ), (const T*) x.ptr( level ), M.ctx->scalars.p + (int64_t) slot * NR, sign );
+}
+
+template < typename T, int NR >
+void xpbyScalarT( Function& p, Function& r, int level, int flag, int slot )
+{
+   Mesh&        M  = *p.mesh;
+   LevelPlan&   P  = M.plan( level );
+   int64_t      nr = 0;
+   const Range* R  = M.closureRanges( P, flag, &nr );
+   if ( nr == 0 ) return;
+   hipLaunchKernelGGL( ( kXpbyScalar< T, NR > ), dim3( (unsigned) nr ), dim3( 256 ), 0, M.ctx->stream, R,
+                       (T*) p.ptr( level ), (const T*) r.ptr( level ), M.ctx->scalars.p + (int64_t) slot * NR );
+}
+
+// dispatch on (precision, nrhs)
+template < template < typename, int > class F, typename... Args >
+void dispatch( int prec, int nrhs, Args&&... args )
+{
+   if ( prec == kF64 )
+   {
+      if ( nrhs == 1 ) return F< double, 1 >::run( args... );
+      if ( nrhs == 32 ) return F< double, 32 >::run( args... );
+   }
+   else
+   {
+      if ( nrhs == 1 ) return F< float, 1 >::run( args... );
+      if ( nrhs == 32 ) return F< float, 32 >::run( args... );
+   }
+   throw std::runtime_error( "unsupported precision / nrhs combination (nrhs must be 1 or 32)" );
+}
+
+template < typename T, int NR >
+struct RefreshF
+{
+   static void run( Function& f, int level, int mask ) { refreshT< T, NR >( f, level, mask ); }
+};
+template < typename T, int NR >
+struct ApplyF
+{
+   static void run( Operator& op, Function& s, Function& d, const Function* b, int L, int flag, int mode )
+   {
+      applyT< T, NR >( op, s, d, b, L, flag, mode );
+   }
+};
+template < typename T, int NR >
+struct SorF
+{
+   static void run( Operator& op, Function& x, const Function& b, double omega, int L, int flag )
+   {
+      sorT< T, NR >( op, x, b, omega, L, flag );
+   }
+};
+template < typename T, int NR >
+struct RestrictF
+{
+   static void run( Function& f, Function& d, int L, int flag ) { restrictT< T, NR >( f, d, L, flag ); }
+};
+template < typename T, int NR >
+struct ProlongF
+{
+   static void run( Function& f, int L, int flag ) { prolongT< T, NR >( f, L, flag ); }
+};
+template < typename T, int NR >
+struct DotF
+{
+   static void run( Function& a, Function& b, int L, int flag, int slot ) { dotT< T, NR >( a, b, L, flag, slot ); }
+};
+template < typename T, int NR >
+struct LincombF
+{
+   static void run( Function& d, int L, int flag, int n, const double* c, Function* const* s, bool acc, double k )
+   {
+      lincombT< T, T, NR >( d, L, flag, n, c, s, acc, k );
+   }
+};
+template < typename T, int NR >
+struct AxpyF
+{
+   static void run( Function& y, Function& x, int L, int flag, int slot, double sign )
+   {
+      axpyScalarT< T, NR >( y, x, L, flag, slot, sign );
+   }
+};
+template < typename T, int NR >
+struct XpbyF
+{
+   static void run( Function& p, Function& r, int L, int flag, int slot ) { xpbyScalarT< T, NR >( p, r, L, flag, slot ); }
+};
+
+} // namespace
+
+void launchRefresh( Function& f, int level, int mask ) { dispatch< RefreshF >( f.prec, f.nrhs, f, level, mask ); }
+
+void launchApply( Operator& op, Function& src, Function& dst, const Function* rhs, int level, int flag, int mode )
+{
+   dispatch< ApplyF >( src.prec, src.nrhs, op, src, dst, rhs, level, flag, mode );
+}
+
+void launchSor( Operator& op, Function& x, const Function& b, double omega, int level, int flag )
+{
+   dispatch< SorF >( x.prec, x.nrhs, op, x, b, omega, level, flag );
+}
+
+void launchRestrict( Function& f, Function& dst, int srcLevel, int flag )
+{
+   if ( dst.prec != f.prec || dst.nrhs != f.nrhs || dst.mesh != f.mesh )
+      throw std::runtime_error( "restrict: source and destination must share mesh, precision and nrhs" );
+   dispatch< RestrictF >( f.prec, f.nrhs, f, dst, srcLevel, flag );
+}
+
+void launchProlongateAdd( Function& f, int srcLevel, int flag )
+{
+   dispatch< ProlongF >( f.prec, f.nrhs, f, srcLevel, flag );
+}
+
+void launchLincomb( Function& dst, int level, int flag, int n, const double* c, Function* const* src, bool accumulate,
+                    double constant )
+{
+   dispatch< LincombF >( dst.prec, dst.nrhs, dst, level, flag, n, c, src, accumulate, constant );
+}
+
+void launchDot( Function& a, Function& b, int level, int flag, int slot )
+{
+   dispatch< DotF >( a.prec, a.nrhs, a, b, level, flag, slot );
+}
+
+void launchAxpyScalar( Function& y, Function& x, int level, int flag, int slot, double sign )
+{
+   dispatch< AxpyF >( y.prec, y.nrhs, y, x, level, flag, slot, sign );
+}
+
+void launchXpbyScalar( Function& p, Function& r, int level, int flag, int slot )
+{
+   dispatch< XpbyF >( p.prec, p.nrhs, p, r, level, flag, slot );
+}
+
+void launchScalarRatio( Context& ctx, int nrhs, int dst, int num, int den, int gate, double gateMin )
+{
+   hipLaunchKernelGGL( kScalarRatio, dim3( 1 ), dim3( 64 ), 0, ctx.stream, ctx.scalars.p, nrhs, dst, num, den, gate,
+                       gateMin );
+}
+
+void launchScalarCopy( Context& ctx, int nrhs, int dst, int src )
+{
+   hipLaunchKernelGGL( kScalarCopy, dim3( 1 ), dim3( 64 ), 0, ctx.stream, ctx.scalars.p, nrhs, dst, src );
+}
+
+void launchConvert( Function& dst, int dstLevel, Function& src, int srcLevel, int flag )
+{
+   if ( dstLevel != srcLevel || dst.nrhs != src.nrhs || dst.mesh != src.mesh )
+      throw std::runtime_error( "convert: functions must share mesh, level and nrhs" );
+   Mesh&        M  = *dst.mesh;
+   LevelPlan&   P  = M.plan( dstLevel );
+   int64_t      nr = 0;
+   const Range* R  = M.closureRanges( P, flag, &nr );
+   if ( nr == 0 ) return;
+   const double c1[1] = { 1.0 };
+   hipStream_t  s     = M.ctx->stream;
+   auto         go    = [&]( auto* d, auto* sp, auto nrTag ) {
+      using TD            = std::remove_pointer_t< decltype( d ) >;
+      using TS            = std::remove_const_t< std::remove_pointer_t< decltype( sp ) > >;
+      constexpr int NRv   = decltype( nrTag )::value;
+      hipLaunchKernelGGL( ( kLincomb< TD, TS, NRv > ), dim3( (unsigned) nr ), dim3( 256 ), 0, s, R, d, 1, sp,
+                          (const TS*) nullptr, (const TS*) nullptr, c1[0], 0.0, 0.0, 0, 0.0 );
+   };
+   auto run = [&]( auto nrTag ) {
+      if ( dst.prec == kF64 && src.prec == kF32 )
+         go( (double*) dst.ptr( dstLevel ), (const float*) src.ptr( srcLevel ), nrTag );
+      else if ( dst.prec == kF32 && src.prec == kF64 )
+         go( (float*) dst.ptr( dstLevel ), (const double*) src.ptr( srcLevel ), nrTag );
+      else if ( dst.prec == kF64 )
+         go( (double*) dst.ptr( dstLevel ), (const double*) src.ptr( srcLevel ), nrTag );
+      else
+         go( (float*) dst.ptr( dstLevel ), (const float*) src.ptr( srcLevel ), nrTag );
+   };
+   if ( dst.nrhs == 1 )
+      run( std::integral_constant< int, 1 >{} );
+   else if ( dst.nrhs == 32 )
+      run( std::integral_constant< int, 32 >{} );
+   else
+      throw std::runtime_error( "unsupported nrhs" );
+}
+
+void launchAddFrom( Function& x, Function& e, int level, int flag )
+{
+   if ( x.nrhs != e.nrhs || x.mesh != e.mesh ) throw std::runtime_error( "add: functions must share mesh and nrhs" );
+   Mesh&        M  = *x.mesh;
+   LevelPlan&   P  = M.plan( level );
+   int64_t      nr = 0;
+   const Range* R  = M.closureRanges( P, flag, &nr );
+   if ( nr == 0 ) return;
+   hipStream_t s = M.ctx->stream;
+   auto        go = [&]( auto* d, auto* sp, auto nrTag ) {
+      using TD          = std::remove_pointer_t< decltype( d ) >;
+      using TS          = std::remove_const_t< std::remove_pointer_t< decltype( sp ) > >;
+      constexpr int NRv = decltype( nrTag )::value;
+      hipLaunchKernelGGL( ( kLincomb< TD, TS, NRv > ), dim3( (unsigned) nr ), dim3( 256 ), 0, s, R, d, 1, sp,
+                          (const TS*) nullptr, (const TS*) nullptr, 1.0, 0.0, 0.0, 1, 0.0 );
+   };
+   auto run = [&]( auto nrTag ) {
+      if ( x.prec == kF64 && e.prec == kF32 )
+         go( (double*) x.ptr( level ), (const float*) e.ptr( level ), nrTag );
+      else if ( x.prec == kF64 && e.prec == kF64 )
+         go( (double*) x.ptr( level ), (const double*) e.ptr( level ), nrTag );
+      else if ( x.prec == kF32 && e.prec == kF32 )
+         go( (float*) x.ptr( level ), (const float*) e.ptr( level ), nrTag );
+      else
+         go( (float*) x.ptr( level ), (const double*) e.ptr( level ), nrTag );
+   };
+   if ( x.nrhs == 1 )
+      run( std::integral_constant< int, 1 >{} );
+   else if ( x.nrhs == 32 )
+      run( std::integral_constant< int, 32 >{} );
+   else
+      throw std::runtime_error( "unsupported nrhs" );
+}
+
+} // namespace hymi

@@ -1,0 +1,41 @@
+// Launchers of the HIP kernels (kernels.hip).  All launches go to the context's stream; none synchronises.
+#pragma once
+
+#include "internal.hpp"
+
+namespace hymi {
+
+enum ApplyMode : int
+{
+   kReplace  = 0,
+   kAdd      = 1,
+   kResidual = 2 // dst = b - A src
+};
+
+// copy owned values into the closure copies of higher-dimensional primitives: bit 0 V->E, bit 1 E->F,
+// bit 2 F->C (executed in that order)
+void launchRefresh( Function& f, int level, int mask );
+void launchApply( Operator& op, Function& src, Function& dst, const Function* rhs, int level, int flag, int mode );
+// one multicolour SOR sweep in the reference's V -> E -> F -> C order (refreshes included)
+void launchSor( Operator& op, Function& x, const Function& b, double omega, int level, int flag );
+// coarse level of dst <- restriction of the fine level of f (dst may be f)
+void launchRestrict( Function& f, Function& dst, int srcLevel, int flag );
+void launchProlongateAdd( Function& f, int srcLevel, int flag );
+// dst = sum_k c[k] * src[k] (+ dst if accumulate) + constant, on the closures of primitives matching flag
+void launchLincomb( Function& dst, int level, int flag, int n, const double* c, Function* const* src, bool accumulate,
+                    double constant );
+// per-rhs dot product over owned DoFs of primitives matching flag; result in ctx scalars[slot*nrhs + r]
+void launchDot( Function& a, Function& b, int level, int flag, int slot );
+// y += sign * scalars[slot*nrhs + r] * x   (closures of primitives matching flag)
+void launchAxpyScalar( Function& y, Function& x, int level, int flag, int slot, double sign );
+// p = r + scalars[slot*nrhs + r] * p
+void launchXpbyScalar( Function& p, Function& r, int level, int flag, int slot );
+// scalars[dst] = (scalars[den] != 0 && scalars[gate] > gateMin) ? scalars[num] / scalars[den] : 0, per rhs
+void launchScalarRatio( Context& ctx, int nrhs, int dst, int num, int den, int gate, double gateMin );
+void launchScalarCopy( Context& ctx, int nrhs, int dst, int src );
+// mixed precision: dst (any precision) = src (any precision) on closures matching flag
+void launchConvert( Function& dst, int dstLevel, Function& src, int srcLevel, int flag );
+// x (FP64) += e (FP32 or FP64)
+void launchAddFrom( Function& x, Function& e, int level, int flag );
+
+} // namespace hymi

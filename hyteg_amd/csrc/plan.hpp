@@ -1,0 +1,94 @@
+// Per-level plans: the HBM layout of a P1Function on one refinement level and the precomputed tables the
+// HIP kernels walk (row segments of macro-cell interiors, neighbour maps of faces, edges and vertices).
+//
+// Layout of one level of a function (elements; every element holds `nrhs` interleaved values):
+//   [ vertices | edges (closure, w each) | faces (closure, triangular) | cells (closure, tetrahedral) ]
+// Each primitive's closure array keeps copies of its lower-dimensional boundary, refreshed by the
+// V->E, E->F and F->C copy kernels; the owned interiors are the unknowns.  Upward reads (an edge reading
+// its faces' first layer, a face reading its cells' first layer) go straight to the neighbour's array, so
+// no face or edge ghost layers exist.
+#pragma once
+
+#include <cstdint>
+
+namespace hymi {
+
+// A run of one cell row: cell `cell`, slice z, row y, x from x0 (length fixed by the launch).
+struct CellSeg
+{
+   int32_t cell, z, y, x0;
+};
+
+// contiguous element range
+struct Range
+{
+   int64_t off;
+   int32_t len;
+   int32_t pad;
+};
+
+// one side of a face: face (i, j) at layer l  ->  cell (x, y, z) = base + i*a + j*b + l*o
+struct FaceCellMap
+{
+   int32_t cell;
+   int32_t base[3];
+   int32_t a[3], b[3], o[3];
+};
+
+// geometry of a face: its neighbour cells and the four off-plane neighbours per side
+struct FaceGeom
+{
+   int32_t     face, nSides, type, pad;
+   FaceCellMap side[2];
+   int32_t     offCol[2][4], offRow[2][4];
+};
+
+// neighbour term of an edge point x: kind 0 = face array at (col,row) = base + x*inc,
+// kind 1 = cell array at (x,y,z) = base + x*inc
+struct EdgeTerm
+{
+   int32_t kind, prim;
+   int32_t base[3], inc[3];
+   float   wr; // restriction weight (0.5 if the micro-edge exists, else 0)
+   int32_t pad;
+};
+
+struct EdgeGeom
+{
+   int32_t edge, nTerms, termOff, type;
+};
+
+struct VertexGeom
+{
+   int32_t vertex, nTerms, termOff, type;
+};
+
+// edge closure endpoints <- vertex values
+struct V2E
+{
+   int32_t edge, v0, v1, pad;
+};
+
+// face boundary <- edge closure: face (col,row) = base + t*inc, t = 0..w-1
+struct E2F
+{
+   int32_t face, edge;
+   int32_t base[2], inc[2];
+};
+
+// cell boundary <- face closure
+struct F2C
+{
+   int32_t     face, pad;
+   FaceCellMap m;
+};
+
+// Operator weights aligned with the geometry tables.
+//   face: 15 per face: in-plane C, W, E, N, S, NW, SE (summed over sides), then side0 off[4], side1 off[4]
+//   edge: 3 (W, C, E) per edge followed by one weight per EdgeTerm
+//   vertex: 1 (centre) per vertex followed by one weight per vertex term
+//   cell: 15 per cell in the direction order C W E N S NW SE TC TW TS TSE BC BN BE BNW
+constexpr int kFaceW = 15;
+constexpr int kCellW = 15;
+
+} // namespace hymi

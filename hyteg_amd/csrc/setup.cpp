@@ -1,0 +1,654 @@
+// Host-side setup: per-level layout/geometry plans and constant stencil weights.
+//
+// Stencil weights follow the reference's assembly: the weight of a micro-vertex neighbour is the sum of the
+// first-row entries of the local element matrices of the micro-tetrahedra around the vertex
+// (src/hyteg/p1functionspace/P1Elements.hpp:86-136 for the 24 micro-tetrahedra, :214-284 for the ones
+// inside one macro-cell, :430-475 for the sum); macro-cell weights are taken at (1,1,1)
+// (src/hyteg/p1functionspace/P1ConstantOperator.cpp:549-556), face weights per neighbour cell at the face
+// point (1,1) (P1Operator.hpp:2031-2042), edge weights at the edge point 1 (P1Elements.hpp:829-964) and
+// vertex weights from the single corner micro-tetrahedron of every neighbour cell (P1Elements.hpp:601-683).
+#include <algorithm>
+#include <cmath>
+#include <stdexcept>
+
+#include "internal.hpp"
+
+namespace hymi {
+
+namespace {
+
+enum Dir
+{
+   C,
+   W,
+   E,
+   N,
+   S,
+   NW,
+   SE,
+   TC,
+   TW,
+   TS,
+   TSE,
+   BC,
+   BN,
+   BE,
+   BNW,
+   NDIR
+};
+const int DOFF[NDIR][3] = { { 0, 0, 0 },  { -1, 0, 0 }, { 1, 0, 0 },  { 0, 1, 0 },  { 0, -1, 0 },
+                            { -1, 1, 0 }, { 1, -1, 0 }, { 0, 0, 1 },  { -1, 0, 1 }, { 0, -1, 1 },
+                            { 1, -1, 1 }, { 0, 0, -1 }, { 0, 1, -1 }, { 1, 0, -1 }, { -1, 1, -1 } };
+// micro-tetrahedra around an inner micro-vertex, each listed from the centre vertex
+const int TETS[24][4] = { { C, BC, BE, BN }, { C, S, SE, TS },   { C, W, NW, TW },   { C, N, E, TC },
+                          { C, W, BC, S },   { C, E, SE, BE },   { C, N, NW, BN },   { C, TS, TC, TW },
+                          { C, BC, BN, BNW }, { C, W, S, TS },   { C, E, SE, TSE },  { C, NW, N, TC },
+                          { C, BC, S, SE },  { C, W, NW, BNW },  { C, E, BN, N },    { C, TC, TS, TSE },
+                          { C, W, BC, BNW }, { C, E, BE, BN },   { C, TC, TW, NW },  { C, SE, TS, TSE },
+                          { C, BC, BE, SE }, { C, BN, BNW, NW }, { C, E, TSE, TC },  { C, W, TS, TW } };
+
+int64_t roundUp( int64_t v, int64_t a ) { return ( v + a - 1 ) / a * a; }
+
+// barycentric change of a cell-frame direction: (db0, db1, db2, db3)
+void dirBary( int d, int out[4] )
+{
+   out[1] = DOFF[d][0];
+   out[2] = DOFF[d][1];
+   out[3] = DOFF[d][2];
+   out[0] = -( out[1] + out[2] + out[3] );
+}
+
+int oppositeOf( const int* fm )
+{
+   int used = ( 1 << fm[0] ) | ( 1 << fm[1] ) | ( 1 << fm[2] );
+   for ( int k = 0; k < 4; k++ )
+      if ( !( used & ( 1 << k ) ) ) return k;
+   return -1;
+}
+
+// first row of the P1 element matrix of tetrahedron P (P[0] is the row vertex); form 0 diffusion, 1 mass
+void elementRow( int form, const double P[4][3], double row[4] )
+{
+   double J[3][3];
+   for ( int r = 0; r < 3; r++ )
+      for ( int c = 0; c < 3; c++ )
+         J[r][c] = P[c + 1][r] - P[0][r];
+   const double cof00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+   const double cof01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+   const double cof02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+   const double det   = J[0][0] * cof00 + J[0][1] * cof01 + J[0][2] * cof02;
+   const double vol   = std::fabs( det ) / 6.0;
+   if ( form == 1 )
+   {
+      row[0] = vol / 10.0;
+      row[1] = row[2] = row[3] = vol / 20.0;
+      return;
+   }
+   // gradients of the barycentric functions 1..3 are the rows of J^{-1} = adj(J)/det
+   double g[4][3];
+   g[1][0] = cof00 / det;
+   g[1][1] = ( J[0][2] * J[2][1] - J[0][1] * J[2][2] ) / det;
+   g[1][2] = ( J[0][1] * J[1][2] - J[0][2] * J[1][1] ) / det;
+   g[2][0] = cof01 / det;
+   g[2][1] = ( J[0][0] * J[2][2] - J[0][2] * J[2][0] ) / det;
+   g[2][2] = ( J[0][2] * J[1][0] - J[0][0] * J[1][2] ) / det;
+   g[3][0] = cof02 / det;
+   g[3][1] = ( J[0][1] * J[2][0] - J[0][0] * J[2][1] ) / det;
+   g[3][2] = ( J[0][0] * J[1][1] - J[0][1] * J[1][0] ) / det;
+   for ( int d = 0; d < 3; d++ )
+      g[0][d] = -( g[1][d] + g[2][d] + g[3][d] );
+   for ( int j = 0; j < 4; j++ )
+      row[j] = vol * ( g[0][0] * g[j][0] + g[0][1] * g[j][1] + g[0][2] * g[j][2] );
+}
+
+// stencil of cell-frame point p of cell c from the micro-tetrahedra inside the cell
+void pointStencil( const Topology& T, int c, int form, int w, const int p[3], double st[NDIR], bool present[NDIR] )
+{
+   const auto& V0 = T.vx[T.cv[c][0]];
+   const auto& V1 = T.vx[T.cv[c][1]];
+   const auto& V2 = T.vx[T.cv[c][2]];
+   const auto& V3 = T.vx[T.cv[c][3]];
+   const double h = 1.0 / ( w - 1 );
+   for ( int d = 0; d < NDIR; d++ )
+   {
+      st[d]      = 0.0;
+      present[d] = false;
+   }
+   for ( const auto& t : TETS )
+   {
+      double P[4][3];
+      bool   inside = true;
+      for ( int k = 0; k < 4 && inside; k++ )
+      {
+         const int x = p[0] + DOFF[t[k]][0], y = p[1] + DOFF[t[k]][1], z = p[2] + DOFF[t[k]][2];
+         if ( x < 0 || y < 0 || z < 0 || x + y + z > w - 1 )
+         {
+            inside = false;
+            break;
+         }
+         for ( int d = 0; d < 3; d++ )
+            P[k][d] = V0[d] + h * ( x * ( V1[d] - V0[d] ) + y * ( V2[d] - V0[d] ) + z * ( V3[d] - V0[d] ) );
+      }
+      if ( !inside ) continue;
+      double row[4];
+      elementRow( form, P, row );
+      for ( int k = 0; k < 4; k++ )
+      {
+         st[t[k]] += row[k];
+         present[t[k]] = true;
+      }
+   }
+}
+
+// face-frame (col,row) of point t on local face edge le at distance o from that edge
+void faceEdgePoint( const Topology& T, int f, int le, int w, int t, int o, int out[2] )
+{
+   static const int FE[3][2] = { { 0, 1 }, { 0, 2 }, { 1, 2 } };
+   int              a = FE[le][0], b = FE[le][1];
+   if ( T.fo[f][le] < 0 ) std::swap( a, b );
+   const int third = 3 - FE[le][0] - FE[le][1];
+   int       bb[3];
+   bb[a]     = w - 1 - t - o;
+   bb[b]     = t;
+   bb[third] = o;
+   out[0]    = bb[1];
+   out[1]    = bb[2];
+}
+
+FaceCellMap numericMap( const Topology& T, int c, int lf, int w )
+{
+   FaceToCellMap M = faceToCellMap( T, c, lf );
+   FaceCellMap   R{};
+   R.cell = c;
+   for ( int d = 0; d < 3; d++ )
+   {
+      R.base[d] = M.kx[d] * ( w - 1 );
+      R.a[d]    = M.a[d];
+      R.b[d]    = M.b[d];
+      R.o[d]    = M.o[d];
+   }
+   return R;
+}
+
+bool inCell( int w, int x, int y, int z ) { return x >= 0 && y >= 0 && z >= 0 && x + y + z <= w - 1; }
+bool inFace( int w, int i, int j ) { return i >= 0 && j >= 0 && i + j <= w - 1; }
+
+// Every affine index map used by a kernel is checked at its extreme arguments on the host (the maps are
+// affine, so the extremes bound all arguments); a wrong map becomes an exception here, never an
+// out-of-bounds access on the device.
+void validatePlan( const Topology& T, const LevelPlan& P )
+{
+   const int w  = P.w;
+   auto      ok = [&]( bool c, const char* what ) {
+      if ( !c ) throw std::runtime_error( std::string( "plan validation failed: " ) + what );
+   };
+   for ( int f = 0; f < T.nF(); f++ )
+   {
+      const FaceGeom& g = P.hFaceGeom[f];
+      ok( g.nSides >= 1 && g.nSides <= 2, "face sides" );
+      for ( int n = 0; n < g.nSides; n++ )
+      {
+         const FaceCellMap& m = g.side[n];
+         ok( m.cell >= 0 && m.cell < T.nC(), "face side cell" );
+         // closure triangle at layer 0 and the layer-1 triangle of width w-1
+         const int corners0[3][2] = { { 0, 0 }, { w - 1, 0 }, { 0, w - 1 } };
+         for ( const auto& c : corners0 )
+            ok( inCell( w, m.base[0] + c[0] * m.a[0] + c[1] * m.b[0], m.base[1] + c[0] * m.a[1] + c[1] * m.b[1],
+                        m.base[2] + c[0] * m.a[2] + c[1] * m.b[2] ),
+                "face-to-cell map (layer 0)" );
+         if ( w >= 2 )
+         {
+            const int corners1[3][2] = { { 0, 0 }, { w - 2, 0 }, { 0, w - 2 } };
+            for ( const auto& c : corners1 )
+               ok( inCell( w, m.base[0] + c[0] * m.a[0] + c[1] * m.b[0] + m.o[0],
+                           m.base[1] + c[0] * m.a[1] + c[1] * m.b[1] + m.o[1],
+                           m.base[2] + c[0] * m.a[2] + c[1] * m.b[2] + m.o[2] ),
+                   "face-to-cell map (layer 1)" );
+         }
+         for ( int k = 0; k < 4; k++ )
+            ok( std::abs( g.offCol[n][k] ) <= 1 && std::abs( g.offRow[n][k] ) <= 1, "off-plane offset" );
+      }
+   }
+   for ( int e = 0; e < T.nE(); e++ )
+   {
+      const EdgeGeom& g = P.hEdgeGeom[e];
+      if ( w < 3 ) continue;
+      for ( int k = 0; k < g.nTerms; k++ )
+      {
+         const EdgeTerm& t = P.hEdgeTerms[g.termOff + k];
+         for ( int x : { 1, w - 2 } )
+         {
+            if ( t.kind == 0 )
+               ok( t.prim >= 0 && t.prim < T.nF() && inFace( w, t.base[0] + x * t.inc[0], t.base[1] + x * t.inc[1] ),
+                   "edge face term" );
+            else
+               ok( t.prim >= 0 && t.prim < T.nC() &&
+                       inCell( w, t.base[0] + x * t.inc[0], t.base[1] + x * t.inc[1], t.base[2] + x * t.inc[2] ),
+                   "edge cell term" );
+         }
+      }
+   }
+   for ( int64_t idx : P.hVertexTerms )
+      ok( idx >= P.offE && idx < P.offF, "vertex term" );
+}
+
+} // namespace
+
+void buildPlan( Mesh& M, LevelPlan& P, int level )
+{
+   const Topology& T = M.T;
+   const int       w = widthOf( level );
+   P.L               = level;
+   P.w               = w;
+   P.eStride         = roundUp( w, 16 );
+   P.fStride         = roundUp( triSize( w ), 32 );
+   P.cStride         = roundUp( tetSize( w ), 64 );
+   P.offE            = roundUp( T.nV(), 64 );
+   P.offF            = roundUp( P.offE + T.nE() * P.eStride, 64 );
+   P.offC            = roundUp( P.offF + T.nF() * P.fStride, 64 );
+   P.total           = P.offC + (int64_t) T.nC() * P.cStride;
+   hipStream_t s     = M.ctx->stream;
+   const bool  dev   = !M.ctx->hostOnly;
+
+   std::vector< V2E > v2e;
+   for ( int e = 0; e < T.nE(); e++ )
+      v2e.push_back( { e, T.ev[e][0], T.ev[e][1], 0 } );
+   if ( dev ) P.dV2E.upload( v2e, s );
+
+   std::vector< E2F > e2f;
+   for ( int f = 0; f < T.nF(); f++ )
+      for ( int le = 0; le < 3; le++ )
+      {
+         int p0[2], p1[2];
+         faceEdgePoint( T, f, le, w, 0, 0, p0 );
+         faceEdgePoint( T, f, le, w, 1, 0, p1 );
+         e2f.push_back( { f, T.fe[f][le], { p0[0], p0[1] }, { p1[0] - p0[0], p1[1] - p0[1] } } );
+         const E2F& m = e2f.back();
+         if ( !inFace( w, m.base[0], m.base[1] ) ||
+              !inFace( w, m.base[0] + ( w - 1 ) * m.inc[0], m.base[1] + ( w - 1 ) * m.inc[1] ) )
+            throw std::runtime_error( "plan validation failed: edge-to-face map" );
+      }
+   if ( dev ) P.dE2F.upload( e2f, s );
+
+   std::vector< F2C > f2c;
+   for ( int c = 0; c < T.nC(); c++ )
+      for ( int lf = 0; lf < 4; lf++ )
+      {
+         F2C x{};
+         x.face = T.cf[c][lf];
+         x.m    = numericMap( T, c, lf, w );
+         const int corners[3][2] = { { 0, 0 }, { w - 1, 0 }, { 0, w - 1 } };
+         for ( const auto& k : corners )
+            if ( !inCell( w, x.m.base[0] + k[0] * x.m.a[0] + k[1] * x.m.b[0], x.m.base[1] + k[0] * x.m.a[1] + k[1] * x.m.b[1],
+                          x.m.base[2] + k[0] * x.m.a[2] + k[1] * x.m.b[2] ) )
+               throw std::runtime_error( "plan validation failed: face-to-cell copy map" );
+         f2c.push_back( x );
+      }
+   if ( dev ) P.dF2C.upload( f2c, s );
+
+   // faces: neighbour-cell maps and the four off-plane neighbours per side (in direction order)
+   P.hFaceGeom.clear();
+   for ( int f = 0; f < T.nF(); f++ )
+   {
+      FaceGeom g{};
+      g.face   = f;
+      g.type   = T.fType[f];
+      g.nSides = (int) T.fCells[f].size();
+      for ( int n = 0; n < g.nSides; n++ )
+      {
+         const int  c   = T.fCells[f][n];
+         const int  lf  = T.localFace( c, f );
+         const int* fm  = &T.cfmap[c][3 * lf];
+         const int  opp = oppositeOf( fm );
+         g.side[n]      = numericMap( T, c, lf, w );
+         int k          = 0;
+         for ( int d = 0; d < NDIR; d++ )
+         {
+            int db[4];
+            dirBary( d, db );
+            if ( db[opp] != 1 ) continue;
+            if ( k >= 4 ) throw std::runtime_error( "more than four off-plane face neighbours" );
+            g.offCol[n][k] = db[fm[1]];
+            g.offRow[n][k] = db[fm[2]];
+            k++;
+         }
+         if ( k != 4 ) throw std::runtime_error( "face side without four off-plane neighbours" );
+      }
+      P.hFaceGeom.push_back( g );
+   }
+   if ( dev ) P.dFaceGeom.upload( P.hFaceGeom, s );
+
+   // edges: face-layer terms (W at edge point x-1, E at x) per neighbour face, diagonal term per cell
+   P.hEdgeGeom.clear();
+   P.hEdgeTerms.clear();
+   for ( int e = 0; e < T.nE(); e++ )
+   {
+      EdgeGeom g{};
+      g.edge    = e;
+      g.type    = T.eType[e];
+      g.termOff = (int) P.hEdgeTerms.size();
+      for ( int f : T.eFaces[e] )
+      {
+         const int le = ( T.fe[f][0] == e ) ? 0 : ( T.fe[f][1] == e ) ? 1 : 2;
+         for ( int side = 0; side < 2; side++ ) // 0: W (t = x-1), 1: E (t = x)
+         {
+            int q0[2], q1[2];
+            faceEdgePoint( T, f, le, w, 0 - ( side == 0 ? 1 : 0 ), 1, q0 );
+            faceEdgePoint( T, f, le, w, 1 - ( side == 0 ? 1 : 0 ), 1, q1 );
+            EdgeTerm t{};
+            t.kind    = 0;
+            t.prim    = f;
+            t.base[0] = q0[0];
+            t.base[1] = q0[1];
+            t.inc[0]  = q1[0] - q0[0];
+            t.inc[1]  = q1[1] - q0[1];
+            t.wr      = 0.5f;
+            P.hEdgeTerms.push_back( t );
+         }
+      }
+      for ( int c : T.eCells[e] )
+      {
+         const int  le = T.localEdge( c, e );
+         const int* em = &T.cemap[c][2 * le];
+         int        bx0[4] = { 0, 0, 0, 0 }, bx1[4] = { 0, 0, 0, 0 };
+         for ( int k = 0; k < 4; k++ )
+            if ( k != em[0] && k != em[1] ) bx0[k] = bx1[k] = 1;
+         // diagonal neighbour of edge point x: b[em0] = w-2-x, b[em1] = x-1
+         bx0[em[0]] = w - 2;
+         bx0[em[1]] = -1;
+         bx1[em[0]] = w - 3;
+         bx1[em[1]] = 0;
+         EdgeTerm t{};
+         t.kind = 1;
+         t.prim = c;
+         for ( int d = 0; d < 3; d++ )
+         {
+            t.base[d] = bx0[d + 1];
+            t.inc[d]  = bx1[d + 1] - bx0[d + 1];
+         }
+         const int lo = std::min( em[0], em[1] ), hi = std::max( em[0], em[1] );
+         t.wr         = ( ( lo == 0 && hi == 2 ) || ( lo == 1 && hi == 3 ) ) ? 0.5f : 0.0f;
+         P.hEdgeTerms.push_back( t );
+      }
+      g.nTerms = (int) P.hEdgeTerms.size() - g.termOff;
+      P.hEdgeGeom.push_back( g );
+   }
+   if ( dev ) P.dEdgeGeom.upload( P.hEdgeGeom, s );
+   if ( dev ) P.dEdgeTerms.upload( P.hEdgeTerms, s );
+
+   P.hVertexGeom.clear();
+   P.hVertexTerms.clear();
+   for ( int v = 0; v < T.nV(); v++ )
+   {
+      VertexGeom g{};
+      g.vertex  = v;
+      g.type    = T.vType[v];
+      g.termOff = (int) P.hVertexTerms.size();
+      for ( int e : T.vEdges[v] )
+         P.hVertexTerms.push_back( P.edgeOff( e ) + ( T.ev[e][0] == v ? 1 : w - 2 ) );
+      g.nTerms = (int) P.hVertexTerms.size() - g.termOff;
+      P.hVertexGeom.push_back( g );
+   }
+   if ( dev ) P.dVertexGeom.upload( P.hVertexGeom, s );
+   if ( dev ) P.dVertexTerms.upload( P.hVertexTerms, s );
+
+   validatePlan( T, P );
+
+   std::vector< int32_t > ct( T.cType.begin(), T.cType.end() );
+   if ( dev ) P.dCellType.upload( ct, s );
+   if ( dev ) HYMI_HIP( hipStreamSynchronize( s ) );
+}
+
+LevelPlan& Mesh::plan( int level )
+{
+   auto it = plans.find( level );
+   if ( it != plans.end() ) return *it->second;
+   auto P = std::make_unique< LevelPlan >();
+   buildPlan( *this, *P, level );
+   auto& ref = *P;
+   plans[level] = std::move( P );
+   return ref;
+}
+
+// Row segments of all macro-cell interiors: rows (c, z, y), x = 1 .. w-2-y-z, cut into runs of `len`.
+// Segments are ordered so that the ones processed by one XCD are contiguous in memory: the launch maps
+// block b to segment group (b % 8) * (nb / 8) + b / 8.
+const CellSeg* Mesh::segments( LevelPlan& P, int len, int64_t* count )
+{
+   auto it = P.segs.find( len );
+   if ( it == P.segs.end() )
+   {
+      std::vector< CellSeg > v;
+      const int              w = P.w;
+      for ( int c = 0; c < T.nC(); c++ )
+         for ( int z = 1; z <= w - 4; z++ )
+            for ( int y = 1; y + z <= w - 3; y++ )
+            {
+               const int n = w - 2 - y - z;
+               for ( int x0 = 1; x0 <= n; x0 += len )
+                  v.push_back( { c, z, y, x0 } );
+            }
+      auto buf = std::make_unique< DevBuf< CellSeg > >();
+      buf->upload( v, ctx->stream );
+      HYMI_HIP( hipStreamSynchronize( ctx->stream ) );
+      P.segCount[len] = (int64_t) v.size();
+      it              = P.segs.emplace( len, std::move( buf ) ).first;
+   }
+   *count = P.segCount[len];
+   return it->second->p;
+}
+
+// Closure ranges (owned part plus copies) of the primitives whose DoF type matches `flag`, cut into chunks.
+const Range* Mesh::closureRanges( LevelPlan& P, int flag, int64_t* count )
+{
+   auto it = P.ranges.find( flag );
+   if ( it == P.ranges.end() )
+   {
+      std::vector< Range > v;
+      auto                 push = [&]( int64_t off, int64_t len ) {
+         const int64_t chunk = 4096;
+         for ( int64_t o = 0; o < len; o += chunk )
+            v.push_back( { off + o, (int32_t) std::min( chunk, len - o ), 0 } );
+      };
+      int64_t runStart = -1, runLen = 0;
+      for ( int vtx = 0; vtx < T.nV(); vtx++ )
+      {
+         if ( T.vType[vtx] & flag )
+         {
+            if ( runStart < 0 ) runStart = vtx;
+            runLen++;
+         }
+         else if ( runStart >= 0 )
+         {
+            push( runStart, runLen );
+            runStart = -1;
+            runLen   = 0;
+         }
+      }
+      if ( runStart >= 0 ) push( runStart, runLen );
+      for ( int e = 0; e < T.nE(); e++ )
+         if ( T.eType[e] & flag ) push( P.edgeOff( e ), P.w );
+      for ( int f = 0; f < T.nF(); f++ )
+         if ( T.fType[f] & flag ) push( P.faceOff( f ), triSize( P.w ) );
+      for ( int c = 0; c < T.nC(); c++ )
+         if ( T.cType[c] & flag ) push( P.cellOff( c ), tetSize( P.w ) );
+      auto buf = std::make_unique< DevBuf< Range > >();
+      buf->upload( v, ctx->stream );
+      HYMI_HIP( hipStreamSynchronize( ctx->stream ) );
+      P.rangeCount[flag] = (int64_t) v.size();
+      it                 = P.ranges.emplace( flag, std::move( buf ) ).first;
+   }
+   *count = P.rangeCount[flag];
+   return it->second->p;
+}
+
+void buildOperatorWeights( Operator& op, int level )
+{
+   Mesh&           M = *op.mesh;
+   const Topology& T = M.T;
+   LevelPlan&      P = M.plan( level );
+   const int       w = P.w;
+   auto&           W = op.weights( level );
+   double          st[NDIR];
+   bool            present[NDIR];
+
+   // cells
+   W.hCell.assign( (size_t) T.nC() * kCellW, 0.0 );
+   if ( level >= 2 )
+      for ( int c = 0; c < T.nC(); c++ )
+      {
+         const int p[3] = { 1, 1, 1 };
+         pointStencil( T, c, op.form, w, p, st, present );
+         for ( int d = 0; d < NDIR; d++ )
+            W.hCell[(size_t) c * kCellW + d] = st[d];
+      }
+
+   // faces: in-plane weights summed over sides, off-plane weights per side in FaceGeom order
+   W.hFace.assign( (size_t) T.nF() * kFaceW, 0.0 );
+   if ( level >= 2 )
+      for ( int f = 0; f < T.nF(); f++ )
+      {
+         const FaceGeom& g  = P.hFaceGeom[f];
+         double*         fw = &W.hFace[(size_t) f * kFaceW];
+         for ( int n = 0; n < g.nSides; n++ )
+         {
+            const int  c   = T.fCells[f][n];
+            const int  lf  = T.localFace( c, f );
+            const int* fm  = &T.cfmap[c][3 * lf];
+            const int  opp = oppositeOf( fm );
+            int        p[3];
+            FaceToCellMap( faceToCellMap( T, c, lf ) ).eval( w, 1, 1, 0, p );
+            pointStencil( T, c, op.form, w, p, st, present );
+            for ( int d = 0; d < NDIR; d++ )
+            {
+               if ( !present[d] ) continue;
+               int db[4];
+               dirBary( d, db );
+               const int dc = db[fm[1]], dr = db[fm[2]];
+               if ( db[opp] == 0 )
+               {
+                  static const int IN[7][2] = { { 0, 0 }, { -1, 0 }, { 1, 0 }, { 0, 1 }, { 0, -1 }, { -1, 1 }, { 1, -1 } };
+                  int              k        = 0;
+                  while ( k < 7 && !( IN[k][0] == dc && IN[k][1] == dr ) )
+                     k++;
+                  if ( k == 7 ) throw std::runtime_error( "in-plane face direction not found" );
+                  fw[k] += st[d];
+               }
+               else
+               {
+                  int k = 0;
+                  while ( k < 4 && !( g.offCol[n][k] == dc && g.offRow[n][k] == dr ) )
+                     k++;
+                  if ( k == 4 ) throw std::runtime_error( "off-plane face direction not found" );
+                  fw[7 + 4 * n + k] += st[d];
+               }
+            }
+         }
+      }
+
+   // edges: W, C, E, then one weight per EdgeTerm
+   std::vector< int > eOff( T.nE() + 1, 0 );
+   for ( int e = 0; e < T.nE(); e++ )
+      eOff[e + 1] = eOff[e] + 3 + P.hEdgeGeom[e].nTerms;
+   W.hEdge.assign( (size_t) eOff[T.nE()], 0.0 );
+   if ( level >= 1 )
+      for ( int e = 0; e < T.nE(); e++ )
+      {
+         double*         ew = &W.hEdge[eOff[e]];
+         const EdgeGeom& g  = P.hEdgeGeom[e];
+         const int       nF = (int) T.eFaces[e].size();
+         for ( int ci = 0; ci < (int) T.eCells[e].size(); ci++ )
+         {
+            const int  c  = T.eCells[e][ci];
+            const int  le = T.localEdge( c, e );
+            const int* em = &T.cemap[c][2 * le];
+            int        b[4] = { 0, 0, 0, 0 };
+            b[em[0]]        = w - 2;
+            b[em[1]]        = 1;
+            const int p[3]  = { b[1], b[2], b[3] };
+            pointStencil( T, c, op.form, w, p, st, present );
+            int others[2], no = 0;
+            for ( int k = 0; k < 4; k++ )
+               if ( k != em[0] && k != em[1] ) others[no++] = k;
+            for ( int d = 0; d < NDIR; d++ )
+            {
+               if ( !present[d] ) continue;
+               int db[4];
+               dirBary( d, db );
+               int q[4];
+               for ( int k = 0; k < 4; k++ )
+                  q[k] = b[k] + db[k];
+               const int xo = q[em[1]] - 1;
+               if ( q[others[0]] == 0 && q[others[1]] == 0 )
+               {
+                  ew[xo == 0 ? 1 : ( xo == 1 ? 2 : 0 )] += st[d];
+               }
+               else if ( q[others[0]] == 0 || q[others[1]] == 0 )
+               {
+                  const int third = q[others[0]] != 0 ? others[0] : others[1];
+                  const int face  = T.findFace( T.cv[c][em[0]], T.cv[c][em[1]], T.cv[c][third] );
+                  const int fi = int( std::find( T.eFaces[e].begin(), T.eFaces[e].end(), face ) - T.eFaces[e].begin() );
+                  // terms 2*fi (W, edge point x-1) and 2*fi+1 (E, edge point x)
+                  ew[3 + 2 * fi + ( xo == 0 ? 1 : 0 )] += st[d];
+               }
+               else
+               {
+                  ew[3 + 2 * nF + ci] += st[d];
+               }
+            }
+         }
+         (void) g;
+      }
+
+   // vertices: centre then one weight per incident edge
+   std::vector< int > vOff( T.nV() + 1, 0 );
+   for ( int v = 0; v < T.nV(); v++ )
+      vOff[v + 1] = vOff[v] + 1 + P.hVertexGeom[v].nTerms;
+   W.hVertex.assign( (size_t) vOff[T.nV()], 0.0 );
+   for ( int v = 0; v < T.nV(); v++ )
+   {
+      double* vw = &W.hVertex[vOff[v]];
+      for ( int c : T.vCells[v] )
+      {
+         int lv = 0;
+         while ( T.cv[c][lv] != v )
+            lv++;
+         int b[4] = { 0, 0, 0, 0 };
+         b[lv]    = w - 1;
+         const int p[3] = { b[1], b[2], b[3] };
+         pointStencil( T, c, op.form, w, p, st, present );
+         for ( int d = 0; d < NDIR; d++ )
+         {
+            if ( !present[d] ) continue;
+            if ( d == C )
+            {
+               vw[0] += st[d];
+               continue;
+            }
+            int db[4];
+            dirBary( d, db );
+            int other = -1;
+            for ( int k = 0; k < 4; k++ )
+               if ( k != lv && b[k] + db[k] != 0 ) other = k;
+            const int e  = T.findEdge( v, T.cv[c][other] );
+            const int ei = int( std::find( T.vEdges[v].begin(), T.vEdges[v].end(), e ) - T.vEdges[v].begin() );
+            vw[1 + ei] += st[d];
+         }
+      }
+   }
+
+   if ( M.ctx->hostOnly ) return;
+   hipStream_t s = M.ctx->stream;
+   auto        up = [&]( const std::vector< double >& h, DevBuf< double >& d64, DevBuf< float >& d32 ) {
+      d64.upload( h, s );
+      std::vector< float > f( h.begin(), h.end() );
+      d32.upload( f, s );
+      HYMI_HIP( hipStreamSynchronize( s ) );
+   };
+   up( W.hCell, W.cell64, W.cell32 );
+   up( W.hFace, W.face64, W.face32 );
+   up( W.hEdge, W.edge64, W.edge32 );
+   up( W.hVertex, W.vertex64, W.vertex32 );
+}
+
+} // namespace hymi

@@ -1,0 +1,151 @@
+/*
+ * hymi.h -- C ABI of the MI355X-native matrix-free P1 geometric multigrid path (libhymi.so).
+ *
+ * Plain C types only: opaque handles, pointers and sizes.  Every entry point returns an int status
+ * (HYMI_OK == 0); on failure hymi_last_error_string() describes the error.  No entry point aborts the
+ * process.  Work is enqueued on the context's HIP stream; entry points that return host-visible results
+ * (downloads, dot products) synchronise that stream.  Calls on one handle are not thread-safe; different
+ * contexts may be used concurrently.
+ *
+ * Each entry point names the reference interface it replaces (paths relative to the HyTeG source root).
+ */
+#ifndef HYMI_H
+#define HYMI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define HYMI_OK 0
+#define HYMI_ERR_INVALID 1      /* bad handle or argument */
+#define HYMI_ERR_LEVEL 2        /* level not allocated for this function / operator */
+#define HYMI_ERR_OOM 3          /* device allocation failed or HBM budget exceeded */
+#define HYMI_ERR_HIP 4          /* HIP runtime error */
+#define HYMI_ERR_UNSUPPORTED 5  /* precision / nrhs combination not built */
+#define HYMI_ERR_INTERNAL 6
+
+/* DoFType bits (src/hyteg/types/types.hpp:35-44) */
+#define HYMI_INNER 1
+#define HYMI_DIRICHLET 2
+#define HYMI_NEUMANN 4
+#define HYMI_FREESLIP 8
+#define HYMI_ALL 15
+
+/* UpdateType (src/hyteg/types/types.hpp:29-32) */
+#define HYMI_REPLACE 0
+#define HYMI_ADD 1
+
+/* storage precision of a function */
+#define HYMI_F64 0
+#define HYMI_F32 1
+
+/* primitive classes for per-primitive upload/download */
+#define HYMI_VERTEX 0
+#define HYMI_EDGE 1
+#define HYMI_FACE 2
+#define HYMI_CELL 3
+
+typedef struct hymi_ctx_s*      hymi_ctx;
+typedef struct hymi_mesh_s*     hymi_mesh;
+typedef struct hymi_function_s* hymi_function;
+typedef struct hymi_operator_s* hymi_operator;
+typedef struct hymi_gmg_s*      hymi_gmg;
+
+const char* hymi_last_error_string( void );
+int         hymi_version( void );
+
+/* ---- context: device + stream ------------------------------------------------------------------ */
+int hymi_ctx_create( int device, hymi_ctx* out );
+int hymi_ctx_destroy( hymi_ctx ctx );
+int hymi_ctx_synchronize( hymi_ctx ctx );
+/* bytes of device memory held by meshes, functions and operators of this context */
+int hymi_ctx_bytes_in_use( hymi_ctx ctx, int64_t* out );
+
+/* ---- mesh + primitive storage -------------------------------------------------------------------
+ * Replaces MeshInfo::meshCuboid (src/hyteg/mesh/MeshGenCuboid.cpp:32-171) / MeshInfo::fromGmshFile
+ * (src/hyteg/mesh/MeshInfo.hpp:421) + SetupPrimitiveStorage (src/hyteg/primitivestorage/
+ * SetupPrimitiveStorage.cpp:36-420) + setMeshBoundaryFlagsOnBoundary(1, 0, true) (:735-746) with
+ * BoundaryCondition::create0123BC (src/hyteg/boundary/BoundaryConditions.cpp:26-33). */
+int hymi_mesh_create_cuboid( hymi_ctx ctx, const double lo[3], const double hi[3], int nx, int ny, int nz,
+                             hymi_mesh* out );
+int hymi_mesh_create( hymi_ctx ctx, int nv, const double* coords, int nc, const int* cell_vertices, hymi_mesh* out );
+int hymi_mesh_from_gmsh( hymi_ctx ctx, const char* path, hymi_mesh* out );
+int hymi_mesh_destroy( hymi_mesh mesh );
+/* numbers of macro vertices, edges, faces, cells */
+int hymi_mesh_counts( hymi_mesh mesh, int out[4] );
+/* DoF type of a macro primitive (PrimitiveStorage boundary flag -> DoFType) */
+int hymi_mesh_primitive_type( hymi_mesh mesh, int cls, int idx, int* out );
+/* owned-DoF numbering of one level: vertices, then edge, face and cell interiors in primitive order
+ * (numberOfGlobalDoFs, src/hyteg/p1functionspace/VertexDoFFunction.cpp:1410-1498) */
+int hymi_mesh_owned_count( hymi_mesh mesh, int level, int64_t* out );
+int hymi_mesh_owned_coords( hymi_mesh mesh, int level, double* xyz );
+int hymi_mesh_owned_types( hymi_mesh mesh, int level, int* types );
+
+/* ---- P1Function -----------------------------------------------------------------------------------
+ * Replaces P1Function<real_t>(name, storage, minLevel, maxLevel) (src/hyteg/p1functionspace/
+ * VertexDoFFunction.hpp:66-77, memory VertexDoFMemory.hpp:44-69); nrhs > 1 stores nrhs right-hand sides
+ * interleaved per DoF (supported: 1, 32). */
+int hymi_function_create( hymi_mesh mesh, int min_level, int max_level, int precision, int nrhs, hymi_function* out );
+int hymi_function_destroy( hymi_function f );
+/* host <-> device in owned order; values are double on the host, nrhs interleaved */
+int hymi_function_upload_owned( hymi_function f, int level, const double* host );
+int hymi_function_download_owned( hymi_function f, int level, double* host );
+/* per-primitive transfer in the reference's FunctionMemory layout (vertex [1+nE], edge
+ * [w + nF(w-1) + nC(w-2)], face [face + nC ghost layers], cell [tetrahedral]); ghost slots are filled
+ * from the neighbours on download and ignored on upload.  nrhs must be 1. */
+int hymi_function_primitive_size( hymi_function f, int level, int cls, int idx, int64_t* out );
+int hymi_function_download_primitive( hymi_function f, int level, int cls, int idx, double* host );
+int hymi_function_upload_primitive( hymi_function f, int level, int cls, int idx, const double* host );
+/* interpolate(constant, level, flag)  (VertexDoFFunction.cpp:238-300) */
+int hymi_interpolate_const( hymi_function f, int level, int flag, double value );
+/* assign / add linear combinations (VertexDoFFunction.cpp:979-1074); n <= 3 */
+int hymi_assign( hymi_function dst, int level, int flag, int n, const double* coeffs, const hymi_function* srcs );
+int hymi_add( hymi_function dst, int level, int flag, int n, const double* coeffs, const hymi_function* srcs );
+/* dotGlobal (VertexDoFFunction.cpp:1410-1498); writes nrhs values */
+int hymi_dot( hymi_function a, hymi_function b, int level, int flag, double* out );
+/* copy between functions of different precision on one level (mixed-precision refinement) */
+int hymi_convert( hymi_function dst, hymi_function src, int level, int flag );
+
+/* ---- P1ConstantLaplaceOperator / P1ConstantMassOperator ------------------------------------------
+ * Replaces P1ConstantOperator<P1FenicsForm<p1_diffusion/mass...>> (src/hyteg/p1functionspace/
+ * P1ConstantOperator.hpp:165-166,188-189; assembly P1ConstantOperator.cpp:543-595). */
+int hymi_laplace_create( hymi_mesh mesh, int min_level, int max_level, hymi_operator* out );
+int hymi_mass_create( hymi_mesh mesh, int min_level, int max_level, hymi_operator* out );
+int hymi_operator_destroy( hymi_operator op );
+/* stencil weights for checks: cell 15 (C W E N S NW SE TC TW TS TSE BC BN BE BNW), face 15 (in-plane
+ * C W E N S NW SE, then 4 off-plane per side), edge 3 + terms, vertex 1 + incident edges */
+int hymi_operator_stencil( hymi_operator op, int level, int cls, int idx, double* out, int* count );
+/* P1Operator::apply (src/hyteg/p1functionspace/P1Operator.hpp:197-325) */
+int hymi_apply( hymi_operator op, hymi_function src, hymi_function dst, int level, int flag, int update );
+/* r = b - A x on flag DoFs (GeometricMultigridSolver.hpp:185-186 fused) */
+int hymi_residual( hymi_operator op, hymi_function x, hymi_function b, hymi_function r, int level, int flag );
+/* P1Operator::smooth_sor (P1Operator.hpp:341-420), multicolour order inside each primitive */
+int hymi_smooth_sor( hymi_operator op, hymi_function x, hymi_function b, double omega, int level, int flag );
+
+/* ---- grid transfer ---------------------------------------------------------------------------------
+ * P1toP1LinearRestriction::restrict (src/hyteg/gridtransferoperators/P1toP1LinearRestriction.cpp:163-245)
+ * P1toP1LinearProlongation::prolongateAndAdd (P1toP1LinearProlongation.cpp:189-397, UpdateType Add) */
+int hymi_restrict( hymi_function f, int src_level, int flag );
+int hymi_prolongate_add( hymi_function f, int src_level, int flag );
+
+/* ---- solvers ---------------------------------------------------------------------------------------
+ * CGSolver::solve (src/hyteg/solvers/CGSolver.hpp:69-150) run entirely on the device: max_iter iterations
+ * with device-side scalars, updates switched off once ||r|| < tol (the reference's early exit). */
+int hymi_cg_solve( hymi_operator op, hymi_function x, hymi_function b, int level, int max_iter, double tol );
+/* GeometricMultigridSolver (src/hyteg/solvers/GeometricMultigridSolver.hpp:42-235) with SORSmoother and a
+ * CG coarse solver; cg_max_iter <= 0 means "number of coarse unknowns". */
+int hymi_gmg_create( hymi_operator op, hymi_function x_template, int min_level, int max_level, int nu1, int nu2,
+                     double omega, int cg_max_iter, double cg_tol, hymi_gmg* out );
+int hymi_gmg_destroy( hymi_gmg g );
+/* one V-cycle: solveRecursively(A, x, b, level) */
+int hymi_gmg_vcycle( hymi_gmg g, hymi_function x, hymi_function b, int level );
+/* capture one V-cycle for (x, b, level) into a hipGraph and replay it `count` times */
+int hymi_gmg_vcycle_graph( hymi_gmg g, hymi_function x, hymi_function b, int level, int count );
+
+#ifdef __cplusplus
+}
+#endif
+#endif
