@@ -70,6 +70,7 @@ SIGNATURES = {
     "hymi_gmg_destroy": [_vp],
     "hymi_gmg_vcycle": [_vp, _vp, _vp, _i],
     "hymi_gmg_vcycle_graph": [_vp, _vp, _vp, _i, _i],
+    "hymi_time_kernel": [_vp, _vp, _vp, _vp, _vp, _i, _i, _i, _P(_d)],
 }
 
 _lib = None

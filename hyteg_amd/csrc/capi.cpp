@@ -1050,4 +1050,44 @@ int hymi_gmg_vcycle_graph( hymi_gmg g, hymi_function x, hymi_function b, int lev
    } );
 }
 
+int hymi_time_kernel( hymi_operator op, hymi_gmg gmg, hymi_function x, hymi_function b, hymi_function y, int level,
+                      int which, int reps, double* ms )
+{
+   return guarded( [&] {
+      require( op && x && ms && reps > 0, "bad argument" );
+      checkOpFunc( op, x, level );
+      if ( which == 1 || which == 3 || which == 4 ) checkOpFunc( op, b, level );
+      if ( which == 0 || which == 2 ) checkOpFunc( op, y, level );
+      if ( which == 4 ) checkGmg( gmg, x, b, level );
+      Operator&   A = op->op;
+      hipStream_t s = A.mesh->ctx->stream;
+      const int   flag = kInner | kNeumann;
+      auto        once = [&]() {
+         switch ( which )
+         {
+         case 0: launchCellApplyOnly( A, x->f, y->f, level, flag ); break;
+         case 1: launchCellSorOnly( A, x->f, b->f, 1.0, level, flag, 0 ); break;
+         case 2: launchApply( A, x->f, y->f, nullptr, level, flag, kReplace ); break;
+         case 3: launchSor( A, x->f, b->f, 1.0, level, flag ); break;
+         case 4: gmg->recurse( x->f, b->f, level ); break;
+         default: throw std::invalid_argument( "bad kernel selector" );
+         }
+      };
+      once(); // warm-up (also builds any lazily created table)
+      hipEvent_t e0, e1;
+      HYMI_HIP( hipEventCreate( &e0 ) );
+      HYMI_HIP( hipEventCreate( &e1 ) );
+      HYMI_HIP( hipEventRecord( e0, s ) );
+      for ( int k = 0; k < reps; k++ )
+         once();
+      HYMI_HIP( hipEventRecord( e1, s ) );
+      HYMI_HIP( hipEventSynchronize( e1 ) );
+      float t = 0.f;
+      HYMI_HIP( hipEventElapsedTime( &t, e0, e1 ) );
+      (void) hipEventDestroy( e0 );
+      (void) hipEventDestroy( e1 );
+      *ms = (double) t / reps;
+   } );
+}
+
 } // extern "C"

@@ -1214,6 +1214,51 @@ void launchConvert( Function& dst, int dstLevel, Function& src, int srcLevel, in
       throw std::runtime_error( "unsupported nrhs" );
 }
 
+namespace {
+template < typename T, int NR >
+struct CellApplyOnlyF
+{
+   static void run( Operator& op, Function& src, Function& dst, int level, int flag )
+   {
+      Mesh&          M    = *op.mesh;
+      LevelPlan&     P    = M.plan( level );
+      int64_t        nseg = 0;
+      const CellSeg* sg   = M.segments( P, segLen< NR >(), &nseg );
+      auto&          Wt   = op.weights( level );
+      if ( nseg == 0 ) return;
+      hipLaunchKernelGGL( ( kCellApply< T, NR, kReplace > ), dim3( (unsigned) xcdGrid( ( nseg + 3 ) / 4 ) ), dim3( 256 ), 0,
+                          M.ctx->stream, viewOf( P ), sg, nseg, wsel< T >( Wt.cell64, Wt.cell32 ), P.dCellType.p, flag,
+                          (const T*) src.ptr( level ), (T*) dst.ptr( level ), (const T*) nullptr );
+   }
+};
+template < typename T, int NR >
+struct CellSorOnlyF
+{
+   static void run( Operator& op, Function& x, const Function& b, double omega, int level, int flag, int colour )
+   {
+      Mesh&          M    = *op.mesh;
+      LevelPlan&     P    = M.plan( level );
+      int64_t        nseg = 0;
+      const CellSeg* sg   = M.segments( P, segLen< NR >(), &nseg );
+      auto&          Wt   = op.weights( level );
+      if ( nseg == 0 ) return;
+      hipLaunchKernelGGL( ( kCellSor< T, NR > ), dim3( (unsigned) xcdGrid( ( nseg + 3 ) / 4 ) ), dim3( 256 ), 0,
+                          M.ctx->stream, viewOf( P ), sg, nseg, wsel< T >( Wt.cell64, Wt.cell32 ), P.dCellType.p, flag,
+                          (T*) x.ptr( level ), (const T*) b.ptr( level ), (T) omega, colour );
+   }
+};
+} // namespace
+
+void launchCellApplyOnly( Operator& op, Function& src, Function& dst, int level, int flag )
+{
+   dispatch< CellApplyOnlyF >( src.prec, src.nrhs, op, src, dst, level, flag );
+}
+
+void launchCellSorOnly( Operator& op, Function& x, const Function& b, double omega, int level, int flag, int colour )
+{
+   dispatch< CellSorOnlyF >( x.prec, x.nrhs, op, x, b, omega, level, flag, colour );
+}
+
 void launchAddFrom( Function& x, Function& e, int level, int flag )
 {
    if ( x.nrhs != e.nrhs || x.mesh != e.mesh ) throw std::runtime_error( "add: functions must share mesh and nrhs" );

@@ -37,5 +37,8 @@ void launchScalarCopy( Context& ctx, int nrhs, int dst, int src );
 void launchConvert( Function& dst, int dstLevel, Function& src, int srcLevel, int flag );
 // x (FP64) += e (FP32 or FP64)
 void launchAddFrom( Function& x, Function& e, int level, int flag );
+// single macro-cell kernels (no refresh, no lower-dimensional primitives) for measurement
+void launchCellApplyOnly( Operator& op, Function& src, Function& dst, int level, int flag );
+void launchCellSorOnly( Operator& op, Function& x, const Function& b, double omega, int level, int flag, int colour );
 
 } // namespace hymi

@@ -145,6 +145,14 @@ int hymi_gmg_vcycle( hymi_gmg g, hymi_function x, hymi_function b, int level );
 /* capture one V-cycle for (x, b, level) into a hipGraph and replay it `count` times */
 int hymi_gmg_vcycle_graph( hymi_gmg g, hymi_function x, hymi_function b, int level, int count );
 
+/* ---- measurement -----------------------------------------------------------------------------------
+ * No reference counterpart (the reference times with its WcTimingTree, src/hyteg/operators/Operator.hpp:
+ * 99-115).  Average duration in ms of `reps` back-to-back launches on the context stream, bracketed by
+ * hipEvents recorded on that stream.  which: 0 macro-cell apply kernel (x -> y), 1 macro-cell SOR colour
+ * pass (x, b; colour 0), 2 full apply (x -> y), 3 full SOR sweep (x, b), 4 one V-cycle of gmg (x, b). */
+int hymi_time_kernel( hymi_operator op, hymi_gmg gmg, hymi_function x, hymi_function b, hymi_function y, int level,
+                      int which, int reps, double* ms_per_launch );
+
 #ifdef __cplusplus
 }
 #endif

@@ -82,6 +82,8 @@ def lib():
             "om_gmg_create": (vp, [vp, i, i, i, i, d, i, i, d]),
             "om_gmg_free": (None, [vp]),
             "om_gmg_solve": (None, [vp, vp, vp, vp, i]),
+            "om_cell_index": (l, [l, l, l, l]),
+            "om_face_index": (l, [l, l, l]),
             "om_wtime": (d, []),
             "om_num_threads": (i, []),
         }
@@ -298,6 +300,14 @@ class GMG:
 
     def solve(self, A, x, b, level):
         lib().om_gmg_solve(self.h, A.h, x.h, b.h, level)
+
+
+def cell_index(w, x, y, z):
+    return lib().om_cell_index(w, x, y, z)
+
+
+def face_index(w, x, y):
+    return lib().om_face_index(w, x, y)
 
 
 def wtime():

@@ -87,6 +87,8 @@ static inline long cidx(long w, long x, long y, long z)
 }
 static inline long fidx(long w, long x, long y) { return y * (w + 1) - y * (y + 1) / 2 + x; }
 static inline int width_of(int level) { return (1 << level) + 1; }
+long om_cell_index(long w, long x, long y, long z) { return cidx(w, x, y, z); }
+long om_face_index(long w, long x, long y) { return fidx(w, x, y); }
 
 /* ------------------------------------------------------------------ */
 struct om_mesh

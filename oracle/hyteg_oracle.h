@@ -46,6 +46,10 @@ void om_face_cells(const om_mesh* m, int f, int out[2]);
 void om_cell_faces(const om_mesh* m, int c, int out[4]);
 void om_cell_face_map(const om_mesh* m, int c, int out[12]);
 
+/* linear layouts: tetrahedral cell index and triangular face index for width w */
+long om_cell_index(long w, long x, long y, long z);
+long om_face_index(long w, long x, long y);
+
 /* ---- owned-DoF numbering (vertices, edge/face/cell interiors in primitive order) ---- */
 long om_owned_count(const om_mesh* m, int level);
 void om_owned_coords(const om_mesh* m, int level, double* xyz);

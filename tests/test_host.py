@@ -1,0 +1,120 @@
+"""CPU tests of the product's host side: the C-ABI library loads and exports every declared entry point,
+topology / owned numbering / stencil weights agree with the oracle (host-only context, no GPU needed),
+and errors come back as status codes."""
+import ctypes
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import mesh_pair
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_symbols():
+    text = open(os.path.join(ROOT, "include", "hymi.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(hymi_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    import hyteg_amd._lib as L
+    lib = L.load()
+    names = _declared_symbols()
+    assert len(names) >= 40
+    for n in names:
+        assert hasattr(lib, n), n
+    assert lib.hymi_version() == 1
+
+
+def test_every_declared_symbol_is_bound_in_python():
+    import hyteg_amd._lib as L
+    bound = set(L.SIGNATURES) | {"hymi_last_error_string"}
+    assert set(_declared_symbols()) <= bound
+
+
+@pytest.fixture(scope="module")
+def H():
+    import hyteg_amd
+    return hyteg_amd
+
+
+@pytest.fixture(scope="module")
+def hctx(H):
+    return H.Context(-1)
+
+
+MESHES = [("cube", 1), ("cube", 2), ("cube", 3), ("gmsh", "regular_octahedron_8el.msh"), ("gmsh", "cube_48el.msh"),
+          ("gmsh", "pyramid_2el.msh"), ("gmsh", "tet_1el.msh")]
+
+
+@pytest.mark.parametrize("kind", MESHES)
+def test_topology_and_owned_numbering_match_oracle(H, hctx, oracle_lib, kind):
+    om, S = mesh_pair(kind, hctx)
+    assert S.counts() == om.counts()
+    nV, nE, nF, nC = om.counts()
+    for cls, n in ((0, nV), (1, nE), (2, nF), (3, nC)):
+        for i in range(n):
+            assert S.primitive_type(cls, i) == om.prim_type(cls, i)
+    for L in range(0, 5):
+        assert S.owned_count(L) == om.owned_count(L)
+        assert np.array_equal(S.owned_types(L), om.owned_types(L))
+        assert np.abs(S.owned_coords(L) - om.owned_coords(L)).max() <= 1e-15
+
+
+@pytest.mark.parametrize("kind", MESHES)
+@pytest.mark.parametrize("form", [0, 1])
+def test_stencil_weights_match_oracle(H, hctx, oracle_lib, kind, form):
+    O = oracle_lib
+    om, S = mesh_pair(kind, hctx)
+    A = (H.P1ConstantLaplaceOperator if form == 0 else H.P1ConstantMassOperator)(S, 0, 4)
+    B = O.Operator(om, 0, 4, form)
+    nV, nE, nF, nC = om.counts()
+    for L in range(1, 5):
+        def close(a, b):
+            return a.shape == b.shape and np.abs(a - b).max() <= 1e-14 * np.abs(b).max()
+
+        for v in range(nV):
+            assert close(A.stencil(L, H.VERTEX, v), B.vertex_stencil(L, v))
+        for e in range(nE):
+            b = B.edge_stencil(L, e)  # reference layout [f0 W, f0 E, W, C, E, f1 W, f1 E, ..., cells]
+            assert close(A.stencil(L, H.EDGE, e), np.concatenate([b[2:5], b[0:2], b[5:]]))
+        if L < 2:
+            continue
+        for c in range(nC):
+            assert close(A.stencil(L, H.CELL, c), B.cell_stencil(L, c))
+        for f in range(nF):
+            a = A.stencil(L, H.FACE, f)
+            sides = [B.face_stencil(L, f, n) for n in range(om.face_cells(f).tolist().count(-1) == 0 and 2 or 1)]
+            ref = sum(sides)
+            scale = abs(ref[0])
+            assert abs(a[0] - ref[0]) <= 1e-14 * scale
+            assert abs(a.sum() - ref.sum()) <= 1e-14 * scale
+
+
+def test_errors_are_status_codes(H, hctx):
+    import hyteg_amd._lib as L
+    lib = L.load()
+    out = ctypes.c_void_p()
+    lo = (ctypes.c_double * 3)(0, 0, 0)
+    hi = (ctypes.c_double * 3)(1, 1, 1)
+    assert lib.hymi_mesh_create_cuboid(hctx.h, lo, hi, 0, 1, 1, ctypes.byref(out)) == L.ERR_INTERNAL
+    assert b"sub-cube" in lib.hymi_last_error_string()
+    assert lib.hymi_mesh_counts(None, None) == L.ERR_INVALID
+    assert lib.hymi_mesh_from_gmsh(hctx.h, b"/nonexistent.msh", ctypes.byref(out)) != L.OK
+    S = H.PrimitiveStorage(H.MeshInfo.meshCuboid((0, 0, 0), (1, 1, 1), 1, 1, 1), hctx)
+    f = ctypes.c_void_p()
+    # functions need a device: a host-only context refuses with a status code
+    assert lib.hymi_function_create(S.h, 0, 2, 0, 1, ctypes.byref(f)) == L.ERR_INVALID
+    with pytest.raises(H.HymiError):
+        H.P1ConstantLaplaceOperator(S, 3, 1)
+
+
+def test_bench_refuses_multi_gpu():
+    res = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True)
+    assert res.returncode != 0
+    assert "one GPU" in res.stderr

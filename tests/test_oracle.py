@@ -1,0 +1,356 @@
+"""Pinning the CPU oracle (runs on the CPU).
+
+Every oracle function is checked against what fixes it independently of its own code: the reference
+tests' known-answer values, a brute-force global assembly from micro-tetrahedra, closed-form facts
+(exactness on linears, transpose relations, tiling volumes), and a brute-force Gauss-Seidel sweep.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import bruteforce as BF
+from helpers import mesh_pair, read_gmsh, sine_exact, sine_rhs, MESHES
+
+import os
+
+
+@pytest.fixture(scope="module")
+def O(oracle_lib):
+    return oracle_lib
+
+
+# --- indexing known-answer values (tests/hyteg/Indexing/VertexDoFMacroCellIndexingTest.cpp:62-127) ---
+KAT_L2 = {(0, 0, 0): 20, (1, 0, 0): 21, (-1, 0, 0): 19, (0, 1, 0): 23, (0, -1, 0): 16, (-1, 1, 0): 22, (1, -1, 0): 17,
+          (0, 0, 1): 29, (-1, 0, 1): 28, (0, -1, 1): 26, (1, -1, 1): 27, (0, 0, -1): 6, (1, 0, -1): 7, (0, 1, -1): 10,
+          (-1, 1, -1): 9}
+KAT_L3_FIRST = {(0, 0, 0): 54, (1, 0, 0): 55, (-1, 0, 0): 53, (0, 1, 0): 61, (0, -1, 0): 46, (-1, 1, 0): 60,
+                (1, -1, 0): 47, (0, 0, 1): 89, (-1, 0, 1): 88, (0, -1, 1): 82, (1, -1, 1): 83, (0, 0, -1): 10,
+                (1, 0, -1): 11, (0, 1, -1): 18, (-1, 1, -1): 17}
+KAT_L3_TOP = {(0, 0, 0): 150, (1, 0, 0): 151, (-1, 0, 0): 149, (0, 1, 0): 153, (0, -1, 0): 146, (-1, 1, 0): 152,
+              (1, -1, 0): 147, (0, 0, 1): 159, (-1, 0, 1): 158, (0, -1, 1): 156, (1, -1, 1): 157, (0, 0, -1): 136,
+              (1, 0, -1): 137, (0, 1, -1): 140, (-1, 1, -1): 139}
+
+
+@pytest.mark.parametrize("level,centre,kat", [(2, (1, 1, 1), KAT_L2), (3, (1, 1, 1), KAT_L3_FIRST),
+                                              (3, (1, 1, 5), KAT_L3_TOP)])
+def test_cell_index_known_answers(O, level, centre, kat):
+    w = 2 ** level + 1
+    for d, idx in kat.items():
+        assert O.cell_index(w, centre[0] + d[0], centre[1] + d[1], centre[2] + d[2]) == idx
+
+
+def test_level_counts_known_answers(O):
+    """tests/hyteg/LevelinfoTest.cpp / VertexDoFMacroCellIndexingTest.cpp:116-127 micro-vertex counts."""
+    vertices_per_edge = [2, 3, 5, 9, 17]
+    vertices_per_face = [3, 6, 15, 45, 153]
+    m, _ = mesh_pair(("gmsh", "tet_1el.msh"), host_ctx=None) if False else (None, None)
+    X, C = read_gmsh(os.path.join(MESHES, "tet_1el.msh"))
+    tet = O.Mesh.from_arrays(X, C)
+    for L in range(5):
+        w = vertices_per_edge[L]
+        assert w == 2 ** L + 1
+        assert O.face_index(w, 0, w - 1) + 1 == vertices_per_face[L]
+        # single macro-tet: every micro-vertex is owned exactly once
+        assert tet.owned_count(L) == w * (w + 1) * (w + 2) // 6
+
+
+@pytest.mark.parametrize("n,counts", [(1, (8, 19, 18, 6)), (2, (27, 98, 120, 48))])
+def test_cuboid_topology(O, n, counts):
+    assert O.Mesh.cuboid(n).counts() == counts
+
+
+@pytest.mark.parametrize("level", [1, 2, 3])
+def test_micro_tetrahedra_tile_the_macro_cell(level):
+    """The 24 micro-tetrahedra around a vertex, filtered to one macro-cell, tile it: 8^L tets of equal
+    volume, every interior micro-face shared by exactly two tets, boundary micro-faces on the macro faces."""
+    tets = BF.micro_tets(level)
+    assert len(tets) == 8 ** level
+    w = 2 ** level + 1
+    V = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], float)
+    vols = [BF.tet_volume(V[0] + (t[:, 0:1] * (V[1] - V[0]) + t[:, 1:2] * (V[2] - V[0]) + t[:, 2:3] * (V[3] - V[0]))
+                          / (w - 1)) for t in tets]
+    assert np.allclose(vols, (1 / 6) / 8 ** level, rtol=1e-12)
+    faces = {}
+    for t in tets:
+        for skip in range(4):
+            f = tuple(sorted(tuple(p) for k, p in enumerate(t) if k != skip))
+            faces[f] = faces.get(f, 0) + 1
+    for f, cnt in faces.items():
+        P = np.array(f)
+        on_boundary = any((P[:, k] == 0).all() for k in range(3)) or (P.sum(axis=1) == w - 1).all()
+        assert cnt == (1 if on_boundary else 2)
+
+
+MESHES_ALL = [("cube", 1), ("cube", 2), ("gmsh", "regular_octahedron_8el.msh"), ("gmsh", "pyramid_2el.msh"),
+              ("gmsh", "cube_48el.msh")]
+
+
+@pytest.mark.parametrize("kind", MESHES_ALL)
+@pytest.mark.parametrize("form", [0, 1])
+@pytest.mark.parametrize("level", [2, 3])
+def test_apply_equals_bruteforce_matrix(O, kind, form, level):
+    om, _ = mesh_pair(kind, host_ctx=_host())
+    K = BF.assemble(om, level, form)
+    A = O.Operator(om, 0, level, form)
+    x = np.random.default_rng(level).standard_normal(om.owned_count(level))
+    u, y = O.Function(om, 0, level), O.Function(om, 0, level)
+    u.set(level, x)
+    A.apply(u, y, level, flag=O.ALL)
+    ref = K @ x
+    assert np.abs(y.get(level) - ref).max() <= 1e-13 * np.abs(ref).max()
+    # flag Inner leaves Dirichlet rows untouched
+    y.set(level, np.full(len(x), 7.0))
+    A.apply(u, y, level, flag=O.INNER)
+    types = om.owned_types(level)
+    got = y.get(level)
+    assert np.all(got[types != O.INNER] == 7.0)
+    assert np.abs(got[types == O.INNER] - ref[types == O.INNER]).max() <= 1e-13 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("kind", MESHES_ALL)
+@pytest.mark.parametrize("level", [2, 3])
+def test_laplace_of_linears_vanishes(O, kind, level):
+    """tests/hyteg/P1/P1LaplaceOperator3DTest.cpp:47,112-134: apply of 0, 1, 42x, 42x+y+1337z is zero in
+    the interior, discrete L2 < 2.8e-13."""
+    om, _ = mesh_pair(kind, host_ctx=_host())
+    A = O.Operator(om, 0, level)
+    X = om.owned_coords(level)
+    inner = om.owned_types(level) == O.INNER
+    for f in (lambda P: 0 * P[:, 0], lambda P: 1 + 0 * P[:, 0], lambda P: 42 * P[:, 0],
+              lambda P: 42 * P[:, 0] + P[:, 1] + 1337 * P[:, 2]):
+        u, y = O.Function(om, 0, level), O.Function(om, 0, level)
+        u.set(level, f(X))
+        A.apply(u, y, level, flag=O.INNER)
+        r = y.get(level)[inner]
+        assert np.sqrt(np.sum(r ** 2) / max(1, inner.sum())) < 2.8e-13
+
+
+@pytest.mark.parametrize("kind", MESHES_ALL[:3])
+def test_stencil_invariants(O, kind):
+    """VertexDoFStencilAssemblyTest.cpp:81-226: row sums vanish, weights scale as h (2 w_L == w_{L-1}),
+    and the cell stencil is symmetric (weight of d equals weight of -d)."""
+    om, _ = mesh_pair(kind, host_ctx=_host())
+    A = O.Operator(om, 0, 6)
+    nV, nE, nF, nC = om.counts()
+    opp = [0, 2, 1, 4, 3, 6, 5, 11, 13, 12, 14, 7, 9, 8, 10]  # C W E N S NW SE TC TW TS TSE BC BN BE BNW
+    for L in range(3, 7):
+        for c in range(nC):
+            s = A.cell_stencil(L, c)
+            assert abs(s.sum()) <= 1e-14 * abs(s[0])
+            assert np.allclose(s, s[opp], rtol=0, atol=1e-14 * abs(s[0]))
+            assert np.allclose(2 * s, A.cell_stencil(L - 1, c), rtol=1e-12)
+        for e in range(nE):
+            s = A.edge_stencil(L, e)
+            assert abs(s.sum()) <= 1e-13 * abs(s[3])
+            assert np.allclose(2 * s, A.edge_stencil(L - 1, e), rtol=1e-12)
+        for v in range(nV):
+            s = A.vertex_stencil(L, v)
+            assert abs(s.sum()) <= 1e-13 * abs(s[0])
+
+
+@pytest.mark.parametrize("kind", MESHES_ALL[:3])
+@pytest.mark.parametrize("level", [1, 2, 3])
+def test_prolongation_exact_on_linears(O, kind, level):
+    """VertexDoFLinearProlongation3DTest.cpp:48: prolongation reproduces constants and linears."""
+    om, _ = mesh_pair(kind, host_ctx=_host())
+    f = O.Function(om, 0, level + 1)
+    Xc, Xf = om.owned_coords(level), om.owned_coords(level + 1)
+    for g in (lambda P: 1 + 0 * P[:, 0], lambda P: 42 * P[:, 0] + P[:, 1] + 1337 * P[:, 2] + 3):
+        f.set(level, g(Xc))
+        f.set(level + 1, np.zeros(len(Xf)))
+        O.prolongate(f, level, flag=O.ALL, add=True)
+        ref = g(Xf)
+        assert np.abs(f.get(level + 1) - ref).max() <= 1e-15 * np.abs(ref).max() * 8
+
+
+@pytest.mark.parametrize("kind", MESHES_ALL[:3])
+@pytest.mark.parametrize("level", [2, 3])
+def test_restriction_is_transpose_of_prolongation(O, kind, level):
+    om, _ = mesh_pair(kind, host_ctx=_host())
+    f = O.Function(om, 0, level)
+    nc, nf = om.owned_count(level - 1), om.owned_count(level)
+    P = np.zeros((nf, nc))
+    R = np.zeros((nc, nf))
+    for j in range(nc):
+        e = np.zeros(nc)
+        e[j] = 1
+        f.set(level - 1, e)
+        f.set(level, np.zeros(nf))
+        O.prolongate(f, level - 1, flag=O.INNER, add=True)
+        P[:, j] = f.get(level)
+    for j in range(nf):
+        e = np.zeros(nf)
+        e[j] = 1
+        f.set(level, e)
+        f.set(level - 1, np.zeros(nc))
+        O.restrict(f, level, flag=O.INNER)
+        R[:, j] = f.get(level - 1)
+    ic = om.owned_types(level - 1) == O.INNER
+    iff = om.owned_types(level) == O.INNER
+    assert np.abs(R[np.ix_(ic, iff)] - P[np.ix_(iff, ic)].T).max() == 0.0
+    # P interpolates: every fine row sums to one
+    assert np.allclose(P[np.ix_(iff, np.ones(nc, bool))].sum(axis=1), 1.0)
+
+
+def _owners(om, level):
+    """(class, primitive, within-primitive lexicographic key) of every owned DoF."""
+    nV, nE, nF, nC = om.counts()
+    w = 2 ** level + 1
+    cls, prim = [], []
+    cls += [0] * nV
+    prim += list(range(nV))
+    for e in range(nE):
+        cls += [1] * (w - 2)
+        prim += [e] * (w - 2)
+    nfp = max(0, (w - 3) * (w - 2) // 2)
+    for f in range(nF):
+        cls += [2] * nfp
+        prim += [f] * nfp
+    ncp = max(0, (w - 4) * (w - 3) * (w - 2) // 6)
+    for c in range(nC):
+        cls += [3] * ncp
+        prim += [c] * ncp
+    return np.array(cls), np.array(prim)
+
+
+@pytest.mark.parametrize("kind", [("cube", 1), ("cube", 2), ("gmsh", "regular_octahedron_8el.msh")])
+@pytest.mark.parametrize("level", [2, 3])
+@pytest.mark.parametrize("order", [0, 1])
+def test_sor_equals_bruteforce_gauss_seidel(O, kind, level, order):
+    """One SOR sweep equals a Gauss-Seidel sweep of the assembled matrix in the reference's order: classes
+    V, E, F, C in turn; within a primitive lexicographic (order 0) or colour by colour (order 1); a point
+    reads current values inside the closure of its own primitive (the primitive and its boundary) and the
+    values from the start of the sweep everywhere else, because those reach it through ghost layers that are
+    refreshed only at the start of the sweep (P1Operator.hpp:375-411)."""
+    om, _ = mesh_pair(kind, host_ctx=_host())
+    nV, nE, nF, nC = om.counts()
+    verts = ([{v} for v in range(nV)] + [set(om.edge_vertices(e)) for e in range(nE)]
+             + [set(om.face_vertices(f)) for f in range(nF)] + [set(om.cell_vertices(c)) for c in range(nC)])
+    base = [0, nV, nV + nE, nV + nE + nF]
+    K = BF.assemble(om, level).tocsr()
+    A = O.Operator(om, 0, level)
+    n = om.owned_count(level)
+    rng = np.random.default_rng(level + 10 * order)
+    x0, b = rng.standard_normal((2, n))
+    omega = 0.8
+    cls, prim = _owners(om, level)
+    types = om.owned_types(level)
+    X = om.owned_coords(level)
+    # within-primitive order: owned order is lexicographic per primitive; colour order groups by colour
+    w = 2 ** level + 1
+    keys = np.arange(n)
+    if order == 1:
+        colour = np.zeros(n, int)
+        # recover local indices from owned order
+        nV, nE, nF, nC = om.counts()
+        k = nV
+        for e in range(nE):
+            for xx in range(1, w - 1):
+                colour[k] = xx % 2
+                k += 1
+        for f in range(nF):
+            for y in range(1, w - 2):
+                for xx in range(1, w - 1 - y):
+                    colour[k] = (xx + 2 * y) % 3
+                    k += 1
+        for c in range(nC):
+            for z in range(1, w - 3):
+                for y in range(1, w - 2 - z):
+                    for xx in range(1, w - 1 - y - z):
+                        colour[k] = (xx + 2 * y + 3 * z) % 4
+                        k += 1
+        keys = np.lexsort((np.arange(n), colour, prim, cls))
+    else:
+        keys = np.lexsort((np.arange(n), prim, cls))
+    x = x0.copy()
+    for i in keys:
+        if types[i] != O.INNER:
+            continue
+        row = K.getrow(i)
+        s = 0.0
+        pi = base[cls[i]] + prim[i]
+        for j, a in zip(row.indices, row.data):
+            if j == i:
+                continue
+            pj = base[cls[j]] + prim[j]
+            in_closure = pj == pi or (cls[j] < cls[i] and verts[pj] <= verts[pi])
+            s += a * (x[j] if in_closure else x0[j])
+        x[i] = (1 - omega) * x[i] + omega * (b[i] - s) / K[i, i]
+    u, f = O.Function(om, 0, level), O.Function(om, 0, level)
+    u.set(level, x0)
+    f.set(level, b)
+    A.smooth_sor(u, f, level, omega=omega, flag=O.INNER, order=order)
+    assert np.abs(u.get(level) - x).max() <= 1e-12 * np.abs(x).max()
+
+
+def test_cg_solves_coarse_problem(O):
+    om, _ = mesh_pair(("cube", 2), host_ctx=_host())
+    L = 2
+    K = BF.assemble(om, L)
+    inner = om.owned_types(L) == O.INNER
+    b = np.random.default_rng(0).standard_normal(om.owned_count(L)) * inner
+    A = O.Operator(om, 0, L)
+    x, f = O.Function(om, 0, L), O.Function(om, 0, L)
+    f.set(L, b)
+    O.CG(om, L, L, 500, 1e-13).solve(A, x, f, L)
+    Ki = K[inner][:, inner].toarray()
+    ref = np.linalg.solve(Ki, b[inner])
+    assert np.abs(x.get(L)[inner] - ref).max() <= 1e-10 * np.abs(ref).max()
+
+
+def test_gmg_rate_bound_octahedron(O):
+    """tests/hyteg/convergence/P1GMG3DConvergenceTest.cpp:49-157: octahedron (8 tets), levels 0 -> 3,
+    V(3,3) Gauss-Seidel, CG coarse solver, u = sin(x) sinh(y) z on the boundary, random interior start,
+    f = 0: the ratio of squared discrete residual norms per cycle stays below 3.2e-2."""
+    om, _ = mesh_pair(("gmsh", "regular_octahedron_8el.msh"), host_ctx=_host())
+    L = 3
+    A = O.Operator(om, 0, L)
+    X = om.owned_coords(L)
+    types = om.owned_types(L)
+    exact = np.sin(X[:, 0]) * np.sinh(X[:, 1]) * X[:, 2]
+    x0 = np.where(types == O.INNER, np.random.default_rng(42).uniform(0, 1, len(X)), exact)
+    x, f, r = O.Function(om, 0, L), O.Function(om, 0, L), O.Function(om, 0, L)
+    x.set(L, x0)
+    g = O.GMG(om, 0, L, 3, 3, 1.0, 0)
+
+    def res2():
+        r.interpolate(L, 0.0)
+        A.apply(x, r, L, flag=O.INNER)
+        return r.dot(r, L, O.INNER)
+
+    last = res2()
+    for _ in range(10):
+        g.solve(A, x, f, L)
+        cur = res2()
+        assert cur / last < 3.2e-2
+        last = cur
+
+
+def test_discretisation_error_is_second_order(O):
+    """Manufactured sine solution on the 6-tet cube: the discrete L2 error falls by ~4 per level."""
+    errs = []
+    om, _ = mesh_pair(("cube", 1), host_ctx=_host())
+    for L in (4, 5, 6):
+        A, M = O.Operator(om, 0, L), O.Operator(om, 0, L, 1)
+        X = om.owned_coords(L)
+        x, b, t = O.Function(om, 0, L), O.Function(om, 0, L), O.Function(om, 0, L)
+        t.set(L, sine_rhs(X))
+        M.apply(t, b, L, flag=O.ALL)
+        g = O.GMG(om, 0, L, 3, 3, 1.0, 0)
+        for _ in range(12):
+            g.solve(A, x, b, L)
+        inner = om.owned_types(L) == O.INNER
+        errs.append(np.sqrt(np.sum((x.get(L) - sine_exact(X))[inner] ** 2) / len(X)))
+    rates = [errs[i] / errs[i + 1] for i in range(2)]
+    assert all(3.5 < r < 4.5 for r in rates), rates
+
+
+_HOST = None
+
+
+def _host():
+    global _HOST
+    if _HOST is None:
+        import hyteg_amd as H
+        _HOST = H.Context(-1)
+    return _HOST
