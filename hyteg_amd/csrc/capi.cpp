@@ -261,6 +261,14 @@ struct hymi_gmg_s
          (void) hipGraphExecDestroy( kv.second );
    }
 
+   int maxL_invoked = -1;
+
+   void cycle( Function& x, Function& b, int L )
+   {
+      maxL_invoked = L;
+      recurse( x, b, L );
+   }
+
    void recurse( Function& x, Function& b, int L )
    {
       const int flag = kInner | kNeumann;
@@ -269,16 +277,19 @@ struct hymi_gmg_s
          cgSolve( *A, x, b, L, cgIters, cgTol, cg );
          return;
       }
+      // ghost copies of x are current after a sweep and after the coarse-level reset; only the caller's x on
+      // the finest level, x after prolongation and x after the coarse solver need a refresh
+      const bool finest = ( L == maxL_invoked );
       for ( int i = 0; i < nu1; i++ )
-         launchSor( *A, x, b, omega, L, flag );
-      launchApply( *A, x, *tmp, &b, L, flag, kResidual );
+         launchSor( *A, x, b, omega, L, flag, i == 0 && finest );
+      launchApply( *A, x, *tmp, &b, L, flag, kResidual, nu1 == 0 && finest );
       // restriction written straight into b on the coarse level (== restrict + b.assign on flag DoFs)
       launchRestrict( *tmp, b, L, flag );
       launchLincomb( x, L - 1, kAll, 0, nullptr, nullptr, false, 0.0 );
       recurse( x, b, L - 1 );
-      launchProlongateAdd( x, L - 1, flag );
+      launchProlongateAdd( x, L - 1, flag, L - 1 == minL );
       for ( int i = 0; i < nu2; i++ )
-         launchSor( *A, x, b, omega, L, flag );
+         launchSor( *A, x, b, omega, L, flag, i == 0 );
    }
 };
 
@@ -975,6 +986,7 @@ int hymi_gmg_create( hymi_operator op, hymi_function xt, int minL, int maxL, int
          int64_t    n = 0;
          M.segments( P, segLen, &n );
          maxBlocks = std::max( maxBlocks, ( n + 3 ) / 4 + 64 );
+         M.marchUnits( P, &n );
          M.closureRanges( P, kInner | kNeumann, &n );
          M.closureRanges( P, kInner | kNeumann | kFreeslip, &n );
          M.closureRanges( P, kAll, &n );
@@ -1013,7 +1025,7 @@ int hymi_gmg_vcycle( hymi_gmg g, hymi_function x, hymi_function b, int level )
 {
    return guarded( [&] {
       checkGmg( g, x, b, level );
-      g->recurse( x->f, b->f, level );
+      g->cycle( x->f, b->f, level );
    } );
 }
 
@@ -1031,7 +1043,7 @@ int hymi_gmg_vcycle_graph( hymi_gmg g, hymi_function x, hymi_function b, int lev
          HYMI_HIP( hipStreamBeginCapture( s, hipStreamCaptureModeThreadLocal ) );
          try
          {
-            g->recurse( x->f, b->f, level );
+            g->cycle( x->f, b->f, level );
          } catch ( ... )
          {
             hipGraph_t dummy = nullptr;
@@ -1069,7 +1081,7 @@ int hymi_time_kernel( hymi_operator op, hymi_gmg gmg, hymi_function x, hymi_func
          case 1: launchCellSorOnly( A, x->f, b->f, 1.0, level, flag, 0 ); break;
          case 2: launchApply( A, x->f, y->f, nullptr, level, flag, kReplace ); break;
          case 3: launchSor( A, x->f, b->f, 1.0, level, flag ); break;
-         case 4: gmg->recurse( x->f, b->f, level ); break;
+         case 4: gmg->cycle( x->f, b->f, level ); break;
          default: throw std::invalid_argument( "bad kernel selector" );
          }
       };

@@ -84,6 +84,7 @@ struct LevelPlan
    std::vector< EdgeTerm >   hEdgeTerms;
    std::vector< VertexGeom > hVertexGeom;
    std::vector< int64_t >    hVertexTerms; // neighbour element index per vertex term
+   std::vector< int64_t >    hVertexEnds;  // the vertex's own copy in that edge's closure
 
    DevBuf< V2E >        dV2E;
    DevBuf< E2F >        dE2F;
@@ -93,11 +94,16 @@ struct LevelPlan
    DevBuf< EdgeTerm >   dEdgeTerms;
    DevBuf< VertexGeom > dVertexGeom;
    DevBuf< int64_t >    dVertexTerms;
+   DevBuf< int64_t >    dVertexEnds;
    DevBuf< int32_t >    dCellType;
 
    // cell row segments, keyed by segment length (in points)
    std::map< int, std::unique_ptr< DevBuf< CellSeg > > > segs;
    std::map< int, int64_t >                              segCount;
+   // work units of the row-marching macro-cell kernels
+   std::unique_ptr< DevBuf< MarchUnit > > march;
+   int64_t                                marchCount = -1;
+   int                                    marchRows  = 32;
    // closure ranges of primitives matching a flag, keyed by flag
    std::map< int, std::unique_ptr< DevBuf< Range > > > ranges;
    std::map< int, int64_t >                            rangeCount;
@@ -115,6 +121,7 @@ struct Mesh
    std::map< int, std::unique_ptr< LevelPlan > > plans;
    LevelPlan& plan( int level );
    const CellSeg* segments( LevelPlan& P, int len, int64_t* count );
+   const MarchUnit* marchUnits( LevelPlan& P, int64_t* count );
    const Range*   closureRanges( LevelPlan& P, int flag, int64_t* count );
 };
 

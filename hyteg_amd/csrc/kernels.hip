@@ -227,6 +227,146 @@ __global__ __launch_bounds__( 256 ) void kCellSor( LV g, const CellSeg* __restri
 }
 
 // --------------------------------------------------------------------------------------------------------
+// macro-cell interior, row-marching form (one right-hand side)
+//
+// One wave owns the window x = x0-1 .. x0+62 of slab z of one macro-cell and marches along y over a chunk of
+// rows.  Each step loads one new row of slab z, of slab z+1 and of slab z-1; the x-1 / x+1 neighbours come
+// from the adjacent lanes (ds_bpermute shuffles), the other rows from registers rotated from earlier steps.
+// Lanes 1..62 produce outputs.  Every row value is read from memory once per wave instead of up to five
+// times, and all per-row index arithmetic is scalar.  Loads are issued two steps ahead of their use and
+// shuffled one step ahead, so a step never waits on the load it just issued.
+
+template < typename T >
+struct Row3
+{
+   T l, c, r; // values at x-1, x, x+1
+};
+
+template < typename T >
+__device__ __forceinline__ T loadRaw( const T* __restrict__ u, int64_t rowBase, int x, int rowEnd )
+{
+   return ( x <= rowEnd ) ? u[rowBase + x] : T( 0 );
+}
+
+template < typename T >
+__device__ __forceinline__ Row3< T > spread( T v )
+{
+   Row3< T > R;
+   R.c = v;
+   R.l = __shfl_up( v, 1, 64 );
+   R.r = __shfl_down( v, 1, 64 );
+   return R;
+}
+
+constexpr int kMarchSor = 3; // MODE value of the in-place multicolour SOR pass
+// finest level whose faces are small enough for one workgroup per face in the fused SOR phase
+constexpr int kFaceFusedMaxLevel = 4;
+
+// MODE kReplace / kAdd / kResidual: y = A u, y += A u, y = b - A u.
+// MODE kMarchSor: in-place SOR update of the points of `colour` (y == u).
+template < typename T, int MODE >
+__global__ __launch_bounds__( 64 ) void kCellMarch( LV g, const MarchUnit* __restrict__ units, int64_t nUnits,
+                                                    const T* __restrict__ W, const int* __restrict__ ctype, int flag,
+                                                    const T* u, T* y, const T* b, T omega, int colour, int rows )
+{
+   const int64_t ui = xcdBlock( nUnits );
+   if ( ui >= nUnits ) return;
+   const MarchUnit U = units[ui];
+   if ( !( ctype[U.cell] & flag ) ) return;
+   const int     lane = threadIdx.x;
+   const int     w = g.w, z = U.z;
+   const int     x    = U.x0 - 1 + lane;
+   const int64_t base = g.offC + (int64_t) U.cell * g.cS;
+   const int64_t t0   = tet( w );
+   const T*      c    = W + (int64_t) U.cell * 15;
+   const T cC = c[dC], cW = c[dW], cE = c[dE], cN = c[dN], cS = c[dS], cNW = c[dNW], cSE = c[dSE];
+   const T cTC = c[dTC], cTW = c[dTW], cTS = c[dTS], cTSE = c[dTSE], cBC = c[dBC], cBN = c[dBN], cBE = c[dBE],
+           cBNW = c[dBNW];
+   const T   relax = omega / cC;
+   const int yLast = min( U.y0 + rows - 1, w - 2 - z - U.x0 );
+   int       yy    = U.y0;
+   // rows held for step yy: slab z rows yy-1, yy, yy+1; slab z+1 rows yy-1, yy; slab z-1 rows yy, yy+1
+   int64_t       rz0 = base + crow( w, t0, yy, z );
+   const int64_t rzm = base + crow( w, t0, yy - 1, z ), rzp = base + crow( w, t0, yy + 1, z );
+   const int64_t rtm = base + crow( w, t0, yy - 1, z + 1 ), rt0 = base + crow( w, t0, yy, z + 1 );
+   const int64_t rb0 = base + crow( w, t0, yy, z - 1 ), rbp = base + crow( w, t0, yy + 1, z - 1 );
+   Row3< T >     Zm = spread( loadRaw( u, rzm, x, w - 1 - ( yy - 1 ) - z ) );
+   Row3< T >     Z0 = spread( loadRaw( u, rz0, x, w - 1 - yy - z ) );
+   Row3< T >     Zp = spread( loadRaw( u, rzp, x, w - 1 - ( yy + 1 ) - z ) );
+   Row3< T >     Tm = spread( loadRaw( u, rtm, x, w - 1 - ( yy - 1 ) - ( z + 1 ) ) );
+   Row3< T >     T0 = spread( loadRaw( u, rt0, x, w - 1 - yy - ( z + 1 ) ) );
+   Row3< T >     B0 = spread( loadRaw( u, rb0, x, w - 1 - yy - ( z - 1 ) ) );
+   Row3< T >     Bp = spread( loadRaw( u, rbp, x, w - 1 - ( yy + 1 ) - ( z - 1 ) ) );
+   // raw rows for step yy+1: slab z row yy+2, slab z+1 row yy+1, slab z-1 row yy+2
+   int64_t nz = rzp + ( w - ( yy + 1 ) - z );
+   int64_t nt = rt0 + ( w - yy - ( z + 1 ) );
+   int64_t nb = rbp + ( w - ( yy + 1 ) - ( z - 1 ) );
+   T       zr = loadRaw( u, nz, x, w - 1 - ( yy + 2 ) - z );
+   T       tr = loadRaw( u, nt, x, w - 1 - ( yy + 1 ) - ( z + 1 ) );
+   T       br = loadRaw( u, nb, x, w - 1 - ( yy + 2 ) - ( z - 1 ) );
+   for ( ; yy <= yLast; yy++ )
+   {
+      // raw rows for step yy+2
+      const int64_t nz2 = nz + ( w - ( yy + 2 ) - z );
+      const int64_t nt2 = nt + ( w - ( yy + 1 ) - ( z + 1 ) );
+      const int64_t nb2 = nb + ( w - ( yy + 2 ) - ( z - 1 ) );
+      const T       zr2 = loadRaw( u, nz2, x, w - 1 - ( yy + 3 ) - z );
+      const T       tr2 = loadRaw( u, nt2, x, w - 1 - ( yy + 2 ) - ( z + 1 ) );
+      const T       br2 = loadRaw( u, nb2, x, w - 1 - ( yy + 3 ) - ( z - 1 ) );
+
+      const bool out = lane >= 1 && lane <= 62 && x <= w - 2 - yy - z;
+      if ( MODE == kMarchSor )
+      {
+         if ( out && ( ( x + 2 * yy + 3 * z ) & 3 ) == colour )
+         {
+            T acc = cW * Z0.l + cE * Z0.r;
+            acc += cN * Zp.c + cNW * Zp.l;
+            acc += cS * Zm.c + cSE * Zm.r;
+            acc += cTC * T0.c + cTW * T0.l;
+            acc += cTS * Tm.c + cTSE * Tm.r;
+            acc += cBC * B0.c + cBE * B0.r;
+            acc += cBN * Bp.c + cBNW * Bp.l;
+            const int64_t o = rz0 + x;
+            y[o]            = ( T( 1 ) - omega ) * Z0.c + relax * ( b[o] - acc );
+         }
+      }
+      else if ( out )
+      {
+         T acc = cC * Z0.c;
+         acc += cW * Z0.l + cE * Z0.r;
+         acc += cN * Zp.c + cNW * Zp.l;
+         acc += cS * Zm.c + cSE * Zm.r;
+         acc += cTC * T0.c + cTW * T0.l;
+         acc += cTS * Tm.c + cTSE * Tm.r;
+         acc += cBC * B0.c + cBE * B0.r;
+         acc += cBN * Bp.c + cBNW * Bp.l;
+         const int64_t o = rz0 + x;
+         if ( MODE == kReplace )
+            y[o] = acc;
+         else if ( MODE == kAdd )
+            y[o] += acc;
+         else
+            y[o] = b[o] - acc;
+      }
+      // advance to step yy+1
+      Zm = Z0;
+      Z0 = Zp;
+      Zp = spread( zr );
+      Tm = T0;
+      T0 = spread( tr );
+      B0 = Bp;
+      Bp = spread( br );
+      zr = zr2;
+      tr = tr2;
+      br = br2;
+      rz0 += w - yy - z;
+      nz = nz2;
+      nt = nt2;
+      nb = nb2;
+   }
+}
+
+// --------------------------------------------------------------------------------------------------------
 // macro-face interior
 
 template < typename T, int NR >
@@ -414,6 +554,97 @@ __global__ void kVertexSor( LV g, const VertexGeom* __restrict__ vg, const int64
       acc += vw[1 + k] * u[terms[G.termOff + k] * NR + r];
    const int64_t o = (int64_t) G.vertex * NR + r;
    u[o]            = ( T( 1 ) - omega ) * u[o] + omega / vw[0] * ( b[o] - acc );
+}
+
+// --------------------------------------------------------------------------------------------------------
+// fused SOR phases of the lower-dimensional primitives
+//
+// A sweep runs kVertexSorPush (vertices, one thread each; the new value is also written to the incident
+// edges' end points), kEdgeSorFused (one workgroup per edge, colours 0/1 separated by a barrier),
+// kFaceSorFused (one workgroup per face: first copies the final edge values into its own boundary, then
+// colours 0/1/2 separated by barriers), the face-to-cell copy, and the four macro-cell colour passes.
+// Within a phase, reads of other primitives of the same class go to closure copies that no kernel of that
+// phase writes, so they see start-of-sweep values exactly as the reference's ghost layers do.  Primitives
+// not selected by `flag` keep their values, so their copies stay valid and they are skipped.
+
+template < typename T >
+__global__ void kVertexSorPush( LV g, const VertexGeom* __restrict__ vg, const int64_t* __restrict__ terms,
+                                const int64_t* __restrict__ ends, int nV, const T* __restrict__ VW, int flag,
+                                T* __restrict__ u, const T* __restrict__ b, T omega )
+{
+   const int v = blockIdx.x * blockDim.x + threadIdx.x;
+   if ( v >= nV ) return;
+   const VertexGeom G = vg[v];
+   if ( !( G.type & flag ) ) return;
+   const T* vw  = VW + G.vertex + G.termOff;
+   T        acc = T( 0 );
+   for ( int k = 0; k < G.nTerms; k++ )
+      acc += vw[1 + k] * u[terms[G.termOff + k]];
+   const T nv = ( T( 1 ) - omega ) * u[G.vertex] + omega / vw[0] * ( b[G.vertex] - acc );
+   u[G.vertex] = nv;
+   // end-point copies in the incident edges (read by no vertex on levels >= 1)
+   for ( int k = 0; k < G.nTerms; k++ )
+      u[ends[G.termOff + k]] = nv;
+}
+
+template < typename T >
+__global__ __launch_bounds__( 128 ) void kEdgeSorFused( LV g, const EdgeGeom* __restrict__ eg,
+                                                        const EdgeTerm* __restrict__ terms, const T* __restrict__ EW,
+                                                        int flag, T* __restrict__ u, const T* __restrict__ b, T omega )
+{
+   const EdgeGeom& G = eg[blockIdx.x];
+   if ( !( G.type & flag ) ) return;
+   const T*      ew = EW + 3 * (int64_t) G.edge + G.termOff;
+   const int64_t eo = g.offE + (int64_t) G.edge * g.eS;
+   const T       rl = omega / ew[1];
+   for ( int colour = 0; colour < 2; colour++ )
+   {
+      // colour 0 = even edge points, colour 1 = odd ones
+      for ( int x = 2 - colour + 2 * (int) threadIdx.x; x <= g.w - 2; x += 2 * blockDim.x )
+      {
+         T acc = ew[0] * u[eo + x - 1] + ew[2] * u[eo + x + 1];
+         acc += edgeTermsSum< T, 1 >( g, G, terms, ew + 3, u, x, 0 );
+         u[eo + x] = ( T( 1 ) - omega ) * u[eo + x] + rl * ( b[eo + x] - acc );
+      }
+      __syncthreads();
+   }
+}
+
+template < typename T >
+__global__ __launch_bounds__( 256 ) void kFaceSorFused( LV g, const FaceGeom* __restrict__ fg, const E2F* __restrict__ e2f,
+                                                        const T* __restrict__ FW, int flag, T* __restrict__ u,
+                                                        const T* __restrict__ b, T omega )
+{
+   const FaceGeom& G = fg[blockIdx.x];
+   if ( !( G.type & flag ) ) return;
+   const int64_t fo = g.offF + (int64_t) G.face * g.fS;
+   // own boundary <- final edge values (the E2F table holds the three edges of face f at 3f .. 3f+2)
+   for ( int le = 0; le < 3; le++ )
+   {
+      const E2F m = e2f[3 * G.face + le];
+      for ( int p = threadIdx.x; p < g.w; p += blockDim.x )
+         u[fo + fidx( g.w, m.base[0] + p * m.inc[0], m.base[1] + p * m.inc[1] )] =
+             u[g.offE + (int64_t) m.edge * g.eS + p];
+   }
+   __syncthreads();
+   const int     m  = g.w - 3;
+   const int64_t np = (int64_t) m * ( m + 1 ) / 2;
+   const T*      fw = FW + (int64_t) G.face * kFaceW;
+   const T       rl = omega / fw[0];
+   for ( int colour = 0; colour < 3; colour++ )
+   {
+      for ( int64_t k = threadIdx.x; k < np; k += blockDim.x )
+      {
+         int a, bb;
+         triPos( m, k, a, bb );
+         const int i = a + 1, j = bb + 1;
+         if ( ( i + 2 * j ) % 3 != colour ) continue;
+         const T       acc = faceStencilSum< T, 1 >( g, G, fw, u, fo, i, j, 0, false );
+         const int64_t o   = fo + fidx( g.w, i, j );
+         u[o]              = ( T( 1 ) - omega ) * u[o] + rl * ( b[o] - acc );
+      }
+      __syncthreads();
+   }
 }
 
 // --------------------------------------------------------------------------------------------------------
@@ -821,6 +1052,66 @@ const double* wsel< double >( const DevBuf< double >& d64, const DevBuf< float >
 template <>
 const float* wsel< float >( const DevBuf< double >&, const DevBuf< float >& d32 ) { return d32.p; }
 
+// macro-cell apply / residual over all interior points: row-marching kernel for one right-hand side,
+// segment kernel (lanes along x * rhs) for interleaved right-hand sides
+template < typename T, int NR >
+void cellApplyLaunch( Mesh& M, LevelPlan& P, const T* CW, int flag, int mode, const T* u, T* y, const T* b )
+{
+   hipStream_t s = M.ctx->stream;
+   LV          g = viewOf( P );
+   if ( NR == 1 )
+   {
+      int64_t          n  = 0;
+      const MarchUnit* mu = M.marchUnits( P, &n );
+      if ( n == 0 ) return;
+      const dim3 grid( (unsigned) xcdGrid( n ) ), block( 64 );
+      if ( mode == kReplace )
+         hipLaunchKernelGGL( ( kCellMarch< T, kReplace > ), grid, block, 0, s, g, mu, n, CW, P.dCellType.p, flag, u, y, b,
+                             T( 0 ), 0, P.marchRows );
+      else if ( mode == kAdd )
+         hipLaunchKernelGGL( ( kCellMarch< T, kAdd > ), grid, block, 0, s, g, mu, n, CW, P.dCellType.p, flag, u, y, b,
+                             T( 0 ), 0, P.marchRows );
+      else
+         hipLaunchKernelGGL( ( kCellMarch< T, kResidual > ), grid, block, 0, s, g, mu, n, CW, P.dCellType.p, flag, u, y,
+                             b, T( 0 ), 0, P.marchRows );
+      return;
+   }
+   int64_t        nseg = 0;
+   const CellSeg* sg   = M.segments( P, 64 / NR, &nseg );
+   if ( nseg == 0 ) return;
+   const dim3 grid( (unsigned) xcdGrid( ( nseg + 3 ) / 4 ) ), block( 256 );
+   if ( mode == kReplace )
+      hipLaunchKernelGGL( ( kCellApply< T, NR, kReplace > ), grid, block, 0, s, g, sg, nseg, CW, P.dCellType.p, flag, u,
+                          y, b );
+   else if ( mode == kAdd )
+      hipLaunchKernelGGL( ( kCellApply< T, NR, kAdd > ), grid, block, 0, s, g, sg, nseg, CW, P.dCellType.p, flag, u, y,
+                          b );
+   else
+      hipLaunchKernelGGL( ( kCellApply< T, NR, kResidual > ), grid, block, 0, s, g, sg, nseg, CW, P.dCellType.p, flag, u,
+                          y, b );
+}
+
+template < typename T, int NR >
+void cellSorLaunch( Mesh& M, LevelPlan& P, const T* CW, int flag, T* u, const T* b, T omega, int colour )
+{
+   hipStream_t s = M.ctx->stream;
+   LV          g = viewOf( P );
+   if ( NR == 1 )
+   {
+      int64_t          n  = 0;
+      const MarchUnit* mu = M.marchUnits( P, &n );
+      if ( n == 0 ) return;
+      hipLaunchKernelGGL( ( kCellMarch< T, kMarchSor > ), dim3( (unsigned) xcdGrid( n ) ), dim3( 64 ), 0, s, g, mu, n, CW,
+                          P.dCellType.p, flag, u, u, b, omega, colour, P.marchRows );
+      return;
+   }
+   int64_t        nseg = 0;
+   const CellSeg* sg   = M.segments( P, 64 / NR, &nseg );
+   if ( nseg == 0 ) return;
+   hipLaunchKernelGGL( ( kCellSor< T, NR > ), dim3( (unsigned) xcdGrid( ( nseg + 3 ) / 4 ) ), dim3( 256 ), 0, s, g, sg,
+                       nseg, CW, P.dCellType.p, flag, u, b, omega, colour );
+}
+
 template < typename T, int NR >
 void refreshT( Function& f, int level, int mask )
 {
@@ -841,7 +1132,8 @@ void refreshT( Function& f, int level, int mask )
 }
 
 template < typename T, int NR >
-void applyT( Operator& op, Function& src, Function& dst, const Function* rhs, int level, int flag, int mode )
+void applyT( Operator& op, Function& src, Function& dst, const Function* rhs, int level, int flag, int mode,
+             bool refreshSrc )
 {
    Mesh&       M  = *op.mesh;
    LevelPlan&  P  = M.plan( level );
@@ -851,7 +1143,7 @@ void applyT( Operator& op, Function& src, Function& dst, const Function* rhs, in
    const T*    u  = (const T*) src.ptr( level );
    T*          y  = (T*) dst.ptr( level );
    const T*    b  = rhs ? (const T*) rhs->ptr( level ) : nullptr;
-   refreshT< T, NR >( src, level, 7 );
+   if ( refreshSrc ) refreshT< T, NR >( src, level, 7 );
    const T* VW = wsel< T >( Wt.vertex64, Wt.vertex32 );
    const T* EW = wsel< T >( Wt.edge64, Wt.edge32 );
    const T* FW = wsel< T >( Wt.face64, Wt.face32 );
@@ -878,17 +1170,13 @@ void applyT( Operator& op, Function& src, Function& dst, const Function* rhs, in
       if ( m > 0 && nF > 0 )
          LAUNCH_MODES( kFaceApply, dim3( blocksFor( m * ( m + 1 ) / 2 * NR, 256 ), nF ), dim3( 256 ), g, P.dFaceGeom.p,
                        FW, flag, u, y, b );
-      int64_t        nseg = 0;
-      const CellSeg* sg   = M.segments( P, segLen< NR >(), &nseg );
-      if ( nseg > 0 )
-         LAUNCH_MODES( kCellApply, dim3( (unsigned) xcdGrid( ( nseg + 3 ) / 4 ) ), dim3( 256 ), g, sg, nseg, CW,
-                       P.dCellType.p, flag, u, y, b );
+      cellApplyLaunch< T, NR >( M, P, CW, flag, mode, u, y, b );
    }
 #undef LAUNCH_MODES
 }
 
 template < typename T, int NR >
-void sorT( Operator& op, Function& xf, const Function& bf, double omega, int level, int flag )
+void sorT( Operator& op, Function& xf, const Function& bf, double omega, int level, int flag, bool refreshFirst )
 {
    Mesh&       M  = *op.mesh;
    LevelPlan&  P  = M.plan( level );
@@ -899,7 +1187,40 @@ void sorT( Operator& op, Function& xf, const Function& bf, double omega, int lev
    const T*    b  = (const T*) bf.ptr( level );
    const T     om = (T) omega;
    const int   nV = M.T.nV(), nE = M.T.nE(), nF = M.T.nF();
-   refreshT< T, NR >( xf, level, 7 );
+   if ( refreshFirst ) refreshT< T, NR >( xf, level, 7 );
+   if ( NR == 1 && level >= 1 )
+   {
+      hipLaunchKernelGGL( ( kVertexSorPush< T > ), dim3( blocksFor( nV, 64 ) ), dim3( 64 ), 0, s, g, P.dVertexGeom.p,
+                          P.dVertexTerms.p, P.dVertexEnds.p, nV, wsel< T >( Wt.vertex64, Wt.vertex32 ), flag, u, b, om );
+      hipLaunchKernelGGL( ( kEdgeSorFused< T > ), dim3( nE ), dim3( 128 ), 0, s, g, P.dEdgeGeom.p, P.dEdgeTerms.p,
+                          wsel< T >( Wt.edge64, Wt.edge32 ), flag, u, b, om );
+      if ( level >= 2 )
+      {
+         if ( nF > 0 && level <= kFaceFusedMaxLevel )
+            hipLaunchKernelGGL( ( kFaceSorFused< T > ), dim3( nF ), dim3( 256 ), 0, s, g, P.dFaceGeom.p, P.dE2F.p,
+                                wsel< T >( Wt.face64, Wt.face32 ), flag, u, b, om );
+         else if ( nF > 0 )
+         {
+            // large faces: one workgroup per face would leave most of the chip idle
+            refreshT< T, NR >( xf, level, 2 );
+            const int64_t m = P.w - 3;
+            for ( int c = 0; c < 3; c++ )
+               hipLaunchKernelGGL( ( kFaceSor< T, NR > ), dim3( blocksFor( m * ( m + 1 ) / 2 * NR, 256 ), nF ),
+                                   dim3( 256 ), 0, s, g, P.dFaceGeom.p, wsel< T >( Wt.face64, Wt.face32 ), flag, u,
+                                   b, om, c );
+         }
+         refreshT< T, NR >( xf, level, 4 );
+         for ( int c = 0; c < 4; c++ )
+            cellSorLaunch< T, NR >( M, P, wsel< T >( Wt.cell64, Wt.cell32 ), flag, u, b, om, c );
+      }
+      else
+      {
+         // level 1: faces and cells have no interior points; keep the face closures current
+         refreshT< T, NR >( xf, level, 2 | 4 );
+      }
+      return;
+   }
+   if ( !refreshFirst ) refreshT< T, NR >( xf, level, 7 );
    hipLaunchKernelGGL( ( kVertexSor< T, NR > ), dim3( blocksFor( (int64_t) nV * NR, 256 ) ), dim3( 256 ), 0, s, g,
                        P.dVertexGeom.p, P.dVertexTerms.p, nV, wsel< T >( Wt.vertex64, Wt.vertex32 ), flag, u, b, om );
    refreshT< T, NR >( xf, level, 1 );
@@ -916,12 +1237,8 @@ void sorT( Operator& op, Function& xf, const Function& bf, double omega, int lev
             hipLaunchKernelGGL( ( kFaceSor< T, NR > ), dim3( blocksFor( m * ( m + 1 ) / 2 * NR, 256 ), nF ), dim3( 256 ), 0,
                                 s, g, P.dFaceGeom.p, wsel< T >( Wt.face64, Wt.face32 ), flag, u, b, om, c );
       refreshT< T, NR >( xf, level, 4 );
-      int64_t        nseg = 0;
-      const CellSeg* sg   = M.segments( P, segLen< NR >(), &nseg );
-      if ( nseg > 0 )
-         for ( int c = 0; c < 4; c++ )
-            hipLaunchKernelGGL( ( kCellSor< T, NR > ), dim3( (unsigned) xcdGrid( ( nseg + 3 ) / 4 ) ), dim3( 256 ), 0, s,
-                                g, sg, nseg, wsel< T >( Wt.cell64, Wt.cell32 ), P.dCellType.p, flag, u, b, om, c );
+      for ( int c = 0; c < 4; c++ )
+         cellSorLaunch< T, NR >( M, P, wsel< T >( Wt.cell64, Wt.cell32 ), flag, u, b, om, c );
    }
 }
 
@@ -954,7 +1271,7 @@ void restrictT( Function& f, Function& dst, int srcLevel, int flag )
 }
 
 template < typename T, int NR >
-void prolongT( Function& f, int srcLevel, int flag )
+void prolongT( Function& f, int srcLevel, int flag, bool refreshCoarse )
 {
    Mesh&       M  = *f.mesh;
    LevelPlan&  Pc = M.plan( srcLevel );
@@ -963,7 +1280,7 @@ void prolongT( Function& f, int srcLevel, int flag )
    hipStream_t s  = M.ctx->stream;
    const T*    uc = (const T*) f.ptr( srcLevel );
    T*          uf = (T*) f.ptr( srcLevel + 1 );
-   refreshT< T, NR >( f, srcLevel, 7 );
+   if ( refreshCoarse ) refreshT< T, NR >( f, srcLevel, 7 );
    const int nV = M.T.nV(), nE = M.T.nE(), nF = M.T.nF();
    hipLaunchKernelGGL( ( kProlongVertex< T, NR > ), dim3( blocksFor( (int64_t) nV * NR, 256 ) ), dim3( 256 ), 0, s,
                        Pf.dVertexGeom.p, nV, flag, uc, uf );
@@ -1070,17 +1387,17 @@ struct RefreshF
 template < typename T, int NR >
 struct ApplyF
 {
-   static void run( Operator& op, Function& s, Function& d, const Function* b, int L, int flag, int mode )
+   static void run( Operator& op, Function& s, Function& d, const Function* b, int L, int flag, int mode, bool refresh )
    {
-      applyT< T, NR >( op, s, d, b, L, flag, mode );
+      applyT< T, NR >( op, s, d, b, L, flag, mode, refresh );
    }
 };
 template < typename T, int NR >
 struct SorF
 {
-   static void run( Operator& op, Function& x, const Function& b, double omega, int L, int flag )
+   static void run( Operator& op, Function& x, const Function& b, double omega, int L, int flag, bool refresh )
    {
-      sorT< T, NR >( op, x, b, omega, L, flag );
+      sorT< T, NR >( op, x, b, omega, L, flag, refresh );
    }
 };
 template < typename T, int NR >
@@ -1091,7 +1408,7 @@ struct RestrictF
 template < typename T, int NR >
 struct ProlongF
 {
-   static void run( Function& f, int L, int flag ) { prolongT< T, NR >( f, L, flag ); }
+   static void run( Function& f, int L, int flag, bool refresh ) { prolongT< T, NR >( f, L, flag, refresh ); }
 };
 template < typename T, int NR >
 struct DotF
@@ -1124,14 +1441,15 @@ struct XpbyF
 
 void launchRefresh( Function& f, int level, int mask ) { dispatch< RefreshF >( f.prec, f.nrhs, f, level, mask ); }
 
-void launchApply( Operator& op, Function& src, Function& dst, const Function* rhs, int level, int flag, int mode )
+void launchApply( Operator& op, Function& src, Function& dst, const Function* rhs, int level, int flag, int mode,
+                  bool refreshSrc )
 {
-   dispatch< ApplyF >( src.prec, src.nrhs, op, src, dst, rhs, level, flag, mode );
+   dispatch< ApplyF >( src.prec, src.nrhs, op, src, dst, rhs, level, flag, mode, refreshSrc );
 }
 
-void launchSor( Operator& op, Function& x, const Function& b, double omega, int level, int flag )
+void launchSor( Operator& op, Function& x, const Function& b, double omega, int level, int flag, bool refreshFirst )
 {
-   dispatch< SorF >( x.prec, x.nrhs, op, x, b, omega, level, flag );
+   dispatch< SorF >( x.prec, x.nrhs, op, x, b, omega, level, flag, refreshFirst );
 }
 
 void launchRestrict( Function& f, Function& dst, int srcLevel, int flag )
@@ -1141,9 +1459,9 @@ void launchRestrict( Function& f, Function& dst, int srcLevel, int flag )
    dispatch< RestrictF >( f.prec, f.nrhs, f, dst, srcLevel, flag );
 }
 
-void launchProlongateAdd( Function& f, int srcLevel, int flag )
+void launchProlongateAdd( Function& f, int srcLevel, int flag, bool refreshCoarse )
 {
-   dispatch< ProlongF >( f.prec, f.nrhs, f, srcLevel, flag );
+   dispatch< ProlongF >( f.prec, f.nrhs, f, srcLevel, flag, refreshCoarse );
 }
 
 void launchLincomb( Function& dst, int level, int flag, int n, const double* c, Function* const* src, bool accumulate,
@@ -1220,15 +1538,11 @@ struct CellApplyOnlyF
 {
    static void run( Operator& op, Function& src, Function& dst, int level, int flag )
    {
-      Mesh&          M    = *op.mesh;
-      LevelPlan&     P    = M.plan( level );
-      int64_t        nseg = 0;
-      const CellSeg* sg   = M.segments( P, segLen< NR >(), &nseg );
-      auto&          Wt   = op.weights( level );
-      if ( nseg == 0 ) return;
-      hipLaunchKernelGGL( ( kCellApply< T, NR, kReplace > ), dim3( (unsigned) xcdGrid( ( nseg + 3 ) / 4 ) ), dim3( 256 ), 0,
-                          M.ctx->stream, viewOf( P ), sg, nseg, wsel< T >( Wt.cell64, Wt.cell32 ), P.dCellType.p, flag,
-                          (const T*) src.ptr( level ), (T*) dst.ptr( level ), (const T*) nullptr );
+      Mesh&      M  = *op.mesh;
+      LevelPlan& P  = M.plan( level );
+      auto&      Wt = op.weights( level );
+      cellApplyLaunch< T, NR >( M, P, wsel< T >( Wt.cell64, Wt.cell32 ), flag, kReplace, (const T*) src.ptr( level ),
+                                (T*) dst.ptr( level ), (const T*) nullptr );
    }
 };
 template < typename T, int NR >
@@ -1236,15 +1550,11 @@ struct CellSorOnlyF
 {
    static void run( Operator& op, Function& x, const Function& b, double omega, int level, int flag, int colour )
    {
-      Mesh&          M    = *op.mesh;
-      LevelPlan&     P    = M.plan( level );
-      int64_t        nseg = 0;
-      const CellSeg* sg   = M.segments( P, segLen< NR >(), &nseg );
-      auto&          Wt   = op.weights( level );
-      if ( nseg == 0 ) return;
-      hipLaunchKernelGGL( ( kCellSor< T, NR > ), dim3( (unsigned) xcdGrid( ( nseg + 3 ) / 4 ) ), dim3( 256 ), 0,
-                          M.ctx->stream, viewOf( P ), sg, nseg, wsel< T >( Wt.cell64, Wt.cell32 ), P.dCellType.p, flag,
-                          (T*) x.ptr( level ), (const T*) b.ptr( level ), (T) omega, colour );
+      Mesh&      M  = *op.mesh;
+      LevelPlan& P  = M.plan( level );
+      auto&      Wt = op.weights( level );
+      cellSorLaunch< T, NR >( M, P, wsel< T >( Wt.cell64, Wt.cell32 ), flag, (T*) x.ptr( level ), (const T*) b.ptr( level ),
+                              (T) omega, colour );
    }
 };
 } // namespace

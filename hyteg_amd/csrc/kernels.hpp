@@ -15,12 +15,16 @@ enum ApplyMode : int
 // copy owned values into the closure copies of higher-dimensional primitives: bit 0 V->E, bit 1 E->F,
 // bit 2 F->C (executed in that order)
 void launchRefresh( Function& f, int level, int mask );
-void launchApply( Operator& op, Function& src, Function& dst, const Function* rhs, int level, int flag, int mode );
-// one multicolour SOR sweep in the reference's V -> E -> F -> C order (refreshes included)
-void launchSor( Operator& op, Function& x, const Function& b, double omega, int level, int flag );
+// refreshSrc = false skips the ghost-copy refresh of src when the caller knows its copies are current
+void launchApply( Operator& op, Function& src, Function& dst, const Function* rhs, int level, int flag, int mode,
+                  bool refreshSrc = true );
+// one multicolour SOR sweep in the reference's V -> E -> F -> C order; refreshFirst = false skips the
+// initial ghost-copy refresh when the caller knows the copies of x are current (they are after a sweep)
+void launchSor( Operator& op, Function& x, const Function& b, double omega, int level, int flag,
+                bool refreshFirst = true );
 // coarse level of dst <- restriction of the fine level of f (dst may be f)
 void launchRestrict( Function& f, Function& dst, int srcLevel, int flag );
-void launchProlongateAdd( Function& f, int srcLevel, int flag );
+void launchProlongateAdd( Function& f, int srcLevel, int flag, bool refreshCoarse = true );
 // dst = sum_k c[k] * src[k] (+ dst if accumulate) + constant, on the closures of primitives matching flag
 void launchLincomb( Function& dst, int level, int flag, int n, const double* c, Function* const* src, bool accumulate,
                     double constant );

@@ -19,6 +19,14 @@ struct CellSeg
    int32_t cell, z, y, x0;
 };
 
+// Work unit of the row-marching macro-cell kernels: one wave owns x = x0-1 .. x0+62 of slab z of a cell
+// and marches over rows y0 .. y0 + rows - 1 (clipped to the slab; rows is chosen per level).
+struct MarchUnit
+{
+   int32_t cell, z, x0, y0;
+};
+constexpr int kMarchOutput = 62; // output lanes per wave (lanes 1..62)
+
 // contiguous element range
 struct Range
 {

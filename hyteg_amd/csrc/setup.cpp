@@ -230,6 +230,8 @@ void validatePlan( const Topology& T, const LevelPlan& P )
    }
    for ( int64_t idx : P.hVertexTerms )
       ok( idx >= P.offE && idx < P.offF, "vertex term" );
+   for ( int64_t idx : P.hVertexEnds )
+      ok( idx >= P.offE && idx < P.offF, "vertex end" );
 }
 
 } // namespace
@@ -378,6 +380,7 @@ void buildPlan( Mesh& M, LevelPlan& P, int level )
 
    P.hVertexGeom.clear();
    P.hVertexTerms.clear();
+   P.hVertexEnds.clear();
    for ( int v = 0; v < T.nV(); v++ )
    {
       VertexGeom g{};
@@ -385,12 +388,17 @@ void buildPlan( Mesh& M, LevelPlan& P, int level )
       g.type    = T.vType[v];
       g.termOff = (int) P.hVertexTerms.size();
       for ( int e : T.vEdges[v] )
-         P.hVertexTerms.push_back( P.edgeOff( e ) + ( T.ev[e][0] == v ? 1 : w - 2 ) );
+      {
+         const bool first = T.ev[e][0] == v;
+         P.hVertexTerms.push_back( P.edgeOff( e ) + ( first ? 1 : w - 2 ) );
+         P.hVertexEnds.push_back( P.edgeOff( e ) + ( first ? 0 : w - 1 ) );
+      }
       g.nTerms = (int) P.hVertexTerms.size() - g.termOff;
       P.hVertexGeom.push_back( g );
    }
    if ( dev ) P.dVertexGeom.upload( P.hVertexGeom, s );
    if ( dev ) P.dVertexTerms.upload( P.hVertexTerms, s );
+   if ( dev ) P.dVertexEnds.upload( P.hVertexEnds, s );
 
    validatePlan( T, P );
 
@@ -436,6 +444,31 @@ const CellSeg* Mesh::segments( LevelPlan& P, int len, int64_t* count )
    }
    *count = P.segCount[len];
    return it->second->p;
+}
+
+// Row-marching units: for every cell, interior slab z and x-window x0 (stride kMarchOutput), the rows that
+// still have interior points at x >= x0 are cut into chunks of kMarchRows.  Units of one cell and slab are
+// adjacent, so the XCD-aware block order keeps neighbouring slabs on one XCD.
+const MarchUnit* Mesh::marchUnits( LevelPlan& P, int64_t* count )
+{
+   if ( P.marchCount < 0 )
+   {
+      std::vector< MarchUnit > v;
+      const int                w = P.w;
+      // shorter chunks on small levels trade prologue loads for enough waves to fill the GPU
+      P.marchRows = P.L >= 7 ? 32 : ( P.L == 6 ? 8 : 4 );
+      for ( int c = 0; c < T.nC(); c++ )
+         for ( int z = 1; z <= w - 4; z++ )
+            for ( int x0 = 1; x0 <= w - 3 - z; x0 += kMarchOutput )
+               for ( int y0 = 1; y0 <= w - 2 - z - x0; y0 += P.marchRows )
+                  v.push_back( { c, z, x0, y0 } );
+      P.march = std::make_unique< DevBuf< MarchUnit > >();
+      P.march->upload( v, ctx->stream );
+      HYMI_HIP( hipStreamSynchronize( ctx->stream ) );
+      P.marchCount = (int64_t) v.size();
+   }
+   *count = P.marchCount;
+   return P.march->p;
 }
 
 // Closure ranges (owned part plus copies) of the primitives whose DoF type matches `flag`, cut into chunks.

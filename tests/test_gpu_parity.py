@@ -79,7 +79,7 @@ def test_mass_and_residual_fp64(kind, gpu_ctx, oracle_lib):
 
 
 @pytest.mark.parametrize("kind", MESH_CASES)
-@pytest.mark.parametrize("level", [2, 3])
+@pytest.mark.parametrize("level", [1, 2, 3, 5])
 def test_sor_multicolour_matches_oracle_schedule(kind, level, gpu_ctx, oracle_lib):
     import hyteg_amd as H
     O = oracle_lib
