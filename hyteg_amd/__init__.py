@@ -51,7 +51,7 @@ class Context:
         return v.value
 
     def __del__(self):
-        if getattr(self, "h", None) and _lib._lib is not None:
+        if getattr(self, "h", None) and _lib is not None and _lib._lib is not None:
             _lib._lib.hymi_ctx_destroy(self.h)
             self.h = None
 
@@ -103,7 +103,7 @@ class PrimitiveStorage:
         self.h = h
 
     def __del__(self):
-        if getattr(self, "h", None) and _lib._lib is not None:
+        if getattr(self, "h", None) and _lib is not None and _lib._lib is not None:
             _lib._lib.hymi_mesh_destroy(self.h)
             self.h = None
 
@@ -132,6 +132,15 @@ class PrimitiveStorage:
         _lib.call("hymi_mesh_owned_types", self.h, level, _ip(out))
         return out
 
+    def set_schedule(self, level, rows=0, waves=0, block=0):
+        """Launch schedule of the macro-cell row-marching kernels on `level` (0 = level default)."""
+        _lib.call("hymi_mesh_set_schedule", self.h, level, rows, waves, block)
+
+    def get_schedule(self, level):
+        out = np.zeros(3, np.int32)
+        _lib.call("hymi_mesh_get_schedule", self.h, level, _ip(out))
+        return {"rows": int(out[0]), "waves": int(out[1]), "block": int(out[2])}
+
 
 class P1Function:
     """P1Function<real_t>(name, storage, minLevel, maxLevel); precision F64/F32, nrhs interleaved RHS."""
@@ -145,7 +154,7 @@ class P1Function:
         self.h = h
 
     def __del__(self):
-        if getattr(self, "h", None) and _lib._lib is not None:
+        if getattr(self, "h", None) and _lib is not None and _lib._lib is not None:
             _lib._lib.hymi_function_destroy(self.h)
             self.h = None
 
@@ -171,6 +180,16 @@ class P1Function:
     def upload_primitive(self, level, cls, idx, values):
         v = np.ascontiguousarray(values, np.float64)
         _lib.call("hymi_function_upload_primitive", self.h, level, cls, idx, _dp(v))
+
+    def interpolate_modes(self, modes, level, flag=ALL):
+        """interpolate(sum of Fourier modes) on the device; modes: (nModes, 4) or (nrhs, nModes, 4) rows of
+        (amplitude, k, l, m) for a * sin(k pi x) sin(l pi y) sin(m pi z)."""
+        m = np.ascontiguousarray(modes, dtype=np.float64)
+        if m.ndim == 2:
+            m = np.broadcast_to(m, (self.nrhs,) + m.shape).copy()
+        if m.shape[0] != self.nrhs or m.shape[2] != 4:
+            raise ValueError("modes must have shape (nModes, 4) or (nrhs, nModes, 4)")
+        _lib.call("hymi_interpolate_modes", self.h, level, flag, m.shape[1], _dp(m))
 
     def interpolate(self, value, level, flag=ALL):
         """interpolate(constant | callable(xyz)->values, level, flag)."""
@@ -220,7 +239,7 @@ class _P1ConstantOperator:
         self.h = h
 
     def __del__(self):
-        if getattr(self, "h", None) and _lib._lib is not None:
+        if getattr(self, "h", None) and _lib is not None and _lib._lib is not None:
             _lib._lib.hymi_operator_destroy(self.h)
             self.h = None
 
@@ -314,8 +333,42 @@ class GeometricMultigridSolver:
         _lib.call("hymi_gmg_vcycle_graph", self._handle(A, x), x.h, b.h, level, count)
 
     def __del__(self):
-        if _lib._lib is None:
+        if _lib is None or _lib._lib is None:
             return
         for h, _ in getattr(self, "_h", {}).values():
             _lib._lib.hymi_gmg_destroy(h)
         self._h = {}
+
+
+class MixedPrecisionRefinementSolver:
+    """Iterative refinement: residual, convergence check and update in the precision of x (FP64), the
+    correction from `cyclesPerOuter` V-cycles of `innerSolver` run in `innerPrecision` (FP32).
+
+    solve(A, x, b, level) iterates until max_rhs ||b - A x|| / ||b - A x_0|| <= tolerance or maxOuter
+    correction steps; it returns the final relative residual per right-hand side and records
+    `outerSteps` and `history` (relative residual after 0, 1, ... steps)."""
+
+    def __init__(self, storage, innerSolver, tolerance=1e-10, maxOuter=50, cyclesPerOuter=2, innerPrecision=F32):
+        self.storage, self.inner = storage, innerSolver
+        self.tolerance, self.maxOuter, self.cyclesPerOuter = tolerance, maxOuter, cyclesPerOuter
+        self.innerPrecision = innerPrecision
+        self._templates = {}
+        self.outerSteps, self.history = 0, []
+
+    def _innerHandle(self, A, nrhs):
+        if nrhs not in self._templates:
+            # the solver takes only precision and nrhs from its template; one coarse level suffices
+            self._templates[nrhs] = P1Function("refinement-template", self.storage, self.inner.minLevel,
+                                               self.inner.minLevel, precision=self.innerPrecision, nrhs=nrhs)
+        return self.inner._handle(A, self._templates[nrhs])
+
+    def solve(self, A, x, b, level):
+        h = self._innerHandle(A, x.nrhs)
+        outer = ctypes.c_int()
+        rel = np.zeros(x.nrhs)
+        hist = np.zeros(self.maxOuter + 1)
+        _lib.call("hymi_solve_refined", h, x.h, b.h, level, float(self.tolerance), int(self.maxOuter),
+                  int(self.cyclesPerOuter), ctypes.byref(outer), _dp(rel), _dp(hist))
+        self.outerSteps = outer.value
+        self.history = hist[: outer.value + 1].tolist()
+        return rel

@@ -254,19 +254,68 @@ struct hymi_gmg_s
    int                         cgIters = 0;
    std::unique_ptr< Function > tmp;
    CgWork                      cg;
-   std::map< std::tuple< void*, void*, int >, hipGraphExec_t > graphs;
+   struct Captured
+   {
+      hipGraphExec_t exec  = nullptr;
+      uint64_t       epoch = 0; // mesh schedule epoch at capture
+   };
+   std::map< std::tuple< void*, void*, int >, Captured > graphs;
    ~hymi_gmg_s()
    {
       for ( auto& kv : graphs )
-         (void) hipGraphExecDestroy( kv.second );
+         (void) hipGraphExecDestroy( kv.second.exec );
    }
 
    int maxL_invoked = -1;
+
+   // iterative refinement work: residual in the outer precision, right-hand side and correction in the
+   // solver's own precision (allocated on first use, kept for later solves)
+   std::unique_ptr< Function > refR, refRc, refEc;
+   int                         refPrec = -1;
 
    void cycle( Function& x, Function& b, int L )
    {
       maxL_invoked = L;
       recurse( x, b, L );
+   }
+
+   // `count` V-cycles for (x, b, L) from a hipGraph captured on first use; a graph captured under an older
+   // schedule epoch (the work-unit tables it points to were replaced) is captured again
+   void replay( Function& x, Function& b, int L, int count )
+   {
+      hipStream_t    s     = A->mesh->ctx->stream;
+      auto           key   = std::make_tuple( (void*) &x, (void*) &b, L );
+      const uint64_t epoch = A->mesh->scheduleEpoch;
+      auto           it    = graphs.find( key );
+      if ( it != graphs.end() && it->second.epoch != epoch )
+      {
+         HYMI_HIP( hipStreamSynchronize( s ) );
+         (void) hipGraphExecDestroy( it->second.exec );
+         graphs.erase( it );
+         it = graphs.end();
+      }
+      if ( it == graphs.end() )
+      {
+         hipGraph_t graph = nullptr;
+         HYMI_HIP( hipStreamBeginCapture( s, hipStreamCaptureModeThreadLocal ) );
+         try
+         {
+            cycle( x, b, L );
+         } catch ( ... )
+         {
+            hipGraph_t dummy = nullptr;
+            (void) hipStreamEndCapture( s, &dummy );
+            if ( dummy ) (void) hipGraphDestroy( dummy );
+            throw;
+         }
+         HYMI_HIP( hipStreamEndCapture( s, &graph ) );
+         hipGraphExec_t exec = nullptr;
+         HYMI_HIP( hipGraphInstantiate( &exec, graph, nullptr, nullptr, 0 ) );
+         (void) hipGraphDestroy( graph );
+         it = graphs.emplace( key, Captured{ exec, epoch } ).first;
+      }
+      for ( int k = 0; k < count; k++ )
+         HYMI_HIP( hipGraphLaunch( it->second.exec, s ) );
    }
 
    void recurse( Function& x, Function& b, int L )
@@ -497,6 +546,30 @@ int hymi_mesh_owned_coords( hymi_mesh mesh, int level, double* xyz )
                   put( q[0], q[1], q[2] );
                }
       }
+   } );
+}
+
+int hymi_mesh_set_schedule( hymi_mesh mesh, int level, int rows, int waves, int block )
+{
+   return guarded( [&] {
+      require( mesh && level >= 0 && level <= 12, "bad argument" );
+      MarchSchedule S;
+      S.rows  = rows;
+      S.waves = waves;
+      S.block = block;
+      mesh->m.setSchedule( level, S );
+   } );
+}
+
+int hymi_mesh_get_schedule( hymi_mesh mesh, int level, int out[3] )
+{
+   return guarded( [&] {
+      require( mesh && out && level >= 0 && level <= 12, "bad argument" );
+      LevelPlan&    P = mesh->m.plan( level );
+      MarchSchedule S = P.sched.rows ? P.sched : defaultSchedule( level );
+      out[0]          = S.rows;
+      out[1]          = S.waves;
+      out[2]          = S.block;
    } );
 }
 
@@ -785,6 +858,25 @@ int hymi_dot( hymi_function a, hymi_function b, int level, int flag, double* out
    } );
 }
 
+int hymi_interpolate_modes( hymi_function f, int level, int flag, int n_modes, const double* modes )
+{
+   return guarded( [&] {
+      require( f && ( modes || n_modes == 0 ) && n_modes >= 0, "bad argument" );
+      requireLevel( f->f, level );
+      Context&     ctx = *f->f.mesh->ctx;
+      const size_t n   = (size_t) f->f.nrhs * (size_t) n_modes * 4;
+      if ( ctx.modes.n < n ) ctx.modes.alloc( n );
+      if ( n > 0 )
+      {
+         // ordered after earlier readers of the staging table on this stream; the host table may be released
+         // when the call returns
+         HYMI_HIP( hipMemcpyAsync( ctx.modes.p, modes, n * sizeof( double ), hipMemcpyHostToDevice, ctx.stream ) );
+         HYMI_HIP( hipStreamSynchronize( ctx.stream ) );
+      }
+      launchInterpolateModes( f->f, level, flag, n_modes, ctx.modes.p );
+   } );
+}
+
 int hymi_convert( hymi_function dst, hymi_function src, int level, int flag )
 {
    return guarded( [&] {
@@ -1034,31 +1126,69 @@ int hymi_gmg_vcycle_graph( hymi_gmg g, hymi_function x, hymi_function b, int lev
    return guarded( [&] {
       checkGmg( g, x, b, level );
       require( count >= 0, "negative count" );
-      hipStream_t s   = g->A->mesh->ctx->stream;
-      auto        key = std::make_tuple( (void*) x, (void*) b, level );
-      auto        it  = g->graphs.find( key );
-      if ( it == g->graphs.end() )
+      g->replay( x->f, b->f, level, count );
+   } );
+}
+
+int hymi_solve_refined( hymi_gmg g, hymi_function x, hymi_function b, int level, double tol, int maxOuter,
+                        int cyclesPerOuter, int* outerDone, double* relRes, double* history )
+{
+   return guarded( [&] {
+      require( g && x && b && outerDone && relRes, "null argument" );
+      require( level >= g->minL && level <= g->maxL, "level outside the solver's range" );
+      require( x->f.mesh == g->A->mesh && b->f.mesh == g->A->mesh, "functions live on another mesh" );
+      require( x->f.prec == b->f.prec, "x and b must share a precision" );
+      require( x->f.nrhs == g->nrhs && b->f.nrhs == g->nrhs, "nrhs differs from the solver's" );
+      requireLevel( x->f, level );
+      requireLevel( b->f, level );
+      require( tol > 0.0 && maxOuter >= 0 && cyclesPerOuter >= 1, "bad tolerance or iteration counts" );
+      Mesh&     M    = *g->A->mesh;
+      Context&  ctx  = *M.ctx;
+      const int nr   = g->nrhs;
+      const int flag = kInner | kNeumann;
+      if ( !g->refR || g->refPrec != x->f.prec )
       {
-         hipGraph_t graph = nullptr;
-         HYMI_HIP( hipStreamBeginCapture( s, hipStreamCaptureModeThreadLocal ) );
-         try
-         {
-            g->cycle( x->f, b->f, level );
-         } catch ( ... )
-         {
-            hipGraph_t dummy = nullptr;
-            (void) hipStreamEndCapture( s, &dummy );
-            if ( dummy ) (void) hipGraphDestroy( dummy );
-            throw;
-         }
-         HYMI_HIP( hipStreamEndCapture( s, &graph ) );
-         hipGraphExec_t exec = nullptr;
-         HYMI_HIP( hipGraphInstantiate( &exec, graph, nullptr, nullptr, 0 ) );
-         (void) hipGraphDestroy( graph );
-         it = g->graphs.emplace( key, exec ).first;
+         g->refR    = makeFunction( M, g->minL, g->maxL, x->f.prec, nr );
+         g->refRc   = makeFunction( M, g->minL, g->maxL, g->prec, nr );
+         g->refEc   = makeFunction( M, g->minL, g->maxL, g->prec, nr );
+         g->refPrec = x->f.prec;
       }
-      for ( int k = 0; k < count; k++ )
-         HYMI_HIP( hipGraphLaunch( it->second, s ) );
+      std::vector< double > r0( nr ), rn( nr );
+      // r = b - A x in the outer precision, and ||r|| per right-hand side
+      auto residualNorms = [&]( std::vector< double >& out ) {
+         launchApply( *g->A, x->f, *g->refR, &b->f, level, flag, kResidual, true );
+         launchDot( *g->refR, *g->refR, level, flag, sUser );
+         HYMI_HIP( hipMemcpyAsync( out.data(), ctx.scalars.p + sUser * nr, sizeof( double ) * nr, hipMemcpyDeviceToHost,
+                                   ctx.stream ) );
+         HYMI_HIP( hipStreamSynchronize( ctx.stream ) );
+         for ( double& v : out )
+            v = std::sqrt( std::max( v, 0.0 ) );
+      };
+      auto worst = [&]() {
+         double m = 0.0;
+         for ( int r = 0; r < nr; r++ )
+            m = std::max( m, r0[r] > 0.0 ? rn[r] / r0[r] : 0.0 );
+         return m;
+      };
+      residualNorms( r0 );
+      rn         = r0;
+      double rel = worst();
+      if ( history ) history[0] = rel;
+      int k = 0;
+      for ( ; k < maxOuter && rel > tol; k++ )
+      {
+         // correction equation A e = r in the solver's precision, e from zero
+         launchConvert( *g->refRc, level, *g->refR, level, kAll );
+         launchLincomb( *g->refEc, level, kAll, 0, nullptr, nullptr, false, 0.0 );
+         g->replay( *g->refEc, *g->refRc, level, cyclesPerOuter );
+         launchAddFrom( x->f, *g->refEc, level, flag );
+         residualNorms( rn );
+         rel = worst();
+         if ( history ) history[k + 1] = rel;
+      }
+      *outerDone = k;
+      for ( int r = 0; r < nr; r++ )
+         relRes[r] = r0[r] > 0.0 ? rn[r] / r0[r] : 0.0;
    } );
 }
 

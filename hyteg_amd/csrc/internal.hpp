@@ -66,6 +66,7 @@ struct Context
    // reduction scratch: block partials and scalar slots
    DevBuf< double > partials;
    DevBuf< double > scalars;
+   DevBuf< double > modes; // staging of per-call coefficient tables (interpolation of Fourier modes)
    size_t           bytesInUse = 0;
 };
 
@@ -100,10 +101,11 @@ struct LevelPlan
    // cell row segments, keyed by segment length (in points)
    std::map< int, std::unique_ptr< DevBuf< CellSeg > > > segs;
    std::map< int, int64_t >                              segCount;
-   // work units of the row-marching macro-cell kernels
+   // work units of the row-marching macro-cell kernels, built for sched.rows
    std::unique_ptr< DevBuf< MarchUnit > > march;
    int64_t                                marchCount = -1;
-   int                                    marchRows  = 32;
+   int                                    marchRowsBuilt = 0;
+   MarchSchedule                          sched;
    // closure ranges of primitives matching a flag, keyed by flag
    std::map< int, std::unique_ptr< DevBuf< Range > > > ranges;
    std::map< int, int64_t >                            rangeCount;
@@ -122,6 +124,14 @@ struct Mesh
    LevelPlan& plan( int level );
    const CellSeg* segments( LevelPlan& P, int len, int64_t* count );
    const MarchUnit* marchUnits( LevelPlan& P, int64_t* count );
+   // replaces the launch schedule of the row-marching kernels on one level (zeros = level default);
+   // bumps scheduleEpoch so that captured graphs holding the old work-unit table are captured again
+   void     setSchedule( int level, const MarchSchedule& S );
+   uint64_t scheduleEpoch = 0;
+   // affine frames of the primitives, level-independent: micro-vertex (x, y, z) of a primitive lies at
+   // o + (x a + y b + z c) / (w - 1); 12 doubles (o, a, b, c) per primitive, vertices, edges, faces, cells
+   const double* frames();
+   DevBuf< double > dFrames;
    const Range*   closureRanges( LevelPlan& P, int flag, int64_t* count );
 };
 
@@ -159,5 +169,6 @@ struct Operator
 // setup (setup.cpp)
 void buildPlan( Mesh& M, LevelPlan& P, int level );
 void buildOperatorWeights( Operator& op, int level );
+MarchSchedule defaultSchedule( int level );
 
 } // namespace hymi

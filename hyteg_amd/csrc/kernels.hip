@@ -230,33 +230,9 @@ __global__ __launch_bounds__( 256 ) void kCellSor( LV g, const CellSeg* __restri
 // macro-cell interior, row-marching form (one right-hand side)
 //
 // One wave owns the window x = x0-1 .. x0+62 of slab z of one macro-cell and marches along y over a chunk of
-// rows.  Each step loads one new row of slab z, of slab z+1 and of slab z-1; the x-1 / x+1 neighbours come
-// from the adjacent lanes (ds_bpermute shuffles), the other rows from registers rotated from earlier steps.
-// Lanes 1..62 produce outputs.  Every row value is read from memory once per wave instead of up to five
-// times, and all per-row index arithmetic is scalar.  Loads are issued two steps ahead of their use and
-// shuffled one step ahead, so a step never waits on the load it just issued.
-
-template < typename T >
-struct Row3
-{
-   T l, c, r; // values at x-1, x, x+1
-};
-
-template < typename T >
-__device__ __forceinline__ T loadRaw( const T* __restrict__ u, int64_t rowBase, int x, int rowEnd )
-{
-   return ( x <= rowEnd ) ? u[rowBase + x] : T( 0 );
-}
-
-template < typename T >
-__device__ __forceinline__ Row3< T > spread( T v )
-{
-   Row3< T > R;
-   R.c = v;
-   R.l = __shfl_up( v, 1, 64 );
-   R.r = __shfl_down( v, 1, 64 );
-   return R;
-}
+// rows.  Each step brings in one new row of slab z, of slab z+1 and of slab z-1; the other rows of the
+// 15-point stencil are registers kept from earlier steps.  Lanes 1..62 produce outputs.  Every row value is
+// read from memory once per wave instead of up to five times, and all per-row index arithmetic is scalar.
 
 constexpr int kMarchSor = 3; // MODE value of the in-place multicolour SOR pass
 // finest level whose faces are small enough for one workgroup per face in the fused SOR phase
@@ -264,105 +240,148 @@ constexpr int kFaceFusedMaxLevel = 4;
 
 // MODE kReplace / kAdd / kResidual: y = A u, y += A u, y = b - A u.
 // MODE kMarchSor: in-place SOR update of the points of `colour` (y == u).
-template < typename T, int MODE >
-__global__ __launch_bounds__( 64 ) void kCellMarch( LV g, const MarchUnit* __restrict__ units, int64_t nUnits,
-                                                    const T* __restrict__ W, const int* __restrict__ ctype, int flag,
-                                                    const T* u, T* y, const T* b, T omega, int colour, int rows )
+//
+// A wave walks its rows in blocks of R output rows.  Each block issues the loads of all 3R+4 rows it needs
+// at once (all independent, so they are in flight together), then computes the R output rows from
+// registers; the x-1 / x+1 neighbours come from DPP wave shifts (lane i reads lane i-1 / i+1, one VALU
+// move per 32-bit half).  No data register lives across blocks, so no step waits on a load issued for a
+// later one; the four rows shared by consecutive blocks are read again from L1/L2.  A workgroup holds
+// blockDim.x / 64 waves, each with its own work unit.
+template < typename T >
+__device__ __forceinline__ T laneDown( T v ) // lane i <- lane i-1 (lane 0 gets 0)
 {
-   const int64_t ui = xcdBlock( nUnits );
+   if constexpr ( sizeof( T ) == 8 )
+   {
+      int2 a = __builtin_bit_cast( int2, v );
+      a.x    = __builtin_amdgcn_mov_dpp( a.x, 0x138, 0xf, 0xf, true );
+      a.y    = __builtin_amdgcn_mov_dpp( a.y, 0x138, 0xf, 0xf, true );
+      return __builtin_bit_cast( T, a );
+   }
+   else
+      return __builtin_bit_cast( T, __builtin_amdgcn_mov_dpp( __builtin_bit_cast( int, v ), 0x138, 0xf, 0xf, true ) );
+}
+
+template < typename T >
+__device__ __forceinline__ T laneUp( T v ) // lane i <- lane i+1 (lane 63 gets 0)
+{
+   if constexpr ( sizeof( T ) == 8 )
+   {
+      int2 a = __builtin_bit_cast( int2, v );
+      a.x    = __builtin_amdgcn_mov_dpp( a.x, 0x130, 0xf, 0xf, true );
+      a.y    = __builtin_amdgcn_mov_dpp( a.y, 0x130, 0xf, 0xf, true );
+      return __builtin_bit_cast( T, a );
+   }
+   else
+      return __builtin_bit_cast( T, __builtin_amdgcn_mov_dpp( __builtin_bit_cast( int, v ), 0x130, 0xf, 0xf, true ) );
+}
+
+// Value at x of the row of cell array `cu` that starts at offset rowOff and ends at rowEnd (both
+// wave-uniform), read without a branch: lanes past the end read the row's last point, and a row that does not
+// exist (rowEnd < 0) reads the cell's first point.  Such values only reach lanes that produce no output.
+template < typename T >
+__device__ __forceinline__ T rowLoad( const T* __restrict__ cu, int rowOff, int x, int rowEnd )
+{
+   const bool has = rowEnd >= 0;
+   return cu[( has ? rowOff : 0 ) + min( x, has ? rowEnd : 0 )];
+}
+
+template < typename T, int MODE, int R >
+__global__ __launch_bounds__( 256 ) void kCellMarch( LV g, const MarchUnit* __restrict__ units, int64_t nUnits,
+                                                     const T* __restrict__ W, const int* __restrict__ ctype, int flag,
+                                                     const T* u, T* y, const T* b, T omega, int colour, int rows )
+{
+   const int     wpb  = blockDim.x >> 6;
+   const int     wave = __builtin_amdgcn_readfirstlane( threadIdx.x >> 6 ); // wave-uniform: unit data stays scalar
+   const int64_t ui   = xcdBlock( ( nUnits + wpb - 1 ) / wpb ) * wpb + wave;
    if ( ui >= nUnits ) return;
    const MarchUnit U = units[ui];
    if ( !( ctype[U.cell] & flag ) ) return;
-   const int     lane = threadIdx.x;
+   const int     lane = threadIdx.x & 63;
    const int     w = g.w, z = U.z;
    const int     x    = U.x0 - 1 + lane;
    const int64_t base = g.offC + (int64_t) U.cell * g.cS;
-   const int64_t t0   = tet( w );
+   const T*      cu   = u + base; // this cell's array; row offsets below are 32-bit (cells hold < 2^31 points)
+   T*            cy   = y + base;
+   const T*      cb   = b ? b + base : nullptr;
+   const int     t0   = (int) tet( w );
    const T*      c    = W + (int64_t) U.cell * 15;
    const T cC = c[dC], cW = c[dW], cE = c[dE], cN = c[dN], cS = c[dS], cNW = c[dNW], cSE = c[dSE];
    const T cTC = c[dTC], cTW = c[dTW], cTS = c[dTS], cTSE = c[dTSE], cBC = c[dBC], cBN = c[dBN], cBE = c[dBE],
            cBNW = c[dBNW];
    const T   relax = omega / cC;
    const int yLast = min( U.y0 + rows - 1, w - 2 - z - U.x0 );
-   int       yy    = U.y0;
-   // rows held for step yy: slab z rows yy-1, yy, yy+1; slab z+1 rows yy-1, yy; slab z-1 rows yy, yy+1
-   int64_t       rz0 = base + crow( w, t0, yy, z );
-   const int64_t rzm = base + crow( w, t0, yy - 1, z ), rzp = base + crow( w, t0, yy + 1, z );
-   const int64_t rtm = base + crow( w, t0, yy - 1, z + 1 ), rt0 = base + crow( w, t0, yy, z + 1 );
-   const int64_t rb0 = base + crow( w, t0, yy, z - 1 ), rbp = base + crow( w, t0, yy + 1, z - 1 );
-   Row3< T >     Zm = spread( loadRaw( u, rzm, x, w - 1 - ( yy - 1 ) - z ) );
-   Row3< T >     Z0 = spread( loadRaw( u, rz0, x, w - 1 - yy - z ) );
-   Row3< T >     Zp = spread( loadRaw( u, rzp, x, w - 1 - ( yy + 1 ) - z ) );
-   Row3< T >     Tm = spread( loadRaw( u, rtm, x, w - 1 - ( yy - 1 ) - ( z + 1 ) ) );
-   Row3< T >     T0 = spread( loadRaw( u, rt0, x, w - 1 - yy - ( z + 1 ) ) );
-   Row3< T >     B0 = spread( loadRaw( u, rb0, x, w - 1 - yy - ( z - 1 ) ) );
-   Row3< T >     Bp = spread( loadRaw( u, rbp, x, w - 1 - ( yy + 1 ) - ( z - 1 ) ) );
-   // raw rows for step yy+1: slab z row yy+2, slab z+1 row yy+1, slab z-1 row yy+2
-   int64_t nz = rzp + ( w - ( yy + 1 ) - z );
-   int64_t nt = rt0 + ( w - yy - ( z + 1 ) );
-   int64_t nb = rbp + ( w - ( yy + 1 ) - ( z - 1 ) );
-   T       zr = loadRaw( u, nz, x, w - 1 - ( yy + 2 ) - z );
-   T       tr = loadRaw( u, nt, x, w - 1 - ( yy + 1 ) - ( z + 1 ) );
-   T       br = loadRaw( u, nb, x, w - 1 - ( yy + 2 ) - ( z - 1 ) );
-   for ( ; yy <= yLast; yy++ )
+   for ( int y0 = U.y0; y0 <= yLast; y0 += R )
    {
-      // raw rows for step yy+2
-      const int64_t nz2 = nz + ( w - ( yy + 2 ) - z );
-      const int64_t nt2 = nt + ( w - ( yy + 1 ) - ( z + 1 ) );
-      const int64_t nb2 = nb + ( w - ( yy + 2 ) - ( z - 1 ) );
-      const T       zr2 = loadRaw( u, nz2, x, w - 1 - ( yy + 3 ) - z );
-      const T       tr2 = loadRaw( u, nt2, x, w - 1 - ( yy + 2 ) - ( z + 1 ) );
-      const T       br2 = loadRaw( u, nb2, x, w - 1 - ( yy + 3 ) - ( z - 1 ) );
-
-      const bool out = lane >= 1 && lane <= 62 && x <= w - 2 - yy - z;
-      if ( MODE == kMarchSor )
+      // all rows of the block, issued together: slab z rows y0-1 .. y0+R, slab z+1 rows y0-1 .. y0+R-1,
+      // slab z-1 rows y0 .. y0+R; row (yy, zz) holds x = 0 .. w-1-yy-zz and the next row follows it
+      T   Zr[R + 2], Tr[R + 1], Br[R + 1];
+      int pz = (int) crow( w, t0, y0 - 1, z ), pt = (int) crow( w, t0, y0 - 1, z + 1 ), pb = (int) crow( w, t0, y0, z - 1 );
+#pragma unroll
+      for ( int k = 0; k < R + 2; k++ )
       {
-         if ( out && ( ( x + 2 * yy + 3 * z ) & 3 ) == colour )
+         const int yy = y0 - 1 + k;
+         Zr[k]        = rowLoad( cu, pz, x, w - 1 - yy - z );
+         pz += w - yy - z;
+      }
+#pragma unroll
+      for ( int k = 0; k < R + 1; k++ )
+      {
+         const int yy = y0 - 1 + k;
+         Tr[k]        = rowLoad( cu, pt, x, w - 1 - yy - ( z + 1 ) );
+         pt += w - yy - ( z + 1 );
+      }
+#pragma unroll
+      for ( int k = 0; k < R + 1; k++ )
+      {
+         const int yy = y0 + k;
+         Br[k]        = rowLoad( cu, pb, x, w - 1 - yy - ( z - 1 ) );
+         pb += w - yy - ( z - 1 );
+      }
+      int rz0 = (int) crow( w, t0, y0, z );
+#pragma unroll
+      for ( int k = 0; k < R; k++ )
+      {
+         const int yy = y0 + k;
+         if ( yy > yLast ) break;
+         const T    Zm = Zr[k], Z0 = Zr[k + 1], Zp = Zr[k + 2], Tm = Tr[k], T0 = Tr[k + 1], B0 = Br[k], Bp = Br[k + 1];
+         const T    Z0l = laneDown( Z0 ), Z0r = laneUp( Z0 ), Zpl = laneDown( Zp ), Zmr = laneUp( Zm );
+         const T    T0l = laneDown( T0 ), Tmr = laneUp( Tm ), B0r = laneUp( B0 ), Bpl = laneDown( Bp );
+         const bool out = lane >= 1 && lane <= 62 && x <= w - 2 - yy - z;
+         if ( MODE == kMarchSor )
          {
-            T acc = cW * Z0.l + cE * Z0.r;
-            acc += cN * Zp.c + cNW * Zp.l;
-            acc += cS * Zm.c + cSE * Zm.r;
-            acc += cTC * T0.c + cTW * T0.l;
-            acc += cTS * Tm.c + cTSE * Tm.r;
-            acc += cBC * B0.c + cBE * B0.r;
-            acc += cBN * Bp.c + cBNW * Bp.l;
-            const int64_t o = rz0 + x;
-            y[o]            = ( T( 1 ) - omega ) * Z0.c + relax * ( b[o] - acc );
+            if ( out && ( ( x + 2 * yy + 3 * z ) & 3 ) == colour )
+            {
+               T acc = cW * Z0l + cE * Z0r;
+               acc += cN * Zp + cNW * Zpl;
+               acc += cS * Zm + cSE * Zmr;
+               acc += cTC * T0 + cTW * T0l;
+               acc += cTS * Tm + cTSE * Tmr;
+               acc += cBC * B0 + cBE * B0r;
+               acc += cBN * Bp + cBNW * Bpl;
+               const int o = rz0 + x;
+               cy[o]       = ( T( 1 ) - omega ) * Z0 + relax * ( cb[o] - acc );
+            }
          }
+         else if ( out )
+         {
+            T acc = cC * Z0;
+            acc += cW * Z0l + cE * Z0r;
+            acc += cN * Zp + cNW * Zpl;
+            acc += cS * Zm + cSE * Zmr;
+            acc += cTC * T0 + cTW * T0l;
+            acc += cTS * Tm + cTSE * Tmr;
+            acc += cBC * B0 + cBE * B0r;
+            acc += cBN * Bp + cBNW * Bpl;
+            const int o = rz0 + x;
+            if ( MODE == kReplace )
+               cy[o] = acc;
+            else if ( MODE == kAdd )
+               cy[o] += acc;
+            else
+               cy[o] = cb[o] - acc;
+         }
+         rz0 += w - yy - z;
       }
-      else if ( out )
-      {
-         T acc = cC * Z0.c;
-         acc += cW * Z0.l + cE * Z0.r;
-         acc += cN * Zp.c + cNW * Zp.l;
-         acc += cS * Zm.c + cSE * Zm.r;
-         acc += cTC * T0.c + cTW * T0.l;
-         acc += cTS * Tm.c + cTSE * Tm.r;
-         acc += cBC * B0.c + cBE * B0.r;
-         acc += cBN * Bp.c + cBNW * Bp.l;
-         const int64_t o = rz0 + x;
-         if ( MODE == kReplace )
-            y[o] = acc;
-         else if ( MODE == kAdd )
-            y[o] += acc;
-         else
-            y[o] = b[o] - acc;
-      }
-      // advance to step yy+1
-      Zm = Z0;
-      Z0 = Zp;
-      Zp = spread( zr );
-      Tm = T0;
-      T0 = spread( tr );
-      B0 = Bp;
-      Bp = spread( br );
-      zr = zr2;
-      tr = tr2;
-      br = br2;
-      rz0 += w - yy - z;
-      nz = nz2;
-      nt = nt2;
-      nb = nb2;
    }
 }
 
@@ -648,6 +667,80 @@ __global__ __launch_bounds__( 256 ) void kFaceSorFused( LV g, const FaceGeom* __
 }
 
 // --------------------------------------------------------------------------------------------------------
+// interpolation of a sum of Fourier modes at the micro-vertices (right-hand sides built on the device)
+
+template < typename T, int NR >
+__device__ __forceinline__ void putModes( T* __restrict__ u, int64_t idx, const double* __restrict__ fr, double h,
+                                          int x, int y, int z, const double* __restrict__ modes, int nModes )
+{
+   double p[3];
+#pragma unroll
+   for ( int d = 0; d < 3; d++ )
+      p[d] = fr[d] + h * ( x * fr[3 + d] + y * fr[6 + d] + z * fr[9 + d] );
+   const double pi = 3.141592653589793;
+   for ( int r = 0; r < NR; r++ )
+   {
+      const double* m   = modes + (int64_t) r * nModes * 4;
+      double        acc = 0.0;
+      for ( int k = 0; k < nModes; k++ )
+         acc += m[4 * k] * sin( m[4 * k + 1] * pi * p[0] ) * sin( m[4 * k + 2] * pi * p[1] ) *
+                sin( m[4 * k + 3] * pi * p[2] );
+      u[idx * NR + r] = (T) acc;
+   }
+}
+
+template < typename T, int NR >
+__global__ void kModesVertex( LV g, const VertexGeom* __restrict__ vg, int nV, int flag, const double* __restrict__ fr,
+                              const double* __restrict__ modes, int nModes, T* __restrict__ u )
+{
+   const int v = blockIdx.x * blockDim.x + threadIdx.x;
+   if ( v >= nV || !( vg[v].type & flag ) ) return;
+   putModes< T, NR >( u, v, fr + 12 * (int64_t) v, 0.0, 0, 0, 0, modes, nModes );
+}
+
+template < typename T, int NR >
+__global__ void kModesEdge( LV g, const EdgeGeom* __restrict__ eg, int nV, int flag, const double* __restrict__ fr,
+                            const double* __restrict__ modes, int nModes, T* __restrict__ u )
+{
+   const EdgeGeom& G = eg[blockIdx.y];
+   const int       x = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+   if ( x > g.w - 2 || !( G.type & flag ) ) return;
+   putModes< T, NR >( u, g.offE + (int64_t) G.edge * g.eS + x, fr + 12 * ( (int64_t) nV + G.edge ), 1.0 / ( g.w - 1 ), x,
+                      0, 0, modes, nModes );
+}
+
+template < typename T, int NR >
+__global__ void kModesFace( LV g, const FaceGeom* __restrict__ fg, int frameOff, int flag, const double* __restrict__ fr,
+                            const double* __restrict__ modes, int nModes, T* __restrict__ u )
+{
+   const FaceGeom& G  = fg[blockIdx.y];
+   const int       m  = g.w - 3;
+   const int64_t   np = (int64_t) m * ( m + 1 ) / 2;
+   const int64_t   k  = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+   if ( k >= np || !( G.type & flag ) ) return;
+   int a, bb;
+   triPos( m, k, a, bb );
+   const int i = a + 1, j = bb + 1;
+   putModes< T, NR >( u, g.offF + (int64_t) G.face * g.fS + fidx( g.w, i, j ), fr + 12 * ( (int64_t) frameOff + G.face ),
+                      1.0 / ( g.w - 1 ), i, j, 0, modes, nModes );
+}
+
+template < typename T, int NR >
+__global__ void kModesCell( LV g, const CellSeg* __restrict__ segs, int64_t nseg, const int* __restrict__ ctype,
+                            int frameOff, int flag, const double* __restrict__ fr, const double* __restrict__ modes,
+                            int nModes, T* __restrict__ u )
+{
+   const int64_t s = blockIdx.x * (int64_t) ( blockDim.x >> 6 ) + ( threadIdx.x >> 6 );
+   if ( s >= nseg ) return;
+   const CellSeg sg = segs[s];
+   const int     x  = sg.x0 + ( threadIdx.x & 63 );
+   if ( x > g.w - 2 - sg.y - sg.z || !( ctype[sg.cell] & flag ) ) return;
+   putModes< T, NR >( u, g.offC + (int64_t) sg.cell * g.cS + cidx( g.w, x, sg.y, sg.z ),
+                      fr + 12 * ( (int64_t) frameOff + sg.cell ), 1.0 / ( g.w - 1 ), x, sg.y, sg.z, modes, nModes );
+}
+
+
+// --------------------------------------------------------------------------------------------------------
 // restriction (coarse owned point <- fine neighbourhood of 2p, weights 1 and 1/2)
 
 template < typename T, int NR >
@@ -770,23 +863,16 @@ __global__ void kRestrictVertex( const VertexGeom* __restrict__ vgFine, const in
 // --------------------------------------------------------------------------------------------------------
 // prolongation-and-add (fine owned point += linear interpolation of the coarse closure)
 
-template < typename T, int NR >
-__global__ __launch_bounds__( 256 ) void kProlongCell( LV gc, LV gf, const CellSeg* __restrict__ segs, int64_t nseg,
-                                                       const int* __restrict__ ctype, int flag,
-                                                       const T* __restrict__ uc, T* __restrict__ uf )
+// Fine point p gets the coarse value at p/2 if p is all-even, else the mean of the coarse values at
+// (p - d)/2 and (p + d)/2, where d is the micro-edge direction given by the parity of p (E, N, SE, TC, TW,
+// TS, TSE).  Within one fine row (y, z) the parities of y and z are fixed, so the coarse rows depend only
+// on the parity of x: four wave-uniform row offsets cover every lane.  An all-even point takes
+// 0.5 * (a + a), which is exactly a.
+__device__ __forceinline__ void prolongDir( int px, int py, int pz, int& dx, int& dy, int& dz )
 {
-   const int64_t s = xcdBlock( ( nseg + 3 ) / 4 ) * 4 + ( threadIdx.x >> 6 );
-   if ( s >= nseg ) return;
-   const CellSeg sg = segs[s];
-   if ( !( ctype[sg.cell] & flag ) ) return;
-   const int lane = threadIdx.x & 63;
-   const int x    = sg.x0 + lane / NR;
-   const int r    = lane % NR;
-   if ( x > gf.w - 2 - sg.y - sg.z ) return;
-   const int y = sg.y, z = sg.z;
-   // micro-edge direction with the parity of (x, y, z): E, N, SE, TC, TW, TS, TSE
-   const int px = x & 1, py = y & 1, pz = z & 1;
-   int       dx = px, dy = py, dz = pz;
+   dx = px;
+   dy = py;
+   dz = pz;
    if ( px && py ) dy = -1;
    if ( pz && px && !py ) dx = -1;
    if ( pz && py && !px ) dy = -1;
@@ -796,15 +882,36 @@ __global__ __launch_bounds__( 256 ) void kProlongCell( LV gc, LV gf, const CellS
       dy = -1;
       dz = 1;
    }
-   const int64_t cb = gc.offC + (int64_t) sg.cell * gc.cS;
-   T             v;
-   if ( !px && !py && !pz )
-      v = uc[( cb + cidx( gc.w, x / 2, y / 2, z / 2 ) ) * NR + r];
-   else
-      v = T( 0.5 ) * ( uc[( cb + cidx( gc.w, ( x - dx ) / 2, ( y - dy ) / 2, ( z - dz ) / 2 ) ) * NR + r] +
-                       uc[( cb + cidx( gc.w, ( x + dx ) / 2, ( y + dy ) / 2, ( z + dz ) / 2 ) ) * NR + r] );
-   const int64_t fb = gf.offC + (int64_t) sg.cell * gf.cS;
-   uf[( fb + cidx( gf.w, x, y, z ) ) * NR + r] += v;
+}
+
+template < typename T, int NR >
+__global__ __launch_bounds__( 256 ) void kProlongCell( LV gc, LV gf, const CellSeg* __restrict__ segs, int64_t nseg,
+                                                       const int* __restrict__ ctype, int flag,
+                                                       const T* __restrict__ uc, T* __restrict__ uf )
+{
+   const int64_t s = xcdBlock( ( nseg + 3 ) / 4 ) * 4 + __builtin_amdgcn_readfirstlane( threadIdx.x >> 6 );
+   if ( s >= nseg ) return;
+   const CellSeg sg = segs[s];
+   if ( !( ctype[sg.cell] & flag ) ) return;
+   const int lane = threadIdx.x & 63;
+   const int x    = sg.x0 + lane / NR;
+   const int r    = lane % NR;
+   const int y = sg.y, z = sg.z, py = y & 1, pz = z & 1;
+   // per x-parity class: direction and the coarse rows of (p - d)/2 and (p + d)/2 (offsets within the cell)
+   int       d0x, d0y, d0z, d1x, d1y, d1z;
+   prolongDir( 0, py, pz, d0x, d0y, d0z );
+   prolongDir( 1, py, pz, d1x, d1y, d1z );
+   const int wc = gc.w, tc = (int) tet( wc );
+   const int m0 = (int) crow( wc, tc, ( y - d0y ) / 2, ( z - d0z ) / 2 ), p0 = (int) crow( wc, tc, ( y + d0y ) / 2, ( z + d0z ) / 2 );
+   const int m1 = (int) crow( wc, tc, ( y - d1y ) / 2, ( z - d1z ) / 2 ), p1 = (int) crow( wc, tc, ( y + d1y ) / 2, ( z + d1z ) / 2 );
+   if ( x > gf.w - 2 - y - z ) return;
+   const int px = x & 1;
+   const int dx = px ? d1x : d0x;
+   const int im = ( px ? m1 : m0 ) + ( x - dx ) / 2, ip = ( px ? p1 : p0 ) + ( x + dx ) / 2;
+   const T*  cu = uc + ( gc.offC + (int64_t) sg.cell * gc.cS ) * NR;
+   const T   v  = T( 0.5 ) * ( cu[(int64_t) im * NR + r] + cu[(int64_t) ip * NR + r] );
+   T*        fu = uf + ( gf.offC + (int64_t) sg.cell * gf.cS ) * NR;
+   fu[( (int64_t) crow( gf.w, (int) tet( gf.w ), y, z ) + x ) * NR + r] += v;
 }
 
 template < typename T, int NR >
@@ -1054,6 +1161,35 @@ const float* wsel< float >( const DevBuf< double >&, const DevBuf< float >& d32 
 
 // macro-cell apply / residual over all interior points: row-marching kernel for one right-hand side,
 // segment kernel (lanes along x * rhs) for interleaved right-hand sides
+// launch of the row-marching kernel with the level's schedule (rows per work unit, waves per workgroup,
+// rows per load block)
+template < typename T, int MODE >
+void marchLaunch( Mesh& M, LevelPlan& P, const T* CW, int flag, const T* u, T* y, const T* b, T omega, int colour )
+{
+   int64_t          n  = 0;
+   const MarchUnit* mu = M.marchUnits( P, &n );
+   if ( n == 0 ) return;
+   const MarchSchedule& S   = P.sched;
+   const int            wpb = S.waves;
+   const dim3           grid( (unsigned) xcdGrid( ( n + wpb - 1 ) / wpb ) ), block( 64 * wpb );
+   hipStream_t          s = M.ctx->stream;
+   LV                   g = viewOf( P );
+#define HYMI_MARCH( RR )                                                                                             \
+   hipLaunchKernelGGL( ( kCellMarch< T, MODE, RR > ), grid, block, 0, s, g, mu, n, CW, P.dCellType.p, flag, u, y, b, \
+                       omega, colour, S.rows )
+   switch ( S.block )
+   {
+   case 1: HYMI_MARCH( 1 ); break;
+   case 2: HYMI_MARCH( 2 ); break;
+   case 4: HYMI_MARCH( 4 ); break;
+   case 8: HYMI_MARCH( 8 ); break;
+   default: throw std::invalid_argument( "march block rows must be 1, 2, 4 or 8" );
+   }
+#undef HYMI_MARCH
+}
+
+// macro-cell apply / residual over all interior points: row-marching kernel for one right-hand side,
+// segment kernel (lanes along x * rhs) for interleaved right-hand sides
 template < typename T, int NR >
 void cellApplyLaunch( Mesh& M, LevelPlan& P, const T* CW, int flag, int mode, const T* u, T* y, const T* b )
 {
@@ -1061,19 +1197,12 @@ void cellApplyLaunch( Mesh& M, LevelPlan& P, const T* CW, int flag, int mode, co
    LV          g = viewOf( P );
    if ( NR == 1 )
    {
-      int64_t          n  = 0;
-      const MarchUnit* mu = M.marchUnits( P, &n );
-      if ( n == 0 ) return;
-      const dim3 grid( (unsigned) xcdGrid( n ) ), block( 64 );
       if ( mode == kReplace )
-         hipLaunchKernelGGL( ( kCellMarch< T, kReplace > ), grid, block, 0, s, g, mu, n, CW, P.dCellType.p, flag, u, y, b,
-                             T( 0 ), 0, P.marchRows );
+         marchLaunch< T, kReplace >( M, P, CW, flag, u, y, b, T( 0 ), 0 );
       else if ( mode == kAdd )
-         hipLaunchKernelGGL( ( kCellMarch< T, kAdd > ), grid, block, 0, s, g, mu, n, CW, P.dCellType.p, flag, u, y, b,
-                             T( 0 ), 0, P.marchRows );
+         marchLaunch< T, kAdd >( M, P, CW, flag, u, y, b, T( 0 ), 0 );
       else
-         hipLaunchKernelGGL( ( kCellMarch< T, kResidual > ), grid, block, 0, s, g, mu, n, CW, P.dCellType.p, flag, u, y,
-                             b, T( 0 ), 0, P.marchRows );
+         marchLaunch< T, kResidual >( M, P, CW, flag, u, y, b, T( 0 ), 0 );
       return;
    }
    int64_t        nseg = 0;
@@ -1098,11 +1227,7 @@ void cellSorLaunch( Mesh& M, LevelPlan& P, const T* CW, int flag, T* u, const T*
    LV          g = viewOf( P );
    if ( NR == 1 )
    {
-      int64_t          n  = 0;
-      const MarchUnit* mu = M.marchUnits( P, &n );
-      if ( n == 0 ) return;
-      hipLaunchKernelGGL( ( kCellMarch< T, kMarchSor > ), dim3( (unsigned) xcdGrid( n ) ), dim3( 64 ), 0, s, g, mu, n, CW,
-                          P.dCellType.p, flag, u, u, b, omega, colour, P.marchRows );
+      marchLaunch< T, kMarchSor >( M, P, CW, flag, u, u, b, omega, colour );
       return;
    }
    int64_t        nseg = 0;
@@ -1601,6 +1726,47 @@ void launchAddFrom( Function& x, Function& e, int level, int flag )
       run( std::integral_constant< int, 32 >{} );
    else
       throw std::runtime_error( "unsupported nrhs" );
+}
+
+void launchInterpolateModes( Function& f, int level, int flag, int nModes, const double* dModes )
+{
+   Mesh&          M    = *f.mesh;
+   LevelPlan&     P    = M.plan( level );
+   LV             g    = viewOf( P );
+   hipStream_t    s    = M.ctx->stream;
+   const double*  fr   = M.frames();
+   const int      nV   = M.T.nV(), nE = M.T.nE(), nF = M.T.nF(), nC = M.T.nC();
+   int64_t        nseg = 0;
+   const CellSeg* sg   = nC > 0 && P.w > 4 ? M.segments( P, 64, &nseg ) : nullptr;
+   auto           go   = [&]( auto* u, auto nrTag ) {
+      using T          = std::remove_pointer_t< decltype( u ) >;
+      constexpr int NR = decltype( nrTag )::value;
+      hipLaunchKernelGGL( ( kModesVertex< T, NR > ), dim3( blocksFor( nV, 64 ) ), dim3( 64 ), 0, s, g, P.dVertexGeom.p, nV,
+                          flag, fr, dModes, nModes, u );
+      if ( P.w > 2 && nE > 0 )
+         hipLaunchKernelGGL( ( kModesEdge< T, NR > ), dim3( blocksFor( P.w - 2, 64 ), nE ), dim3( 64 ), 0, s, g,
+                             P.dEdgeGeom.p, nV, flag, fr, dModes, nModes, u );
+      const int64_t m = P.w - 3;
+      if ( m > 0 && nF > 0 )
+         hipLaunchKernelGGL( ( kModesFace< T, NR > ), dim3( blocksFor( m * ( m + 1 ) / 2, 128 ), nF ), dim3( 128 ), 0, s,
+                             g, P.dFaceGeom.p, nV + nE, flag, fr, dModes, nModes, u );
+      if ( nseg > 0 )
+         hipLaunchKernelGGL( ( kModesCell< T, NR > ), dim3( (unsigned) ( ( nseg + 3 ) / 4 ) ), dim3( 256 ), 0, s, g, sg,
+                             nseg, P.dCellType.p, nV + nE + nF, flag, fr, dModes, nModes, u );
+   };
+   auto run = [&]( auto nrTag ) {
+      if ( f.prec == kF64 )
+         go( (double*) f.ptr( level ), nrTag );
+      else
+         go( (float*) f.ptr( level ), nrTag );
+   };
+   if ( f.nrhs == 1 )
+      run( std::integral_constant< int, 1 >{} );
+   else if ( f.nrhs == 32 )
+      run( std::integral_constant< int, 32 >{} );
+   else
+      throw std::runtime_error( "unsupported nrhs" );
+   launchRefresh( f, level, 7 );
 }
 
 } // namespace hymi

@@ -41,6 +41,10 @@ void launchScalarCopy( Context& ctx, int nrhs, int dst, int src );
 void launchConvert( Function& dst, int dstLevel, Function& src, int srcLevel, int flag );
 // x (FP64) += e (FP32 or FP64)
 void launchAddFrom( Function& x, Function& e, int level, int flag );
+// f = sum_m modes[r][m][0] * sin(k pi x) sin(l pi y) sin(m pi z), (k, l, m) = modes[r][m][1..3], for every
+// right-hand side r, on the DoFs of primitives matching flag; closure copies refreshed.  dModes is a device
+// table of nrhs * nModes * 4 doubles.
+void launchInterpolateModes( Function& f, int level, int flag, int nModes, const double* dModes );
 // single macro-cell kernels (no refresh, no lower-dimensional primitives) for measurement
 void launchCellApplyOnly( Operator& op, Function& src, Function& dst, int level, int flag );
 void launchCellSorOnly( Operator& op, Function& x, const Function& b, double omega, int level, int flag, int colour );

@@ -27,6 +27,13 @@ struct MarchUnit
 };
 constexpr int kMarchOutput = 62; // output lanes per wave (lanes 1..62)
 
+// launch schedule of the row-marching kernels on one level: rows per work unit, waves per workgroup
+// (1, 2 or 4) and output rows per load block (1, 2, 4 or 8)
+struct MarchSchedule
+{
+   int rows = 0, waves = 0, block = 0; // 0 = level default
+};
+
 // contiguous element range
 struct Range
 {

@@ -446,29 +446,96 @@ const CellSeg* Mesh::segments( LevelPlan& P, int len, int64_t* count )
    return it->second->p;
 }
 
+// Default launch schedule of the row-marching kernels: shorter chunks on small levels trade prologue loads
+// for enough waves to fill the GPU.
+MarchSchedule defaultSchedule( int level )
+{
+   MarchSchedule S;
+   S.rows  = level >= 7 ? 32 : ( level == 6 ? 8 : 4 );
+   S.waves = 1;
+   S.block = 4;
+   return S;
+}
+
+void checkSchedule( const MarchSchedule& S )
+{
+   if ( S.rows < 1 || S.rows > 4096 ) throw std::invalid_argument( "march rows must be in 1..4096" );
+   if ( S.waves != 1 && S.waves != 2 && S.waves != 4 ) throw std::invalid_argument( "march waves must be 1, 2 or 4" );
+   if ( S.block != 1 && S.block != 2 && S.block != 4 && S.block != 8 )
+      throw std::invalid_argument( "march block rows must be 1, 2, 4 or 8" );
+}
+
 // Row-marching units: for every cell, interior slab z and x-window x0 (stride kMarchOutput), the rows that
-// still have interior points at x >= x0 are cut into chunks of kMarchRows.  Units of one cell and slab are
+// still have interior points at x >= x0 are cut into chunks of sched.rows.  Units of one cell and slab are
 // adjacent, so the XCD-aware block order keeps neighbouring slabs on one XCD.
 const MarchUnit* Mesh::marchUnits( LevelPlan& P, int64_t* count )
 {
-   if ( P.marchCount < 0 )
+   if ( P.sched.rows == 0 ) P.sched = defaultSchedule( P.L );
+   if ( P.marchCount < 0 || P.marchRowsBuilt != P.sched.rows )
    {
       std::vector< MarchUnit > v;
-      const int                w = P.w;
-      // shorter chunks on small levels trade prologue loads for enough waves to fill the GPU
-      P.marchRows = P.L >= 7 ? 32 : ( P.L == 6 ? 8 : 4 );
+      const int                w    = P.w;
+      const int                rows = P.sched.rows;
       for ( int c = 0; c < T.nC(); c++ )
          for ( int z = 1; z <= w - 4; z++ )
             for ( int x0 = 1; x0 <= w - 3 - z; x0 += kMarchOutput )
-               for ( int y0 = 1; y0 <= w - 2 - z - x0; y0 += P.marchRows )
+               for ( int y0 = 1; y0 <= w - 2 - z - x0; y0 += rows )
                   v.push_back( { c, z, x0, y0 } );
+      HYMI_HIP( hipStreamSynchronize( ctx->stream ) ); // no launch may still read the old table
       P.march = std::make_unique< DevBuf< MarchUnit > >();
       P.march->upload( v, ctx->stream );
       HYMI_HIP( hipStreamSynchronize( ctx->stream ) );
-      P.marchCount = (int64_t) v.size();
+      P.marchCount     = (int64_t) v.size();
+      P.marchRowsBuilt = rows;
    }
    *count = P.marchCount;
    return P.march->p;
+}
+
+void Mesh::setSchedule( int level, const MarchSchedule& S )
+{
+   LevelPlan&    P = plan( level );
+   MarchSchedule N = S;
+   const auto    D = defaultSchedule( level );
+   if ( N.rows == 0 ) N.rows = D.rows;
+   if ( N.waves == 0 ) N.waves = D.waves;
+   if ( N.block == 0 ) N.block = D.block;
+   checkSchedule( N );
+   P.sched = N;
+   scheduleEpoch++;
+   int64_t n = 0;
+   if ( !ctx->hostOnly ) marchUnits( P, &n );
+}
+
+const double* Mesh::frames()
+{
+   if ( !dFrames.p )
+   {
+      const int             n = T.nV() + T.nE() + T.nF() + T.nC();
+      std::vector< double > h( 12 * (size_t) n, 0.0 );
+      size_t                k    = 0;
+      auto                  put = [&]( int o, int a, int b, int c ) {
+         double* q = &h[12 * k++];
+         for ( int d = 0; d < 3; d++ )
+         {
+            q[d]     = T.vx[o][d];
+            q[3 + d] = a >= 0 ? T.vx[a][d] - T.vx[o][d] : 0.0;
+            q[6 + d] = b >= 0 ? T.vx[b][d] - T.vx[o][d] : 0.0;
+            q[9 + d] = c >= 0 ? T.vx[c][d] - T.vx[o][d] : 0.0;
+         }
+      };
+      for ( int v = 0; v < T.nV(); v++ )
+         put( v, -1, -1, -1 );
+      for ( int e = 0; e < T.nE(); e++ )
+         put( T.ev[e][0], T.ev[e][1], -1, -1 );
+      for ( int f = 0; f < T.nF(); f++ )
+         put( T.fv[f][0], T.fv[f][1], T.fv[f][2], -1 );
+      for ( int c = 0; c < T.nC(); c++ )
+         put( T.cv[c][0], T.cv[c][1], T.cv[c][2], T.cv[c][3] );
+      dFrames.upload( h, ctx->stream );
+      HYMI_HIP( hipStreamSynchronize( ctx->stream ) );
+   }
+   return dFrames.p;
 }
 
 // Closure ranges (owned part plus copies) of the primitives whose DoF type matches `flag`, cut into chunks.

@@ -101,6 +101,10 @@ int hymi_function_download_primitive( hymi_function f, int level, int cls, int i
 int hymi_function_upload_primitive( hymi_function f, int level, int cls, int idx, const double* host );
 /* interpolate(constant, level, flag)  (VertexDoFFunction.cpp:238-300) */
 int hymi_interpolate_const( hymi_function f, int level, int flag, double value );
+/* interpolate(expr, level, flag) (VertexDoFFunction.cpp:238-300) for expr = a sum of Fourier modes,
+ * evaluated on the device: f_r(x) = sum_m a sin(k pi x) sin(l pi y) sin(m pi z) with
+ * (a, k, l, m) = modes[(r * n_modes + m) * 4 + 0..3] for each right-hand side r < nrhs. */
+int hymi_interpolate_modes( hymi_function f, int level, int flag, int n_modes, const double* modes );
 /* assign / add linear combinations (VertexDoFFunction.cpp:979-1074); n <= 3 */
 int hymi_assign( hymi_function dst, int level, int flag, int n, const double* coeffs, const hymi_function* srcs );
 int hymi_add( hymi_function dst, int level, int flag, int n, const double* coeffs, const hymi_function* srcs );
@@ -145,6 +149,18 @@ int hymi_gmg_vcycle( hymi_gmg g, hymi_function x, hymi_function b, int level );
 /* capture one V-cycle for (x, b, level) into a hipGraph and replay it `count` times */
 int hymi_gmg_vcycle_graph( hymi_gmg g, hymi_function x, hymi_function b, int level, int count );
 
+/* ---- mixed precision: FP32 V-cycles inside FP64 iterative refinement -------------------------------
+ * No single reference counterpart; the outer loop is the reference's cycle-and-check driver
+ * (apps/MultigridStudies/MultigridStudies.cpp:616-690: cycle, residual, stop below tolerance) with each
+ * cycle replaced by a correction step.  With g built on an FP32 template and x, b in FP64:
+ *   r = b - A x (FP64), stop if max_rhs ||r|| / ||r_0|| <= tol, else
+ *   e = 0, cycles_per_outer FP32 V-cycles of g on A e = r, x += e (FP64).
+ * The V-cycles replay a captured hipGraph.  outer_done = correction steps taken; rel_res[nrhs] = final
+ * ||r|| / ||r_0|| per right-hand side; history (optional, max_outer + 1 entries) = max over right-hand
+ * sides after 0, 1, ... steps.  Work buffers are allocated on the first call and kept by g. */
+int hymi_solve_refined( hymi_gmg g, hymi_function x, hymi_function b, int level, double tol, int max_outer,
+                        int cycles_per_outer, int* outer_done, double* rel_res, double* history );
+
 /* ---- measurement -----------------------------------------------------------------------------------
  * No reference counterpart (the reference times with its WcTimingTree, src/hyteg/operators/Operator.hpp:
  * 99-115).  Average duration in ms of `reps` back-to-back launches on the context stream, bracketed by
@@ -152,6 +168,15 @@ int hymi_gmg_vcycle_graph( hymi_gmg g, hymi_function x, hymi_function b, int lev
  * pass (x, b; colour 0), 2 full apply (x -> y), 3 full SOR sweep (x, b), 4 one V-cycle of gmg (x, b). */
 int hymi_time_kernel( hymi_operator op, hymi_gmg gmg, hymi_function x, hymi_function b, hymi_function y, int level,
                       int which, int reps, double* ms_per_launch );
+
+/* ---- launch schedules ------------------------------------------------------------------------------
+ * No reference counterpart (the reference's CPU kernels have no launch shape).  The row-marching
+ * macro-cell kernels of one level run with a schedule: rows per work unit (1..4096), waves per
+ * workgroup (1, 2 or 4) and output rows per load block (1, 2, 4 or 8); 0 selects the level default.
+ * Setting a schedule waits for the context stream; V-cycle graphs captured before it are captured again
+ * on their next replay.  out = {rows, waves, block}. */
+int hymi_mesh_set_schedule( hymi_mesh mesh, int level, int rows, int waves, int block );
+int hymi_mesh_get_schedule( hymi_mesh mesh, int level, int out[3] );
 
 #ifdef __cplusplus
 }

@@ -234,6 +234,57 @@ def test_graph_replay_matches_eager(gpu_ctx, oracle_lib):
         assert rel_err(x2.download(L), x.download(L)) == 0.0, k
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_march_schedules_bitwise(prec, gpu_ctx, oracle_lib):
+    """Every launch schedule of the row-marching kernels computes the same values in the same order."""
+    import hyteg_amd as H
+    O = oracle_lib
+    L = 4
+    say(f"[schedules] cube n=2 level={L} precision={prec}")
+    om, S = _pair(("cube", 2), gpu_ctx)
+    rng = np.random.default_rng(5)
+    xh, bh = rng.standard_normal((2, om.owned_count(L)))
+    A = H.P1ConstantLaplaceOperator(S, 0, L)
+    x, b, y = (H.P1Function(n, S, 0, L, precision=prec) for n in "xby")
+    b.upload(L, bh)
+
+    def run():
+        x.upload(L, xh)
+        A.apply(x, y, L, H.INNER)
+        out = [y.download(L)]
+        A.smooth_sor(x, b, 0.8, L, H.INNER)
+        out.append(x.download(L))
+        return out
+
+    S.set_schedule(L)
+    ref = run()
+    for rows, waves, block in [(1, 1, 1), (3, 2, 2), (4, 4, 8), (8, 4, 4), (64, 2, 8), (2, 1, 8), (13, 4, 2), (7, 1, 4)]:
+        S.set_schedule(L, rows, waves, block)
+        got = run()
+        for a, r in zip(got, ref):
+            assert np.array_equal(a, r), (rows, waves, block)
+    S.set_schedule(L)
+
+
+def test_graph_recaptured_after_schedule_change(gpu_ctx, oracle_lib):
+    import hyteg_amd as H
+    O = oracle_lib
+    L = 4
+    say(f"[graph schedule] cube n=2 level={L}")
+    om, S = _pair(("cube", 2), gpu_ctx)
+    A, OA, b, x, ob, ox, gmg, og = _gmg_setup(H, O, om, S, L, order=1)
+    x2 = H.P1Function("x2", S, 0, L)
+    gmg.solve(A, x, b, L)
+    gmg.solve_graph(A, x2, b, L, 1)
+    assert rel_err(x2.download(L), x.download(L)) == 0.0
+    S.set_schedule(L, 2, 4, 3)      # frees the work-unit table the captured graph used
+    gmg.solve(A, x, b, L)
+    gmg.solve_graph(A, x2, b, L, 1)
+    _ = x.dotGlobal(x, L, H.INNER)
+    assert rel_err(x2.download(L), x.download(L)) == 0.0
+    S.set_schedule(L)
+
+
 def test_fp32_apply_and_vcycle(gpu_ctx, oracle_lib):
     import hyteg_amd as H
     O = oracle_lib
@@ -297,3 +348,40 @@ def test_reference_primitive_layout(kind, gpu_ctx, oracle_lib):
             ref = of.prim(L, cls, idx)
             assert got.shape == ref.shape, (cls, idx)
             assert np.array_equal(got, ref), (cls, idx)
+
+
+@pytest.mark.parametrize("nrhs,prec", [(1, 0), (32, 0), (1, 1)])
+@pytest.mark.parametrize("kind", [("cube", 2), ("gmsh", "regular_octahedron_8el.msh")])
+def test_interpolate_modes_matches_host(kind, nrhs, prec, gpu_ctx):
+    """Device evaluation of Fourier-mode sums equals host evaluation at the owned coordinates; only flag DoFs
+    are written and the closure copies are current (an apply sees the same values as after an upload)."""
+    import hyteg_amd as H
+    L = 3
+    say(f"[modes] mesh={kind} level={L} nrhs={nrhs} precision={prec}")
+    _, S = _pair(kind, gpu_ctx)
+    rng = np.random.default_rng(17)
+    modes = np.zeros((nrhs, 4, 4))
+    modes[:, :, 0] = rng.standard_normal((nrhs, 4))
+    modes[:, :, 1:] = rng.integers(1, 9, (nrhs, 4, 3))
+    X = S.owned_coords(L)
+    want = np.zeros((len(X), nrhs))
+    for r in range(nrhs):
+        for a, k, l, m in modes[r]:
+            want[:, r] += a * np.sin(k * np.pi * X[:, 0]) * np.sin(l * np.pi * X[:, 1]) * np.sin(m * np.pi * X[:, 2])
+    want = want[:, 0] if nrhs == 1 else want
+    tol = 1e-13 if prec == 0 else 1e-6
+    f = H.P1Function("f", S, 0, L, precision=prec, nrhs=nrhs)
+    f.interpolate_modes(modes, L, H.ALL)
+    assert rel_err(f.download(L), want) <= tol
+    inner = S.owned_types(L) == H.INNER
+    g = H.P1Function("g", S, 0, L, precision=prec, nrhs=nrhs)
+    g.interpolate_modes(modes, L, H.INNER)
+    got = g.download(L)
+    assert np.all(got[~inner] == 0)
+    assert rel_err(got[inner], want[inner]) <= tol
+    A = H.P1ConstantLaplaceOperator(S, 0, L)
+    h, y1, y2 = (H.P1Function(n, S, 0, L, precision=prec, nrhs=nrhs) for n in ("h", "y1", "y2"))
+    h.upload(L, f.download(L))
+    A.apply(f, y1, L, H.INNER)
+    A.apply(h, y2, L, H.INNER)
+    assert np.array_equal(y1.download(L), y2.download(L))

@@ -114,7 +114,40 @@ def test_errors_are_status_codes(H, hctx):
         H.P1ConstantLaplaceOperator(S, 3, 1)
 
 
+def test_schedule_arguments(H, hctx):
+    S = H.PrimitiveStorage(H.MeshInfo.meshCuboid((0, 0, 0), (1, 1, 1), 1, 1, 1), hctx)
+    for L in (2, 5, 7, 8):
+        d = S.get_schedule(L)
+        assert 1 <= d["rows"] <= 4096 and d["waves"] in (1, 2, 4) and d["block"] in (1, 2, 4, 8)
+    S.set_schedule(5, rows=16, waves=4, block=8)
+    assert S.get_schedule(5) == {"rows": 16, "waves": 4, "block": 8}
+    for bad in (dict(waves=3), dict(block=3), dict(rows=5000), dict(rows=-1)):
+        with pytest.raises(H.HymiError):
+            S.set_schedule(5, **bad)
+    assert S.get_schedule(5) == {"rows": 16, "waves": 4, "block": 8}   # a refused call changes nothing
+    S.set_schedule(5)
+    assert S.get_schedule(5) == S.get_schedule(5) and S.get_schedule(5)["rows"] >= 1
+
+
 def test_bench_refuses_multi_gpu():
     res = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True)
     assert res.returncode != 0
     assert "one GPU" in res.stderr
+
+
+def test_serving_counts_and_stream(H, hctx):
+    from hyteg_amd.serving import cuboid_counts, hierarchy_bytes, poisson_stream
+    for n in (1, 2, 3):
+        S = H.PrimitiveStorage(H.MeshInfo.meshCuboid((0, 0, 0), (1, 1, 1), n, n, n), hctx)
+        assert cuboid_counts(n) == S.counts()
+    # budget estimate grows ~8x per level and covers the finest level's three FP64 and three FP32 copies
+    for n in (1, 2, 3):
+        fixed = 8 << 20                                            # tables and scalars
+        assert 6.5 < (hierarchy_bytes(n, 7) - fixed) / (hierarchy_bytes(n, 6) - fixed) < 8.5
+        assert hierarchy_bytes(n, 7) > 36 * ((n * 2 ** 7 + 1) ** 3)
+    a, b = poisson_stream(2000, rate=200.0, seed=3), poisson_stream(2000, rate=200.0, seed=3)
+    assert [(r.arrival, r.n, r.level) for r in a] == [(r.arrival, r.n, r.level) for r in b]
+    gaps = np.diff([0.0] + [r.arrival for r in a])
+    assert abs(gaps.mean() - 1 / 200.0) < 0.1 / 200.0          # exponential inter-arrival, mean 1/rate
+    assert {r.n for r in a} == {1, 2, 3} and {r.level for r in a} == {4, 5, 6, 7}
+    assert all(r.modes.shape == (4, 4) and 1 <= r.modes[:, 1:].min() and r.modes[:, 1:].max() <= 8 for r in a)
