@@ -51,7 +51,7 @@ class Context:
         return v.value
 
     def __del__(self):
-        if getattr(self, "h", None) and _lib is not None and _lib._lib is not None:
+        if getattr(self, "h", None) and _lib is not None and _lib.releasable():
             _lib._lib.hymi_ctx_destroy(self.h)
             self.h = None
 
@@ -103,7 +103,7 @@ class PrimitiveStorage:
         self.h = h
 
     def __del__(self):
-        if getattr(self, "h", None) and _lib is not None and _lib._lib is not None:
+        if getattr(self, "h", None) and _lib is not None and _lib.releasable():
             _lib._lib.hymi_mesh_destroy(self.h)
             self.h = None
 
@@ -154,7 +154,7 @@ class P1Function:
         self.h = h
 
     def __del__(self):
-        if getattr(self, "h", None) and _lib is not None and _lib._lib is not None:
+        if getattr(self, "h", None) and _lib is not None and _lib.releasable():
             _lib._lib.hymi_function_destroy(self.h)
             self.h = None
 
@@ -239,7 +239,7 @@ class _P1ConstantOperator:
         self.h = h
 
     def __del__(self):
-        if getattr(self, "h", None) and _lib is not None and _lib._lib is not None:
+        if getattr(self, "h", None) and _lib is not None and _lib.releasable():
             _lib._lib.hymi_operator_destroy(self.h)
             self.h = None
 
@@ -333,7 +333,7 @@ class GeometricMultigridSolver:
         _lib.call("hymi_gmg_vcycle_graph", self._handle(A, x), x.h, b.h, level, count)
 
     def __del__(self):
-        if _lib is None or _lib._lib is None:
+        if _lib is None or not _lib.releasable():
             return
         for h, _ in getattr(self, "_h", {}).values():
             _lib._lib.hymi_gmg_destroy(h)

@@ -3,6 +3,7 @@
 The library is loaded from the package directory; a missing library raises instead of falling back to
 anything else.
 """
+import atexit
 import ctypes
 import os
 
@@ -14,6 +15,25 @@ INNER, DIRICHLET, NEUMANN, FREESLIP, ALL = 1, 2, 4, 8, 15
 REPLACE, ADD = 0, 1
 F64, F32 = 0, 1
 VERTEX, EDGE, FACE, CELL = 0, 1, 2, 3
+
+
+# Set when the interpreter starts shutting down.  From then on the Python wrappers release nothing: module
+# teardown destroys objects in no particular order (a context could go before the functions on it), and the
+# process exit returns every device allocation anyway.
+closing = False
+
+
+def _mark_closing():
+    global closing
+    closing = True
+
+
+atexit.register(_mark_closing)
+
+
+def releasable():
+    """True while handles may still be destroyed (library loaded, interpreter not shutting down)."""
+    return _lib is not None and not closing
 
 
 class HymiError(RuntimeError):

@@ -586,43 +586,83 @@ __global__ void kVertexSor( LV g, const VertexGeom* __restrict__ vg, const int64
 // phase writes, so they see start-of-sweep values exactly as the reference's ghost layers do.  Primitives
 // not selected by `flag` keep their values, so their copies stay valid and they are skipped.
 
+// sum over the 64 lanes of a wave in a fixed butterfly order (every lane gets the total)
 template < typename T >
-__global__ void kVertexSorPush( LV g, const VertexGeom* __restrict__ vg, const int64_t* __restrict__ terms,
-                                const int64_t* __restrict__ ends, int nV, const T* __restrict__ VW, int flag,
-                                T* __restrict__ u, const T* __restrict__ b, T omega )
+__device__ __forceinline__ T waveSum( T v )
 {
-   const int v = blockIdx.x * blockDim.x + threadIdx.x;
-   if ( v >= nV ) return;
-   const VertexGeom G = vg[v];
+#pragma unroll
+   for ( int m = 32; m >= 1; m >>= 1 )
+      v += __shfl_xor( v, m, 64 );
+   return v;
+}
+
+// one wave per vertex: lanes take the incident-edge terms, a butterfly sums them
+template < typename T >
+__global__ __launch_bounds__( 64 ) void kVertexSorPush( LV g, const VertexGeom* __restrict__ vg,
+                                                        const int64_t* __restrict__ terms,
+                                                        const int64_t* __restrict__ ends, int nV,
+                                                        const T* __restrict__ VW, int flag, T* __restrict__ u,
+                                                        const T* __restrict__ b, T omega )
+{
+   const VertexGeom G = vg[blockIdx.x];
    if ( !( G.type & flag ) ) return;
-   const T* vw  = VW + G.vertex + G.termOff;
-   T        acc = T( 0 );
-   for ( int k = 0; k < G.nTerms; k++ )
-      acc += vw[1 + k] * u[terms[G.termOff + k]];
-   const T nv = ( T( 1 ) - omega ) * u[G.vertex] + omega / vw[0] * ( b[G.vertex] - acc );
-   u[G.vertex] = nv;
+   const int lane = threadIdx.x;
+   const T*  vw   = VW + G.vertex + G.termOff;
+   T         part = T( 0 );
+   for ( int k = lane; k < G.nTerms; k += 64 )
+      part += vw[1 + k] * u[terms[G.termOff + k]];
+   const T acc = waveSum( part );
+   const T nv  = ( T( 1 ) - omega ) * u[G.vertex] + omega / vw[0] * ( b[G.vertex] - acc );
+   if ( lane == 0 ) u[G.vertex] = nv;
    // end-point copies in the incident edges (read by no vertex on levels >= 1)
-   for ( int k = 0; k < G.nTerms; k++ )
+   for ( int k = lane; k < G.nTerms; k += 64 )
       u[ends[G.termOff + k]] = nv;
 }
 
+// one workgroup per edge; the edge's neighbour terms and weights are staged in LDS once, so the per-point
+// sums issue independent value loads
 template < typename T >
 __global__ __launch_bounds__( 128 ) void kEdgeSorFused( LV g, const EdgeGeom* __restrict__ eg,
                                                         const EdgeTerm* __restrict__ terms, const T* __restrict__ EW,
                                                         int flag, T* __restrict__ u, const T* __restrict__ b, T omega )
 {
+   __shared__ int64_t sPrim[kEdgeTermsMax];                  // element offset of the neighbour primitive
+   __shared__ int     sB[3][kEdgeTermsMax], sI[3][kEdgeTermsMax], sKind[kEdgeTermsMax];
+   __shared__ T       sW[kEdgeTermsMax];
    const EdgeGeom& G = eg[blockIdx.x];
    if ( !( G.type & flag ) ) return;
    const T*      ew = EW + 3 * (int64_t) G.edge + G.termOff;
    const int64_t eo = g.offE + (int64_t) G.edge * g.eS;
-   const T       rl = omega / ew[1];
+   const int     nT = G.nTerms; // <= kEdgeTermsMax (checked when the plan is built)
+   // term k at edge point x: face element fidx(w, B0 + x I0, B1 + x I1) or cell element
+   // cidx(w, B0 + x I0, B1 + x I1, B2 + x I2) of the neighbour primitive
+   for ( int k = threadIdx.x; k < nT; k += blockDim.x )
+   {
+      const EdgeTerm t = terms[G.termOff + k];
+      sPrim[k]         = t.kind == 0 ? g.offF + (int64_t) t.prim * g.fS : g.offC + (int64_t) t.prim * g.cS;
+      sKind[k]         = t.kind;
+      for ( int d = 0; d < 3; d++ )
+      {
+         sB[d][k] = t.base[d];
+         sI[d][k] = t.inc[d];
+      }
+      sW[k] = ew[3 + k];
+   }
+   __syncthreads();
+   const T rl = omega / ew[1];
    for ( int colour = 0; colour < 2; colour++ )
    {
       // colour 0 = even edge points, colour 1 = odd ones
       for ( int x = 2 - colour + 2 * (int) threadIdx.x; x <= g.w - 2; x += 2 * blockDim.x )
       {
          T acc = ew[0] * u[eo + x - 1] + ew[2] * u[eo + x + 1];
-         acc += edgeTermsSum< T, 1 >( g, G, terms, ew + 3, u, x, 0 );
+#pragma unroll 4
+         for ( int k = 0; k < nT; k++ )
+         {
+            const int     c0  = sB[0][k] + x * sI[0][k], c1 = sB[1][k] + x * sI[1][k];
+            const int64_t idx = sPrim[k] + ( sKind[k] == 0 ? fidx( g.w, c0, c1 ) : cidx( g.w, c0, c1, sB[2][k] + x * sI[2][k] ) );
+            acc += sW[k] * u[idx];
+         }
          u[eo + x] = ( T( 1 ) - omega ) * u[eo + x] + rl * ( b[eo + x] - acc );
       }
       __syncthreads();
@@ -1315,7 +1355,7 @@ void sorT( Operator& op, Function& xf, const Function& bf, double omega, int lev
    if ( refreshFirst ) refreshT< T, NR >( xf, level, 7 );
    if ( NR == 1 && level >= 1 )
    {
-      hipLaunchKernelGGL( ( kVertexSorPush< T > ), dim3( blocksFor( nV, 64 ) ), dim3( 64 ), 0, s, g, P.dVertexGeom.p,
+      hipLaunchKernelGGL( ( kVertexSorPush< T > ), dim3( nV ), dim3( 64 ), 0, s, g, P.dVertexGeom.p,
                           P.dVertexTerms.p, P.dVertexEnds.p, nV, wsel< T >( Wt.vertex64, Wt.vertex32 ), flag, u, b, om );
       hipLaunchKernelGGL( ( kEdgeSorFused< T > ), dim3( nE ), dim3( 128 ), 0, s, g, P.dEdgeGeom.p, P.dEdgeTerms.p,
                           wsel< T >( Wt.edge64, Wt.edge32 ), flag, u, b, om );

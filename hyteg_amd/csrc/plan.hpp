@@ -68,6 +68,9 @@ struct EdgeTerm
    int32_t pad;
 };
 
+// largest number of neighbour terms of one edge (the fused edge SOR stages them in LDS)
+constexpr int kEdgeTermsMax = 128;
+
 struct EdgeGeom
 {
    int32_t edge, nTerms, termOff, type;

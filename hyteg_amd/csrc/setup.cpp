@@ -212,6 +212,7 @@ void validatePlan( const Topology& T, const LevelPlan& P )
    for ( int e = 0; e < T.nE(); e++ )
    {
       const EdgeGeom& g = P.hEdgeGeom[e];
+      ok( g.nTerms <= kEdgeTermsMax, "edge with more neighbour terms than kEdgeTermsMax" );
       if ( w < 3 ) continue;
       for ( int k = 0; k < g.nTerms; k++ )
       {
@@ -446,13 +447,14 @@ const CellSeg* Mesh::segments( LevelPlan& P, int len, int64_t* count )
    return it->second->p;
 }
 
-// Default launch schedule of the row-marching kernels: shorter chunks on small levels trade prologue loads
-// for enough waves to fill the GPU.
+// Default launch schedule of the row-marching kernels, from a sweep of rows x waves x block on MI355X
+// (48 macro-cells, levels 5-7, FP64 apply and SOR colour pass): 8-row units with 4-row load blocks are
+// fastest from level 6 up; smaller levels need shorter units to fill the GPU.
 MarchSchedule defaultSchedule( int level )
 {
    MarchSchedule S;
-   S.rows  = level >= 7 ? 32 : ( level == 6 ? 8 : 4 );
-   S.waves = 1;
+   S.rows  = level >= 6 ? 8 : 4;
+   S.waves = level >= 7 ? 1 : 2;
    S.block = 4;
    return S;
 }

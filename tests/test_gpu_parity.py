@@ -277,7 +277,7 @@ def test_graph_recaptured_after_schedule_change(gpu_ctx, oracle_lib):
     gmg.solve(A, x, b, L)
     gmg.solve_graph(A, x2, b, L, 1)
     assert rel_err(x2.download(L), x.download(L)) == 0.0
-    S.set_schedule(L, 2, 4, 3)      # frees the work-unit table the captured graph used
+    S.set_schedule(L, 2, 4, 8)      # frees the work-unit table the captured graph used
     gmg.solve(A, x, b, L)
     gmg.solve_graph(A, x2, b, L, 1)
     _ = x.dotGlobal(x, L, H.INNER)
