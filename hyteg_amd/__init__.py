@@ -132,14 +132,15 @@ class PrimitiveStorage:
         _lib.call("hymi_mesh_owned_types", self.h, level, _ip(out))
         return out
 
-    def set_schedule(self, level, rows=0, waves=0, block=0):
-        """Launch schedule of the macro-cell row-marching kernels on `level` (0 = level default)."""
-        _lib.call("hymi_mesh_set_schedule", self.h, level, rows, waves, block)
+    def set_schedule(self, level, rows=0, waves=0, block=0, kind=-1):
+        """Launch schedule of the macro-cell kernels on `level`: kind 0 row windows, 1 flat slab windows;
+        -1 / 0 = level default."""
+        _lib.call("hymi_mesh_set_schedule", self.h, level, kind, rows, waves, block)
 
     def get_schedule(self, level):
-        out = np.zeros(3, np.int32)
+        out = np.zeros(4, np.int32)
         _lib.call("hymi_mesh_get_schedule", self.h, level, _ip(out))
-        return {"rows": int(out[0]), "waves": int(out[1]), "block": int(out[2])}
+        return {"rows": int(out[0]), "waves": int(out[1]), "block": int(out[2]), "kind": int(out[3])}
 
 
 class P1Function:

@@ -91,7 +91,7 @@ SIGNATURES = {
     "hymi_gmg_vcycle": [_vp, _vp, _vp, _i],
     "hymi_gmg_vcycle_graph": [_vp, _vp, _vp, _i, _i],
     "hymi_time_kernel": [_vp, _vp, _vp, _vp, _vp, _i, _i, _i, _P(_d)],
-    "hymi_mesh_set_schedule": [_vp, _i, _i, _i, _i],
+    "hymi_mesh_set_schedule": [_vp, _i, _i, _i, _i, _i],
     "hymi_interpolate_modes": [_vp, _i, _i, _i, _P(_d)],
     "hymi_solve_refined": [_vp, _vp, _vp, _i, _d, _i, _i, _P(_i), _P(_d), _P(_d)],
     "hymi_mesh_get_schedule": [_vp, _i, _P(_i)],

@@ -19,6 +19,12 @@
 
 using namespace hymi;
 
+std::recursive_mutex& hymi::legacyLock()
+{
+   static std::recursive_mutex m;
+   return m;
+}
+
 struct hymi_ctx_s
 {
    Context c;
@@ -118,6 +124,7 @@ std::vector< double > downloadLevel( Function& f, int level )
    const int64_t         n = P.total * f.nrhs;
    std::vector< double > out( (size_t) n );
    HYMI_HIP( hipStreamSynchronize( M.ctx->stream ) );
+   HYMI_LEGACY_LOCK;
    if ( f.prec == kF64 )
       HYMI_HIP( hipMemcpy( out.data(), f.ptr( level ), (size_t) n * 8, hipMemcpyDeviceToHost ) );
    else
@@ -135,6 +142,7 @@ void uploadLevel( Function& f, int level, const std::vector< double >& in )
    LevelPlan&    P = M.plan( level );
    const int64_t n = P.total * f.nrhs;
    HYMI_HIP( hipStreamSynchronize( M.ctx->stream ) );
+   HYMI_LEGACY_LOCK;
    if ( f.prec == kF64 )
       HYMI_HIP( hipMemcpy( f.ptr( level ), in.data(), (size_t) n * 8, hipMemcpyHostToDevice ) );
    else
@@ -202,6 +210,7 @@ std::unique_ptr< Function > makeFunction( Mesh& M, int minL, int maxL, int prec,
       LevelPlan&   P     = M.plan( L );
       const size_t bytes = (size_t) P.total * nrhs * F->elemBytes();
       void*        d     = nullptr;
+      HYMI_LEGACY_LOCK;
       hipError_t   e     = hipMalloc( &d, bytes );
       if ( e != hipSuccess )
       {
@@ -262,6 +271,7 @@ struct hymi_gmg_s
    std::map< std::tuple< void*, void*, int >, Captured > graphs;
    ~hymi_gmg_s()
    {
+      HYMI_LEGACY_LOCK;
       for ( auto& kv : graphs )
          (void) hipGraphExecDestroy( kv.second.exec );
    }
@@ -290,12 +300,14 @@ struct hymi_gmg_s
       if ( it != graphs.end() && it->second.epoch != epoch )
       {
          HYMI_HIP( hipStreamSynchronize( s ) );
+         HYMI_LEGACY_LOCK;
          (void) hipGraphExecDestroy( it->second.exec );
          graphs.erase( it );
          it = graphs.end();
       }
       if ( it == graphs.end() )
       {
+         HYMI_LEGACY_LOCK; // no other thread touches the legacy stream while this stream captures
          hipGraph_t graph = nullptr;
          HYMI_HIP( hipStreamBeginCapture( s, hipStreamCaptureModeThreadLocal ) );
          try
@@ -367,7 +379,10 @@ int hymi_ctx_create( int device, hymi_ctx* out )
       HYMI_HIP( hipSetDevice( device ) );
       auto c      = std::make_unique< hymi_ctx_s >();
       c->c.device = device;
-      HYMI_HIP( hipStreamCreateWithFlags( &c->c.stream, hipStreamNonBlocking ) );
+      {
+         HYMI_LEGACY_LOCK;
+         HYMI_HIP( hipStreamCreateWithFlags( &c->c.stream, hipStreamNonBlocking ) );
+      }
       ensureScalars( c->c, 32 );
       c->c.partials.alloc( 1 << 16 );
       HYMI_HIP( hipStreamSynchronize( c->c.stream ) );
@@ -382,10 +397,13 @@ int hymi_ctx_destroy( hymi_ctx ctx )
       if ( !ctx->c.hostOnly )
       {
          (void) hipStreamSynchronize( ctx->c.stream );
+         HYMI_LEGACY_LOCK;
          ctx->c.partials.release();
          ctx->c.scalars.release();
+         ctx->c.modes.release();
          (void) hipStreamDestroy( ctx->c.stream );
       }
+      HYMI_LEGACY_LOCK;
       delete ctx;
    } );
 }
@@ -549,11 +567,12 @@ int hymi_mesh_owned_coords( hymi_mesh mesh, int level, double* xyz )
    } );
 }
 
-int hymi_mesh_set_schedule( hymi_mesh mesh, int level, int rows, int waves, int block )
+int hymi_mesh_set_schedule( hymi_mesh mesh, int level, int kind, int rows, int waves, int block )
 {
    return guarded( [&] {
       require( mesh && level >= 0 && level <= 12, "bad argument" );
       MarchSchedule S;
+      S.kind  = kind;
       S.rows  = rows;
       S.waves = waves;
       S.block = block;
@@ -561,7 +580,7 @@ int hymi_mesh_set_schedule( hymi_mesh mesh, int level, int rows, int waves, int 
    } );
 }
 
-int hymi_mesh_get_schedule( hymi_mesh mesh, int level, int out[3] )
+int hymi_mesh_get_schedule( hymi_mesh mesh, int level, int out[4] )
 {
    return guarded( [&] {
       require( mesh && out && level >= 0 && level <= 12, "bad argument" );
@@ -570,6 +589,7 @@ int hymi_mesh_get_schedule( hymi_mesh mesh, int level, int out[3] )
       out[0]          = S.rows;
       out[1]          = S.waves;
       out[2]          = S.block;
+      out[3]          = S.kind;
    } );
 }
 
@@ -616,6 +636,7 @@ int hymi_function_create( hymi_mesh mesh, int minL, int maxL, int precision, int
 
 Function::~Function()
 {
+   HYMI_LEGACY_LOCK;
    for ( size_t k = 0; k < data.size(); k++ )
    {
       if ( data[k] )
@@ -852,6 +873,8 @@ int hymi_dot( hymi_function a, hymi_function b, int level, int flag, double* out
       requireLevel( b->f, level );
       Context& ctx = *a->f.mesh->ctx;
       launchDot( a->f, b->f, level, flag, sUser );
+      HYMI_HIP( hipStreamSynchronize( ctx.stream ) );
+      HYMI_LEGACY_LOCK;
       HYMI_HIP( hipMemcpyAsync( out, ctx.scalars.p + sUser * a->f.nrhs, sizeof( double ) * a->f.nrhs,
                                 hipMemcpyDeviceToHost, ctx.stream ) );
       HYMI_HIP( hipStreamSynchronize( ctx.stream ) );
@@ -868,8 +891,10 @@ int hymi_interpolate_modes( hymi_function f, int level, int flag, int n_modes, c
       if ( ctx.modes.n < n ) ctx.modes.alloc( n );
       if ( n > 0 )
       {
-         // ordered after earlier readers of the staging table on this stream; the host table may be released
+         // earlier readers of the staging table on this stream finish first; the host table may be released
          // when the call returns
+         HYMI_HIP( hipStreamSynchronize( ctx.stream ) );
+         HYMI_LEGACY_LOCK;
          HYMI_HIP( hipMemcpyAsync( ctx.modes.p, modes, n * sizeof( double ), hipMemcpyHostToDevice, ctx.stream ) );
          HYMI_HIP( hipStreamSynchronize( ctx.stream ) );
       }
@@ -1158,6 +1183,8 @@ int hymi_solve_refined( hymi_gmg g, hymi_function x, hymi_function b, int level,
       auto residualNorms = [&]( std::vector< double >& out ) {
          launchApply( *g->A, x->f, *g->refR, &b->f, level, flag, kResidual, true );
          launchDot( *g->refR, *g->refR, level, flag, sUser );
+         HYMI_HIP( hipStreamSynchronize( ctx.stream ) );
+         HYMI_LEGACY_LOCK;
          HYMI_HIP( hipMemcpyAsync( out.data(), ctx.scalars.p + sUser * nr, sizeof( double ) * nr, hipMemcpyDeviceToHost,
                                    ctx.stream ) );
          HYMI_HIP( hipStreamSynchronize( ctx.stream ) );

@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -28,6 +29,13 @@ struct HipError
       if ( e_ != hipSuccess ) throw ::hymi::HipError{ e_, std::string( #call ) };        \
    } while ( 0 )
 
+// Graph capture on a context's stream, and calls that may synchronise with the legacy stream (hipMalloc,
+// hipFree, copies from or to pageable host memory, stream creation and destruction), must not overlap
+// across threads: HIP refuses such a call while another thread captures.  Every such call in the library
+// holds this lock; a thread never waits for its stream's queued work while holding it.
+std::recursive_mutex& legacyLock();
+#define HYMI_LEGACY_LOCK std::lock_guard< std::recursive_mutex > hymiLegacyLock_( ::hymi::legacyLock() )
+
 // device buffer owned by RAII
 template < typename T >
 struct DevBuf
@@ -40,12 +48,14 @@ struct DevBuf
    ~DevBuf() { release(); }
    void release()
    {
+      HYMI_LEGACY_LOCK;
       if ( p ) (void) hipFree( p );
       p = nullptr;
       n = 0;
    }
    void alloc( size_t count )
    {
+      HYMI_LEGACY_LOCK;
       release();
       if ( count == 0 ) count = 1;
       HYMI_HIP( hipMalloc( (void**) &p, count * sizeof( T ) ) );
@@ -53,6 +63,7 @@ struct DevBuf
    }
    void upload( const std::vector< T >& h, hipStream_t s )
    {
+      HYMI_LEGACY_LOCK;
       alloc( h.size() );
       if ( !h.empty() ) HYMI_HIP( hipMemcpyAsync( p, h.data(), h.size() * sizeof( T ), hipMemcpyHostToDevice, s ) );
    }
@@ -105,6 +116,7 @@ struct LevelPlan
    std::unique_ptr< DevBuf< MarchUnit > > march;
    int64_t                                marchCount = -1;
    int                                    marchRowsBuilt = 0;
+   int                                    marchKindBuilt = -1;
    MarchSchedule                          sched;
    // closure ranges of primitives matching a flag, keyed by flag
    std::map< int, std::unique_ptr< DevBuf< Range > > > ranges;

@@ -385,6 +385,113 @@ __global__ __launch_bounds__( 256 ) void kCellMarch( LV g, const MarchUnit* __re
    }
 }
 
+// Flat-window form of the macro-cell kernel.  Slab z of a cell (width W = w - z) is one contiguous array of
+// W(W+1)/2 points, rows y = 0 .. W-1 of length W - y.  A window is 64 consecutive slab elements
+// e = ws-1 .. ws+62 (lanes 0..63) whose lanes 1..62 produce outputs, so short rows no longer leave most of
+// a wave idle.  Each lane finds its row y from the window's first row y0 by a short uniform loop; its
+// stencil rows then sit at closed-form offsets from e (s(y) = y(W+1) - y(y+1)/2 starts row y):
+//   slab z:   (x, y+1) = e + W - y          (x, y-1) = e - (W - y + 1)
+//   slab z+1: (x, y)   = e - y              (x, y-1) = e - W            (slab z+1 has width W-1)
+//   slab z-1: (x, y)   = e + y              (x, y+1) = e + W + 1        (slab z-1 has width W+1)
+// An output lane has x >= 1 and x + 1 <= W - 1 - y, so lanes i-1 and i+1 lie in the same row and the
+// x-1 / x+1 values come from DPP wave shifts as in the row form.  Indices of lanes that feed no output are
+// clamped into the cell.  A unit is `rows` consecutive windows of one slab, taken R at a time so that the
+// loads of R windows are in flight together.  MarchUnit fields here: cell, z, x0 = first output element
+// ws of the first window, y0 = row of element ws - 1.
+template < typename T, int MODE, int R >
+__global__ __launch_bounds__( 256 ) void kCellFlat( LV g, const MarchUnit* __restrict__ units, int64_t nUnits,
+                                                    const T* __restrict__ W8, const int* __restrict__ ctype, int flag,
+                                                    const T* u, T* y, const T* b, T omega, int colour, int rows )
+{
+   const int     wpb  = blockDim.x >> 6;
+   const int     wave = __builtin_amdgcn_readfirstlane( threadIdx.x >> 6 );
+   const int64_t ui   = xcdBlock( ( nUnits + wpb - 1 ) / wpb ) * wpb + wave;
+   if ( ui >= nUnits ) return;
+   const MarchUnit U = units[ui];
+   if ( !( ctype[U.cell] & flag ) ) return;
+   const int     lane = threadIdx.x & 63;
+   const int     w = g.w, z = U.z, Wd = w - z;
+   const int64_t base = g.offC + (int64_t) U.cell * g.cS;
+   const T*      cu   = u + base;
+   T*            cy   = y + base;
+   const T*      cb   = b ? b + base : nullptr;
+   const int     t0   = (int) tet( w );
+   const int     S0   = t0 - (int) tet( Wd );     // slab z, z+1, z-1 starts within the cell
+   const int     S1   = t0 - (int) tet( Wd - 1 );
+   const int     Sm   = t0 - (int) tet( Wd + 1 );
+   const int     last = (int) tet( w ) - 1;        // clamp bound of cell element indices
+   const int     nEl  = Wd * ( Wd + 1 ) / 2;       // slab elements
+   const T*      c    = W8 + (int64_t) U.cell * 15;
+   const T cC = c[dC], cW = c[dW], cE = c[dE], cN = c[dN], cS = c[dS], cNW = c[dNW], cSE = c[dSE];
+   const T cTC = c[dTC], cTW = c[dTW], cTS = c[dTS], cTSE = c[dTSE], cBC = c[dBC], cBN = c[dBN], cBE = c[dBE],
+           cBNW = c[dBNW];
+   const T relax = omega / cC;
+   auto    rowStart = [&]( int yy ) { return yy * ( Wd + 1 ) - yy * ( yy + 1 ) / 2; };
+   int     ws = U.x0, y0 = U.y0;                   // first window; y0 = row of element ws - 1
+   for ( int jb = 0; jb < rows; jb += R )
+   {
+#pragma unroll
+   for ( int j = 0; j < R; j++, ws += kMarchOutput )
+   {
+      if ( jb + j >= rows || ws - 1 >= nEl ) break;
+      // advance y0 (wave-uniform) to the row of element ws - 1, and find the row of the last lane
+      while ( rowStart( y0 + 1 ) <= ws - 1 )
+         y0++;
+      int yL = y0;
+      while ( yL + 1 < Wd && rowStart( yL + 1 ) <= ws + 62 )
+         yL++;
+      const int e  = ws - 1 + lane;
+      int       yy = y0, next = rowStart( y0 + 1 );
+      for ( int it = y0; it < yL; it++ )
+         if ( e >= next )
+         {
+            yy++;
+            next += Wd - yy;
+         }
+      const int  x   = e - rowStart( yy );
+      const bool out = lane >= 1 && lane <= 62 && e < nEl && x >= 1 && yy >= 1 && x + yy <= Wd - 2;
+      auto cl = [&]( int i ) { return min( max( i, 0 ), last ); };
+      const T Z0 = cu[cl( S0 + e )], Zp = cu[cl( S0 + e + Wd - yy )], Zm = cu[cl( S0 + e - ( Wd - yy + 1 ) )];
+      const T T0 = cu[cl( S1 + e - yy )], Tm = cu[cl( S1 + e - Wd )];
+      const T B0 = cu[cl( Sm + e + yy )], Bp = cu[cl( Sm + e + Wd + 1 )];
+      const T Z0l = laneDown( Z0 ), Z0r = laneUp( Z0 ), Zpl = laneDown( Zp ), Zmr = laneUp( Zm );
+      const T T0l = laneDown( T0 ), Tmr = laneUp( Tm ), B0r = laneUp( B0 ), Bpl = laneDown( Bp );
+      const int o = S0 + e;
+      if ( MODE == kMarchSor )
+      {
+         if ( out && ( ( x + 2 * yy + 3 * z ) & 3 ) == colour )
+         {
+            T acc = cW * Z0l + cE * Z0r;
+            acc += cN * Zp + cNW * Zpl;
+            acc += cS * Zm + cSE * Zmr;
+            acc += cTC * T0 + cTW * T0l;
+            acc += cTS * Tm + cTSE * Tmr;
+            acc += cBC * B0 + cBE * B0r;
+            acc += cBN * Bp + cBNW * Bpl;
+            cy[o] = ( T( 1 ) - omega ) * Z0 + relax * ( cb[o] - acc );
+         }
+      }
+      else if ( out )
+      {
+         T acc = cC * Z0;
+         acc += cW * Z0l + cE * Z0r;
+         acc += cN * Zp + cNW * Zpl;
+         acc += cS * Zm + cSE * Zmr;
+         acc += cTC * T0 + cTW * T0l;
+         acc += cTS * Tm + cTSE * Tmr;
+         acc += cBC * B0 + cBE * B0r;
+         acc += cBN * Bp + cBNW * Bpl;
+         if ( MODE == kReplace )
+            cy[o] = acc;
+         else if ( MODE == kAdd )
+            cy[o] += acc;
+         else
+            cy[o] = cb[o] - acc;
+      }
+   }
+   }
+}
+
 // --------------------------------------------------------------------------------------------------------
 // macro-face interior
 
@@ -1214,16 +1321,30 @@ void marchLaunch( Mesh& M, LevelPlan& P, const T* CW, int flag, const T* u, T* y
    const dim3           grid( (unsigned) xcdGrid( ( n + wpb - 1 ) / wpb ) ), block( 64 * wpb );
    hipStream_t          s = M.ctx->stream;
    LV                   g = viewOf( P );
-#define HYMI_MARCH( RR )                                                                                             \
-   hipLaunchKernelGGL( ( kCellMarch< T, MODE, RR > ), grid, block, 0, s, g, mu, n, CW, P.dCellType.p, flag, u, y, b, \
+#define HYMI_MARCH( KERNEL, RR )                                                                                   \
+   hipLaunchKernelGGL( ( KERNEL< T, MODE, RR > ), grid, block, 0, s, g, mu, n, CW, P.dCellType.p, flag, u, y, b,    \
                        omega, colour, S.rows )
-   switch ( S.block )
+   if ( S.kind == 1 )
    {
-   case 1: HYMI_MARCH( 1 ); break;
-   case 2: HYMI_MARCH( 2 ); break;
-   case 4: HYMI_MARCH( 4 ); break;
-   case 8: HYMI_MARCH( 8 ); break;
-   default: throw std::invalid_argument( "march block rows must be 1, 2, 4 or 8" );
+      switch ( S.block )
+      {
+      case 1: HYMI_MARCH( kCellFlat, 1 ); break;
+      case 2: HYMI_MARCH( kCellFlat, 2 ); break;
+      case 4: HYMI_MARCH( kCellFlat, 4 ); break;
+      case 8: HYMI_MARCH( kCellFlat, 8 ); break;
+      default: throw std::invalid_argument( "march block must be 1, 2, 4 or 8" );
+      }
+   }
+   else
+   {
+      switch ( S.block )
+      {
+      case 1: HYMI_MARCH( kCellMarch, 1 ); break;
+      case 2: HYMI_MARCH( kCellMarch, 2 ); break;
+      case 4: HYMI_MARCH( kCellMarch, 4 ); break;
+      case 8: HYMI_MARCH( kCellMarch, 8 ); break;
+      default: throw std::invalid_argument( "march block rows must be 1, 2, 4 or 8" );
+      }
    }
 #undef HYMI_MARCH
 }

@@ -27,11 +27,12 @@ struct MarchUnit
 };
 constexpr int kMarchOutput = 62; // output lanes per wave (lanes 1..62)
 
-// launch schedule of the row-marching kernels on one level: rows per work unit, waves per workgroup
-// (1, 2 or 4) and output rows per load block (1, 2, 4 or 8)
+// launch schedule of the macro-cell kernels on one level.  kind 0: row march (units of `rows` rows of one
+// x-window, loaded `block` rows at a time); kind 1: flat windows (units of `rows` 64-element windows of one
+// slab, loaded `block` windows at a time).  waves = waves per workgroup (1, 2 or 4).
 struct MarchSchedule
 {
-   int rows = 0, waves = 0, block = 0; // 0 = level default
+   int rows = 0, waves = 0, block = 0, kind = -1; // 0 / -1 = level default
 };
 
 // contiguous element range

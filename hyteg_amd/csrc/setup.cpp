@@ -447,12 +447,13 @@ const CellSeg* Mesh::segments( LevelPlan& P, int len, int64_t* count )
    return it->second->p;
 }
 
-// Default launch schedule of the row-marching kernels, from a sweep of rows x waves x block on MI355X
+// Default launch schedule of the macro-cell kernels, from a sweep of kind x rows x waves x block on MI355X
 // (48 macro-cells, levels 5-7, FP64 apply and SOR colour pass): 8-row units with 4-row load blocks are
 // fastest from level 6 up; smaller levels need shorter units to fill the GPU.
 MarchSchedule defaultSchedule( int level )
 {
    MarchSchedule S;
+   S.kind  = 0;
    S.rows  = level >= 6 ? 8 : 4;
    S.waves = level >= 7 ? 1 : 2;
    S.block = 4;
@@ -461,34 +462,62 @@ MarchSchedule defaultSchedule( int level )
 
 void checkSchedule( const MarchSchedule& S )
 {
+   if ( S.kind != 0 && S.kind != 1 ) throw std::invalid_argument( "schedule kind must be 0 (rows) or 1 (flat)" );
    if ( S.rows < 1 || S.rows > 4096 ) throw std::invalid_argument( "march rows must be in 1..4096" );
    if ( S.waves != 1 && S.waves != 2 && S.waves != 4 ) throw std::invalid_argument( "march waves must be 1, 2 or 4" );
    if ( S.block != 1 && S.block != 2 && S.block != 4 && S.block != 8 )
       throw std::invalid_argument( "march block rows must be 1, 2, 4 or 8" );
 }
 
-// Row-marching units: for every cell, interior slab z and x-window x0 (stride kMarchOutput), the rows that
-// still have interior points at x >= x0 are cut into chunks of sched.rows.  Units of one cell and slab are
-// adjacent, so the XCD-aware block order keeps neighbouring slabs on one XCD.
+namespace {
+// first element of row y in a slab of width W (rows of length W - y)
+int slabRowStart( int W, int y ) { return y * ( W + 1 ) - y * ( y + 1 ) / 2; }
+} // namespace
+
+// Work units of the macro-cell kernels, ordered cell -> slab -> position so that the XCD-aware block order
+// keeps neighbouring slabs of a cell on one XCD.
+//   kind 0: for every interior slab z and x-window x0 (stride kMarchOutput), the rows that still have
+//           interior points at x >= x0, cut into chunks of sched.rows rows;
+//   kind 1: the slab elements from the first interior point (1, 1) to the last, (1, W-3), cut into
+//           windows of kMarchOutput outputs, sched.rows windows per unit; y0 = row of element ws - 1.
 const MarchUnit* Mesh::marchUnits( LevelPlan& P, int64_t* count )
 {
    if ( P.sched.rows == 0 ) P.sched = defaultSchedule( P.L );
-   if ( P.marchCount < 0 || P.marchRowsBuilt != P.sched.rows )
+   if ( P.marchCount < 0 || P.marchRowsBuilt != P.sched.rows || P.marchKindBuilt != P.sched.kind )
    {
       std::vector< MarchUnit > v;
       const int                w    = P.w;
       const int                rows = P.sched.rows;
       for ( int c = 0; c < T.nC(); c++ )
          for ( int z = 1; z <= w - 4; z++ )
-            for ( int x0 = 1; x0 <= w - 3 - z; x0 += kMarchOutput )
-               for ( int y0 = 1; y0 <= w - 2 - z - x0; y0 += rows )
-                  v.push_back( { c, z, x0, y0 } );
+         {
+            if ( P.sched.kind == 0 )
+            {
+               for ( int x0 = 1; x0 <= w - 3 - z; x0 += kMarchOutput )
+                  for ( int y0 = 1; y0 <= w - 2 - z - x0; y0 += rows )
+                     v.push_back( { c, z, x0, y0 } );
+            }
+            else
+            {
+               const int W     = w - z;
+               const int first = slabRowStart( W, 1 ) + 1;
+               const int lastE = slabRowStart( W, W - 3 ) + 1;
+               int       y     = 0;
+               for ( int ws = first; ws <= lastE; ws += kMarchOutput * rows )
+               {
+                  while ( slabRowStart( W, y + 1 ) <= ws - 1 )
+                     y++;
+                  v.push_back( { c, z, ws, y } );
+               }
+            }
+         }
       HYMI_HIP( hipStreamSynchronize( ctx->stream ) ); // no launch may still read the old table
       P.march = std::make_unique< DevBuf< MarchUnit > >();
       P.march->upload( v, ctx->stream );
       HYMI_HIP( hipStreamSynchronize( ctx->stream ) );
       P.marchCount     = (int64_t) v.size();
       P.marchRowsBuilt = rows;
+      P.marchKindBuilt = P.sched.kind;
    }
    *count = P.marchCount;
    return P.march->p;
@@ -499,6 +528,7 @@ void Mesh::setSchedule( int level, const MarchSchedule& S )
    LevelPlan&    P = plan( level );
    MarchSchedule N = S;
    const auto    D = defaultSchedule( level );
+   if ( N.kind < 0 ) N.kind = D.kind;
    if ( N.rows == 0 ) N.rows = D.rows;
    if ( N.waves == 0 ) N.waves = D.waves;
    if ( N.block == 0 ) N.block = D.block;

@@ -258,11 +258,13 @@ def test_march_schedules_bitwise(prec, gpu_ctx, oracle_lib):
 
     S.set_schedule(L)
     ref = run()
-    for rows, waves, block in [(1, 1, 1), (3, 2, 2), (4, 4, 8), (8, 4, 4), (64, 2, 8), (2, 1, 8), (13, 4, 2), (7, 1, 4)]:
-        S.set_schedule(L, rows, waves, block)
+    for rows, waves, block, kind in [(1, 1, 1, 0), (3, 2, 2, 0), (4, 4, 8, 0), (8, 4, 4, 0), (64, 2, 8, 0),
+                                     (2, 1, 8, 0), (13, 4, 2, 0), (7, 1, 4, 0), (1, 1, 1, 1), (4, 2, 2, 1),
+                                     (8, 4, 4, 1), (3, 1, 8, 1), (2, 2, 1, 1), (64, 4, 4, 1)]:
+        S.set_schedule(L, rows, waves, block, kind)
         got = run()
         for a, r in zip(got, ref):
-            assert np.array_equal(a, r), (rows, waves, block)
+            assert np.array_equal(a, r), (rows, waves, block, kind)
     S.set_schedule(L)
 
 
