@@ -33,6 +33,14 @@ def _ip(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
 
 
+def _releasable():
+    """True while native handles may be destroyed; False during interpreter shutdown."""
+    try:
+        return _lib is not None and _lib.releasable()
+    except Exception:
+        return False
+
+
 class Context:
     """A device + HIP stream (device=-1: host-only context for topology and stencil queries)."""
 
@@ -50,8 +58,8 @@ class Context:
         _lib.call("hymi_ctx_bytes_in_use", self.h, ctypes.byref(v))
         return v.value
 
-    def __del__(self):
-        if getattr(self, "h", None) and _lib is not None and _lib.releasable():
+    def __del__(self, _releasable=_releasable):
+        if getattr(self, "h", None) and _releasable():
             _lib._lib.hymi_ctx_destroy(self.h)
             self.h = None
 
@@ -102,8 +110,8 @@ class PrimitiveStorage:
             _lib.call("hymi_mesh_create", self.ctx.h, len(X), _dp(X), len(C), _ip(C), ctypes.byref(h))
         self.h = h
 
-    def __del__(self):
-        if getattr(self, "h", None) and _lib is not None and _lib.releasable():
+    def __del__(self, _releasable=_releasable):
+        if getattr(self, "h", None) and _releasable():
             _lib._lib.hymi_mesh_destroy(self.h)
             self.h = None
 
@@ -154,8 +162,8 @@ class P1Function:
         _lib.call("hymi_function_create", storage.h, minLevel, maxLevel, precision, nrhs, ctypes.byref(h))
         self.h = h
 
-    def __del__(self):
-        if getattr(self, "h", None) and _lib is not None and _lib.releasable():
+    def __del__(self, _releasable=_releasable):
+        if getattr(self, "h", None) and _releasable():
             _lib._lib.hymi_function_destroy(self.h)
             self.h = None
 
@@ -239,8 +247,8 @@ class _P1ConstantOperator:
         _lib.call(self._create, storage.h, minLevel, maxLevel, ctypes.byref(h))
         self.h = h
 
-    def __del__(self):
-        if getattr(self, "h", None) and _lib is not None and _lib.releasable():
+    def __del__(self, _releasable=_releasable):
+        if getattr(self, "h", None) and _releasable():
             _lib._lib.hymi_operator_destroy(self.h)
             self.h = None
 
@@ -333,8 +341,8 @@ class GeometricMultigridSolver:
         """Replay a hipGraph-captured V-cycle `count` times."""
         _lib.call("hymi_gmg_vcycle_graph", self._handle(A, x), x.h, b.h, level, count)
 
-    def __del__(self):
-        if _lib is None or not _lib.releasable():
+    def __del__(self, _releasable=_releasable):
+        if not _releasable():
             return
         for h, _ in getattr(self, "_h", {}).values():
             _lib._lib.hymi_gmg_destroy(h)

@@ -1104,6 +1104,7 @@ int hymi_gmg_create( hymi_operator op, hymi_function xt, int minL, int maxL, int
          M.segments( P, segLen, &n );
          maxBlocks = std::max( maxBlocks, ( n + 3 ) / 4 + 64 );
          M.marchUnits( P, &n );
+         M.transferUnits( P, &n );
          M.closureRanges( P, kInner | kNeumann, &n );
          M.closureRanges( P, kInner | kNeumann | kFreeslip, &n );
          M.closureRanges( P, kAll, &n );

@@ -117,6 +117,11 @@ struct LevelPlan
    int64_t                                marchCount = -1;
    int                                    marchRowsBuilt = 0;
    int                                    marchKindBuilt = -1;
+   int                                    marchBlockBuilt = 0;
+   // flat 64-point windows (no halo lanes) of every interior slab, kTransferWindows per unit, for the
+   // grid-transfer kernels; fields as for the flat macro-cell kernel (x0 = first element, y0 = its row)
+   std::unique_ptr< DevBuf< MarchUnit > > transfer;
+   int64_t                                transferCount = -1;
    MarchSchedule                          sched;
    // closure ranges of primitives matching a flag, keyed by flag
    std::map< int, std::unique_ptr< DevBuf< Range > > > ranges;
@@ -136,6 +141,7 @@ struct Mesh
    LevelPlan& plan( int level );
    const CellSeg* segments( LevelPlan& P, int len, int64_t* count );
    const MarchUnit* marchUnits( LevelPlan& P, int64_t* count );
+   const MarchUnit* transferUnits( LevelPlan& P, int64_t* count );
    // replaces the launch schedule of the row-marching kernels on one level (zeros = level default);
    // bumps scheduleEpoch so that captured graphs holding the old work-unit table are captured again
    void     setSchedule( int level, const MarchSchedule& S );

@@ -48,6 +48,13 @@ __device__ __forceinline__ int64_t crow( int64_t w, int64_t t0, int64_t y, int64
    const int64_t wz = w - z;
    return t0 - tet( wz ) + y * ( wz + 1 ) - y * ( y + 1 ) / 2;
 }
+// 32-bit forms for cell-relative offsets (a cell of width w <= 1025 has fewer than 2^31 points)
+__device__ __forceinline__ int tet32( int w ) { return w * ( w + 1 ) / 2 * ( w + 2 ) / 3; }
+__device__ __forceinline__ int crow32( int w, int t0, int y, int z )
+{
+   const int wz = w - z;
+   return t0 - tet32( wz ) + y * ( wz + 1 ) - ( y * ( y + 1 ) >> 1 );
+}
 // (a, b) of flat index k in the triangle of width m (row-major, row b has m - b points)
 __device__ __forceinline__ void triPos( int64_t m, int64_t k, int& a, int& b )
 {
@@ -122,23 +129,25 @@ __global__ void kE2F( LV g, const E2F* __restrict__ tab, int n, T* __restrict__ 
    u[dst * NR + r]   = u[src * NR + r];
 }
 
+// one workgroup per (face row j, cell-face pair): face point (i, j), i = 0 .. w-1-j, is copied to the cell
+// point base + i a + j b of the pair's face-to-cell map
 template < typename T, int NR >
 __global__ void kF2C( LV g, const F2C* __restrict__ tab, T* __restrict__ u )
 {
-   const F2C     m  = tab[blockIdx.y];
-   const int64_t np = (int64_t) g.w * ( g.w + 1 ) / 2;
-   const int64_t t  = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
-   const int     r  = (int) ( t % NR );
-   const int64_t k  = t / NR;
-   if ( k >= np ) return;
-   int i, j;
-   triPos( g.w, k, i, j );
-   const int x = m.m.base[0] + i * m.m.a[0] + j * m.m.b[0];
-   const int y = m.m.base[1] + i * m.m.a[1] + j * m.m.b[1];
-   const int z = m.m.base[2] + i * m.m.a[2] + j * m.m.b[2];
-   const int64_t dst = g.offC + (int64_t) m.m.cell * g.cS + cidx( g.w, x, y, z );
-   const int64_t src = g.offF + (int64_t) m.face * g.fS + k;
-   u[dst * NR + r]   = u[src * NR + r];
+   const F2C m  = tab[blockIdx.y];
+   const int j  = blockIdx.x;
+   const int w  = g.w;
+   const int t0 = tet32( w );
+   const T*  fs = u + ( g.offF + (int64_t) m.face * g.fS + (int64_t) ( j * ( w + 1 ) - ( j * ( j + 1 ) >> 1 ) ) ) * NR;
+   T*        cu = u + ( g.offC + (int64_t) m.m.cell * g.cS ) * NR;
+   for ( int t = threadIdx.x; t < ( w - j ) * NR; t += blockDim.x )
+   {
+      const int i = t / NR, r = t % NR;
+      const int x = m.m.base[0] + i * m.m.a[0] + j * m.m.b[0];
+      const int y = m.m.base[1] + i * m.m.a[1] + j * m.m.b[1];
+      const int z = m.m.base[2] + i * m.m.a[2] + j * m.m.b[2];
+      cu[(int64_t) ( crow32( w, t0, y, z ) + x ) * NR + r] = fs[t];
+   }
 }
 
 // --------------------------------------------------------------------------------------------------------
@@ -275,15 +284,26 @@ __device__ __forceinline__ T laneUp( T v ) // lane i <- lane i+1 (lane 63 gets 0
       return __builtin_bit_cast( T, __builtin_amdgcn_mov_dpp( __builtin_bit_cast( int, v ), 0x130, 0xf, 0xf, true ) );
 }
 
-// Value at x of the row of cell array `cu` that starts at offset rowOff and ends at rowEnd (both
-// wave-uniform), read without a branch: lanes past the end read the row's last point, and a row that does not
-// exist (rowEnd < 0) reads the cell's first point.  Such values only reach lanes that produce no output.
+// Buffer view of one macro-cell array: loads go through a buffer resource whose range is the cell's
+// (padded) array, so an element index outside the cell reads 0 in hardware instead of touching other
+// memory.  The whole element offset is passed as the per-lane offset (the range check covers it).  Such
+// reads only ever reach lanes that produce no output.  dword3 0x00020000: raw 32-bit data format of gfx9.
 template < typename T >
-__device__ __forceinline__ T rowLoad( const T* __restrict__ cu, int rowOff, int x, int rowEnd )
+struct CellBuf
 {
-   const bool has = rowEnd >= 0;
-   return cu[( has ? rowOff : 0 ) + min( x, has ? rowEnd : 0 )];
-}
+   __amdgpu_buffer_rsrc_t r;
+   __device__ __forceinline__ CellBuf( const T* p, int64_t n )
+       : r( __builtin_amdgcn_make_buffer_rsrc( (void*) p, 0, (int) ( n * (int64_t) sizeof( T ) ), 0x00020000 ) )
+   {}
+   __device__ __forceinline__ T operator[]( int idx ) const
+   {
+      typedef unsigned int u32x2 __attribute__( ( ext_vector_type( 2 ) ) );
+      if constexpr ( sizeof( T ) == 8 )
+         return __builtin_bit_cast( T, __builtin_amdgcn_raw_buffer_load_b64( r, idx * 8, 0, 0 ) );
+      else
+         return __builtin_bit_cast( T, __builtin_amdgcn_raw_buffer_load_b32( r, idx * 4, 0, 0 ) );
+   }
+};
 
 template < typename T, int MODE, int R >
 __global__ __launch_bounds__( 256 ) void kCellMarch( LV g, const MarchUnit* __restrict__ units, int64_t nUnits,
@@ -300,10 +320,10 @@ __global__ __launch_bounds__( 256 ) void kCellMarch( LV g, const MarchUnit* __re
    const int     w = g.w, z = U.z;
    const int     x    = U.x0 - 1 + lane;
    const int64_t base = g.offC + (int64_t) U.cell * g.cS;
-   const T*      cu   = u + base; // this cell's array; row offsets below are 32-bit (cells hold < 2^31 points)
+   const CellBuf< T > cu( u + base, g.cS ); // row offsets below are 32-bit (cells hold < 2^31 points)
    T*            cy   = y + base;
    const T*      cb   = b ? b + base : nullptr;
-   const int     t0   = (int) tet( w );
+   const int     t0   = tet32( w );
    const T*      c    = W + (int64_t) U.cell * 15;
    const T cC = c[dC], cW = c[dW], cE = c[dE], cN = c[dN], cS = c[dS], cNW = c[dNW], cSE = c[dSE];
    const T cTC = c[dTC], cTW = c[dTW], cTS = c[dTS], cTSE = c[dTSE], cBC = c[dBC], cBN = c[dBN], cBE = c[dBE],
@@ -315,29 +335,29 @@ __global__ __launch_bounds__( 256 ) void kCellMarch( LV g, const MarchUnit* __re
       // all rows of the block, issued together: slab z rows y0-1 .. y0+R, slab z+1 rows y0-1 .. y0+R-1,
       // slab z-1 rows y0 .. y0+R; row (yy, zz) holds x = 0 .. w-1-yy-zz and the next row follows it
       T   Zr[R + 2], Tr[R + 1], Br[R + 1];
-      int pz = (int) crow( w, t0, y0 - 1, z ), pt = (int) crow( w, t0, y0 - 1, z + 1 ), pb = (int) crow( w, t0, y0, z - 1 );
+      int pz = crow32( w, t0, y0 - 1, z ), pt = crow32( w, t0, y0 - 1, z + 1 ), pb = crow32( w, t0, y0, z - 1 );
 #pragma unroll
       for ( int k = 0; k < R + 2; k++ )
       {
          const int yy = y0 - 1 + k;
-         Zr[k]        = rowLoad( cu, pz, x, w - 1 - yy - z );
+         Zr[k]        = cu[pz + x];
          pz += w - yy - z;
       }
 #pragma unroll
       for ( int k = 0; k < R + 1; k++ )
       {
          const int yy = y0 - 1 + k;
-         Tr[k]        = rowLoad( cu, pt, x, w - 1 - yy - ( z + 1 ) );
+         Tr[k]        = cu[pt + x];
          pt += w - yy - ( z + 1 );
       }
 #pragma unroll
       for ( int k = 0; k < R + 1; k++ )
       {
          const int yy = y0 + k;
-         Br[k]        = rowLoad( cu, pb, x, w - 1 - yy - ( z - 1 ) );
+         Br[k]        = cu[pb + x];
          pb += w - yy - ( z - 1 );
       }
-      int rz0 = (int) crow( w, t0, y0, z );
+      int rz0 = crow32( w, t0, y0, z );
 #pragma unroll
       for ( int k = 0; k < R; k++ )
       {
@@ -394,8 +414,8 @@ __global__ __launch_bounds__( 256 ) void kCellMarch( LV g, const MarchUnit* __re
 //   slab z+1: (x, y)   = e - y              (x, y-1) = e - W            (slab z+1 has width W-1)
 //   slab z-1: (x, y)   = e + y              (x, y+1) = e + W + 1        (slab z-1 has width W+1)
 // An output lane has x >= 1 and x + 1 <= W - 1 - y, so lanes i-1 and i+1 lie in the same row and the
-// x-1 / x+1 values come from DPP wave shifts as in the row form.  Indices of lanes that feed no output are
-// clamped into the cell.  A unit is `rows` consecutive windows of one slab, taken R at a time so that the
+// x-1 / x+1 values come from DPP wave shifts as in the row form.  Reads go through CellBuf, so indices of
+// lanes that feed no output need no clamping.  A unit is `rows` consecutive windows of one slab, taken R at a time so that the
 // loads of R windows are in flight together.  MarchUnit fields here: cell, z, x0 = first output element
 // ws of the first window, y0 = row of element ws - 1.
 template < typename T, int MODE, int R >
@@ -412,14 +432,13 @@ __global__ __launch_bounds__( 256 ) void kCellFlat( LV g, const MarchUnit* __res
    const int     lane = threadIdx.x & 63;
    const int     w = g.w, z = U.z, Wd = w - z;
    const int64_t base = g.offC + (int64_t) U.cell * g.cS;
-   const T*      cu   = u + base;
+   const CellBuf< T > cu( u + base, g.cS );
    T*            cy   = y + base;
    const T*      cb   = b ? b + base : nullptr;
-   const int     t0   = (int) tet( w );
-   const int     S0   = t0 - (int) tet( Wd );     // slab z, z+1, z-1 starts within the cell
-   const int     S1   = t0 - (int) tet( Wd - 1 );
-   const int     Sm   = t0 - (int) tet( Wd + 1 );
-   const int     last = (int) tet( w ) - 1;        // clamp bound of cell element indices
+   const int     t0   = tet32( w );
+   const int     S0   = t0 - tet32( Wd );         // slab z, z+1, z-1 starts within the cell
+   const int     S1   = t0 - tet32( Wd - 1 );
+   const int     Sm   = t0 - tet32( Wd + 1 );
    const int     nEl  = Wd * ( Wd + 1 ) / 2;       // slab elements
    const T*      c    = W8 + (int64_t) U.cell * 15;
    const T cC = c[dC], cW = c[dW], cE = c[dE], cN = c[dN], cS = c[dS], cNW = c[dNW], cSE = c[dSE];
@@ -428,20 +447,27 @@ __global__ __launch_bounds__( 256 ) void kCellFlat( LV g, const MarchUnit* __res
    const T relax = omega / cC;
    auto    rowStart = [&]( int yy ) { return yy * ( Wd + 1 ) - yy * ( yy + 1 ) / 2; };
    int     ws = U.x0, y0 = U.y0;                   // first window; y0 = row of element ws - 1
+   int     n0 = rowStart( y0 + 1 );                // start of row y0 + 1 (rows advance by W - y)
    for ( int jb = 0; jb < rows; jb += R )
    {
 #pragma unroll
    for ( int j = 0; j < R; j++, ws += kMarchOutput )
    {
       if ( jb + j >= rows || ws - 1 >= nEl ) break;
-      // advance y0 (wave-uniform) to the row of element ws - 1, and find the row of the last lane
-      while ( rowStart( y0 + 1 ) <= ws - 1 )
+      // advance y0 (wave-uniform) to the row of element ws - 1, and find the row yL of the last lane
+      while ( n0 <= ws - 1 )
+      {
          y0++;
-      int yL = y0;
-      while ( yL + 1 < Wd && rowStart( yL + 1 ) <= ws + 62 )
+         n0 += Wd - y0;
+      }
+      int yL = y0, nL = n0;
+      while ( yL + 1 < Wd && nL <= ws + 62 )
+      {
          yL++;
+         nL += Wd - yL;
+      }
       const int e  = ws - 1 + lane;
-      int       yy = y0, next = rowStart( y0 + 1 );
+      int       yy = y0, next = n0;
       for ( int it = y0; it < yL; it++ )
          if ( e >= next )
          {
@@ -450,10 +476,9 @@ __global__ __launch_bounds__( 256 ) void kCellFlat( LV g, const MarchUnit* __res
          }
       const int  x   = e - rowStart( yy );
       const bool out = lane >= 1 && lane <= 62 && e < nEl && x >= 1 && yy >= 1 && x + yy <= Wd - 2;
-      auto cl = [&]( int i ) { return min( max( i, 0 ), last ); };
-      const T Z0 = cu[cl( S0 + e )], Zp = cu[cl( S0 + e + Wd - yy )], Zm = cu[cl( S0 + e - ( Wd - yy + 1 ) )];
-      const T T0 = cu[cl( S1 + e - yy )], Tm = cu[cl( S1 + e - Wd )];
-      const T B0 = cu[cl( Sm + e + yy )], Bp = cu[cl( Sm + e + Wd + 1 )];
+      const T Z0 = cu[S0 + e], Zp = cu[S0 + e + Wd - yy], Zm = cu[S0 + e - ( Wd - yy + 1 )];
+      const T T0 = cu[S1 + e - yy], Tm = cu[S1 + e - Wd];
+      const T B0 = cu[Sm + e + yy], Bp = cu[Sm + e + Wd + 1];
       const T Z0l = laneDown( Z0 ), Z0r = laneUp( Z0 ), Zpl = laneDown( Zp ), Zmr = laneUp( Zm );
       const T T0l = laneDown( T0 ), Tmr = laneUp( Tm ), B0r = laneUp( B0 ), Bpl = laneDown( Bp );
       const int o = S0 + e;
@@ -490,6 +515,153 @@ __global__ __launch_bounds__( 256 ) void kCellFlat( LV g, const MarchUnit* __res
       }
    }
    }
+}
+
+// LDS-staged slab tiles (schedule kind 2).  A workgroup owns rows y in [ya, ya+R) of one macro-cell for the
+// slabs z in [za, zb) and marches along z.  The band of slab s is rows ya-1 .. ya+R of that slab, one
+// contiguous range of the cell array (rows of a slab are stored one after another); it is loaded once, with
+// consecutive threads on consecutive elements, into a ring of four LDS slots: while the outputs of slab z
+// are computed from the bands of slabs z-1, z, z+1, the loads of slab z+2's band are in flight in
+// registers and are stored to the fourth slot afterwards (two register stages alternate, so a request has
+// two steps to arrive).  Each output is computed by one lane of a wave
+// that takes a whole row y (lanes along x) and reads its 15 stencil values from LDS.  TileUnit fields in
+// MarchUnit: cell, z = za, x0 = zb, y0 = ya; R = `rows`.
+template < typename T, int MODE >
+__global__ __launch_bounds__( 256 ) void kCellTile( LV g, const MarchUnit* __restrict__ units, int64_t nUnits,
+                                                    const T* __restrict__ W8, const int* __restrict__ ctype, int flag,
+                                                    const T* u, T* y, const T* b, T omega, int colour, int R, int cap )
+{
+   extern __shared__ __align__( 16 ) unsigned char ldsRaw[];
+   T*            lds = reinterpret_cast< T* >( ldsRaw ); // 4 slots of cap elements
+   const int64_t ui  = xcdBlock( nUnits );
+   if ( ui >= nUnits ) return;
+   const MarchUnit U = units[ui];
+   if ( !( ctype[U.cell] & flag ) ) return;
+   const int w = g.w, za = U.z, zb = U.x0, ya = U.y0;
+   const int t0 = tet32( w );
+   const int64_t base = g.offC + (int64_t) U.cell * g.cS;
+   const CellBuf< T > cu( u + base, g.cS );
+   T*       cy = y + base;
+   const T* cb = b ? b + base : nullptr;
+   const T* c  = W8 + (int64_t) U.cell * 15;
+   const T cC = c[dC], cW = c[dW], cE = c[dE], cN = c[dN], cS = c[dS], cNW = c[dNW], cSE = c[dSE];
+   const T cTC = c[dTC], cTW = c[dTW], cTS = c[dTS], cTSE = c[dTSE], cBC = c[dBC], cBN = c[dBN], cBE = c[dBE],
+           cBNW = c[dBNW];
+   const T relax = omega / cC;
+   const int tid = threadIdx.x, nth = blockDim.x;
+   const int wave = __builtin_amdgcn_readfirstlane( tid >> 6 ), nwave = nth >> 6, lane = tid & 63;
+   // slab s: width Ws = w - s, row yy starts at rs(Ws, yy) within the slab, slab starts at t0 - tet(Ws)
+   auto rs = [&]( int Ws, int yy ) { return yy * ( Ws + 1 ) - ( yy * ( yy + 1 ) >> 1 ); };
+   // band of slab s: rows ya-1 .. min(ya+R, Ws-1); first element (cell-relative) and length
+   auto bandFirst = [&]( int s ) { return t0 - tet32( w - s ) + rs( w - s, ya - 1 ); };
+   auto bandLen   = [&]( int s ) {
+      const int Ws = w - s, last = min( ya + R, Ws - 1 );
+      return last < ya - 1 ? 0 : rs( Ws, last + 1 ) - rs( Ws, ya - 1 );
+   };
+   constexpr int kMaxPer = kTileMaxPer; // elements per thread of one band (cap <= kMaxPer * blockDim, checked on the host)
+   auto loadBand = [&]( int s, T* slot ) {
+      const int f = bandFirst( s ), n = bandLen( s );
+      for ( int k = tid; k < n; k += nth )
+         slot[k] = cu[f + k];
+   };
+   // outputs of slab z from the LDS bands of slabs z-1, z, z+1
+   auto computeSlab = [&]( int z ) {
+      // outputs of slab z: interior rows y in [max(ya, 1), min(ya+R-1, W-3)], one wave per row
+      const T*  Lz = lds + ( z & 3 ) * cap;
+      const T*  Lt = lds + ( ( z + 1 ) & 3 ) * cap;
+      const T*  Lb = lds + ( ( z - 1 ) & 3 ) * cap;
+      const int Wd = w - z;
+      const int S0 = t0 - tet32( Wd );
+      const int yEnd = min( ya + R - 1, Wd - 3 );
+      for ( int yy = max( ya, 1 ) + wave; yy <= yEnd; yy += nwave )
+      {
+         // band-relative starts of the rows used, per slab (widths Wd, Wd - 1, Wd + 1)
+         const int z0 = rs( Wd, yy ) - rs( Wd, ya - 1 ), zm = rs( Wd, yy - 1 ) - rs( Wd, ya - 1 ),
+                   zp = rs( Wd, yy + 1 ) - rs( Wd, ya - 1 );
+         const int t0r = rs( Wd - 1, yy ) - rs( Wd - 1, ya - 1 ), tmr = rs( Wd - 1, yy - 1 ) - rs( Wd - 1, ya - 1 );
+         const int b0r = rs( Wd + 1, yy ) - rs( Wd + 1, ya - 1 ), bpr = rs( Wd + 1, yy + 1 ) - rs( Wd + 1, ya - 1 );
+         const int go  = S0 + rs( Wd, yy ); // cell-relative start of output row (yy, z)
+         for ( int x = 1 + lane; x <= Wd - 2 - yy; x += 64 )
+         {
+            const bool doit = MODE != kMarchSor || ( ( x + 2 * yy + 3 * z ) & 3 ) == colour;
+            if ( !doit ) continue;
+            const T Z0 = Lz[z0 + x], Z0l = Lz[z0 + x - 1], Z0r = Lz[z0 + x + 1];
+            const T Zp = Lz[zp + x], Zpl = Lz[zp + x - 1], Zm = Lz[zm + x], Zmr = Lz[zm + x + 1];
+            const T T0 = Lt[t0r + x], T0l = Lt[t0r + x - 1], Tm = Lt[tmr + x], Tmr = Lt[tmr + x + 1];
+            const T B0 = Lb[b0r + x], B0r = Lb[b0r + x + 1], Bp = Lb[bpr + x], Bpl = Lb[bpr + x - 1];
+            const int o = go + x;
+            if ( MODE == kMarchSor )
+            {
+               T acc = cW * Z0l + cE * Z0r;
+               acc += cN * Zp + cNW * Zpl;
+               acc += cS * Zm + cSE * Zmr;
+               acc += cTC * T0 + cTW * T0l;
+               acc += cTS * Tm + cTSE * Tmr;
+               acc += cBC * B0 + cBE * B0r;
+               acc += cBN * Bp + cBNW * Bpl;
+               cy[o] = ( T( 1 ) - omega ) * Z0 + relax * ( cb[o] - acc );
+            }
+            else
+            {
+               T acc = cC * Z0;
+               acc += cW * Z0l + cE * Z0r;
+               acc += cN * Zp + cNW * Zpl;
+               acc += cS * Zm + cSE * Zmr;
+               acc += cTC * T0 + cTW * T0l;
+               acc += cTS * Tm + cTSE * Tmr;
+               acc += cBC * B0 + cBE * B0r;
+               acc += cBN * Bp + cBNW * Bpl;
+               if ( MODE == kReplace )
+                  cy[o] = acc;
+               else if ( MODE == kAdd )
+                  cy[o] += acc;
+               else
+                  cy[o] = cb[o] - acc;
+            }
+         }
+      }
+   };
+   // preload slabs za-1, za, za+1 into LDS; slab za+2 into registers (stage A)
+   for ( int s = za - 1; s <= za + 1; s++ )
+      loadBand( s, lds + ( s & 3 ) * cap );
+   T    stA[kMaxPer], stB[kMaxPer];
+   auto issue = [&]( int s, T* st ) {
+      const int f = s <= zb ? bandFirst( s ) : 0, n = s <= zb ? bandLen( s ) : 0;
+#pragma unroll
+      for ( int j = 0; j < kMaxPer; j++ )
+      {
+         const int k = tid + j * nth;
+         if ( k < n ) st[j] = cu[f + k];
+      }
+   };
+   auto retire = [&]( int s, const T* st ) {
+      const int n = s <= zb ? bandLen( s ) : 0;
+      T*        L = lds + ( s & 3 ) * cap;
+#pragma unroll
+      for ( int j = 0; j < kMaxPer; j++ )
+      {
+         const int k = tid + j * nth;
+         if ( k < n ) L[k] = st[j];
+      }
+   };
+   issue( za + 2, stA );
+   __syncthreads();
+   // step z: slab z+3 is requested, slab z's outputs computed from the LDS bands of z-1, z, z+1, and slab
+   // z+2 (requested one step earlier) is written to the slot of slab z-2.  Two register stages alternate,
+   // so each request has two steps to arrive.
+   auto step = [&]( int z, T* stNext, const T* stReady ) {
+      issue( z + 3, stNext );
+      computeSlab( z );
+      retire( z + 2, stReady );
+      __syncthreads();
+   };
+   int z = za;
+   for ( ; z + 1 < zb; z += 2 )
+   {
+      step( z, stB, stA );
+      step( z + 1, stA, stB );
+   }
+   if ( z < zb ) step( z, stB, stA );
 }
 
 // --------------------------------------------------------------------------------------------------------
@@ -570,6 +742,50 @@ __global__ void kFaceSor( LV g, const FaceGeom* __restrict__ fg, const T* __rest
    const T       acc = faceStencilSum< T, NR >( g, G, fw, u, fo, i, j, r, false );
    const int64_t o   = ( fo + fidx( g.w, i, j ) ) * NR + r;
    u[o]              = ( T( 1 ) - omega ) * u[o] + omega / fw[0] * ( b[o] - acc );
+}
+
+// Row-wise colour pass for one right-hand side: workgroup (row j, face) visits only the points (i, j) of
+// the colour, i = i0, i0 + 3, ...; face and cell offsets use 32-bit arithmetic.  Same sums in the same
+// order as faceStencilSum.
+template < typename T >
+__global__ __launch_bounds__( 64 ) void kFaceSorRows( LV g, const FaceGeom* __restrict__ fg, const T* __restrict__ FW,
+                                                      int flag, T* __restrict__ u, const T* __restrict__ b, T omega,
+                                                      int colour )
+{
+   const FaceGeom& G = fg[blockIdx.y];
+   if ( !( G.type & flag ) ) return;
+   const int w = g.w, j = 1 + blockIdx.x; // interior rows j = 1 .. w-3
+   if ( j > w - 3 ) return;
+   const int t0 = tet32( w );
+   const T*  fw = FW + (int64_t) G.face * kFaceW;
+   const T   rl = omega / fw[0];
+   T*        fu = u + g.offF + (int64_t) G.face * g.fS;
+   const T*  fb = b + g.offF + (int64_t) G.face * g.fS;
+   auto      fi = [&]( int ii, int jj ) { return jj * ( w + 1 ) - ( jj * ( jj + 1 ) >> 1 ) + ii; };
+   // first i >= 1 with (i + 2j) % 3 == colour
+   const int i0 = 1 + ( ( colour - 1 - 2 * j ) % 3 + 3 ) % 3;
+   for ( int i = i0 + 3 * (int) threadIdx.x; i + j <= w - 2; i += 3 * blockDim.x )
+   {
+      T acc = fw[1] * fu[fi( i - 1, j )] + fw[2] * fu[fi( i + 1, j )];
+      acc += fw[3] * fu[fi( i, j + 1 )] + fw[4] * fu[fi( i, j - 1 )];
+      acc += fw[5] * fu[fi( i - 1, j + 1 )] + fw[6] * fu[fi( i + 1, j - 1 )];
+      for ( int sd = 0; sd < G.nSides; sd++ )
+      {
+         const FaceCellMap& M  = G.side[sd];
+         const T*           cu = u + g.offC + (int64_t) M.cell * g.cS;
+#pragma unroll
+         for ( int k = 0; k < 4; k++ )
+         {
+            const int col = i + G.offCol[sd][k], row = j + G.offRow[sd][k];
+            const int x   = M.base[0] + col * M.a[0] + row * M.b[0] + M.o[0];
+            const int y   = M.base[1] + col * M.a[1] + row * M.b[1] + M.o[1];
+            const int z   = M.base[2] + col * M.a[2] + row * M.b[2] + M.o[2];
+            acc += fw[7 + 4 * sd + k] * cu[crow32( w, t0, y, z ) + x];
+         }
+      }
+      const int o = fi( i, j );
+      fu[o]       = ( T( 1 ) - omega ) * fu[o] + rl * ( fb[o] - acc );
+   }
 }
 
 // --------------------------------------------------------------------------------------------------------
@@ -1061,6 +1277,59 @@ __global__ __launch_bounds__( 256 ) void kProlongCell( LV gc, LV gf, const CellS
    fu[( (int64_t) crow( gf.w, (int) tet( gf.w ), y, z ) + x ) * NR + r] += v;
 }
 
+// Flat form for one right-hand side: a wave takes kTransferWindows windows of 64 consecutive elements of one
+// fine slab z (units from Mesh::transferUnits); each lane finds its row y, and computes the two coarse
+// indices with 32-bit arithmetic.  The coarse slabs (z -/+ pz) / 2 are wave-uniform because the
+// z-component of the direction always equals the parity of z.
+template < typename T >
+__global__ __launch_bounds__( 256 ) void kProlongFlat( LV gc, LV gf, const MarchUnit* __restrict__ units,
+                                                       int64_t nUnits, const int* __restrict__ ctype, int flag,
+                                                       const T* __restrict__ uc, T* __restrict__ uf )
+{
+   const int64_t ui = xcdBlock( ( nUnits + 3 ) / 4 ) * 4 + __builtin_amdgcn_readfirstlane( threadIdx.x >> 6 );
+   if ( ui >= nUnits ) return;
+   const MarchUnit U = units[ui];
+   if ( !( ctype[U.cell] & flag ) ) return;
+   const int lane = threadIdx.x & 63;
+   const int w = gf.w, z = U.z, Wd = w - z, wc = gc.w;
+   const int tf = tet32( w ), tc = tet32( wc );
+   const int nEl = Wd * ( Wd + 1 ) / 2;
+   const int S0  = tf - tet32( Wd );
+   const int pz  = z & 1;
+   // coarse slabs of (p - d) / 2 and (p + d) / 2 and their widths
+   const int Zm = ( z - pz ) / 2, Zp = ( z + pz ) / 2;
+   const int Cm = tc - tet32( wc - Zm ), Cp = tc - tet32( wc - Zp ); // coarse slab starts
+   const int Wm = wc - Zm, Wp = wc - Zp;
+   const T*  cu = uc + gc.offC + (int64_t) U.cell * gc.cS;
+   T*        fu = uf + gf.offC + (int64_t) U.cell * gf.cS;
+   auto      rs = [&]( int WW, int yy ) { return yy * ( WW + 1 ) - ( yy * ( yy + 1 ) >> 1 ); };
+   int       e0 = U.x0, y0 = U.y0;
+   for ( int j = 0; j < kTransferWindows; j++, e0 += 64 )
+   {
+      if ( e0 >= nEl ) break;
+      while ( rs( Wd, y0 + 1 ) <= e0 )
+         y0++;
+      int yL = y0;
+      while ( yL + 1 < Wd && rs( Wd, yL + 1 ) <= e0 + 63 )
+         yL++;
+      const int e  = e0 + lane;
+      int       yy = y0, next = rs( Wd, y0 + 1 );
+      for ( int it = y0; it < yL; it++ )
+         if ( e >= next )
+         {
+            yy++;
+            next += Wd - yy;
+         }
+      const int x = e - rs( Wd, yy );
+      if ( !( e < nEl && x >= 1 && yy >= 1 && x + yy <= Wd - 2 ) ) continue;
+      int dx, dy, dz;
+      prolongDir( x & 1, yy & 1, pz, dx, dy, dz );
+      const int im = Cm + rs( Wm, ( yy - dy ) / 2 ) + ( x - dx ) / 2;
+      const int ip = Cp + rs( Wp, ( yy + dy ) / 2 ) + ( x + dx ) / 2;
+      fu[S0 + e] += T( 0.5 ) * ( cu[im] + cu[ip] );
+   }
+}
+
 template < typename T, int NR >
 __global__ void kProlongFace( LV gc, LV gf, const FaceGeom* __restrict__ fg, int flag, const T* __restrict__ uc,
                               T* __restrict__ uf )
@@ -1324,6 +1593,14 @@ void marchLaunch( Mesh& M, LevelPlan& P, const T* CW, int flag, const T* u, T* y
 #define HYMI_MARCH( KERNEL, RR )                                                                                   \
    hipLaunchKernelGGL( ( KERNEL< T, MODE, RR > ), grid, block, 0, s, g, mu, n, CW, P.dCellType.p, flag, u, y, b,    \
                        omega, colour, S.rows )
+   if ( S.kind == 2 )
+   {
+      const int    cap = tileCap( P.w, S.rows );
+      const size_t lds = 4 * (size_t) cap * sizeof( T );
+      hipLaunchKernelGGL( ( kCellTile< T, MODE > ), dim3( (unsigned) xcdGrid( n ) ), dim3( 64 * S.waves ), lds, s, g, mu, n,
+                          CW, P.dCellType.p, flag, u, y, b, omega, colour, S.rows, cap );
+      return;
+   }
    if ( S.kind == 1 )
    {
       switch ( S.block )
@@ -1413,8 +1690,8 @@ void refreshT( Function& f, int level, int mask )
       hipLaunchKernelGGL( ( kE2F< T, NR > ), dim3( blocksFor( 3 * (int64_t) M.T.nF() * P.w * NR, 256 ) ), dim3( 256 ), 0,
                           s, g, P.dE2F.p, 3 * M.T.nF(), u );
    if ( ( mask & 4 ) && M.T.nC() > 0 )
-      hipLaunchKernelGGL( ( kF2C< T, NR > ), dim3( blocksFor( triSize( P.w ) * NR, 256 ), 4 * M.T.nC() ), dim3( 256 ), 0, s,
-                          g, P.dF2C.p, u );
+      hipLaunchKernelGGL( ( kF2C< T, NR > ), dim3( P.w, 4 * M.T.nC() ), dim3( P.w * NR <= 64 ? 64 : 128 ), 0, s, g,
+                          P.dF2C.p, u );
 }
 
 template < typename T, int NR >
@@ -1489,11 +1766,9 @@ void sorT( Operator& op, Function& xf, const Function& bf, double omega, int lev
          {
             // large faces: one workgroup per face would leave most of the chip idle
             refreshT< T, NR >( xf, level, 2 );
-            const int64_t m = P.w - 3;
             for ( int c = 0; c < 3; c++ )
-               hipLaunchKernelGGL( ( kFaceSor< T, NR > ), dim3( blocksFor( m * ( m + 1 ) / 2 * NR, 256 ), nF ),
-                                   dim3( 256 ), 0, s, g, P.dFaceGeom.p, wsel< T >( Wt.face64, Wt.face32 ), flag, u,
-                                   b, om, c );
+               hipLaunchKernelGGL( ( kFaceSorRows< T > ), dim3( P.w - 3, nF ), dim3( 64 ), 0, s, g, P.dFaceGeom.p,
+                                   wsel< T >( Wt.face64, Wt.face32 ), flag, u, b, om, c );
          }
          refreshT< T, NR >( xf, level, 4 );
          for ( int c = 0; c < 4; c++ )
@@ -1576,6 +1851,15 @@ void prolongT( Function& f, int srcLevel, int flag, bool refreshCoarse )
    if ( m > 0 && nF > 0 )
       hipLaunchKernelGGL( ( kProlongFace< T, NR > ), dim3( blocksFor( m * ( m + 1 ) / 2 * NR, 256 ), nF ), dim3( 256 ), 0,
                           s, gc, gf, Pf.dFaceGeom.p, flag, uc, uf );
+   if ( NR == 1 )
+   {
+      int64_t          nu = 0;
+      const MarchUnit* tu = M.transferUnits( Pf, &nu );
+      if ( nu > 0 )
+         hipLaunchKernelGGL( ( kProlongFlat< T > ), dim3( (unsigned) xcdGrid( ( nu + 3 ) / 4 ) ), dim3( 256 ), 0, s, gc, gf,
+                             tu, nu, Pf.dCellType.p, flag, uc, uf );
+      return;
+   }
    int64_t        nseg = 0;
    const CellSeg* sg   = M.segments( Pf, segLen< NR >(), &nseg );
    if ( nseg > 0 )

@@ -26,10 +26,17 @@ struct MarchUnit
    int32_t cell, z, x0, y0;
 };
 constexpr int kMarchOutput = 62; // output lanes per wave (lanes 1..62)
+constexpr int kTransferWindows = 4; // 64-point windows per unit of the flat grid-transfer kernels
+// LDS slab tiles: elements of one band (rows ya-1 .. ya+R of a slab of width <= w) and the limits the kernel
+// relies on (8 staged elements per thread and stage, 64 KB of LDS for the four slots in FP64)
+inline int tileCap( int w, int R ) { return ( R + 2 ) * w; }
+constexpr int kTileMaxPer = 8;
+constexpr int kTileMaxLds = 64 * 1024;
 
 // launch schedule of the macro-cell kernels on one level.  kind 0: row march (units of `rows` rows of one
 // x-window, loaded `block` rows at a time); kind 1: flat windows (units of `rows` 64-element windows of one
-// slab, loaded `block` windows at a time).  waves = waves per workgroup (1, 2 or 4).
+// slab, loaded `block` windows at a time); kind 2: LDS slab tiles (a band of `rows` rows of one cell marched
+// over `block` slabs by one workgroup).  waves = waves per workgroup (1, 2 or 4).
 struct MarchSchedule
 {
    int rows = 0, waves = 0, block = 0, kind = -1; // 0 / -1 = level default

@@ -448,24 +448,32 @@ const CellSeg* Mesh::segments( LevelPlan& P, int len, int64_t* count )
 }
 
 // Default launch schedule of the macro-cell kernels, from a sweep of kind x rows x waves x block on MI355X
-// (48 macro-cells, levels 5-7, FP64 apply and SOR colour pass): 8-row units with 4-row load blocks are
-// fastest from level 6 up; smaller levels need shorter units to fill the GPU.
+// (48 macro-cells, levels 5-7, FP64 apply and SOR colour pass): flat slab windows win on every level (they
+// keep ~90 % of the lanes busy where row windows keep ~50 %); two windows per unit from level 6 up, one
+// below, where more waves are needed to fill the GPU.
 MarchSchedule defaultSchedule( int level )
 {
    MarchSchedule S;
-   S.kind  = 0;
-   S.rows  = level >= 6 ? 8 : 4;
-   S.waves = level >= 7 ? 1 : 2;
-   S.block = 4;
+   S.kind  = 1;
+   S.rows  = level >= 6 ? 2 : 1;
+   S.waves = level >= 6 ? 2 : 4;
+   S.block = level >= 6 ? 2 : 1;
    return S;
 }
 
-void checkSchedule( const MarchSchedule& S )
+void checkSchedule( const MarchSchedule& S, int level )
 {
-   if ( S.kind != 0 && S.kind != 1 ) throw std::invalid_argument( "schedule kind must be 0 (rows) or 1 (flat)" );
+   if ( S.kind < 0 || S.kind > 2 ) throw std::invalid_argument( "schedule kind must be 0 (rows), 1 (flat) or 2 (tiles)" );
    if ( S.rows < 1 || S.rows > 4096 ) throw std::invalid_argument( "march rows must be in 1..4096" );
    if ( S.waves != 1 && S.waves != 2 && S.waves != 4 ) throw std::invalid_argument( "march waves must be 1, 2 or 4" );
-   if ( S.block != 1 && S.block != 2 && S.block != 4 && S.block != 8 )
+   if ( S.kind == 2 )
+   {
+      const int cap = tileCap( widthOf( level ), S.rows );
+      if ( S.block < 1 || S.block > 4096 ) throw std::invalid_argument( "tile slabs must be in 1..4096" );
+      if ( cap > kTileMaxPer * 64 * S.waves || 4 * cap * 8 > kTileMaxLds )
+         throw std::invalid_argument( "tile band too large for this level (fewer rows or more waves)" );
+   }
+   else if ( S.block != 1 && S.block != 2 && S.block != 4 && S.block != 8 )
       throw std::invalid_argument( "march block rows must be 1, 2, 4 or 8" );
 }
 
@@ -483,7 +491,8 @@ int slabRowStart( int W, int y ) { return y * ( W + 1 ) - y * ( y + 1 ) / 2; }
 const MarchUnit* Mesh::marchUnits( LevelPlan& P, int64_t* count )
 {
    if ( P.sched.rows == 0 ) P.sched = defaultSchedule( P.L );
-   if ( P.marchCount < 0 || P.marchRowsBuilt != P.sched.rows || P.marchKindBuilt != P.sched.kind )
+   if ( P.marchCount < 0 || P.marchRowsBuilt != P.sched.rows || P.marchKindBuilt != P.sched.kind ||
+        ( P.sched.kind == 2 && P.marchBlockBuilt != P.sched.block ) )
    {
       std::vector< MarchUnit > v;
       const int                w    = P.w;
@@ -491,7 +500,19 @@ const MarchUnit* Mesh::marchUnits( LevelPlan& P, int64_t* count )
       for ( int c = 0; c < T.nC(); c++ )
          for ( int z = 1; z <= w - 4; z++ )
          {
-            if ( P.sched.kind == 0 )
+            if ( P.sched.kind == 2 )
+            {
+               // one unit per (band of R rows, segment of `block` slabs); slabs with interior rows in the band
+               if ( z != 1 ) continue;
+               const int R = rows, Zc = P.sched.block;
+               for ( int ya = 1; ya <= w - 4; ya += R )
+               {
+                  const int zmax = w - 3 - ya;
+                  for ( int za = 1; za <= zmax; za += Zc )
+                     v.push_back( { c, za, std::min( za + Zc, zmax + 1 ), ya } );
+               }
+            }
+            else if ( P.sched.kind == 0 )
             {
                for ( int x0 = 1; x0 <= w - 3 - z; x0 += kMarchOutput )
                   for ( int y0 = 1; y0 <= w - 2 - z - x0; y0 += rows )
@@ -518,9 +539,39 @@ const MarchUnit* Mesh::marchUnits( LevelPlan& P, int64_t* count )
       P.marchCount     = (int64_t) v.size();
       P.marchRowsBuilt = rows;
       P.marchKindBuilt = P.sched.kind;
+      P.marchBlockBuilt = P.sched.block;
    }
    *count = P.marchCount;
    return P.march->p;
+}
+
+const MarchUnit* Mesh::transferUnits( LevelPlan& P, int64_t* count )
+{
+   if ( P.transferCount < 0 )
+   {
+      std::vector< MarchUnit > v;
+      const int                w = P.w;
+      for ( int c = 0; c < T.nC(); c++ )
+         for ( int z = 1; z <= w - 4; z++ )
+         {
+            const int W     = w - z;
+            const int first = slabRowStart( W, 1 ) + 1;
+            const int lastE = slabRowStart( W, W - 3 ) + 1;
+            int       y     = 0;
+            for ( int e0 = first; e0 <= lastE; e0 += 64 * kTransferWindows )
+            {
+               while ( slabRowStart( W, y + 1 ) <= e0 )
+                  y++;
+               v.push_back( { c, z, e0, y } );
+            }
+         }
+      P.transfer = std::make_unique< DevBuf< MarchUnit > >();
+      P.transfer->upload( v, ctx->stream );
+      HYMI_HIP( hipStreamSynchronize( ctx->stream ) );
+      P.transferCount = (int64_t) v.size();
+   }
+   *count = P.transferCount;
+   return P.transfer->p;
 }
 
 void Mesh::setSchedule( int level, const MarchSchedule& S )
@@ -532,7 +583,7 @@ void Mesh::setSchedule( int level, const MarchSchedule& S )
    if ( N.rows == 0 ) N.rows = D.rows;
    if ( N.waves == 0 ) N.waves = D.waves;
    if ( N.block == 0 ) N.block = D.block;
-   checkSchedule( N );
+   checkSchedule( N, level );
    P.sched = N;
    scheduleEpoch++;
    int64_t n = 0;

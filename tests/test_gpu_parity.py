@@ -260,7 +260,8 @@ def test_march_schedules_bitwise(prec, gpu_ctx, oracle_lib):
     ref = run()
     for rows, waves, block, kind in [(1, 1, 1, 0), (3, 2, 2, 0), (4, 4, 8, 0), (8, 4, 4, 0), (64, 2, 8, 0),
                                      (2, 1, 8, 0), (13, 4, 2, 0), (7, 1, 4, 0), (1, 1, 1, 1), (4, 2, 2, 1),
-                                     (8, 4, 4, 1), (3, 1, 8, 1), (2, 2, 1, 1), (64, 4, 4, 1)]:
+                                     (8, 4, 4, 1), (3, 1, 8, 1), (2, 2, 1, 1), (64, 4, 4, 1),
+                                     (4, 4, 1, 2), (3, 4, 2, 2), (8, 2, 5, 2), (12, 4, 64, 2), (2, 1, 3, 2)]:
         S.set_schedule(L, rows, waves, block, kind)
         got = run()
         for a, r in zip(got, ref):
