@@ -341,16 +341,14 @@ struct hymi_gmg_s
       // ghost copies of x are current after a sweep and after the coarse-level reset; only the caller's x on
       // the finest level, x after prolongation and x after the coarse solver need a refresh
       const bool finest = ( L == maxL_invoked );
-      for ( int i = 0; i < nu1; i++ )
-         launchSor( *A, x, b, omega, L, flag, i == 0 && finest );
+      launchSorSweeps( *A, x, b, omega, L, flag, nu1, finest );
       launchApply( *A, x, *tmp, &b, L, flag, kResidual, nu1 == 0 && finest );
       // restriction written straight into b on the coarse level (== restrict + b.assign on flag DoFs)
       launchRestrict( *tmp, b, L, flag );
       launchLincomb( x, L - 1, kAll, 0, nullptr, nullptr, false, 0.0 );
       recurse( x, b, L - 1 );
       launchProlongateAdd( x, L - 1, flag, L - 1 == minL );
-      for ( int i = 0; i < nu2; i++ )
-         launchSor( *A, x, b, omega, L, flag, i == 0 );
+      launchSorSweeps( *A, x, b, omega, L, flag, nu2, true );
    }
 };
 

@@ -122,6 +122,7 @@ struct LevelPlan
    // grid-transfer kernels; fields as for the flat macro-cell kernel (x0 = first element, y0 = its row)
    std::unique_ptr< DevBuf< MarchUnit > > transfer;
    int64_t                                transferCount = -1;
+
    MarchSchedule                          sched;
    // closure ranges of primitives matching a flag, keyed by flag
    std::map< int, std::unique_ptr< DevBuf< Range > > > ranges;
@@ -142,6 +143,7 @@ struct Mesh
    const CellSeg* segments( LevelPlan& P, int len, int64_t* count );
    const MarchUnit* marchUnits( LevelPlan& P, int64_t* count );
    const MarchUnit* transferUnits( LevelPlan& P, int64_t* count );
+
    // replaces the launch schedule of the row-marching kernels on one level (zeros = level default);
    // bumps scheduleEpoch so that captured graphs holding the old work-unit table are captured again
    void     setSchedule( int level, const MarchSchedule& S );

@@ -1103,6 +1103,7 @@ __global__ void kModesCell( LV g, const CellSeg* __restrict__ segs, int64_t nseg
 }
 
 
+
 // --------------------------------------------------------------------------------------------------------
 // restriction (coarse owned point <- fine neighbourhood of 2p, weights 1 and 1/2)
 
@@ -1804,6 +1805,14 @@ void sorT( Operator& op, Function& xf, const Function& bf, double omega, int lev
 }
 
 template < typename T, int NR >
+void sorSweepsT( Operator& op, Function& xf, const Function& bf, double omega, int level, int flag, int count,
+                 bool refreshFirst )
+{
+   for ( int k = 0; k < count; k++ )
+      sorT< T, NR >( op, xf, bf, omega, level, flag, refreshFirst && k == 0 );
+}
+
+template < typename T, int NR >
 void restrictT( Function& f, Function& dst, int srcLevel, int flag )
 {
    Mesh&       M  = *f.mesh;
@@ -1971,6 +1980,14 @@ struct SorF
    }
 };
 template < typename T, int NR >
+struct SorSweepsF
+{
+   static void run( Operator& op, Function& x, const Function& b, double omega, int L, int flag, int count, bool refresh )
+   {
+      sorSweepsT< T, NR >( op, x, b, omega, L, flag, count, refresh );
+   }
+};
+template < typename T, int NR >
 struct RestrictF
 {
    static void run( Function& f, Function& d, int L, int flag ) { restrictT< T, NR >( f, d, L, flag ); }
@@ -2020,6 +2037,12 @@ void launchApply( Operator& op, Function& src, Function& dst, const Function* rh
 void launchSor( Operator& op, Function& x, const Function& b, double omega, int level, int flag, bool refreshFirst )
 {
    dispatch< SorF >( x.prec, x.nrhs, op, x, b, omega, level, flag, refreshFirst );
+}
+
+void launchSorSweeps( Operator& op, Function& x, const Function& b, double omega, int level, int flag, int count,
+                      bool refreshFirst )
+{
+   dispatch< SorSweepsF >( x.prec, x.nrhs, op, x, b, omega, level, flag, count, refreshFirst );
 }
 
 void launchRestrict( Function& f, Function& dst, int srcLevel, int flag )

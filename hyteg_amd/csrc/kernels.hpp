@@ -22,6 +22,9 @@ void launchApply( Operator& op, Function& src, Function& dst, const Function* rh
 // initial ghost-copy refresh when the caller knows the copies of x are current (they are after a sweep)
 void launchSor( Operator& op, Function& x, const Function& b, double omega, int level, int flag,
                 bool refreshFirst = true );
+// `count` SOR sweeps (refreshFirst applies to the first)
+void launchSorSweeps( Operator& op, Function& x, const Function& b, double omega, int level, int flag, int count,
+                      bool refreshFirst = true );
 // coarse level of dst <- restriction of the fine level of f (dst may be f)
 void launchRestrict( Function& f, Function& dst, int srcLevel, int flag );
 void launchProlongateAdd( Function& f, int srcLevel, int flag, bool refreshCoarse = true );

@@ -170,13 +170,17 @@ int hymi_time_kernel( hymi_operator op, hymi_gmg gmg, hymi_function x, hymi_func
                       int which, int reps, double* ms_per_launch );
 
 /* ---- launch schedules ------------------------------------------------------------------------------
- * No reference counterpart (the reference's CPU kernels have no launch shape).  The row-marching
- * macro-cell kernels of one level run with a schedule: rows per work unit (1..4096), waves per
- * workgroup (1, 2 or 4) and output rows per load block (1, 2, 4 or 8); 0 selects the level default.
- * Setting a schedule waits for the context stream; V-cycle graphs captured before it are captured again
- * on their next replay.  out = {rows, waves, block}. */
-int hymi_mesh_set_schedule( hymi_mesh mesh, int level, int rows, int waves, int block );
-int hymi_mesh_get_schedule( hymi_mesh mesh, int level, int out[3] );
+ * No reference counterpart (the reference's CPU kernels have no launch shape).  The macro-cell kernels of
+ * one level run with a schedule: kind 0 walks x-windows of one row at a time (units of `rows` rows, loaded
+ * `block` rows at a time, block 1, 2, 4 or 8), kind 1 walks each slab as one flat array (units of `rows`
+ * windows of 64 elements, loaded `block` windows at a time, block 1, 2, 4 or 8), kind 2 stages slab bands
+ * in LDS (a workgroup marches a band of `rows` rows of one cell over `block` slabs, 1..4096); waves per
+ * workgroup 1, 2 or 4; rows 1..4096.  kind -1 and zeros select the level default.  A schedule the level
+ * cannot run (a kind-2 band too large for LDS) is refused.  Results do not depend on the schedule.  Setting a
+ * schedule waits for the context stream; V-cycle graphs captured before it are captured again on their
+ * next replay.  out = {rows, waves, block, kind}. */
+int hymi_mesh_set_schedule( hymi_mesh mesh, int level, int kind, int rows, int waves, int block );
+int hymi_mesh_get_schedule( hymi_mesh mesh, int level, int out[4] );
 
 #ifdef __cplusplus
 }
