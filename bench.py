@@ -42,12 +42,18 @@ def time_kernel(H, A, gmg_h, x, b, y, level, which, reps):
     return ms.value
 
 
-def traffic_from_profile(kernel):
-    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE), if a summary exists."""
+KIND_NAMES = {0: "kCellMarch (row windows)", 1: "kCellFlat (flat slab windows)", 2: "kCellTile (LDS slab tiles)"}
+
+
+def traffic_from_profile(kernel, kernel_name):
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE), if a summary for the kernel
+    that runs exists (profiles/traffic_<kernel>.json names the kernel it was measured on)."""
     path = os.path.join(ROOT, "profiles", f"traffic_{kernel}.json")
     if os.path.exists(path):
         with open(path) as fh:
-            return json.load(fh).get("bytes_per_launch")
+            d = json.load(fh)
+        if d.get("kernel", "").split("<")[0] == kernel_name.split(" ")[0]:
+            return d.get("bytes_per_launch")
     return None
 
 
@@ -84,12 +90,7 @@ def kernels_only(H, n, L, reps):
     b.interpolate(1.0, L, H.ALL)
     t_apply = time_kernel(H, A, None, x, None, y, L, 0, reps)
     t_sor = time_kernel(H, A, None, x, b, None, L, 1, reps)
-    S.ctx.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        y.assign([1.0], [x], L, H.ALL)
-    S.ctx.synchronize()
-    t_copy = (time.perf_counter() - t0) * 1e3 / reps
+    t_copy = time_kernel(H, A, None, x, None, y, L, 5, reps)
     copy_bytes = S.ctx.bytes_in_use() // 3       # one level of one function (x, b, y are alike)
     return {"apply_ms": t_apply, "sor_pass_ms": t_sor, "copy_ms": t_copy, "copy_bytes_each_way": copy_bytes,
             "copy_GBs": 2 * copy_bytes / (t_copy * 1e-3) / 1e9}
@@ -302,15 +303,17 @@ def main():
     t_apply = time_kernel(H, A, None, x, None, y, L, 0, reps)
     t_sor = time_kernel(H, A, None, x, b, None, L, 1, reps)
     t_vc = time_kernel(H, A, gh, x, b, None, L, 4, 3)
+    sched = S.get_schedule(L)
+    kname = KIND_NAMES[sched["kind"]]
     s = 8
     apply_bytes = 2 * s * Ncell                 # read x once, write y once
     sor_bytes = 3 * s * Ncell / 4               # one colour pass = a quarter of a sweep's read x, b / write x
     sor_share = 6 * 4 * t_sor / t_vc            # V(3,3): 6 sweeps x 4 colour passes on the finest level
     roof_sor = {"bound": "hbm", "achieved": sor_bytes / (t_sor * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "traffic": traffic_from_profile("cell_sor")}
+                "traffic": traffic_from_profile("cell_sor", kname)}
     roof_sor["frac"] = roof_sor["achieved"] / HBM_PEAK_GBS
     roof_apply = {"bound": "hbm", "achieved": apply_bytes / (t_apply * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
-                  "unit": "GB/s", "traffic": traffic_from_profile("cell_apply")}
+                  "unit": "GB/s", "traffic": traffic_from_profile("cell_apply", kname)}
     roof_apply["frac"] = roof_apply["achieved"] / HBM_PEAK_GBS
 
     out = {
@@ -337,10 +340,10 @@ def main():
             "rel_residual_after_all_cycles": rel_res,
             "cycles_total": 1 + args.warmup + args.steps + reps * 0,
         },
-        "roofline": dict(roof_sor, kernel="kCellSor (one colour pass)",
+        "roofline": dict(roof_sor, kernel=f"{kname}, SOR, one colour pass", schedule=sched,
                          algorithmic_bytes_per_launch=sor_bytes, ms_per_launch=t_sor,
                          share_of_vcycle=sor_share),
-        "roofline_apply": dict(roof_apply, kernel="kCellApply (15-point stencil)",
+        "roofline_apply": dict(roof_apply, kernel=f"{kname}, 15-point apply", schedule=sched,
                                algorithmic_bytes_per_launch=apply_bytes, ms_per_launch=t_apply),
         "vcycle_ms_events": t_vc,
     }

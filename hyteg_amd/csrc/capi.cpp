@@ -1225,7 +1225,7 @@ int hymi_time_kernel( hymi_operator op, hymi_gmg gmg, hymi_function x, hymi_func
       require( op && x && ms && reps > 0, "bad argument" );
       checkOpFunc( op, x, level );
       if ( which == 1 || which == 3 || which == 4 ) checkOpFunc( op, b, level );
-      if ( which == 0 || which == 2 ) checkOpFunc( op, y, level );
+      if ( which == 0 || which == 2 || which == 5 ) checkOpFunc( op, y, level );
       if ( which == 4 ) checkGmg( gmg, x, b, level );
       Operator&   A = op->op;
       hipStream_t s = A.mesh->ctx->stream;
@@ -1238,6 +1238,14 @@ int hymi_time_kernel( hymi_operator op, hymi_gmg gmg, hymi_function x, hymi_func
          case 2: launchApply( A, x->f, y->f, nullptr, level, flag, kReplace ); break;
          case 3: launchSor( A, x->f, b->f, 1.0, level, flag ); break;
          case 4: gmg->cycle( x->f, b->f, level ); break;
+         case 5:
+         {
+            // plain copy y = x of the whole level (closures of all primitives): a bandwidth reference
+            const double one[1] = { 1.0 };
+            Function*    src[1] = { &x->f };
+            launchLincomb( y->f, level, kAll, 1, one, src, false, 0.0 );
+            break;
+         }
          default: throw std::invalid_argument( "bad kernel selector" );
          }
       };

@@ -165,7 +165,8 @@ int hymi_solve_refined( hymi_gmg g, hymi_function x, hymi_function b, int level,
  * No reference counterpart (the reference times with its WcTimingTree, src/hyteg/operators/Operator.hpp:
  * 99-115).  Average duration in ms of `reps` back-to-back launches on the context stream, bracketed by
  * hipEvents recorded on that stream.  which: 0 macro-cell apply kernel (x -> y), 1 macro-cell SOR colour
- * pass (x, b; colour 0), 2 full apply (x -> y), 3 full SOR sweep (x, b), 4 one V-cycle of gmg (x, b). */
+ * pass (x, b; colour 0), 2 full apply (x -> y), 3 full SOR sweep (x, b), 4 one V-cycle of gmg (x, b),
+ * 5 plain copy y = x of the whole level (a bandwidth reference). */
 int hymi_time_kernel( hymi_operator op, hymi_gmg gmg, hymi_function x, hymi_function b, hymi_function y, int level,
                       int which, int reps, double* ms_per_launch );
 
