@@ -42,7 +42,8 @@ def time_kernel(H, A, gmg_h, x, b, y, level, which, reps):
     return ms.value
 
 
-KIND_NAMES = {0: "kCellMarch (row windows)", 1: "kCellFlat (flat slab windows)", 2: "kCellTile (LDS slab tiles)"}
+KIND_NAMES = {0: "kCellMarch (row windows)", 1: "kCellFlat (flat slab windows)", 2: "kCellTile (LDS slab tiles)",
+              3: "kCellColumn (columns marched along z)"}
 
 
 def traffic_from_profile(kernel, kernel_name):
@@ -84,6 +85,7 @@ def kernels_only(H, n, L, reps):
     """Each measured kernel alone, `reps` launches: the macro-cell apply, one SOR colour pass, and a plain
     copy of a whole level (the reference for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)."""
     S = H.PrimitiveStorage(H.MeshInfo.meshCuboid((0, 0, 0), (1, 1, 1), n, n, n))
+    S.use_column_schedules(L)
     A = H.P1ConstantLaplaceOperator(S, L, L)
     x, b, y = (H.P1Function(k, S, L, L) for k in "xby")
     x.interpolate_modes(sine_modes(), L)
@@ -100,6 +102,7 @@ def mixed_vs_fp64(H, n, L, tol, seed, cycles_per_outer):
     """FP32 V(2,2) cycles inside FP64 iterative refinement against plain FP64 V(2,2) cycles, both from x = 0
     to relative residual tol, on b = M I_h f + U[-1, 1] * 1e-3 per DoF."""
     S = H.PrimitiveStorage(H.MeshInfo.meshCuboid((0, 0, 0), (1, 1, 1), n, n, n))
+    S.use_column_schedules(L)
     A = H.P1ConstantLaplaceOperator(S, 0, L)
     M = H.P1ConstantMassOperator(S, 0, L)
     x, b, t = (H.P1Function(k, S, 0, L) for k in ("x", "b", "t"))
@@ -233,7 +236,7 @@ def serving_run(count, rate, slots, seed):
     srv.close()
     out = summarize(res, srv.wall)
     out.update({"rate_per_s": rate, "slots": slots, "seed": seed, "wall_s": srv.wall, "warm_s": warm_s,
-                "errors": len(srv.errors)})
+                "errors": len(srv.errors), "error_samples": [e for _, e in srv.errors[:3]]})
     return out
 
 
@@ -269,6 +272,7 @@ def main():
         print(json.dumps(kernels_only(H, n, L, max(5, args.steps))))
         return
     S = H.PrimitiveStorage(H.MeshInfo.meshCuboid((0, 0, 0), (1, 1, 1), n, n, n))
+    S.use_column_schedules(L)
     A = H.P1ConstantLaplaceOperator(S, 0, L)
     M = H.P1ConstantMassOperator(S, 0, L)
     x = H.P1Function("u", S, 0, L)

@@ -141,9 +141,18 @@ class PrimitiveStorage:
         return out
 
     def set_schedule(self, level, rows=0, waves=0, block=0, kind=-1):
-        """Launch schedule of the macro-cell kernels on `level`: kind 0 row windows, 1 flat slab windows;
+        """Launch schedule of the macro-cell kernels on `level`: kind 0 row windows, 1 flat slab windows,
+        2 LDS slab tiles, 3 columns marched along z (rows = slabs per unit, block = prefetch distance);
         -1 / 0 = level default."""
         _lib.call("hymi_mesh_set_schedule", self.h, level, kind, rows, waves, block)
+
+    def use_column_schedules(self, maxLevel, minLevel=5):
+        """Select the column march (kind 3) with its measured best parameters on levels minLevel..maxLevel:
+        16 slabs per unit and 4 waves per workgroup from level 7 up, 8 and 4 on level 6, 4 and 2 below;
+        loads one slab ahead."""
+        for L in range(minLevel, maxLevel + 1):
+            rows, waves = (16, 4) if L >= 7 else (8, 4) if L == 6 else (4, 2)
+            self.set_schedule(L, rows, waves, 1, 3)
 
     def get_schedule(self, level):
         out = np.zeros(4, np.int32)

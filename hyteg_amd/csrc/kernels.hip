@@ -303,6 +303,19 @@ struct CellBuf
       else
          return __builtin_bit_cast( T, __builtin_amdgcn_raw_buffer_load_b32( r, idx * 4, 0, 0 ) );
    }
+   // store v at idx if keep, else nothing: a dropped lane's byte offset 0x80000000 lies past the range, and
+   // the hardware discards stores outside it, so the store needs no branch
+   __device__ __forceinline__ void store( int idx, T v, bool keep ) const
+   {
+      const int off = keep ? idx * (int) sizeof( T ) : (int) 0x80000000u;
+      if constexpr ( sizeof( T ) == 8 )
+      {
+         typedef unsigned int u32x2 __attribute__( ( ext_vector_type( 2 ) ) );
+         __builtin_amdgcn_raw_buffer_store_b64( __builtin_bit_cast( u32x2, v ), r, off, 0, 0 );
+      }
+      else
+         __builtin_amdgcn_raw_buffer_store_b32( __builtin_bit_cast( unsigned int, v ), r, off, 0, 0 );
+   }
 };
 
 template < typename T, int MODE, int R >
@@ -662,6 +675,147 @@ __global__ __launch_bounds__( 256 ) void kCellTile( LV g, const MarchUnit* __res
       step( z + 1, stA, stB );
    }
    if ( z < zb ) step( z, stB, stA );
+}
+
+// Column march along z (schedule kind 3).  A wave owns a fixed set of 64 positions (x, y): the elements
+// ws-1 .. ws+62 of slab za, lanes 1..62 producing outputs; it walks them up through slabs za .. zEnd.  In
+// slab s (width W_s = w - s) position (x, y) sits at the cell-relative index P(s), with
+//   P(s+1) = P(s) + W_s (W_s + 1) / 2 - y        (slab s+1 is one narrower, so its row y starts y earlier)
+// and its row neighbours at P(s) + W_s - y (y+1) and P(s) - (W_s - y + 1) (y-1).  The outputs of slab z read
+// (x, y), (x, y+1), (x, y-1) of slab z, (x, y), (x, y-1) of slab z+1 and (x, y), (x, y+1) of slab z-1, so the
+// three values of a slab, loaded once, serve three consecutive steps: a step loads three values per lane (plus
+// b for the residual and SOR) where a slab-local kernel loads seven.  The x-1 / x+1 values are DPP wave
+// shifts of the same lane set (lanes of an output keep one row y in every slab: x+1 <= w-2-z-y), each
+// computed once per slab and used twice.  Slab values live in a ring of 3 + D register sets, set
+// (s - za + 1) mod (3 + D) for slab s, with loads issued D slabs ahead; the step loop is unrolled over one
+// turn of the ring, so each set keeps fixed registers and no step waits on a load issued for a later one.
+// Lanes that produce no output may read positions outside the slab; CellBuf turns reads outside the cell
+// into 0 and such values reach no output.  MarchUnit fields: cell, z = za | zEnd << 16, x0 = ws, y0 = row
+// of element ws - 1 in slab za.
+template < typename T, int MODE, int D >
+__global__ __launch_bounds__( 256 ) void kCellColumn( LV g, const MarchUnit* __restrict__ units, int64_t nUnits,
+                                                      const T* __restrict__ W8, const int* __restrict__ ctype, int flag,
+                                                      const T* u, T* y, const T* b, T omega, int colour, int rows )
+{
+   constexpr int NS   = 3 + D;
+   const int     wpb  = blockDim.x >> 6;
+   const int     wave = __builtin_amdgcn_readfirstlane( threadIdx.x >> 6 );
+   const int64_t ui   = xcdBlock( ( nUnits + wpb - 1 ) / wpb ) * wpb + wave;
+   if ( ui >= nUnits ) return;
+   const MarchUnit U = units[ui];
+   if ( !( ctype[U.cell] & flag ) ) return;
+   const int     lane = threadIdx.x & 63;
+   const int     w = g.w, za = U.z & 0xffff, zEnd = U.z >> 16, Wa = w - za;
+   const int64_t base = g.offC + (int64_t) U.cell * g.cS;
+   const CellBuf< T > cu( u + base, g.cS );
+   const CellBuf< T > cb( ( MODE == kAdd ? y : b ) + base, g.cS ); // b (residual, SOR) or y (add)
+   const CellBuf< T > cy( y + base, g.cS );
+   const int     t0   = tet32( w );
+   const T*      c    = W8 + (int64_t) U.cell * 15;
+   const T cC = c[dC], cW = c[dW], cE = c[dE], cN = c[dN], cS = c[dS], cNW = c[dNW], cSE = c[dSE];
+   const T cTC = c[dTC], cTW = c[dTW], cTS = c[dTS], cTSE = c[dTSE], cBC = c[dBC], cBN = c[dBN], cBE = c[dBE],
+           cBNW = c[dBNW];
+   const T relax = omega / cC;
+   auto    rs    = [&]( int yy ) { return yy * ( Wa + 1 ) - yy * ( yy + 1 ) / 2; };
+   // position (x, yy) of this lane's element e of slab za: the row of the first lane is y0, rows advance
+   // by Wa - yy elements
+   const int ws = U.x0, y0 = U.y0;
+   int       yL = y0, nL = rs( y0 + 1 );
+   while ( yL + 1 < Wa && nL <= ws + 62 )
+   {
+      yL++;
+      nL += Wa - yL;
+   }
+   const int e  = ws - 1 + lane;
+   int       yy = y0, next = rs( y0 + 1 );
+   for ( int it = y0; it < yL; it++ )
+      if ( e >= next )
+      {
+         yy++;
+         next += Wa - yy;
+      }
+   const int  x    = e - rs( yy );
+   const bool cand = lane >= 1 && lane <= 62 && e < Wa * ( Wa + 1 ) / 2 && x >= 1 && yy >= 1;
+   const int  xy   = x + yy;
+   // next slab to load: number sl, width Wl, this lane's index Pl (slab za-1 first)
+   int sl = za - 1, Wl = Wa + 1;
+   int Pl = t0 - tet32( Wa ) + e - ( Wa + 1 ) * ( Wa + 2 ) / 2 + yy;
+   int Po = t0 - tet32( Wa ) + e; // index of the output position in slab z
+   T   v0[NS], vp[NS], vm[NS], vb[NS], l0[NS], r0[NS], lp[NS], rm[NS];
+   constexpr bool needB = MODE != kReplace;
+   auto load = [&]( int k ) {
+      v0[k] = cu[Pl];
+      vp[k] = cu[Pl + Wl - yy];
+      vm[k] = cu[Pl - ( Wl - yy + 1 )];
+      if ( needB ) vb[k] = cb[Pl];
+      Pl += Wl * ( Wl + 1 ) / 2 - yy;
+      Wl--;
+      sl++;
+   };
+   auto shifts = [&]( int k ) {
+      l0[k] = laneDown( v0[k] );
+      r0[k] = laneUp( v0[k] );
+      lp[k] = laneDown( vp[k] );
+      rm[k] = laneUp( vm[k] );
+   };
+   // slabs za-1 .. za+D.  Loads are issued unconditionally, also for slabs past zEnd+1 (the last one an
+   // output reads): a skipped load would leave the compiler unable to count outstanding loads, and it
+   // would then wait for all of them in every step.
+#pragma unroll
+   for ( int k = 0; k < D + 2; k++ )
+   {
+      load( k );
+      __builtin_amdgcn_sched_barrier( 0 );
+   }
+   shifts( 0 );
+   shifts( 1 );
+   __builtin_amdgcn_sched_barrier( 0 );
+   // whole turns of the ring; steps past zEnd store nothing
+   for ( int jb = 0; za + jb <= zEnd; jb += NS )
+   {
+#pragma unroll
+      for ( int k = 0; k < NS; k++ )
+      {
+         const int z = za + jb + k;
+         const int sB = k % NS, sZ = ( k + 1 ) % NS, sT = ( k + 2 ) % NS, sL = ( k + 2 + D ) % NS;
+         load( sL ); // slab z+1+D
+         // keep the step's loads ahead of its arithmetic: the scheduler may otherwise sink them past
+         // the waits for older slabs, which then have to wait for everything
+         __builtin_amdgcn_sched_barrier( 0 );
+         shifts( sT );
+         const bool out = cand && xy <= w - 2 - z && z <= zEnd;
+         const T    Z0 = v0[sZ], Zp = vp[sZ], Zm = vm[sZ], T0 = v0[sT], Tm = vm[sT], B0 = v0[sB], Bp = vp[sB];
+         const T    Z0l = l0[sZ], Z0r = r0[sZ], Zpl = lp[sZ], Zmr = rm[sZ];
+         const T    T0l = l0[sT], Tmr = rm[sT], B0r = r0[sB], Bpl = lp[sB];
+         // every lane computes, only output lanes store (branch-free, see CellBuf::store)
+         if ( MODE == kMarchSor )
+         {
+            T acc = cW * Z0l + cE * Z0r;
+            acc += cN * Zp + cNW * Zpl;
+            acc += cS * Zm + cSE * Zmr;
+            acc += cTC * T0 + cTW * T0l;
+            acc += cTS * Tm + cTSE * Tmr;
+            acc += cBC * B0 + cBE * B0r;
+            acc += cBN * Bp + cBNW * Bpl;
+            cy.store( Po, ( T( 1 ) - omega ) * Z0 + relax * ( vb[sZ] - acc ),
+                      out && ( ( x + 2 * yy + 3 * z ) & 3 ) == colour );
+         }
+         else
+         {
+            T acc = cC * Z0;
+            acc += cW * Z0l + cE * Z0r;
+            acc += cN * Zp + cNW * Zpl;
+            acc += cS * Zm + cSE * Zmr;
+            acc += cTC * T0 + cTW * T0l;
+            acc += cTS * Tm + cTSE * Tmr;
+            acc += cBC * B0 + cBE * B0r;
+            acc += cBN * Bp + cBNW * Bpl;
+            cy.store( Po, MODE == kReplace ? acc : MODE == kAdd ? vb[sZ] + acc : vb[sZ] - acc, out );
+         }
+         Po += ( w - z ) * ( w - z + 1 ) / 2 - yy;
+         __builtin_amdgcn_sched_barrier( 0 );
+      }
+   }
 }
 
 // --------------------------------------------------------------------------------------------------------
@@ -1602,7 +1756,17 @@ void marchLaunch( Mesh& M, LevelPlan& P, const T* CW, int flag, const T* u, T* y
                           CW, P.dCellType.p, flag, u, y, b, omega, colour, S.rows, cap );
       return;
    }
-   if ( S.kind == 1 )
+   if ( S.kind == 3 )
+   {
+      switch ( S.block )
+      {
+      case 1: HYMI_MARCH( kCellColumn, 1 ); break;
+      case 2: HYMI_MARCH( kCellColumn, 2 ); break;
+      case 4: HYMI_MARCH( kCellColumn, 4 ); break;
+      default: throw std::invalid_argument( "column prefetch distance must be 1, 2 or 4" );
+      }
+   }
+   else if ( S.kind == 1 )
    {
       switch ( S.block )
       {

@@ -448,9 +448,11 @@ const CellSeg* Mesh::segments( LevelPlan& P, int len, int64_t* count )
 }
 
 // Default launch schedule of the macro-cell kernels, from a sweep of kind x rows x waves x block on MI355X
-// (48 macro-cells, levels 5-7, FP64 apply and SOR colour pass): flat slab windows win on every level (they
-// keep ~90 % of the lanes busy where row windows keep ~50 %); two windows per unit from level 6 up, one
-// below, where more waves are needed to fill the GPU.
+// (48 macro-cells, levels 5-7, FP64 apply and SOR colour pass): flat slab windows (they keep ~90 % of the
+// lanes busy where row windows keep ~50 %); two windows per unit from level 6 up, one below, where more
+// waves are needed to fill the GPU.  The column march (kind 3) is faster from level 5 up (level 7 FP64
+// apply 81 us against 105 us) but is selected explicitly, not by default: a serving run of many concurrent
+// hierarchies with it as the default ended in a GPU fault whose cause has not been found.
 MarchSchedule defaultSchedule( int level )
 {
    MarchSchedule S;
@@ -463,7 +465,8 @@ MarchSchedule defaultSchedule( int level )
 
 void checkSchedule( const MarchSchedule& S, int level )
 {
-   if ( S.kind < 0 || S.kind > 2 ) throw std::invalid_argument( "schedule kind must be 0 (rows), 1 (flat) or 2 (tiles)" );
+   if ( S.kind < 0 || S.kind > 3 )
+      throw std::invalid_argument( "schedule kind must be 0 (rows), 1 (flat), 2 (tiles) or 3 (columns)" );
    if ( S.rows < 1 || S.rows > 4096 ) throw std::invalid_argument( "march rows must be in 1..4096" );
    if ( S.waves != 1 && S.waves != 2 && S.waves != 4 ) throw std::invalid_argument( "march waves must be 1, 2 or 4" );
    if ( S.kind == 2 )
@@ -472,6 +475,11 @@ void checkSchedule( const MarchSchedule& S, int level )
       if ( S.block < 1 || S.block > 4096 ) throw std::invalid_argument( "tile slabs must be in 1..4096" );
       if ( cap > kTileMaxPer * 64 * S.waves || 4 * cap * 8 > kTileMaxLds )
          throw std::invalid_argument( "tile band too large for this level (fewer rows or more waves)" );
+   }
+   else if ( S.kind == 3 )
+   {
+      if ( S.block != 1 && S.block != 2 && S.block != 4 )
+         throw std::invalid_argument( "column prefetch distance (block) must be 1, 2 or 4" );
    }
    else if ( S.block != 1 && S.block != 2 && S.block != 4 && S.block != 8 )
       throw std::invalid_argument( "march block rows must be 1, 2, 4 or 8" );
@@ -500,7 +508,36 @@ const MarchUnit* Mesh::marchUnits( LevelPlan& P, int64_t* count )
       for ( int c = 0; c < T.nC(); c++ )
          for ( int z = 1; z <= w - 4; z++ )
          {
-            if ( P.sched.kind == 2 )
+            if ( P.sched.kind == 3 )
+            {
+               // one unit per 64-element window of slab za (za = 1, 1 + Zc, ...), marched up to the last
+               // slab of its chunk in which one of its positions is still interior
+               const int Zc = rows;
+               if ( ( z - 1 ) % Zc != 0 ) continue;
+               const int W     = w - z;
+               const int nEl   = W * ( W + 1 ) / 2;
+               const int first = slabRowStart( W, 1 ) + 1;
+               const int lastE = slabRowStart( W, W - 3 ) + 1;
+               int       y     = 0;
+               for ( int ws = first; ws <= lastE; ws += kMarchOutput )
+               {
+                  while ( slabRowStart( W, y + 1 ) <= ws - 1 )
+                     y++;
+                  int m = 1 << 30, yy = y;
+                  for ( int e = ws; e < ws + kMarchOutput && e < nEl; e++ )
+                  {
+                     while ( slabRowStart( W, yy + 1 ) <= e )
+                        yy++;
+                     const int xx = e - slabRowStart( W, yy );
+                     if ( xx >= 1 && yy >= 1 ) m = std::min( m, xx + yy );
+                  }
+                  if ( m == 1 << 30 ) continue;
+                  const int zEnd = std::min( z + Zc - 1, w - 2 - m );
+                  if ( zEnd < z ) continue;
+                  v.push_back( { c, z | ( zEnd << 16 ), ws, y } );
+               }
+            }
+            else if ( P.sched.kind == 2 )
             {
                // one unit per (band of R rows, segment of `block` slabs); slabs with interior rows in the band
                if ( z != 1 ) continue;

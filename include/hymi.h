@@ -175,8 +175,9 @@ int hymi_time_kernel( hymi_operator op, hymi_gmg gmg, hymi_function x, hymi_func
  * one level run with a schedule: kind 0 walks x-windows of one row at a time (units of `rows` rows, loaded
  * `block` rows at a time, block 1, 2, 4 or 8), kind 1 walks each slab as one flat array (units of `rows`
  * windows of 64 elements, loaded `block` windows at a time, block 1, 2, 4 or 8), kind 2 stages slab bands
- * in LDS (a workgroup marches a band of `rows` rows of one cell over `block` slabs, 1..4096); waves per
- * workgroup 1, 2 or 4; rows 1..4096.  kind -1 and zeros select the level default.  A schedule the level
+ * in LDS (a workgroup marches a band of `rows` rows of one cell over `block` slabs, 1..4096), kind 3
+ * marches a wave's 64 (x, y) positions up through `rows` slabs with loads issued `block` slabs ahead (block
+ * 1, 2 or 4); waves per workgroup 1, 2 or 4; rows 1..4096.  kind -1 and zeros select the level default.  A schedule the level
  * cannot run (a kind-2 band too large for LDS) is refused.  Results do not depend on the schedule.  Setting a
  * schedule waits for the context stream; V-cycle graphs captured before it are captured again on their
  * next replay.  out = {rows, waves, block, kind}. */

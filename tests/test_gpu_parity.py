@@ -261,11 +261,47 @@ def test_march_schedules_bitwise(prec, gpu_ctx, oracle_lib):
     for rows, waves, block, kind in [(1, 1, 1, 0), (3, 2, 2, 0), (4, 4, 8, 0), (8, 4, 4, 0), (64, 2, 8, 0),
                                      (2, 1, 8, 0), (13, 4, 2, 0), (7, 1, 4, 0), (1, 1, 1, 1), (4, 2, 2, 1),
                                      (8, 4, 4, 1), (3, 1, 8, 1), (2, 2, 1, 1), (64, 4, 4, 1),
-                                     (4, 4, 1, 2), (3, 4, 2, 2), (8, 2, 5, 2), (12, 4, 64, 2), (2, 1, 3, 2)]:
+                                     (4, 4, 1, 2), (3, 4, 2, 2), (8, 2, 5, 2), (12, 4, 64, 2), (2, 1, 3, 2),
+                                     (1, 1, 1, 3), (2, 2, 2, 3), (3, 4, 4, 3), (5, 1, 2, 3), (16, 2, 2, 3),
+                                     (64, 4, 1, 3)]:
         S.set_schedule(L, rows, waves, block, kind)
         got = run()
         for a, r in zip(got, ref):
             assert np.array_equal(a, r), (rows, waves, block, kind)
+    S.set_schedule(L)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_column_schedules_bitwise_level6(prec, gpu_ctx):
+    """The column march at its production parameters (and neighbours of them) on level 6 equals the default
+    schedule bit for bit: apply, residual and an SOR sweep."""
+    import hyteg_amd as H
+    L = 6
+    say(f"[column schedules] cube n=2 level={L} precision={prec}")
+    S = H.PrimitiveStorage(H.MeshInfo.meshCuboid((0, 0, 0), (1, 1, 1), 2, 2, 2), gpu_ctx)
+    rng = np.random.default_rng(11)
+    xh, bh = rng.standard_normal((2, S.owned_count(L)))
+    A = H.P1ConstantLaplaceOperator(S, L, L)
+    x, b, y = (H.P1Function(n, S, L, L, precision=prec) for n in "xby")
+    b.upload(L, bh)
+
+    def run():
+        x.upload(L, xh)
+        A.apply(x, y, L, H.INNER)
+        out = [y.download(L)]
+        A.residual(x, b, y, L, H.INNER)
+        out.append(y.download(L))
+        A.smooth_sor(x, b, 1.0, L, H.INNER)
+        out.append(x.download(L))
+        return out
+
+    S.set_schedule(L)
+    ref = run()
+    for rows, waves, block in [(8, 4, 1), (16, 4, 1), (4, 2, 1), (8, 1, 2), (32, 2, 4), (1, 4, 1)]:
+        S.set_schedule(L, rows, waves, block, 3)
+        got = run()
+        for a, r in zip(got, ref):
+            assert np.array_equal(a, r), (rows, waves, block)
     S.set_schedule(L)
 
 

@@ -119,11 +119,11 @@ def test_schedule_arguments(H, hctx):
     for L in (2, 5, 7, 8):
         d = S.get_schedule(L)
         assert 1 <= d["rows"] <= 4096 and d["waves"] in (1, 2, 4) and d["block"] in (1, 2, 4, 8)
-        assert d["kind"] in (0, 1, 2)
+        assert d["kind"] in (0, 1, 2, 3)
     S.set_schedule(5, rows=16, waves=4, block=8, kind=1)
     assert S.get_schedule(5) == {"rows": 16, "waves": 4, "block": 8, "kind": 1}
-    for bad in (dict(waves=3), dict(block=3), dict(rows=5000), dict(rows=-1), dict(kind=3),
-                dict(kind=2, rows=200, waves=4, block=8)):
+    for bad in (dict(waves=3), dict(block=3), dict(rows=5000), dict(rows=-1), dict(kind=4),
+                dict(kind=2, rows=200, waves=4, block=8), dict(kind=3, block=8)):
         with pytest.raises(H.HymiError):
             S.set_schedule(5, **bad)
     assert S.get_schedule(5) == {"rows": 16, "waves": 4, "block": 8, "kind": 1}   # a refused call changes nothing
