@@ -55,6 +55,9 @@ __device__ __forceinline__ int crow32( int w, int t0, int y, int z )
    const int wz = w - z;
    return t0 - tet32( wz ) + y * ( wz + 1 ) - ( y * ( y + 1 ) >> 1 );
 }
+// cell-relative index of (x, y, z) and face-relative index of (i, j), 32-bit (t0 = tet32(w))
+__device__ __forceinline__ int cellRel( int w, int t0, int x, int y, int z ) { return crow32( w, t0, y, z ) + x; }
+__device__ __forceinline__ int faceRel( int w, int i, int j ) { return j * ( w + 1 ) - ( j * ( j + 1 ) >> 1 ) + i; }
 // (a, b) of flat index k in the triangle of width m (row-major, row b has m - b points)
 __device__ __forceinline__ void triPos( int64_t m, int64_t k, int& a, int& b )
 {
@@ -305,16 +308,18 @@ struct CellBuf
    }
    // store v at idx if keep, else nothing: a dropped lane's byte offset 0x80000000 lies past the range, and
    // the hardware discards stores outside it, so the store needs no branch
+   // AUX = cache policy bits of the instruction (2: non-temporal)
+   template < int AUX = 0 >
    __device__ __forceinline__ void store( int idx, T v, bool keep ) const
    {
       const int off = keep ? idx * (int) sizeof( T ) : (int) 0x80000000u;
       if constexpr ( sizeof( T ) == 8 )
       {
          typedef unsigned int u32x2 __attribute__( ( ext_vector_type( 2 ) ) );
-         __builtin_amdgcn_raw_buffer_store_b64( __builtin_bit_cast( u32x2, v ), r, off, 0, 0 );
+         __builtin_amdgcn_raw_buffer_store_b64( __builtin_bit_cast( u32x2, v ), r, off, 0, AUX );
       }
       else
-         __builtin_amdgcn_raw_buffer_store_b32( __builtin_bit_cast( unsigned int, v ), r, off, 0, 0 );
+         __builtin_amdgcn_raw_buffer_store_b32( __builtin_bit_cast( unsigned int, v ), r, off, 0, AUX );
    }
 };
 
@@ -690,10 +695,10 @@ __global__ __launch_bounds__( 256 ) void kCellTile( LV g, const MarchUnit* __res
 // (s - za + 1) mod (3 + D) for slab s, with loads issued D slabs ahead; the step loop is unrolled over one
 // turn of the ring, so each set keeps fixed registers and no step waits on a load issued for a later one.
 // Lanes that produce no output may read positions outside the slab; CellBuf turns reads outside the cell
-// into 0 and such values reach no output.  MarchUnit fields: cell, z = za | zEnd << 16, x0 = ws, y0 = row
-// of element ws - 1 in slab za.
-template < typename T, int MODE, int D >
-__global__ __launch_bounds__( 256 ) void kCellColumn( LV g, const MarchUnit* __restrict__ units, int64_t nUnits,
+// into 0 and such values reach no output.  NT: stores marked non-temporal (streamed past the caches).
+// MarchUnit fields: cell, z = za | zEnd << 16, x0 = ws, y0 = row of element ws - 1 in slab za.
+template < typename T, int MODE, int D, int NT >
+__global__ __launch_bounds__( 1024 ) void kCellColumn( LV g, const MarchUnit* __restrict__ units, int64_t nUnits,
                                                       const T* __restrict__ W8, const int* __restrict__ ctype, int flag,
                                                       const T* u, T* y, const T* b, T omega, int colour, int rows )
 {
@@ -797,8 +802,8 @@ __global__ __launch_bounds__( 256 ) void kCellColumn( LV g, const MarchUnit* __r
             acc += cTS * Tm + cTSE * Tmr;
             acc += cBC * B0 + cBE * B0r;
             acc += cBN * Bp + cBNW * Bpl;
-            cy.store( Po, ( T( 1 ) - omega ) * Z0 + relax * ( vb[sZ] - acc ),
-                      out && ( ( x + 2 * yy + 3 * z ) & 3 ) == colour );
+            cy.template store< NT ? 2 : 0 >( Po, ( T( 1 ) - omega ) * Z0 + relax * ( vb[sZ] - acc ),
+                                              out && ( ( x + 2 * yy + 3 * z ) & 3 ) == colour );
          }
          else
          {
@@ -810,7 +815,8 @@ __global__ __launch_bounds__( 256 ) void kCellColumn( LV g, const MarchUnit* __r
             acc += cTS * Tm + cTSE * Tmr;
             acc += cBC * B0 + cBE * B0r;
             acc += cBN * Bp + cBNW * Bpl;
-            cy.store( Po, MODE == kReplace ? acc : MODE == kAdd ? vb[sZ] + acc : vb[sZ] - acc, out );
+            cy.template store< NT ? 2 : 0 >( Po, MODE == kReplace ? acc : MODE == kAdd ? vb[sZ] + acc : vb[sZ] - acc,
+                                              out );
          }
          Po += ( w - z ) * ( w - z + 1 ) / 2 - yy;
          __builtin_amdgcn_sched_barrier( 0 );
@@ -832,6 +838,7 @@ __device__ __forceinline__ T faceStencilSum( const LV& g, const FaceGeom& G, con
    acc += fw[5] * UF( i - 1, j + 1 ) + fw[6] * UF( i + 1, j - 1 );
    if ( withCentre ) acc += fw[0] * UF( i, j );
 #undef UF
+   const int t0 = tet32( w );
    for ( int s = 0; s < G.nSides; s++ )
    {
       const FaceCellMap& M  = G.side[s];
@@ -843,7 +850,7 @@ __device__ __forceinline__ T faceStencilSum( const LV& g, const FaceGeom& G, con
          const int x   = M.base[0] + col * M.a[0] + row * M.b[0] + M.o[0];
          const int y   = M.base[1] + col * M.a[1] + row * M.b[1] + M.o[1];
          const int z   = M.base[2] + col * M.a[2] + row * M.b[2] + M.o[2];
-         acc += fw[7 + 4 * s + k] * u[( cb + cidx( w, x, y, z ) ) * NR + r];
+         acc += fw[7 + 4 * s + k] * u[( cb + cellRel( w, t0, x, y, z ) ) * NR + r];
       }
    }
    return acc;
@@ -949,16 +956,17 @@ template < typename T, int NR >
 __device__ __forceinline__ T edgeTermsSum( const LV& g, const EdgeGeom& G, const EdgeTerm* __restrict__ terms,
                                            const T* __restrict__ tw, const T* __restrict__ u, int x, int r )
 {
-   T acc = T( 0 );
+   T         acc = T( 0 );
+   const int w = g.w, t0 = tet32( w );
    for ( int k = 0; k < G.nTerms; k++ )
    {
       const EdgeTerm& e = terms[G.termOff + k];
       int64_t         idx;
       if ( e.kind == 0 )
-         idx = g.offF + (int64_t) e.prim * g.fS + fidx( g.w, e.base[0] + x * e.inc[0], e.base[1] + x * e.inc[1] );
+         idx = g.offF + (int64_t) e.prim * g.fS + faceRel( w, e.base[0] + x * e.inc[0], e.base[1] + x * e.inc[1] );
       else
          idx = g.offC + (int64_t) e.prim * g.cS +
-               cidx( g.w, e.base[0] + x * e.inc[0], e.base[1] + x * e.inc[1], e.base[2] + x * e.inc[2] );
+               cellRel( w, t0, e.base[0] + x * e.inc[0], e.base[1] + x * e.inc[1], e.base[2] + x * e.inc[2] );
       acc += tw[k] * u[idx * NR + r];
    }
    return acc;
@@ -1126,19 +1134,30 @@ __global__ __launch_bounds__( 128 ) void kEdgeSorFused( LV g, const EdgeGeom* __
       sW[k] = ew[3 + k];
    }
    __syncthreads();
-   const T rl = omega / ew[1];
+   const T   rl = omega / ew[1];
+   const int w = g.w, t0 = tet32( w );
    for ( int colour = 0; colour < 2; colour++ )
    {
       // colour 0 = even edge points, colour 1 = odd ones
-      for ( int x = 2 - colour + 2 * (int) threadIdx.x; x <= g.w - 2; x += 2 * blockDim.x )
+      for ( int x = 2 - colour + 2 * (int) threadIdx.x; x <= w - 2; x += 2 * blockDim.x )
       {
          T acc = ew[0] * u[eo + x - 1] + ew[2] * u[eo + x + 1];
-#pragma unroll 4
-         for ( int k = 0; k < nT; k++ )
+         // terms in groups of 8: the eight values are requested before the first is summed (32-bit index
+         // arithmetic keeps the address math short); the summation order is the term order
+         for ( int k0 = 0; k0 < nT; k0 += 8 )
          {
-            const int     c0  = sB[0][k] + x * sI[0][k], c1 = sB[1][k] + x * sI[1][k];
-            const int64_t idx = sPrim[k] + ( sKind[k] == 0 ? fidx( g.w, c0, c1 ) : cidx( g.w, c0, c1, sB[2][k] + x * sI[2][k] ) );
-            acc += sW[k] * u[idx];
+            T v[8];
+#pragma unroll
+            for ( int q = 0; q < 8; q++ )
+            {
+               const int k = min( k0 + q, nT - 1 );
+               const int c0 = sB[0][k] + x * sI[0][k], c1 = sB[1][k] + x * sI[1][k];
+               const int rel = sKind[k] == 0 ? faceRel( w, c0, c1 ) : cellRel( w, t0, c0, c1, sB[2][k] + x * sI[2][k] );
+               v[q] = u[sPrim[k] + rel];
+            }
+#pragma unroll
+            for ( int q = 0; q < 8; q++ )
+               if ( k0 + q < nT ) acc += sW[k0 + q] * v[q];
          }
          u[eo + x] = ( T( 1 ) - omega ) * u[eo + x] + rl * ( b[eo + x] - acc );
       }
@@ -1326,7 +1345,7 @@ __global__ void kRestrictFace( LV gc, LV gf, const FaceGeom* __restrict__ fgFine
          const int x   = M.base[0] + col * M.a[0] + row * M.b[0] + M.o[0];
          const int y   = M.base[1] + col * M.a[1] + row * M.b[1] + M.o[1];
          const int z   = M.base[2] + col * M.a[2] + row * M.b[2] + M.o[2];
-         acc += uf[( cb + cidx( w, x, y, z ) ) * NR + r];
+         acc += uf[( cb + cellRel( w, tet32( w ), x, y, z ) ) * NR + r];
       }
    }
    const int64_t co = gc.offF + (int64_t) G.face * gc.fS;
@@ -1353,10 +1372,10 @@ __global__ void kRestrictEdge( LV gc, LV gf, const EdgeGeom* __restrict__ egFine
       if ( e.wr == 0.0f ) continue;
       int64_t idx;
       if ( e.kind == 0 )
-         idx = gf.offF + (int64_t) e.prim * gf.fS + fidx( gf.w, e.base[0] + x * e.inc[0], e.base[1] + x * e.inc[1] );
+         idx = gf.offF + (int64_t) e.prim * gf.fS + faceRel( gf.w, e.base[0] + x * e.inc[0], e.base[1] + x * e.inc[1] );
       else
          idx = gf.offC + (int64_t) e.prim * gf.cS +
-               cidx( gf.w, e.base[0] + x * e.inc[0], e.base[1] + x * e.inc[1], e.base[2] + x * e.inc[2] );
+               cellRel( gf.w, tet32( gf.w ), e.base[0] + x * e.inc[0], e.base[1] + x * e.inc[1], e.base[2] + x * e.inc[2] );
       acc += T( e.wr ) * uf[idx * NR + r];
    }
    uc[( gc.offE + (int64_t) G.edge * gc.eS + X ) * NR + r] = acc;
@@ -1748,6 +1767,9 @@ void marchLaunch( Mesh& M, LevelPlan& P, const T* CW, int flag, const T* u, T* y
 #define HYMI_MARCH( KERNEL, RR )                                                                                   \
    hipLaunchKernelGGL( ( KERNEL< T, MODE, RR > ), grid, block, 0, s, g, mu, n, CW, P.dCellType.p, flag, u, y, b,    \
                        omega, colour, S.rows )
+#define HYMI_COLUMN( DD, NT )                                                                                      \
+   hipLaunchKernelGGL( ( kCellColumn< T, MODE, DD, NT > ), grid, block, 0, s, g, mu, n, CW, P.dCellType.p, flag, u, y,  \
+                       b, omega, colour, S.rows )
    if ( S.kind == 2 )
    {
       const int    cap = tileCap( P.w, S.rows );
@@ -1760,10 +1782,14 @@ void marchLaunch( Mesh& M, LevelPlan& P, const T* CW, int flag, const T* u, T* y
    {
       switch ( S.block )
       {
-      case 1: HYMI_MARCH( kCellColumn, 1 ); break;
-      case 2: HYMI_MARCH( kCellColumn, 2 ); break;
-      case 4: HYMI_MARCH( kCellColumn, 4 ); break;
-      default: throw std::invalid_argument( "column prefetch distance must be 1, 2 or 4" );
+      // block = prefetch distance (1, 2 or 4) + 8 for non-temporal stores
+      case 1: HYMI_COLUMN( 1, 0 ); break;
+      case 2: HYMI_COLUMN( 2, 0 ); break;
+      case 4: HYMI_COLUMN( 4, 0 ); break;
+      case 9: HYMI_COLUMN( 1, 1 ); break;
+      case 10: HYMI_COLUMN( 2, 1 ); break;
+      case 12: HYMI_COLUMN( 4, 1 ); break;
+      default: throw std::invalid_argument( "column block must be 1, 2 or 4, plus 8 for non-temporal stores" );
       }
    }
    else if ( S.kind == 1 )
@@ -1789,6 +1815,7 @@ void marchLaunch( Mesh& M, LevelPlan& P, const T* CW, int flag, const T* u, T* y
       }
    }
 #undef HYMI_MARCH
+#undef HYMI_COLUMN
 }
 
 // macro-cell apply / residual over all interior points: row-marching kernel for one right-hand side,

@@ -468,7 +468,8 @@ void checkSchedule( const MarchSchedule& S, int level )
    if ( S.kind < 0 || S.kind > 3 )
       throw std::invalid_argument( "schedule kind must be 0 (rows), 1 (flat), 2 (tiles) or 3 (columns)" );
    if ( S.rows < 1 || S.rows > 4096 ) throw std::invalid_argument( "march rows must be in 1..4096" );
-   if ( S.waves != 1 && S.waves != 2 && S.waves != 4 ) throw std::invalid_argument( "march waves must be 1, 2 or 4" );
+   const bool wavesOk = S.waves == 1 || S.waves == 2 || S.waves == 4 || ( S.kind == 3 && ( S.waves == 8 || S.waves == 16 ) );
+   if ( !wavesOk ) throw std::invalid_argument( "march waves must be 1, 2 or 4 (kind 3: also 8 or 16)" );
    if ( S.kind == 2 )
    {
       const int cap = tileCap( widthOf( level ), S.rows );
@@ -478,8 +479,9 @@ void checkSchedule( const MarchSchedule& S, int level )
    }
    else if ( S.kind == 3 )
    {
-      if ( S.block != 1 && S.block != 2 && S.block != 4 )
-         throw std::invalid_argument( "column prefetch distance (block) must be 1, 2 or 4" );
+      const int d = S.block & 7;
+      if ( ( S.block & ~15 ) || ( d != 1 && d != 2 && d != 4 ) )
+         throw std::invalid_argument( "column block must be a prefetch distance 1, 2 or 4, plus 8 for non-temporal stores" );
    }
    else if ( S.block != 1 && S.block != 2 && S.block != 4 && S.block != 8 )
       throw std::invalid_argument( "march block rows must be 1, 2, 4 or 8" );

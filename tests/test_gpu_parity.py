@@ -263,7 +263,7 @@ def test_march_schedules_bitwise(prec, gpu_ctx, oracle_lib):
                                      (8, 4, 4, 1), (3, 1, 8, 1), (2, 2, 1, 1), (64, 4, 4, 1),
                                      (4, 4, 1, 2), (3, 4, 2, 2), (8, 2, 5, 2), (12, 4, 64, 2), (2, 1, 3, 2),
                                      (1, 1, 1, 3), (2, 2, 2, 3), (3, 4, 4, 3), (5, 1, 2, 3), (16, 2, 2, 3),
-                                     (64, 4, 1, 3)]:
+                                     (64, 4, 1, 3), (4, 8, 9, 3), (16, 16, 1, 3), (2, 16, 10, 3)]:
         S.set_schedule(L, rows, waves, block, kind)
         got = run()
         for a, r in zip(got, ref):
@@ -297,7 +297,8 @@ def test_column_schedules_bitwise_level6(prec, gpu_ctx):
 
     S.set_schedule(L)
     ref = run()
-    for rows, waves, block in [(8, 4, 1), (16, 4, 1), (4, 2, 1), (8, 1, 2), (32, 2, 4), (1, 4, 1)]:
+    for rows, waves, block in [(8, 4, 1), (16, 4, 1), (4, 2, 1), (8, 1, 2), (32, 2, 4), (1, 4, 1), (16, 16, 9),
+                               (8, 8, 1), (16, 8, 12)]:
         S.set_schedule(L, rows, waves, block, 3)
         got = run()
         for a, r in zip(got, ref):
