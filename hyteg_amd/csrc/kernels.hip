@@ -1651,6 +1651,53 @@ __global__ __launch_bounds__( 256 ) void kDotCell( LV g, const CellSeg* __restri
    blockReduceStore< NR >( v, partials + (int64_t) blockIdx.x * NR );
 }
 
+// Cell part of a dot product for one right-hand side: a fixed grid of kDotBlocks workgroups; wave k of the
+// grid takes the flat transfer units k, k + 4 kDotBlocks, ... (64-element windows of one slab, each lane
+// decoding its row as the flat transfer kernels do) and sums a b over the interior points it holds, in FP64;
+// each workgroup stores one partial.  The order of every sum is fixed, so the result does not vary between
+// runs.
+constexpr int kDotBlocks = 2048;
+template < typename T >
+__global__ __launch_bounds__( 256 ) void kDotFlat( LV g, const MarchUnit* __restrict__ units, int64_t nUnits,
+                                                   const int* __restrict__ ctype, int flag, const T* __restrict__ a,
+                                                   const T* __restrict__ b, double* __restrict__ partials )
+{
+   const int lane = threadIdx.x & 63;
+   const int w    = g.w, t0 = tet32( w );
+   double    v    = 0.0;
+   auto      rs   = [&]( int WW, int yy ) { return yy * ( WW + 1 ) - ( yy * ( yy + 1 ) >> 1 ); };
+   for ( int64_t ui = (int64_t) blockIdx.x * 4 + __builtin_amdgcn_readfirstlane( threadIdx.x >> 6 ); ui < nUnits;
+         ui += (int64_t) gridDim.x * 4 )
+   {
+      const MarchUnit U = units[ui];
+      if ( !( ctype[U.cell] & flag ) ) continue;
+      const int      z = U.z, Wd = w - z, nEl = Wd * ( Wd + 1 ) / 2, S0 = t0 - tet32( Wd );
+      const T*       ca = a + g.offC + (int64_t) U.cell * g.cS + S0;
+      const T*       cb = b + g.offC + (int64_t) U.cell * g.cS + S0;
+      int            e0 = U.x0, y0 = U.y0;
+      for ( int j = 0; j < kTransferWindows; j++, e0 += 64 )
+      {
+         if ( e0 >= nEl ) break;
+         while ( rs( Wd, y0 + 1 ) <= e0 )
+            y0++;
+         int yL = y0;
+         while ( yL + 1 < Wd && rs( Wd, yL + 1 ) <= e0 + 63 )
+            yL++;
+         const int e  = e0 + lane;
+         int       yy = y0, next = rs( Wd, y0 + 1 );
+         for ( int it = y0; it < yL; it++ )
+            if ( e >= next )
+            {
+               yy++;
+               next += Wd - yy;
+            }
+         const int x = e - rs( Wd, yy );
+         if ( e < nEl && x >= 1 && yy >= 1 && x + yy <= Wd - 2 ) v += (double) ca[e] * (double) cb[e];
+      }
+   }
+   blockReduceStore< 1 >( v, partials + blockIdx.x );
+}
+
 template < typename T, int NR >
 __global__ __launch_bounds__( 256 ) void kDotLow( LV g, const FaceGeom* __restrict__ fg, int nF,
                                                   const EdgeGeom* __restrict__ eg, int nE,
@@ -2075,16 +2122,33 @@ void dotT( Function& a, Function& b, int level, int flag, int slot )
    LevelPlan&  P   = M.plan( level );
    LV          g   = viewOf( P );
    hipStream_t s   = ctx.stream;
-   int64_t     nseg = 0;
-   const CellSeg* sg = M.segments( P, segLen< NR >(), &nseg );
-   const int64_t nbCell = ( nseg + 3 ) / 4;
-   const int64_t nbLow  = 64;
-   const int64_t need   = ( nbCell + nbLow ) * NR;
-   if ( (int64_t) ctx.partials.n < need ) ctx.partials.alloc( (size_t) need );
-   if ( nbCell > 0 && level >= 2 )
-      hipLaunchKernelGGL( ( kDotCell< T, NR > ), dim3( (unsigned) nbCell ), dim3( 256 ), 0, s, g, sg, nseg, P.dCellType.p,
-                          flag, (const T*) a.ptr( level ), (const T*) b.ptr( level ), ctx.partials.p );
-   const int64_t nbC = ( level >= 2 ) ? nbCell : 0;
+   const int64_t nbLow = 64;
+   int64_t       nbC   = 0;
+   if ( NR == 1 )
+   {
+      // flat windows of the transfer units, a fixed grid (few partials for the final sum)
+      int64_t          nu = 0;
+      const MarchUnit* mu = level >= 2 ? M.transferUnits( P, &nu ) : nullptr;
+      nbC                 = nu > 0 ? std::min< int64_t >( kDotBlocks, ( nu + 3 ) / 4 ) : 0;
+      const int64_t need  = nbC + nbLow;
+      if ( (int64_t) ctx.partials.n < need ) ctx.partials.alloc( (size_t) need );
+      if ( nbC > 0 )
+         hipLaunchKernelGGL( ( kDotFlat< T > ), dim3( (unsigned) nbC ), dim3( 256 ), 0, s, g, mu, nu, P.dCellType.p,
+                             flag, (const T*) a.ptr( level ), (const T*) b.ptr( level ), ctx.partials.p );
+   }
+   else
+   {
+      int64_t        nseg   = 0;
+      const CellSeg* sg     = M.segments( P, segLen< NR >(), &nseg );
+      const int64_t  nbCell = ( nseg + 3 ) / 4;
+      const int64_t  need   = ( nbCell + nbLow ) * NR;
+      if ( (int64_t) ctx.partials.n < need ) ctx.partials.alloc( (size_t) need );
+      if ( nbCell > 0 && level >= 2 )
+         hipLaunchKernelGGL( ( kDotCell< T, NR > ), dim3( (unsigned) nbCell ), dim3( 256 ), 0, s, g, sg, nseg,
+                             P.dCellType.p, flag, (const T*) a.ptr( level ), (const T*) b.ptr( level ),
+                             ctx.partials.p );
+      nbC = ( level >= 2 ) ? nbCell : 0;
+   }
    hipLaunchKernelGGL( ( kDotLow< T, NR > ), dim3( (unsigned) nbLow ), dim3( 256 ), 0, s, g, P.dFaceGeom.p, M.T.nF(),
                        P.dEdgeGeom.p, M.T.nE(), P.dVertexGeom.p, M.T.nV(), flag, (const T*) a.ptr( level ),
                        (const T*) b.ptr( level ), ctx.partials.p + nbC * NR );
