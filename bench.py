@@ -251,7 +251,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--mixed-level", type=int, default=8)
-    ap.add_argument("--mixed-cycles-per-step", type=int, default=2)
+    ap.add_argument("--mixed-cycles-per-step", type=int, default=4)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-requests", type=int, default=200)

@@ -1103,6 +1103,7 @@ int hymi_gmg_create( hymi_operator op, hymi_function xt, int minL, int maxL, int
          maxBlocks = std::max( maxBlocks, ( n + 3 ) / 4 + 64 );
          M.marchUnits( P, &n );
          M.transferUnits( P, &n );
+         if ( g->nrhs == 1 ) M.faceGhost( P );
          M.closureRanges( P, kInner | kNeumann, &n );
          M.closureRanges( P, kInner | kNeumann | kFreeslip, &n );
          M.closureRanges( P, kAll, &n );

@@ -122,6 +122,9 @@ struct LevelPlan
    // grid-transfer kernels; fields as for the flat macro-cell kernel (x0 = first element, y0 = its row)
    std::unique_ptr< DevBuf< MarchUnit > > transfer;
    int64_t                                transferCount = -1;
+   // face ghost layers of the SOR sweep: per face and side, the neighbour cell's first interior layer in
+   // face coordinates (2 nF fStride elements, FP64-sized so that either precision fits)
+   DevBuf< double > faceGhost;
 
    MarchSchedule                          sched;
    // closure ranges of primitives matching a flag, keyed by flag
@@ -143,6 +146,8 @@ struct Mesh
    const CellSeg* segments( LevelPlan& P, int len, int64_t* count );
    const MarchUnit* marchUnits( LevelPlan& P, int64_t* count );
    const MarchUnit* transferUnits( LevelPlan& P, int64_t* count );
+   // face ghost buffer of level P (allocated on first use)
+   void* faceGhost( LevelPlan& P );
 
    // replaces the launch schedule of the row-marching kernels on one level (zeros = level default);
    // bumps scheduleEpoch so that captured graphs holding the old work-unit table are captured again

@@ -908,16 +908,41 @@ __global__ void kFaceSor( LV g, const FaceGeom* __restrict__ fg, const T* __rest
 // Row-wise colour pass for one right-hand side: workgroup (row j, face) visits only the points (i, j) of
 // the colour, i = i0, i0 + 3, ...; face and cell offsets use 32-bit arithmetic.  Same sums in the same
 // order as faceStencilSum.
+// Face ghost layers for the SOR sweep: for face f and side s, gh[(2 f + s) fS + fidx(col, row)] = the cell
+// value at base + col a + row b + o of the side's face-to-cell map, for every (col, row) of the face
+// triangle whose image lies in the cell (a superset of the off-plane neighbours the face stencil reads).
+// One workgroup per (row, face).  The cell reads are scattered for two of the four cell face planes (one
+// point per cell row); gathering them once per sweep replaces scattered reads in each colour pass.
+template < typename T >
+__global__ __launch_bounds__( 128 ) void kFaceGhost( LV g, const FaceGeom* __restrict__ fg, const T* __restrict__ u,
+                                                     T* __restrict__ gh )
+{
+   const FaceGeom& G   = fg[blockIdx.y];
+   const int       w   = g.w, row = blockIdx.x, t0 = tet32( w );
+   for ( int sd = 0; sd < G.nSides; sd++ )
+   {
+      const FaceCellMap& M  = G.side[sd];
+      const T*           cu = u + g.offC + (int64_t) M.cell * g.cS;
+      T*                 go = gh + ( 2 * (int64_t) G.face + sd ) * g.fS;
+      for ( int col = threadIdx.x; col + row <= w - 1; col += blockDim.x )
+      {
+         const int x = M.base[0] + col * M.a[0] + row * M.b[0] + M.o[0];
+         const int y = M.base[1] + col * M.a[1] + row * M.b[1] + M.o[1];
+         const int z = M.base[2] + col * M.a[2] + row * M.b[2] + M.o[2];
+         if ( x >= 0 && y >= 0 && z >= 0 && x + y + z <= w - 1 ) go[faceRel( w, col, row )] = cu[cellRel( w, t0, x, y, z )];
+      }
+   }
+}
+
 template < typename T >
 __global__ __launch_bounds__( 64 ) void kFaceSorRows( LV g, const FaceGeom* __restrict__ fg, const T* __restrict__ FW,
                                                       int flag, T* __restrict__ u, const T* __restrict__ b, T omega,
-                                                      int colour )
+                                                      int colour, const T* __restrict__ gh )
 {
    const FaceGeom& G = fg[blockIdx.y];
    if ( !( G.type & flag ) ) return;
    const int w = g.w, j = 1 + blockIdx.x; // interior rows j = 1 .. w-3
    if ( j > w - 3 ) return;
-   const int t0 = tet32( w );
    const T*  fw = FW + (int64_t) G.face * kFaceW;
    const T   rl = omega / fw[0];
    T*        fu = u + g.offF + (int64_t) G.face * g.fS;
@@ -932,17 +957,12 @@ __global__ __launch_bounds__( 64 ) void kFaceSorRows( LV g, const FaceGeom* __re
       acc += fw[5] * fu[fi( i - 1, j + 1 )] + fw[6] * fu[fi( i + 1, j - 1 )];
       for ( int sd = 0; sd < G.nSides; sd++ )
       {
-         const FaceCellMap& M  = G.side[sd];
-         const T*           cu = u + g.offC + (int64_t) M.cell * g.cS;
+         // off-plane neighbours from the side's ghost layer (the same values as the cell's, gathered
+         // before the face phase; no kernel of the face phase writes a cell)
+         const T* go = gh + ( 2 * (int64_t) G.face + sd ) * g.fS;
 #pragma unroll
          for ( int k = 0; k < 4; k++ )
-         {
-            const int col = i + G.offCol[sd][k], row = j + G.offRow[sd][k];
-            const int x   = M.base[0] + col * M.a[0] + row * M.b[0] + M.o[0];
-            const int y   = M.base[1] + col * M.a[1] + row * M.b[1] + M.o[1];
-            const int z   = M.base[2] + col * M.a[2] + row * M.b[2] + M.o[2];
-            acc += fw[7 + 4 * sd + k] * cu[crow32( w, t0, y, z ) + x];
-         }
+            acc += fw[7 + 4 * sd + k] * go[fi( i + G.offCol[sd][k], j + G.offRow[sd][k] )];
       }
       const int o = fi( i, j );
       fu[o]       = ( T( 1 ) - omega ) * fu[o] + rl * ( fb[o] - acc );
@@ -2005,9 +2025,11 @@ void sorT( Operator& op, Function& xf, const Function& bf, double omega, int lev
          {
             // large faces: one workgroup per face would leave most of the chip idle
             refreshT< T, NR >( xf, level, 2 );
+            T* gh = (T*) M.faceGhost( P );
+            hipLaunchKernelGGL( ( kFaceGhost< T > ), dim3( P.w - 1, nF ), dim3( 128 ), 0, s, g, P.dFaceGeom.p, u, gh );
             for ( int c = 0; c < 3; c++ )
                hipLaunchKernelGGL( ( kFaceSorRows< T > ), dim3( P.w - 3, nF ), dim3( 64 ), 0, s, g, P.dFaceGeom.p,
-                                   wsel< T >( Wt.face64, Wt.face32 ), flag, u, b, om, c );
+                                   wsel< T >( Wt.face64, Wt.face32 ), flag, u, b, om, c, (const T*) gh );
          }
          refreshT< T, NR >( xf, level, 4 );
          for ( int c = 0; c < 4; c++ )

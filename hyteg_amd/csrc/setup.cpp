@@ -613,6 +613,13 @@ const MarchUnit* Mesh::transferUnits( LevelPlan& P, int64_t* count )
    return P.transfer->p;
 }
 
+void* Mesh::faceGhost( LevelPlan& P )
+{
+   const size_t n = 2 * (size_t) T.nF() * (size_t) P.fStride;
+   if ( P.faceGhost.n < n ) P.faceGhost.alloc( n );
+   return P.faceGhost.p;
+}
+
 void Mesh::setSchedule( int level, const MarchSchedule& S )
 {
    LevelPlan&    P = plan( level );
