@@ -1333,6 +1333,60 @@ __global__ __launch_bounds__( 256 ) void kRestrictCell( LV gc, LV gf, const Cell
    uc[( cbase + cidx( gc.w, X, sg.y, sg.z ) ) * NR + r] = acc;
 }
 
+// Flat form for one right-hand side: a wave takes kTransferWindows windows of 64 consecutive elements of one
+// coarse slab Z (units from Mesh::transferUnits of the coarse level); each lane finds its coarse row Y and
+// gathers the fifteen fine values around (2X, 2Y, 2Z) with 32-bit row offsets, summed in the order of
+// kRestrictCell.
+template < typename T >
+__global__ __launch_bounds__( 256 ) void kRestrictFlat( LV gc, LV gf, const MarchUnit* __restrict__ units,
+                                                        int64_t nUnits, const int* __restrict__ ctype, int flag,
+                                                        const T* __restrict__ uf, T* __restrict__ uc )
+{
+   const int64_t ui = xcdBlock( ( nUnits + 3 ) / 4 ) * 4 + __builtin_amdgcn_readfirstlane( threadIdx.x >> 6 );
+   if ( ui >= nUnits ) return;
+   const MarchUnit U = units[ui];
+   if ( !( ctype[U.cell] & flag ) ) return;
+   const int lane = threadIdx.x & 63;
+   const int wc = gc.w, w = gf.w, Z = U.z, Wd = wc - Z;
+   const int tc = tet32( wc ), tf = tet32( w );
+   const int nEl = Wd * ( Wd + 1 ) / 2;
+   const int S0  = tc - tet32( Wd );
+   const int z   = 2 * Z;
+   const CellBuf< T > fu( uf + gf.offC + (int64_t) U.cell * gf.cS, gf.cS );
+   const CellBuf< T > cu( uc + gc.offC + (int64_t) U.cell * gc.cS, gc.cS );
+   auto      rs = [&]( int WW, int yy ) { return yy * ( WW + 1 ) - ( yy * ( yy + 1 ) >> 1 ); };
+   int       e0 = U.x0, y0 = U.y0;
+   for ( int j = 0; j < kTransferWindows; j++, e0 += 64 )
+   {
+      if ( e0 >= nEl ) break;
+      while ( rs( Wd, y0 + 1 ) <= e0 )
+         y0++;
+      int yL = y0;
+      while ( yL + 1 < Wd && rs( Wd, yL + 1 ) <= e0 + 63 )
+         yL++;
+      const int e  = e0 + lane;
+      int       Y  = y0, next = rs( Wd, y0 + 1 );
+      for ( int it = y0; it < yL; it++ )
+         if ( e >= next )
+         {
+            Y++;
+            next += Wd - Y;
+         }
+      const int  X  = e - rs( Wd, Y );
+      const bool ok = e < nEl && X >= 1 && Y >= 1 && X + Y <= Wd - 2;
+      const int  x = 2 * X, y = 2 * Y;
+      // fine rows (y, z), (y+1, z), (y-1, z), (y, z+1), (y-1, z+1), (y, z-1), (y+1, z-1); lanes without an
+      // interior coarse point read through the buffer range check and store nothing
+      const int r0 = crow32( w, tf, y, z ) + x, rN = crow32( w, tf, y + 1, z ) + x, rS = crow32( w, tf, y - 1, z ) + x;
+      const int rT = crow32( w, tf, y, z + 1 ) + x, rTS = crow32( w, tf, y - 1, z + 1 ) + x;
+      const int rB = crow32( w, tf, y, z - 1 ) + x, rBN = crow32( w, tf, y + 1, z - 1 ) + x;
+      T acc = fu[r0 - 1] + fu[r0 + 1] + fu[rN] + fu[rN - 1] + fu[rS] + fu[rS + 1] + fu[rT] + fu[rT - 1] + fu[rTS] +
+              fu[rTS + 1] + fu[rB] + fu[rB + 1] + fu[rBN] + fu[rBN - 1];
+      acc = fu[r0] + T( 0.5 ) * acc;
+      cu.store( S0 + e, acc, ok );
+   }
+}
+
 template < typename T, int NR >
 __global__ void kRestrictFace( LV gc, LV gf, const FaceGeom* __restrict__ fgFine, int flag, const T* __restrict__ uf,
                                T* __restrict__ uc )
@@ -1494,34 +1548,54 @@ __global__ __launch_bounds__( 256 ) void kProlongFlat( LV gc, LV gf, const March
    const int Zm = ( z - pz ) / 2, Zp = ( z + pz ) / 2;
    const int Cm = tc - tet32( wc - Zm ), Cp = tc - tet32( wc - Zp ); // coarse slab starts
    const int Wm = wc - Zm, Wp = wc - Zp;
-   const T*  cu = uc + gc.offC + (int64_t) U.cell * gc.cS;
-   T*        fu = uf + gf.offC + (int64_t) U.cell * gf.cS;
+   const CellBuf< T > cu( uc + gc.offC + (int64_t) U.cell * gc.cS, gc.cS );
+   const CellBuf< T > fu( uf + gf.offC + (int64_t) U.cell * gf.cS, gf.cS );
    auto      rs = [&]( int WW, int yy ) { return yy * ( WW + 1 ) - ( yy * ( yy + 1 ) >> 1 ); };
    int       e0 = U.x0, y0 = U.y0;
+   // all windows' loads are issued before the first sum (twelve loads in flight per lane); lanes without
+   // an interior point read index 0 and store nothing
+   T    fv[kTransferWindows], ca[kTransferWindows], cb[kTransferWindows];
+   int  fo[kTransferWindows];
+   bool ok[kTransferWindows];
+#pragma unroll
    for ( int j = 0; j < kTransferWindows; j++, e0 += 64 )
    {
-      if ( e0 >= nEl ) break;
-      while ( rs( Wd, y0 + 1 ) <= e0 )
-         y0++;
-      int yL = y0;
-      while ( yL + 1 < Wd && rs( Wd, yL + 1 ) <= e0 + 63 )
-         yL++;
-      const int e  = e0 + lane;
-      int       yy = y0, next = rs( Wd, y0 + 1 );
-      for ( int it = y0; it < yL; it++ )
-         if ( e >= next )
+      ok[j] = false;
+      fo[j] = 0;
+      int im = 0, ip = 0;
+      if ( e0 < nEl )
+      {
+         while ( rs( Wd, y0 + 1 ) <= e0 )
+            y0++;
+         int yL = y0;
+         while ( yL + 1 < Wd && rs( Wd, yL + 1 ) <= e0 + 63 )
+            yL++;
+         const int e  = e0 + lane;
+         int       yy = y0, next = rs( Wd, y0 + 1 );
+         for ( int it = y0; it < yL; it++ )
+            if ( e >= next )
+            {
+               yy++;
+               next += Wd - yy;
+            }
+         const int x = e - rs( Wd, yy );
+         ok[j]       = e < nEl && x >= 1 && yy >= 1 && x + yy <= Wd - 2;
+         if ( ok[j] )
          {
-            yy++;
-            next += Wd - yy;
+            int dx, dy, dz;
+            prolongDir( x & 1, yy & 1, pz, dx, dy, dz );
+            im    = Cm + rs( Wm, ( yy - dy ) / 2 ) + ( x - dx ) / 2;
+            ip    = Cp + rs( Wp, ( yy + dy ) / 2 ) + ( x + dx ) / 2;
+            fo[j] = S0 + e;
          }
-      const int x = e - rs( Wd, yy );
-      if ( !( e < nEl && x >= 1 && yy >= 1 && x + yy <= Wd - 2 ) ) continue;
-      int dx, dy, dz;
-      prolongDir( x & 1, yy & 1, pz, dx, dy, dz );
-      const int im = Cm + rs( Wm, ( yy - dy ) / 2 ) + ( x - dx ) / 2;
-      const int ip = Cp + rs( Wp, ( yy + dy ) / 2 ) + ( x + dx ) / 2;
-      fu[S0 + e] += T( 0.5 ) * ( cu[im] + cu[ip] );
+      }
+      fv[j] = fu[fo[j]];
+      ca[j] = cu[im];
+      cb[j] = cu[ip];
    }
+#pragma unroll
+   for ( int j = 0; j < kTransferWindows; j++ )
+      fu.store( fo[j], fv[j] + T( 0.5 ) * ( ca[j] + cb[j] ), ok[j] );
 }
 
 template < typename T, int NR >
@@ -2093,6 +2167,15 @@ void restrictT( Function& f, Function& dst, int srcLevel, int flag )
    if ( m > 0 && nF > 0 )
       hipLaunchKernelGGL( ( kRestrictFace< T, NR > ), dim3( blocksFor( m * ( m + 1 ) / 2 * NR, 256 ), nF ), dim3( 256 ), 0,
                           s, gc, gf, Pf.dFaceGeom.p, flag, uf, uc );
+   if ( NR == 1 )
+   {
+      int64_t          nu = 0;
+      const MarchUnit* mu = M.transferUnits( Pc, &nu );
+      if ( nu > 0 )
+         hipLaunchKernelGGL( ( kRestrictFlat< T > ), dim3( (unsigned) xcdGrid( ( nu + 3 ) / 4 ) ), dim3( 256 ), 0, s, gc, gf,
+                             mu, nu, Pc.dCellType.p, flag, uf, uc );
+      return;
+   }
    int64_t        nseg = 0;
    const CellSeg* sg   = M.segments( Pc, segLen< NR >(), &nseg );
    if ( nseg > 0 )
