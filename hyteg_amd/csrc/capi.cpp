@@ -1179,9 +1179,12 @@ int hymi_solve_refined( hymi_gmg g, hymi_function x, hymi_function b, int level,
          g->refPrec = x->f.prec;
       }
       std::vector< double > r0( nr ), rn( nr );
-      // r = b - A x in the outer precision, and ||r|| per right-hand side
-      auto residualNorms = [&]( std::vector< double >& out ) {
-         launchApply( *g->A, x->f, *g->refR, &b->f, level, flag, kResidual, true );
+      // r = b - A x in the outer precision, and ||r|| per right-hand side.  x's closure copies are refreshed
+      // before the first residual only: the update x += e below runs over every primitive's whole closure
+      // array, and e's copies are current (a V-cycle ends with a smoothing sweep, after which every copy
+      // is), so the update keeps x's copies equal to their owners.
+      auto residualNorms = [&]( std::vector< double >& out, bool refresh ) {
+         launchApply( *g->A, x->f, *g->refR, &b->f, level, flag, kResidual, refresh );
          launchDot( *g->refR, *g->refR, level, flag, sUser );
          HYMI_HIP( hipStreamSynchronize( ctx.stream ) );
          HYMI_LEGACY_LOCK;
@@ -1197,7 +1200,7 @@ int hymi_solve_refined( hymi_gmg g, hymi_function x, hymi_function b, int level,
             m = std::max( m, r0[r] > 0.0 ? rn[r] / r0[r] : 0.0 );
          return m;
       };
-      residualNorms( r0 );
+      residualNorms( r0, true );
       rn         = r0;
       double rel = worst();
       if ( history ) history[0] = rel;
@@ -1208,8 +1211,10 @@ int hymi_solve_refined( hymi_gmg g, hymi_function x, hymi_function b, int level,
          launchConvert( *g->refRc, level, *g->refR, level, kAll );
          launchLincomb( *g->refEc, level, kAll, 0, nullptr, nullptr, false, 0.0 );
          g->replay( *g->refEc, *g->refRc, level, cyclesPerOuter );
-         launchAddFrom( x->f, *g->refEc, level, flag );
-         residualNorms( rn );
+         // e is zero on every Dirichlet value and copy (it starts at zero and no flag kernel writes them),
+         // so adding it over all primitives changes no boundary value
+         launchAddFrom( x->f, *g->refEc, level, kAll );
+         residualNorms( rn, g->nu2 == 0 || level == g->minL ); // without a final sweep e's copies may be stale
          rel = worst();
          if ( history ) history[k + 1] = rel;
       }

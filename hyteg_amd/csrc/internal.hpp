@@ -108,6 +108,10 @@ struct LevelPlan
    DevBuf< int64_t >    dVertexTerms;
    DevBuf< int64_t >    dVertexEnds;
    DevBuf< int32_t >    dCellType;
+   // interior points of one macro-cell grouped by SOR colour (x + 2y + 3z) mod 4, packed x | y << 10 | z << 20,
+   // colour c at [colourStart[c], colourStart[c+1]) (levels <= kCellSmallMaxLevel)
+   DevBuf< int32_t >    dColourPts;
+   int                  colourStart[5] = { 0, 0, 0, 0, 0 };
 
    // cell row segments, keyed by segment length (in points)
    std::map< int, std::unique_ptr< DevBuf< CellSeg > > > segs;

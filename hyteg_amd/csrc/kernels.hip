@@ -682,6 +682,55 @@ __global__ __launch_bounds__( 256 ) void kCellTile( LV g, const MarchUnit* __res
    if ( z < zb ) step( z, stB, stA );
 }
 
+// The whole SOR cell phase of a small level in one launch: one workgroup per macro-cell runs the four colours
+// in order, with a barrier between colours (cells do not couple within the phase: their boundary values are
+// closure copies that no kernel of the phase writes).  pts lists the cell's interior points by colour; every
+// point is computed with the expression of the march kernels, so the result equals four colour passes.
+struct ColourStarts
+{
+   int s[5];
+};
+template < typename T >
+__global__ __launch_bounds__( 256 ) void kCellSorSmall( LV g, const int* __restrict__ pts, ColourStarts cs,
+                                                        const T* __restrict__ W8, const int* __restrict__ ctype, int flag,
+                                                        T* __restrict__ u, const T* __restrict__ b, T omega )
+{
+   const int cell = blockIdx.x;
+   if ( !( ctype[cell] & flag ) ) return;
+   const int     w = g.w, t0 = tet32( w );
+   const int64_t base = g.offC + (int64_t) cell * g.cS;
+   T*            cu   = u + base;
+   const T*      cb   = b + base;
+   const T*      c    = W8 + (int64_t) cell * 15;
+   const T cE = c[dE], cW = c[dW], cN = c[dN], cS = c[dS], cNW = c[dNW], cSE = c[dSE];
+   const T cTC = c[dTC], cTW = c[dTW], cTS = c[dTS], cTSE = c[dTSE], cBC = c[dBC], cBN = c[dBN], cBE = c[dBE],
+           cBNW = c[dBNW];
+   const T relax = omega / c[dC];
+   for ( int colour = 0; colour < 4; colour++ )
+   {
+      for ( int k = cs.s[colour] + (int) threadIdx.x; k < cs.s[colour + 1]; k += blockDim.x )
+      {
+         const int p = pts[k], x = p & 1023, y = ( p >> 10 ) & 1023, z = p >> 20;
+         const int r0 = crow32( w, t0, y, z ) + x, rN = crow32( w, t0, y + 1, z ) + x, rS = crow32( w, t0, y - 1, z ) + x;
+         const int rT = crow32( w, t0, y, z + 1 ) + x, rTS = crow32( w, t0, y - 1, z + 1 ) + x;
+         const int rB = crow32( w, t0, y, z - 1 ) + x, rBN = crow32( w, t0, y + 1, z - 1 ) + x;
+         const T   Z0 = cu[r0], Z0l = cu[r0 - 1], Z0r = cu[r0 + 1], Zp = cu[rN], Zpl = cu[rN - 1], Zm = cu[rS],
+                 Zmr = cu[rS + 1];
+         const T T0 = cu[rT], T0l = cu[rT - 1], Tm = cu[rTS], Tmr = cu[rTS + 1];
+         const T B0 = cu[rB], B0r = cu[rB + 1], Bp = cu[rBN], Bpl = cu[rBN - 1];
+         T       acc = cW * Z0l + cE * Z0r;
+         acc += cN * Zp + cNW * Zpl;
+         acc += cS * Zm + cSE * Zmr;
+         acc += cTC * T0 + cTW * T0l;
+         acc += cTS * Tm + cTSE * Tmr;
+         acc += cBC * B0 + cBE * B0r;
+         acc += cBN * Bp + cBNW * Bpl;
+         cu[r0] = ( T( 1 ) - omega ) * Z0 + relax * ( cb[r0] - acc );
+      }
+      __syncthreads(); // the next colour reads this colour's values
+   }
+}
+
 // Column march along z (schedule kind 3).  A wave owns a fixed set of 64 positions (x, y): the elements
 // ws-1 .. ws+62 of slab za, lanes 1..62 producing outputs; it walks them up through slabs za .. zEnd.  In
 // slab s (width W_s = w - s) position (x, y) sits at the cell-relative index P(s), with
@@ -2106,8 +2155,17 @@ void sorT( Operator& op, Function& xf, const Function& bf, double omega, int lev
                                    wsel< T >( Wt.face64, Wt.face32 ), flag, u, b, om, c, (const T*) gh );
          }
          refreshT< T, NR >( xf, level, 4 );
-         for ( int c = 0; c < 4; c++ )
-            cellSorLaunch< T, NR >( M, P, wsel< T >( Wt.cell64, Wt.cell32 ), flag, u, b, om, c );
+         if ( level <= kCellSmallMaxLevel && P.colourStart[4] > 0 )
+         {
+            ColourStarts cs;
+            for ( int k = 0; k < 5; k++ )
+               cs.s[k] = P.colourStart[k];
+            hipLaunchKernelGGL( ( kCellSorSmall< T > ), dim3( M.T.nC() ), dim3( 256 ), 0, s, g, P.dColourPts.p, cs,
+                                wsel< T >( Wt.cell64, Wt.cell32 ), P.dCellType.p, flag, u, b, om );
+         }
+         else
+            for ( int c = 0; c < 4; c++ )
+               cellSorLaunch< T, NR >( M, P, wsel< T >( Wt.cell64, Wt.cell32 ), flag, u, b, om, c );
       }
       else
       {

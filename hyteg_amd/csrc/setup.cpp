@@ -405,6 +405,20 @@ void buildPlan( Mesh& M, LevelPlan& P, int level )
 
    std::vector< int32_t > ct( T.cType.begin(), T.cType.end() );
    if ( dev ) P.dCellType.upload( ct, s );
+   if ( P.L >= 2 && P.L <= kCellSmallMaxLevel )
+   {
+      std::vector< int32_t > pts;
+      for ( int c = 0; c < 4; c++ )
+      {
+         P.colourStart[c] = (int) pts.size();
+         for ( int z = 1; z <= w - 4; z++ )
+            for ( int y = 1; y <= w - 3 - z; y++ )
+               for ( int x = 1; x <= w - 2 - y - z; x++ )
+                  if ( ( x + 2 * y + 3 * z ) % 4 == c ) pts.push_back( x | ( y << 10 ) | ( z << 20 ) );
+      }
+      P.colourStart[4] = (int) pts.size();
+      if ( dev ) P.dColourPts.upload( pts, s );
+   }
    if ( dev ) HYMI_HIP( hipStreamSynchronize( s ) );
 }
 
