@@ -79,7 +79,7 @@ struct EdgeTerm
 // largest number of neighbour terms of one edge (the fused edge SOR stages them in LDS)
 constexpr int kEdgeTermsMax = 128;
 // levels whose cell SOR phase runs as one workgroup per macro-cell for all four colours
-constexpr int kCellSmallMaxLevel = 4;
+constexpr int kCellSmallMaxLevel = 5;
 
 struct EdgeGeom
 {
