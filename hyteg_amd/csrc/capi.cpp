@@ -1183,9 +1183,16 @@ int hymi_solve_refined( hymi_gmg g, hymi_function x, hymi_function b, int level,
       // before the first residual only: the update x += e below runs over every primitive's whole closure
       // array, and e's copies are current (a V-cycle ends with a smoothing sweep, after which every copy
       // is), so the update keeps x's copies equal to their owners.
+      // With the column schedule on the level, the cell residual goes rounded straight into the FP32
+      // right-hand side and its norm is summed on the way (no FP64 cell residual is written or read again).
+      bool split          = false;
       auto residualNorms = [&]( std::vector< double >& out, bool refresh ) {
-         launchApply( *g->A, x->f, *g->refR, &b->f, level, flag, kResidual, refresh );
-         launchDot( *g->refR, *g->refR, level, flag, sUser );
+         split = launchResidualSplit( *g->A, x->f, b->f, *g->refR, *g->refRc, level, flag, sUser, refresh );
+         if ( !split )
+         {
+            launchApply( *g->A, x->f, *g->refR, &b->f, level, flag, kResidual, refresh );
+            launchDot( *g->refR, *g->refR, level, flag, sUser );
+         }
          HYMI_HIP( hipStreamSynchronize( ctx.stream ) );
          HYMI_LEGACY_LOCK;
          HYMI_HIP( hipMemcpyAsync( out.data(), ctx.scalars.p + sUser * nr, sizeof( double ) * nr, hipMemcpyDeviceToHost,
@@ -1208,7 +1215,7 @@ int hymi_solve_refined( hymi_gmg g, hymi_function x, hymi_function b, int level,
       for ( ; k < maxOuter && rel > tol; k++ )
       {
          // correction equation A e = r in the solver's precision, e from zero
-         launchConvert( *g->refRc, level, *g->refR, level, kAll );
+         if ( !split ) launchConvert( *g->refRc, level, *g->refR, level, kAll );
          launchLincomb( *g->refEc, level, kAll, 0, nullptr, nullptr, false, 0.0 );
          g->replay( *g->refEc, *g->refRc, level, cyclesPerOuter );
          // e is zero on every Dirichlet value and copy (it starts at zero and no flag kernel writes them),

@@ -247,6 +247,9 @@ __global__ __launch_bounds__( 256 ) void kCellSor( LV g, const CellSeg* __restri
 // read from memory once per wave instead of up to five times, and all per-row index arithmetic is scalar.
 
 constexpr int kMarchSor = 3; // MODE value of the in-place multicolour SOR pass
+// MODE value of the column kernel's residual for iterative refinement: r = b - A x computed in T, stored
+// rounded to FP32 (the correction equation's right-hand side) and summed as r^2 in FP64, one partial per wave
+constexpr int kResidualSplit = 4;
 // finest level whose faces are small enough for one workgroup per face in the fused SOR phase
 constexpr int kFaceFusedMaxLevel = 4;
 
@@ -259,6 +262,16 @@ constexpr int kFaceFusedMaxLevel = 4;
 // move per 32-bit half).  No data register lives across blocks, so no step waits on a load issued for a
 // later one; the four rows shared by consecutive blocks are read again from L1/L2.  A workgroup holds
 // blockDim.x / 64 waves, each with its own work unit.
+// butterfly sum over the 64 lanes of a wave (every lane gets the sum)
+template < typename T >
+__device__ __forceinline__ T waveSum( T v )
+{
+#pragma unroll
+   for ( int m = 32; m >= 1; m >>= 1 )
+      v += __shfl_xor( v, m, 64 );
+   return v;
+}
+
 template < typename T >
 __device__ __forceinline__ T laneDown( T v ) // lane i <- lane i-1 (lane 0 gets 0)
 {
@@ -749,7 +762,8 @@ __global__ __launch_bounds__( 256 ) void kCellSorSmall( LV g, const int* __restr
 template < typename T, int MODE, int D, int NT >
 __global__ __launch_bounds__( 1024 ) void kCellColumn( LV g, const MarchUnit* __restrict__ units, int64_t nUnits,
                                                       const T* __restrict__ W8, const int* __restrict__ ctype, int flag,
-                                                      const T* u, T* y, const T* b, T omega, int colour, int rows )
+                                                      const T* u, T* y, const T* b, T omega, int colour, int rows,
+                                                      float* rlow, double* partials )
 {
    constexpr int NS   = 3 + D;
    const int     wpb  = blockDim.x >> 6;
@@ -757,13 +771,19 @@ __global__ __launch_bounds__( 1024 ) void kCellColumn( LV g, const MarchUnit* __
    const int64_t ui   = xcdBlock( ( nUnits + wpb - 1 ) / wpb ) * wpb + wave;
    if ( ui >= nUnits ) return;
    const MarchUnit U = units[ui];
-   if ( !( ctype[U.cell] & flag ) ) return;
+   if ( !( ctype[U.cell] & flag ) )
+   {
+      if ( MODE == kResidualSplit && ( threadIdx.x & 63 ) == 0 ) partials[ui] = 0.0;
+      return;
+   }
    const int     lane = threadIdx.x & 63;
    const int     w = g.w, za = U.z & 0xffff, zEnd = U.z >> 16, Wa = w - za;
    const int64_t base = g.offC + (int64_t) U.cell * g.cS;
    const CellBuf< T > cu( u + base, g.cS );
    const CellBuf< T > cb( ( MODE == kAdd ? y : b ) + base, g.cS ); // b (residual, SOR) or y (add)
-   const CellBuf< T > cy( y + base, g.cS );
+   const CellBuf< T > cy( ( MODE == kResidualSplit ? u : y ) + base, g.cS ); // (unused by kResidualSplit)
+   const CellBuf< float > cr( ( MODE == kResidualSplit ? rlow : (float*) nullptr ) + base, g.cS );
+   double        nsum = 0.0;
    const int     t0   = tet32( w );
    const T*      c    = W8 + (int64_t) U.cell * 15;
    const T cC = c[dC], cW = c[dW], cE = c[dE], cN = c[dN], cS = c[dS], cNW = c[dNW], cSE = c[dSE];
@@ -864,12 +884,24 @@ __global__ __launch_bounds__( 1024 ) void kCellColumn( LV g, const MarchUnit* __
             acc += cTS * Tm + cTSE * Tmr;
             acc += cBC * B0 + cBE * B0r;
             acc += cBN * Bp + cBNW * Bpl;
-            cy.template store< NT ? 2 : 0 >( Po, MODE == kReplace ? acc : MODE == kAdd ? vb[sZ] + acc : vb[sZ] - acc,
-                                              out );
+            if constexpr ( MODE == kResidualSplit )
+            {
+               const T r = vb[sZ] - acc;
+               cr.store( Po, (float) r, out );
+               nsum += out ? (double) r * (double) r : 0.0;
+            }
+            else
+               cy.template store< NT ? 2 : 0 >( Po, MODE == kReplace ? acc : MODE == kAdd ? vb[sZ] + acc : vb[sZ] - acc,
+                                                 out );
          }
          Po += ( w - z ) * ( w - z + 1 ) / 2 - yy;
          __builtin_amdgcn_sched_barrier( 0 );
       }
+   }
+   if constexpr ( MODE == kResidualSplit )
+   {
+      nsum = waveSum( nsum );
+      if ( lane == 0 ) partials[ui] = nsum;
    }
 }
 
@@ -1141,14 +1173,6 @@ __global__ void kVertexSor( LV g, const VertexGeom* __restrict__ vg, const int64
 // not selected by `flag` keep their values, so their copies stay valid and they are skipped.
 
 // sum over the 64 lanes of a wave in a fixed butterfly order (every lane gets the total)
-template < typename T >
-__device__ __forceinline__ T waveSum( T v )
-{
-#pragma unroll
-   for ( int m = 32; m >= 1; m >>= 1 )
-      v += __shfl_xor( v, m, 64 );
-   return v;
-}
 
 // one wave per vertex: lanes take the incident-edge terms, a butterfly sums them
 template < typename T >
@@ -1944,7 +1968,8 @@ const float* wsel< float >( const DevBuf< double >&, const DevBuf< float >& d32 
 // launch of the row-marching kernel with the level's schedule (rows per work unit, waves per workgroup,
 // rows per load block)
 template < typename T, int MODE >
-void marchLaunch( Mesh& M, LevelPlan& P, const T* CW, int flag, const T* u, T* y, const T* b, T omega, int colour )
+void marchLaunch( Mesh& M, LevelPlan& P, const T* CW, int flag, const T* u, T* y, const T* b, T omega, int colour,
+                  float* rlow = nullptr, double* partials = nullptr )
 {
    int64_t          n  = 0;
    const MarchUnit* mu = M.marchUnits( P, &n );
@@ -1959,7 +1984,24 @@ void marchLaunch( Mesh& M, LevelPlan& P, const T* CW, int flag, const T* u, T* y
                        omega, colour, S.rows )
 #define HYMI_COLUMN( DD, NT )                                                                                      \
    hipLaunchKernelGGL( ( kCellColumn< T, MODE, DD, NT > ), grid, block, 0, s, g, mu, n, CW, P.dCellType.p, flag, u, y,  \
-                       b, omega, colour, S.rows )
+                       b, omega, colour, S.rows, rlow, partials )
+   if constexpr ( MODE == kResidualSplit )
+   {
+      if ( S.kind != 3 ) throw std::logic_error( "the split residual needs the column schedule" );
+      switch ( S.block )
+      {
+      case 1: HYMI_COLUMN( 1, 0 ); break;
+      case 2: HYMI_COLUMN( 2, 0 ); break;
+      case 4: HYMI_COLUMN( 4, 0 ); break;
+      case 9: HYMI_COLUMN( 1, 1 ); break;
+      case 10: HYMI_COLUMN( 2, 1 ); break;
+      case 12: HYMI_COLUMN( 4, 1 ); break;
+      default: throw std::invalid_argument( "column block must be 1, 2 or 4, plus 8 for non-temporal stores" );
+      }
+      return;
+   }
+   else
+   {
    if ( S.kind == 2 )
    {
       const int    cap = tileCap( P.w, S.rows );
@@ -2003,6 +2045,7 @@ void marchLaunch( Mesh& M, LevelPlan& P, const T* CW, int flag, const T* u, T* y
       case 8: HYMI_MARCH( kCellMarch, 8 ); break;
       default: throw std::invalid_argument( "march block rows must be 1, 2, 4 or 8" );
       }
+   }
    }
 #undef HYMI_MARCH
 #undef HYMI_COLUMN
@@ -2578,6 +2621,49 @@ void launchCellApplyOnly( Operator& op, Function& src, Function& dst, int level,
 void launchCellSorOnly( Operator& op, Function& x, const Function& b, double omega, int level, int flag, int colour )
 {
    dispatch< CellSorOnlyF >( x.prec, x.nrhs, op, x, b, omega, level, flag, colour );
+}
+
+bool launchResidualSplit( Operator& op, Function& x, const Function& b, Function& r64, Function& rc, int level,
+                          int flag, int slot, bool refreshSrc )
+{
+   if ( x.prec != kF64 || b.prec != kF64 || r64.prec != kF64 || rc.prec != kF32 || x.nrhs != 1 || level < 2 ) return false;
+   Mesh&      M = *op.mesh;
+   LevelPlan& P = M.plan( level );
+   int64_t    n = 0;
+   M.marchUnits( P, &n );
+   Context& ctx = *M.ctx;
+   if ( P.sched.kind != 3 || n == 0 || (int64_t) ctx.partials.n < n + 64 ) return false;
+   hipStream_t   s  = ctx.stream;
+   LV            g  = viewOf( P );
+   auto&         Wt = op.weights( level );
+   const double* u  = (const double*) x.ptr( level );
+   const double* bb = (const double*) b.ptr( level );
+   double*       y  = (double*) r64.ptr( level );
+   float*        yc = (float*) rc.ptr( level );
+   if ( refreshSrc ) refreshT< double, 1 >( x, level, 7 );
+   const int nV = M.T.nV(), nE = M.T.nE(), nF = M.T.nF();
+   // vertices, edges and faces in FP64, as applyT does
+   hipLaunchKernelGGL( ( kVertexApply< double, 1, kResidual > ), dim3( blocksFor( nV, 256 ) ), dim3( 256 ), 0, s, g,
+                       P.dVertexGeom.p, P.dVertexTerms.p, nV, Wt.vertex64.p, flag, u, y, bb );
+   hipLaunchKernelGGL( ( kEdgeApply< double, 1, kResidual > ), dim3( blocksFor( P.w - 2, 64 ), nE ), dim3( 64 ), 0, s, g,
+                       P.dEdgeGeom.p, P.dEdgeTerms.p, Wt.edge64.p, flag, u, y, bb );
+   const int64_t m = P.w - 3;
+   if ( m > 0 && nF > 0 )
+      hipLaunchKernelGGL( ( kFaceApply< double, 1, kResidual > ), dim3( blocksFor( m * ( m + 1 ) / 2, 256 ), nF ),
+                          dim3( 256 ), 0, s, g, P.dFaceGeom.p, Wt.face64.p, flag, u, y, bb );
+   // cells: FP32 residual straight into rc, r^2 per wave into partials[0, n)
+   marchLaunch< double, kResidualSplit >( M, P, Wt.cell64.p, flag, u, nullptr, bb, 0.0, 0, yc, ctx.partials.p );
+   // FP32 copy of the vertex, edge and face residuals (whole arrays)
+   int64_t      nr = 0;
+   const Range* R  = M.closureRanges( P, kAll | kLowerDim, &nr );
+   if ( nr > 0 )
+      hipLaunchKernelGGL( ( kLincomb< float, double, 1 > ), dim3( (unsigned) nr ), dim3( 256 ), 0, s, R, yc, 1, y,
+                          (const double*) nullptr, (const double*) nullptr, 1.0, 0.0, 0.0, 0, 0.0 );
+   // ||r||^2: cell partials, then the vertex, edge and face part, summed in a fixed order
+   hipLaunchKernelGGL( ( kDotLow< double, 1 > ), dim3( 64 ), dim3( 256 ), 0, s, g, P.dFaceGeom.p, nF, P.dEdgeGeom.p, nE,
+                       P.dVertexGeom.p, nV, flag, (const double*) y, (const double*) y, ctx.partials.p + n );
+   hipLaunchKernelGGL( ( kFinishDot< 1 > ), dim3( 1 ), dim3( 256 ), 0, s, ctx.partials.p, n + 64, ctx.scalars.p + slot );
+   return true;
 }
 
 void launchAddFrom( Function& x, Function& e, int level, int flag )

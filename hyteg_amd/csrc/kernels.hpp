@@ -44,6 +44,11 @@ void launchScalarCopy( Context& ctx, int nrhs, int dst, int src );
 void launchConvert( Function& dst, int dstLevel, Function& src, int srcLevel, int flag );
 // x (FP64) += e (FP32 or FP64)
 void launchAddFrom( Function& x, Function& e, int level, int flag );
+// Residual of iterative refinement: r = b - A x with FP64 x and b; vertex, edge and face values go to r64
+// and, rounded, to rc; cell values go rounded straight to rc; ||r||^2 (FP64, owned DoFs of flag primitives)
+// to scalars[slot].  Needs the column schedule on the level; returns false (nothing launched) otherwise.
+bool launchResidualSplit( Operator& op, Function& x, const Function& b, Function& r64, Function& rc, int level,
+                          int flag, int slot, bool refreshSrc );
 // f = sum_m modes[r][m][0] * sin(k pi x) sin(l pi y) sin(m pi z), (k, l, m) = modes[r][m][1..3], for every
 // right-hand side r, on the DoFs of primitives matching flag; closure copies refreshed.  dModes is a device
 // table of nrhs * nModes * 4 doubles.

@@ -693,6 +693,8 @@ const Range* Mesh::closureRanges( LevelPlan& P, int flag, int64_t* count )
          for ( int64_t o = 0; o < len; o += chunk )
             v.push_back( { off + o, (int32_t) std::min( chunk, len - o ), 0 } );
       };
+      const bool cells = !( flag & kLowerDim );
+      flag &= ~kLowerDim;
       int64_t runStart = -1, runLen = 0;
       for ( int vtx = 0; vtx < T.nV(); vtx++ )
       {
@@ -713,15 +715,16 @@ const Range* Mesh::closureRanges( LevelPlan& P, int flag, int64_t* count )
          if ( T.eType[e] & flag ) push( P.edgeOff( e ), P.w );
       for ( int f = 0; f < T.nF(); f++ )
          if ( T.fType[f] & flag ) push( P.faceOff( f ), triSize( P.w ) );
-      for ( int c = 0; c < T.nC(); c++ )
+      for ( int c = 0; c < T.nC() && cells; c++ )
          if ( T.cType[c] & flag ) push( P.cellOff( c ), tetSize( P.w ) );
-      auto buf = std::make_unique< DevBuf< Range > >();
+      const int key = flag | ( cells ? 0 : (int) kLowerDim );
+      auto      buf = std::make_unique< DevBuf< Range > >();
       buf->upload( v, ctx->stream );
       HYMI_HIP( hipStreamSynchronize( ctx->stream ) );
-      P.rangeCount[flag] = (int64_t) v.size();
-      it                 = P.ranges.emplace( flag, std::move( buf ) ).first;
+      P.rangeCount[key] = (int64_t) v.size();
+      it                = P.ranges.emplace( key, std::move( buf ) ).first;
    }
-   *count = P.rangeCount[flag];
+   *count = P.rangeCount[it->first];
    return it->second->p;
 }
 

@@ -24,7 +24,9 @@ enum DoFType : int
    kDirichlet = 2,
    kNeumann   = 4,
    kFreeslip  = 8,
-   kAll       = 15
+   kAll       = 15,
+   // closure-range selector only: vertices, edges and faces matching the other bits, no cells
+   kLowerDim  = 1 << 20
 };
 
 struct Topology

@@ -122,3 +122,33 @@ def test_refined_nrhs32_matches_single(gpu_ctx):
         for _ in range(25):
             g64.solve(A, x1, b1, L)
         assert rel_err(Xb[:, r], x1.download(L)) <= 1e-6, r
+
+
+def test_refined_split_residual_matches_two_pass(gpu_ctx):
+    """With the column schedule the refinement residual is written rounded to FP32 in the cell kernel and its
+    norm is summed on the way; the corrections, and so the solution, equal the two-pass form (FP64 residual,
+    separate conversion and dot) bit for bit, and the recorded norms agree to rounding."""
+    import hyteg_amd as H
+    L = 5
+    say(f"[refine split] cube n=2 level={L}")
+    _, S = mesh_pair(("cube", 2), gpu_ctx)
+    rng = np.random.default_rng(17)
+    A, M = H.P1ConstantLaplaceOperator(S, 0, L), H.P1ConstantMassOperator(S, 0, L)
+    b, _ = _rhs_with_noise(H, S, A, M, L, rng)
+
+    def solve():
+        inner = H.GeometricMultigridSolver(S, H.SORSmoother(1.0), H.CGSolver(S, 0, 0), H.P1toP1LinearRestriction(),
+                                           H.P1toP1LinearProlongation(), 0, L, 2, 2)
+        R = H.MixedPrecisionRefinementSolver(S, inner, tolerance=1e-10, maxOuter=40, cyclesPerOuter=2)
+        x = H.P1Function("x", S, 0, L)
+        rel = R.solve(A, x, b, L)
+        return x.download(L), R.outerSteps, np.array(R.history), float(np.max(rel))
+
+    x1, n1, h1, r1 = solve()                      # level default: two-pass residual
+    S.set_schedule(L, 4, 2, 1, 3)                 # column march on the finest level: split residual
+    x2, n2, h2, r2 = solve()
+    S.set_schedule(L)
+    say(f"  outer steps {n1} / {n2}, final {r1:.2e} / {r2:.2e}")
+    assert n1 == n2 and r2 <= 1e-10
+    assert np.array_equal(x1, x2)
+    assert np.allclose(h1, h2, rtol=1e-12, atol=0.0)
