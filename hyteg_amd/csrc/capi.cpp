@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -199,7 +200,9 @@ std::unique_ptr< Function > makeFunction( Mesh& M, int minL, int maxL, int prec,
    if ( nrhs != 1 && nrhs != 32 ) throw std::runtime_error( "unsupported nrhs (1 or 32)" );
    if ( prec != kF64 && prec != kF32 ) throw std::invalid_argument( "precision must be HYMI_F64 or HYMI_F32" );
    if ( minL < 0 || maxL < minL || maxL > 12 ) throw std::invalid_argument( "invalid level range" );
+   static std::atomic< uint64_t > nextUid{ 1 };
    auto F  = std::make_unique< Function >();
+   F->uid  = nextUid.fetch_add( 1 );
    F->mesh = &M;
    F->minL = minL;
    F->maxL = maxL;
@@ -268,7 +271,8 @@ struct hymi_gmg_s
       hipGraphExec_t exec  = nullptr;
       uint64_t       epoch = 0; // mesh schedule epoch at capture
    };
-   std::map< std::tuple< void*, void*, int >, Captured > graphs;
+   // keyed on the functions' uids: an address may be reused by a later function whose buffers differ
+   std::map< std::tuple< uint64_t, uint64_t, int >, Captured > graphs;
    ~hymi_gmg_s()
    {
       HYMI_LEGACY_LOCK;
@@ -294,7 +298,7 @@ struct hymi_gmg_s
    void replay( Function& x, Function& b, int L, int count )
    {
       hipStream_t    s     = A->mesh->ctx->stream;
-      auto           key   = std::make_tuple( (void*) &x, (void*) &b, L );
+      auto           key   = std::make_tuple( x.uid, b.uid, L );
       const uint64_t epoch = A->mesh->scheduleEpoch;
       auto           it    = graphs.find( key );
       if ( it != graphs.end() && it->second.epoch != epoch )
@@ -382,7 +386,8 @@ int hymi_ctx_create( int device, hymi_ctx* out )
          HYMI_HIP( hipStreamCreateWithFlags( &c->c.stream, hipStreamNonBlocking ) );
       }
       ensureScalars( c->c, 32 );
-      c->c.partials.alloc( 1 << 16 );
+      // fixed for the context's lifetime: captured graphs hold these addresses
+      c->c.partials.alloc( (size_t) kPartialsCap );
       HYMI_HIP( hipStreamSynchronize( c->c.stream ) );
       *out = c.release();
    } );
@@ -1094,13 +1099,11 @@ int hymi_gmg_create( hymi_operator op, hymi_function xt, int minL, int maxL, int
       g->cg.ap  = makeFunction( M, minL, minL, g->prec, g->nrhs );
       // build every table a V-cycle uses now, so that a V-cycle never allocates (graph capture)
       const int segLen = 64 / g->nrhs;
-      int64_t   maxBlocks = 0;
       for ( int L = minL; L <= maxL; L++ )
       {
          LevelPlan& P = M.plan( L );
          int64_t    n = 0;
          M.segments( P, segLen, &n );
-         maxBlocks = std::max( maxBlocks, ( n + 3 ) / 4 + 64 );
          M.marchUnits( P, &n );
          M.transferUnits( P, &n );
          if ( g->nrhs == 1 ) M.faceGhost( P );
@@ -1108,9 +1111,9 @@ int hymi_gmg_create( hymi_operator op, hymi_function xt, int minL, int maxL, int
          M.closureRanges( P, kInner | kNeumann | kFreeslip, &n );
          M.closureRanges( P, kAll, &n );
       }
+      // the context's scalar slots and partial-sum buffer were sized for 32 right-hand sides when it was
+      // created and are never reallocated (graphs of other solvers on this context hold their addresses)
       Context& ctx = *M.ctx;
-      ensureScalars( ctx, g->nrhs );
-      if ( (int64_t) ctx.partials.n < maxBlocks * g->nrhs ) ctx.partials.alloc( (size_t) ( maxBlocks * g->nrhs ) );
       HYMI_HIP( hipStreamSynchronize( ctx.stream ) );
       *out = g.release();
    } );

@@ -129,6 +129,8 @@ struct LevelPlan
    // face ghost layers of the SOR sweep: per face and side, the neighbour cell's first interior layer in
    // face coordinates (2 nF fStride elements, FP64-sized so that either precision fits)
    DevBuf< double > faceGhost;
+   // per-work-unit partial sums of the split refinement residual (never used inside a captured graph)
+   DevBuf< double > normPartials;
 
    MarchSchedule                          sched;
    // closure ranges of primitives matching a flag, keyed by flag
@@ -176,6 +178,7 @@ struct Function
    int                      minL = 0, maxL = 0, nrhs = 1, prec = kF64;
    std::vector< void* >     data; // per level device pointers
    std::vector< int64_t >   count;
+   uint64_t                 uid = 0; // unique over the process (graph caches key on it, never on addresses)
    ~Function();
    void*  ptr( int level ) const { return data.at( level - minL ); }
    size_t elemBytes() const { return prec == kF64 ? 8 : 4; }

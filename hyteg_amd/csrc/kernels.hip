@@ -1801,18 +1801,18 @@ __global__ __launch_bounds__( 256 ) void kDotCell( LV g, const CellSeg* __restri
                                                    const int* __restrict__ ctype, int flag, const T* __restrict__ a,
                                                    const T* __restrict__ b, double* __restrict__ partials )
 {
-   const int64_t s = (int64_t) blockIdx.x * 4 + ( threadIdx.x >> 6 );
-   double        v = 0.0;
-   if ( s < nseg )
+   // wave k of the grid sums segments k, k + 4 gridDim, ... (a fixed grid: few partials, fixed order)
+   double        v    = 0.0;
+   const int     lane = threadIdx.x & 63;
+   for ( int64_t s = (int64_t) blockIdx.x * 4 + ( threadIdx.x >> 6 ); s < nseg; s += (int64_t) gridDim.x * 4 )
    {
-      const CellSeg sg   = segs[s];
-      const int     lane = threadIdx.x & 63;
-      const int     x    = sg.x0 + lane / NR;
-      const int     r    = lane % NR;
+      const CellSeg sg = segs[s];
+      const int     x  = sg.x0 + lane / NR;
+      const int     r  = lane % NR;
       if ( ( ctype[sg.cell] & flag ) && x <= g.w - 2 - sg.y - sg.z )
       {
          const int64_t o = ( g.offC + (int64_t) sg.cell * g.cS + cidx( g.w, x, sg.y, sg.z ) ) * NR + r;
-         v               = (double) a[o] * (double) b[o];
+         v += (double) a[o] * (double) b[o];
       }
    }
    blockReduceStore< NR >( v, partials + (int64_t) blockIdx.x * NR );
@@ -1823,7 +1823,6 @@ __global__ __launch_bounds__( 256 ) void kDotCell( LV g, const CellSeg* __restri
 // decoding its row as the flat transfer kernels do) and sums a b over the interior points it holds, in FP64;
 // each workgroup stores one partial.  The order of every sum is fixed, so the result does not vary between
 // runs.
-constexpr int kDotBlocks = 2048;
 template < typename T >
 __global__ __launch_bounds__( 256 ) void kDotFlat( LV g, const MarchUnit* __restrict__ units, int64_t nUnits,
                                                    const int* __restrict__ ctype, int flag, const T* __restrict__ a,
@@ -2336,8 +2335,6 @@ void dotT( Function& a, Function& b, int level, int flag, int slot )
       int64_t          nu = 0;
       const MarchUnit* mu = level >= 2 ? M.transferUnits( P, &nu ) : nullptr;
       nbC                 = nu > 0 ? std::min< int64_t >( kDotBlocks, ( nu + 3 ) / 4 ) : 0;
-      const int64_t need  = nbC + nbLow;
-      if ( (int64_t) ctx.partials.n < need ) ctx.partials.alloc( (size_t) need );
       if ( nbC > 0 )
          hipLaunchKernelGGL( ( kDotFlat< T > ), dim3( (unsigned) nbC ), dim3( 256 ), 0, s, g, mu, nu, P.dCellType.p,
                              flag, (const T*) a.ptr( level ), (const T*) b.ptr( level ), ctx.partials.p );
@@ -2346,15 +2343,14 @@ void dotT( Function& a, Function& b, int level, int flag, int slot )
    {
       int64_t        nseg   = 0;
       const CellSeg* sg     = M.segments( P, segLen< NR >(), &nseg );
-      const int64_t  nbCell = ( nseg + 3 ) / 4;
-      const int64_t  need   = ( nbCell + nbLow ) * NR;
-      if ( (int64_t) ctx.partials.n < need ) ctx.partials.alloc( (size_t) need );
+      const int64_t  nbCell = std::min< int64_t >( kDotBlocks, ( nseg + 3 ) / 4 );
       if ( nbCell > 0 && level >= 2 )
          hipLaunchKernelGGL( ( kDotCell< T, NR > ), dim3( (unsigned) nbCell ), dim3( 256 ), 0, s, g, sg, nseg,
                              P.dCellType.p, flag, (const T*) a.ptr( level ), (const T*) b.ptr( level ),
                              ctx.partials.p );
       nbC = ( level >= 2 ) ? nbCell : 0;
    }
+   if ( ( nbC + nbLow ) * NR > (int64_t) ctx.partials.n ) throw std::logic_error( "dot: partial-sum buffer too small" );
    hipLaunchKernelGGL( ( kDotLow< T, NR > ), dim3( (unsigned) nbLow ), dim3( 256 ), 0, s, g, P.dFaceGeom.p, M.T.nF(),
                        P.dEdgeGeom.p, M.T.nE(), P.dVertexGeom.p, M.T.nV(), flag, (const T*) a.ptr( level ),
                        (const T*) b.ptr( level ), ctx.partials.p + nbC * NR );
@@ -2632,7 +2628,10 @@ bool launchResidualSplit( Operator& op, Function& x, const Function& b, Function
    int64_t    n = 0;
    M.marchUnits( P, &n );
    Context& ctx = *M.ctx;
-   if ( P.sched.kind != 3 || n == 0 || (int64_t) ctx.partials.n < n + 64 ) return false;
+   if ( P.sched.kind != 3 || n == 0 ) return false;
+   // one partial per work unit, in a buffer of the level (no captured graph uses it)
+   if ( (int64_t) P.normPartials.n < n + 64 ) P.normPartials.alloc( (size_t) ( n + 64 ) );
+   double* part = P.normPartials.p;
    hipStream_t   s  = ctx.stream;
    LV            g  = viewOf( P );
    auto&         Wt = op.weights( level );
@@ -2652,7 +2651,7 @@ bool launchResidualSplit( Operator& op, Function& x, const Function& b, Function
       hipLaunchKernelGGL( ( kFaceApply< double, 1, kResidual > ), dim3( blocksFor( m * ( m + 1 ) / 2, 256 ), nF ),
                           dim3( 256 ), 0, s, g, P.dFaceGeom.p, Wt.face64.p, flag, u, y, bb );
    // cells: FP32 residual straight into rc, r^2 per wave into partials[0, n)
-   marchLaunch< double, kResidualSplit >( M, P, Wt.cell64.p, flag, u, nullptr, bb, 0.0, 0, yc, ctx.partials.p );
+   marchLaunch< double, kResidualSplit >( M, P, Wt.cell64.p, flag, u, nullptr, bb, 0.0, 0, yc, part );
    // FP32 copy of the vertex, edge and face residuals (whole arrays)
    int64_t      nr = 0;
    const Range* R  = M.closureRanges( P, kAll | kLowerDim, &nr );
@@ -2661,8 +2660,8 @@ bool launchResidualSplit( Operator& op, Function& x, const Function& b, Function
                           (const double*) nullptr, (const double*) nullptr, 1.0, 0.0, 0.0, 0, 0.0 );
    // ||r||^2: cell partials, then the vertex, edge and face part, summed in a fixed order
    hipLaunchKernelGGL( ( kDotLow< double, 1 > ), dim3( 64 ), dim3( 256 ), 0, s, g, P.dFaceGeom.p, nF, P.dEdgeGeom.p, nE,
-                       P.dVertexGeom.p, nV, flag, (const double*) y, (const double*) y, ctx.partials.p + n );
-   hipLaunchKernelGGL( ( kFinishDot< 1 > ), dim3( 1 ), dim3( 256 ), 0, s, ctx.partials.p, n + 64, ctx.scalars.p + slot );
+                       P.dVertexGeom.p, nV, flag, (const double*) y, (const double*) y, part + n );
+   hipLaunchKernelGGL( ( kFinishDot< 1 > ), dim3( 1 ), dim3( 256 ), 0, s, part, n + 64, ctx.scalars.p + slot );
    return true;
 }
 

@@ -78,6 +78,10 @@ struct EdgeTerm
 
 // largest number of neighbour terms of one edge (the fused edge SOR stages them in LDS)
 constexpr int kEdgeTermsMax = 128;
+// reductions: the cell part of a dot product runs on at most kDotBlocks workgroups, so a context's partial-sum
+// buffer has a fixed size (it is allocated once, with the context: captured graphs hold its address)
+constexpr int     kDotBlocks     = 2048;
+constexpr int64_t kPartialsCap   = ( kDotBlocks + 64 ) * 32;
 // levels whose cell SOR phase runs as one workgroup per macro-cell for all four colours
 constexpr int kCellSmallMaxLevel = 5;
 

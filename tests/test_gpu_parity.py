@@ -425,3 +425,42 @@ def test_interpolate_modes_matches_host(kind, nrhs, prec, gpu_ctx):
     A.apply(f, y1, L, H.INNER)
     A.apply(h, y2, L, H.INNER)
     assert np.array_equal(y1.download(L), y2.download(L))
+
+
+def test_graph_replays_survive_other_solvers_and_new_functions(gpu_ctx):
+    """A captured V-cycle keeps working after larger solvers are created on the same context (the context's
+    reduction buffers are never reallocated) and is never replayed for a new function that happens to reuse a
+    destroyed function's address (graphs are keyed on function ids)."""
+    import hyteg_amd as H
+    L = 4
+    say(f"[graph lifetime] cube n=1 level={L}, then cube n=2 level 5 on the same context")
+    S = H.PrimitiveStorage(H.MeshInfo.meshCuboid((0, 0, 0), (1, 1, 1), 1, 1, 1), gpu_ctx)
+    A = H.P1ConstantLaplaceOperator(S, 0, L)
+
+    def gmg(St, Lmax):
+        return H.GeometricMultigridSolver(St, H.SORSmoother(1.0), H.CGSolver(St, 0, 0), H.P1toP1LinearRestriction(),
+                                          H.P1toP1LinearProlongation(), 0, Lmax, 2, 2)
+
+    g = gmg(S, L)
+    b = H.P1Function("b", S, 0, L)
+    b.interpolate(1.0, L, H.INNER)
+    x = H.P1Function("x", S, 0, L)
+    g.solve_graph(A, x, b, L, 2)                       # captured here
+    S2 = H.PrimitiveStorage(H.MeshInfo.meshCuboid((0, 0, 0), (1, 1, 1), 2, 2, 2), gpu_ctx)
+    A2 = H.P1ConstantLaplaceOperator(S2, 0, 5)
+    g2 = gmg(S2, 5)
+    x2, b2 = H.P1Function("x2", S2, 0, 5), H.P1Function("b2", S2, 0, 5)
+    b2.interpolate(1.0, 5, H.INNER)
+    g2.solve_graph(A2, x2, b2, 5, 1)
+    ref = H.P1Function("ref", S, 0, L)
+    ref.interpolate(0.0, L, H.ALL)
+    for _ in range(5):
+        g.solve(A, ref, b, L)                          # eager cycles
+    x.interpolate(0.0, L, H.ALL)
+    g.solve_graph(A, x, b, L, 5)                       # replay of the graph captured before g2 existed
+    assert np.array_equal(x.download(L), ref.download(L))
+    for k in range(3):                                 # new functions, possibly at a reused address
+        del x
+        x = H.P1Function("x", S, 0, L)
+        g.solve_graph(A, x, b, L, 5)
+        assert np.array_equal(x.download(L), ref.download(L)), k
