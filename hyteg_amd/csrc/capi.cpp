@@ -298,6 +298,7 @@ struct hymi_gmg_s
    void replay( Function& x, Function& b, int L, int count )
    {
       hipStream_t    s     = A->mesh->ctx->stream;
+      if ( x.uid == 0 || b.uid == 0 ) throw std::logic_error( "graph replay of a function without an id" );
       auto           key   = std::make_tuple( x.uid, b.uid, L );
       const uint64_t epoch = A->mesh->scheduleEpoch;
       auto           it    = graphs.find( key );
@@ -630,6 +631,7 @@ int hymi_function_create( hymi_mesh mesh, int minL, int maxL, int precision, int
       h->f.maxL  = F->maxL;
       h->f.nrhs  = F->nrhs;
       h->f.prec  = F->prec;
+      h->f.uid   = F->uid;
       h->f.data  = std::move( F->data );
       h->f.count = std::move( F->count );
       F->data.clear();
