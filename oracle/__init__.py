@@ -74,6 +74,9 @@ def lib():
             "om_face_stencil": (i, [vp, i, i, i, P(d)]),
             "om_apply": (None, [vp, vp, vp, i, i, i]),
             "om_smooth_sor": (None, [vp, vp, vp, d, i, i, i]),
+            "om_smooth_sor_backward": (None, [vp, vp, vp, d, i, i, i]),
+            "om_gmg_set_variant": (None, [vp, i, i]),
+            "om_fmg_solve": (None, [vp, vp, vp, vp, i, i]),
             "om_restrict": (None, [vp, i, i]),
             "om_prolongate": (None, [vp, i, i, i]),
             "om_cg_create": (vp, [vp, i, i, i, d]),
@@ -267,6 +270,9 @@ class Operator:
     def smooth_sor(self, x, b, level, omega=1.0, flag=INNER | NEUMANN, order=0):
         lib().om_smooth_sor(self.h, x.h, b.h, float(omega), level, flag, order)
 
+    def smooth_sor_backward(self, x, b, level, omega=1.0, flag=INNER | NEUMANN, order=0):
+        lib().om_smooth_sor_backward(self.h, x.h, b.h, float(omega), level, flag, order)
+
 
 def restrict(f, src_level, flag=INNER | NEUMANN):
     lib().om_restrict(f.h, src_level, flag)
@@ -290,8 +296,14 @@ class CG:
 
 
 class GMG:
-    def __init__(self, mesh, minL, maxL, nu1=3, nu2=3, omega=1.0, order=0, cg_max_iter=10000, cg_tol=1e-16):
+    def __init__(self, mesh, minL, maxL, nu1=3, nu2=3, omega=1.0, order=0, cg_max_iter=10000, cg_tol=1e-16,
+                 wcycle=False, symmetric=False):
         self.h = lib().om_gmg_create(mesh.h, minL, maxL, nu1, nu2, float(omega), order, cg_max_iter, cg_tol)
+        lib().om_gmg_set_variant(self.h, int(wcycle), int(symmetric))
+
+    def fmg(self, A, x, b, level, cycles_per_level=1):
+        """FullMultigridSolver: cycles on every level from the coarsest up, prolongation (Replace) between."""
+        lib().om_fmg_solve(self.h, A.h, x.h, b.h, level, cycles_per_level)
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:

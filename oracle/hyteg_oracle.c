@@ -1431,15 +1431,16 @@ void om_apply(const om_op* op, om_func* src, om_func* dst, int L, int flag, int 
 }
 
 /* ------------------------------------------------------------------ */
-/* SOR (P1Operator.hpp:341-420): V -> E -> F -> C with ghost refresh in between */
-void om_smooth_sor(const om_op* op, om_func* X, om_func* B, double omega, int L, int flag, int order)
+/* SOR (P1Operator.hpp:341-420).  Forward: V -> E -> F -> C with a ghost refresh between classes.  Backward
+ * (smooth_sor_backwards, same file; kernels sor_3D_macrocell_P1_backwards.cpp, sor_3D_macroface_P1_backwards
+ * .cpp): C -> F -> E -> V with the downward refreshes between classes, and inside each primitive the
+ * lexicographic order reversed (order 0) or the colours taken in reverse (order 1). */
+static void sor_vertices(const om_op* op, om_func* X, om_func* B, double omega, int L, int flag)
 {
    const om_mesh* m  = op->m;
    int            li = L - op->minL;
-   long           w  = width_of( L );
-   comm_all( X, L );
-   om_level* x = LV( X, L );
-   om_level* b = LV( B, L );
+   om_level*      x  = LV( X, L );
+   om_level*      b  = LV( B, L );
    for ( int v = 0; v < m->nV; v++ )
    {
       if ( !( m->vt[v] & flag ) ) continue;
@@ -1449,94 +1450,154 @@ void om_smooth_sor(const om_op* op, om_func* X, om_func* B, double omega, int L,
          t -= st[k + 1] * x->v[v][k + 1];
       x->v[v][0] = ( 1.0 - omega ) * x->v[v][0] + omega * t / st[0];
    }
-   comm_v2e( X, L );
-   if ( L >= 1 )
-      for ( int e = 0; e < m->nE; e++ )
+}
+static void sor_edges(const om_op* op, om_func* X, om_func* B, double omega, int L, int flag, int order, int bw)
+{
+   const om_mesh* m  = op->m;
+   int            li = L - op->minL;
+   long           w  = width_of( L );
+   om_level*      x  = LV( X, L );
+   om_level*      b  = LV( B, L );
+   if ( L < 1 ) return;
+   for ( int e = 0; e < m->nE; e++ )
+   {
+      if ( !( m->et[e] & flag ) ) continue;
+      const double* st   = op->est[li] + op->est_p[e];
+      double*       a    = x->e[e];
+      const double* r    = b->e[e];
+      int           nF   = nF_of_e( m, e ), nC = nC_of_e( m, e );
+      double        inv  = 1.0 / st[3];
+      int           ncol = order ? 2 : 1;
+      for ( int cc = 0; cc < ncol; cc++ )
       {
-         if ( !( m->et[e] & flag ) ) continue;
-         const double* st  = op->est[li] + op->est_p[e];
-         double*       a   = x->e[e];
-         const double* r   = b->e[e];
-         int           nF  = nF_of_e( m, e ), nC = nC_of_e( m, e );
-         double        inv = 1.0 / st[3];
-         int           ncol = order ? 2 : 1;
-         for ( int col = 0; col < ncol; col++ )
-            for ( long xx = 1; xx <= w - 2; xx++ )
+         int col = bw ? ncol - 1 - cc : cc;
+         for ( long k = 1; k <= w - 2; k++ )
+         {
+            long xx = bw ? w - 1 - k : k;
+            if ( order && ( xx % 2 ) != col ) continue;
+            double t = r[xx];
+            t -= st[2] * a[xx - 1] + st[4] * a[xx + 1];
+            for ( int fi = 0; fi < nF; fi++ )
             {
-               if ( order && ( xx % 2 ) != col ) continue;
-               double t = r[xx];
-               t -= st[2] * a[xx - 1] + st[4] * a[xx + 1];
-               for ( int fi = 0; fi < nF; fi++ )
-               {
-                  int sw = fi == 0 ? 0 : 3 + 2 * fi, se = sw + 1;
-                  t -= st[sw] * a[w + fi * ( w - 1 ) + xx - 1] + st[se] * a[w + fi * ( w - 1 ) + xx];
-               }
-               for ( int ci = 0; ci < nC; ci++ )
-                  t -= st[3 + 2 * nF + ci] * a[w + nF * ( w - 1 ) + ci * ( w - 2 ) + xx - 1];
-               a[xx] = ( 1.0 - omega ) * a[xx] + omega * inv * t;
+               int sw = fi == 0 ? 0 : 3 + 2 * fi, se = sw + 1;
+               t -= st[sw] * a[w + fi * ( w - 1 ) + xx - 1] + st[se] * a[w + fi * ( w - 1 ) + xx];
             }
+            for ( int ci = 0; ci < nC; ci++ )
+               t -= st[3 + 2 * nF + ci] * a[w + nF * ( w - 1 ) + ci * ( w - 2 ) + xx - 1];
+            a[xx] = ( 1.0 - omega ) * a[xx] + omega * inv * t;
+         }
       }
+   }
+}
+static void sor_faces(const om_op* op, om_func* X, om_func* B, double omega, int L, int flag, int order, int bw)
+{
+   const om_mesh* m  = op->m;
+   int            li = L - op->minL;
+   long           w  = width_of( L );
+   om_level*      x  = LV( X, L );
+   om_level*      b  = LV( B, L );
+   if ( L < 2 ) return;
+#pragma omp parallel for schedule( dynamic )
+   for ( int f = 0; f < m->nF; f++ )
+   {
+      if ( !( m->ft[f] & flag ) ) continue;
+      double*       fa  = x->f[f];
+      const double* r   = b->f[f];
+      int           nC  = nC_of_f( m, f );
+      double        ctr = 0;
+      for ( int n = 0; n < nC; n++ )
+         ctr += op->fst[li][( 2 * f + n ) * ND + D_C].wgt;
+      int ncol = order ? 3 : 1;
+      for ( int cc = 0; cc < ncol; cc++ )
+      {
+         int col = bw ? ncol - 1 - cc : cc;
+         for ( long jj = 1; jj <= w - 3; jj++ )
+         {
+            long j = bw ? w - 2 - jj : jj;
+            for ( long ii = 1; ii + j <= w - 2; ii++ )
+            {
+               long i = bw ? w - 1 - j - ii : ii;
+               if ( order && ( ( i + 2 * j ) % 3 ) != col ) continue;
+               double t = r[fidx( w, i, j )];
+               for ( int n = 0; n < nC; n++ )
+               {
+                  const face_term* T = op->fst[li] + ( 2 * f + n ) * ND;
+                  for ( int dd = 1; dd < ND; dd++ )
+                     t -= T[dd].wgt * face_term_value( fa, w, n, &T[dd], i, j );
+               }
+               long idx = fidx( w, i, j );
+               fa[idx]  = ( 1.0 - omega ) * fa[idx] + omega / ctr * t;
+            }
+         }
+      }
+   }
+}
+static void sor_cells(const om_op* op, om_func* X, om_func* B, double omega, int L, int flag, int order, int bw)
+{
+   const om_mesh* m  = op->m;
+   int            li = L - op->minL;
+   long           w  = width_of( L );
+   om_level*      x  = LV( X, L );
+   om_level*      b  = LV( B, L );
+   if ( L < 2 ) return;
+#pragma omp parallel for schedule( dynamic )
+   for ( int c = 0; c < m->nC; c++ )
+   {
+      if ( !( m->ct[c] & flag ) ) continue;
+      const double* st   = op->cst[li] + ND * c;
+      double*       a    = x->c[c];
+      const double* r    = b->c[c];
+      double        inv  = 1.0 / st[D_C];
+      int           ncol = order ? 4 : 1;
+      for ( int cc = 0; cc < ncol; cc++ )
+      {
+         int col = bw ? ncol - 1 - cc : cc;
+         for ( long zz = 1; zz <= w - 4; zz++ )
+         {
+            long z = bw ? w - 3 - zz : zz;
+            for ( long yy = 1; yy + z <= w - 3; yy++ )
+            {
+               long y = bw ? w - 2 - z - yy : yy;
+               long off[ND];
+               for ( int dd = 0; dd < ND; dd++ )
+                  off[dd] = cidx( w, 1 + DOFF[dd][0], y + DOFF[dd][1], z + DOFF[dd][2] ) - 1;
+               for ( long k = 1; k + y + z <= w - 2; k++ )
+               {
+                  long xx = bw ? w - 1 - y - z - k : k;
+                  if ( order && ( ( xx + 2 * y + 3 * z ) % 4 ) != col ) continue;
+                  double t = r[off[D_C] + xx];
+                  for ( int dd = 1; dd < ND; dd++ )
+                     t -= st[dd] * a[off[dd] + xx];
+                  a[off[D_C] + xx] = ( 1.0 - omega ) * a[off[D_C] + xx] + omega * inv * t;
+               }
+            }
+         }
+      }
+   }
+}
+
+void om_smooth_sor(const om_op* op, om_func* X, om_func* B, double omega, int L, int flag, int order)
+{
+   comm_all( X, L );
+   sor_vertices( op, X, B, omega, L, flag );
+   comm_v2e( X, L );
+   sor_edges( op, X, B, omega, L, flag, order, 0 );
    comm_e2f( X, L );
-   if ( L >= 2 )
-   {
-#pragma omp parallel for schedule( dynamic )
-      for ( int f = 0; f < m->nF; f++ )
-      {
-         if ( !( m->ft[f] & flag ) ) continue;
-         double*       fa  = x->f[f];
-         const double* r   = b->f[f];
-         int           nC  = nC_of_f( m, f );
-         double        ctr = 0;
-         for ( int n = 0; n < nC; n++ )
-            ctr += op->fst[li][( 2 * f + n ) * ND + D_C].wgt;
-         int ncol = order ? 3 : 1;
-         for ( int col = 0; col < ncol; col++ )
-            for ( long j = 1; j <= w - 3; j++ )
-               for ( long i = 1; i + j <= w - 2; i++ )
-               {
-                  if ( order && ( ( i + 2 * j ) % 3 ) != col ) continue;
-                  double t = r[fidx( w, i, j )];
-                  for ( int n = 0; n < nC; n++ )
-                  {
-                     const face_term* T = op->fst[li] + ( 2 * f + n ) * ND;
-                     for ( int dd = 1; dd < ND; dd++ )
-                        t -= T[dd].wgt * face_term_value( fa, w, n, &T[dd], i, j );
-                  }
-                  long idx = fidx( w, i, j );
-                  fa[idx]  = ( 1.0 - omega ) * fa[idx] + omega / ctr * t;
-               }
-      }
-   }
+   sor_faces( op, X, B, omega, L, flag, order, 0 );
    comm_f2c( X, L );
-   if ( L >= 2 )
-   {
-#pragma omp parallel for schedule( dynamic )
-      for ( int c = 0; c < m->nC; c++ )
-      {
-         if ( !( m->ct[c] & flag ) ) continue;
-         const double* st  = op->cst[li] + ND * c;
-         double*       a   = x->c[c];
-         const double* r   = b->c[c];
-         double        inv = 1.0 / st[D_C];
-         int           ncol = order ? 4 : 1;
-         for ( int col = 0; col < ncol; col++ )
-            for ( long z = 1; z <= w - 4; z++ )
-               for ( long y = 1; y + z <= w - 3; y++ )
-               {
-                  long off[ND];
-                  for ( int dd = 0; dd < ND; dd++ )
-                     off[dd] = cidx( w, 1 + DOFF[dd][0], y + DOFF[dd][1], z + DOFF[dd][2] ) - 1;
-                  for ( long xx = 1; xx + y + z <= w - 2; xx++ )
-                  {
-                     if ( order && ( ( xx + 2 * y + 3 * z ) % 4 ) != col ) continue;
-                     double t = r[off[D_C] + xx];
-                     for ( int dd = 1; dd < ND; dd++ )
-                        t -= st[dd] * a[off[dd] + xx];
-                     a[off[D_C] + xx] = ( 1.0 - omega ) * a[off[D_C] + xx] + omega * inv * t;
-                  }
-               }
-      }
-   }
+   sor_cells( op, X, B, omega, L, flag, order, 0 );
+}
+
+void om_smooth_sor_backward(const om_op* op, om_func* X, om_func* B, double omega, int L, int flag, int order)
+{
+   comm_all( X, L );
+   sor_cells( op, X, B, omega, L, flag, order, 1 );
+   comm_c2f( X, L );
+   sor_faces( op, X, B, omega, L, flag, order, 1 );
+   comm_f2e( X, L );
+   sor_edges( op, X, B, omega, L, flag, order, 1 );
+   comm_e2v( X, L );
+   sor_vertices( op, X, B, omega, L, flag );
 }
 
 /* ------------------------------------------------------------------ */
@@ -1797,6 +1858,8 @@ struct om_gmg
    om_func* tmp;
    om_cg*   cg;
    int      minL, maxL, nu1, nu2, order;
+   int      wcycle;    /* GeometricMultigridSolver.hpp:201-204: the coarse correction is visited twice */
+   int      symmetric; /* SymmetricSORSmoother.hpp:38-60: a forward then a backward sweep per step */
    double   omega;
 };
 om_gmg* om_gmg_create(om_mesh* m, int minL, int maxL, int nu1, int nu2, double omega, int order, int cgMaxIter,
@@ -1820,6 +1883,16 @@ void om_gmg_free(om_gmg* g)
    om_cg_free( g->cg );
    free( g );
 }
+void om_gmg_set_variant(om_gmg* g, int wcycle, int symmetric)
+{
+   g->wcycle    = wcycle;
+   g->symmetric = symmetric;
+}
+static void gmg_smooth(om_gmg* g, const om_op* A, om_func* x, om_func* b, int L, int flag)
+{
+   om_smooth_sor( A, x, b, g->omega, L, flag, g->order );
+   if ( g->symmetric ) om_smooth_sor_backward( A, x, b, g->omega, L, flag, g->order );
+}
 static void gmg_rec(om_gmg* g, const om_op* A, om_func* x, om_func* b, int L)
 {
    const int flag = OM_INNER | OM_NEUMANN;
@@ -1829,7 +1902,7 @@ static void gmg_rec(om_gmg* g, const om_op* A, om_func* x, om_func* b, int L)
       return;
    }
    for ( int i = 0; i < g->nu1; i++ )
-      om_smooth_sor( A, x, b, g->omega, L, flag, g->order );
+      gmg_smooth( g, A, x, b, L, flag );
    om_apply( A, x, g->tmp, L, flag, 0 );
    double   c2[2] = {1.0, -1.0};
    om_func* bt[2] = {b, g->tmp};
@@ -1840,11 +1913,26 @@ static void gmg_rec(om_gmg* g, const om_op* A, om_func* x, om_func* b, int L)
    om_assign( b, L - 1, flag, 1, c1, t1 );
    om_interpolate_const( x, L - 1, OM_ALL, 0.0 );
    gmg_rec( g, A, x, b, L - 1 );
+   if ( g->wcycle ) gmg_rec( g, A, x, b, L - 1 );
    om_prolongate( x, L - 1, flag, 1 );
    for ( int i = 0; i < g->nu2; i++ )
-      om_smooth_sor( A, x, b, g->omega, L, flag, g->order );
+      gmg_smooth( g, A, x, b, L, flag );
 }
 void om_gmg_solve(om_gmg* g, const om_op* A, om_func* x, om_func* b, int L) { gmg_rec( g, A, x, b, L ); }
+
+/* FullMultigridSolver.hpp:41-68: from the coarsest level up, cyclesPerLevel cycles of g, then x on the next
+ * level = P x (prolongate with Replace on Inner | Neumann DoFs); b must hold the right-hand side on every
+ * level (the cycles overwrite b below the level they run on, which the loop never revisits). */
+void om_fmg_solve(om_gmg* g, const om_op* A, om_func* x, om_func* b, int L, int cyclesPerLevel)
+{
+   const int flag = OM_INNER | OM_NEUMANN;
+   for ( int cur = g->minL; cur <= L; cur++ )
+   {
+      for ( int k = 0; k < cyclesPerLevel; k++ )
+         gmg_rec( g, A, x, b, cur );
+      if ( cur < g->maxL ) om_prolongate( x, cur, flag, 0 );
+   }
+}
 
 double om_wtime(void)
 {
