@@ -278,6 +278,9 @@ class _P1ConstantOperator:
     def smooth_sor(self, dst, rhs, relax, level, flag):
         _lib.call("hymi_smooth_sor", self.h, dst.h, rhs.h, float(relax), level, flag)
 
+    def smooth_sor_backwards(self, dst, rhs, relax, level, flag):
+        _lib.call("hymi_smooth_sor_backwards", self.h, dst.h, rhs.h, float(relax), level, flag)
+
     def smooth_gs(self, dst, rhs, level, flag):
         self.smooth_sor(dst, rhs, 1.0, level, flag)
 
@@ -301,11 +304,29 @@ class P1toP1LinearProlongation:
 
 
 class SORSmoother:
+    symmetric = False
+
     def __init__(self, relax=1.0):
         self.relax = relax
 
     def solve(self, A, x, b, level):
         A.smooth_sor(x, b, self.relax, level, INNER | NEUMANN)
+
+
+class SymmetricSORSmoother(SORSmoother):
+    """SymmetricSORSmoother (src/hyteg/solvers/SymmetricSORSmoother.hpp:38-60): a forward SOR sweep, then a
+    backward one."""
+    symmetric = True
+
+    def solve(self, A, x, b, level):
+        A.smooth_sor(x, b, self.relax, level, INNER | NEUMANN)
+        A.smooth_sor_backwards(x, b, self.relax, level, INNER | NEUMANN)
+
+
+class CycleType:
+    """GeometricMultigridSolver's CycleType (V: one coarse correction per level, W: two)."""
+    VCYCLE = 0
+    WCYCLE = 1
 
 
 class CGSolver:
@@ -322,14 +343,20 @@ class CGSolver:
 
 class GeometricMultigridSolver:
     """GeometricMultigridSolver(storage, smoother, coarseSolver, restriction, prolongation, minLevel, maxLevel,
-    preSmoothSteps, postSmoothSteps) -- V-cycles with SOR smoothing and a device CG coarse solver."""
+    preSmoothSteps, postSmoothSteps, smoothIncrement, cycleType) -- V- or W-cycles with SOR or symmetric SOR
+    smoothing and a device CG coarse solver (GeometricMultigridSolver.hpp:42-235; smoothIncrement must be 0)."""
 
     def __init__(self, storage, smoother, coarseSolver, restrictionOperator, prolongationOperator, minLevel,
-                 maxLevel, preSmoothSteps=3, postSmoothSteps=3):
+                 maxLevel, preSmoothSteps=3, postSmoothSteps=3, smoothIncrement=0, cycleType=CycleType.VCYCLE):
+        if smoothIncrement != 0:
+            raise ValueError("smoothIncrement != 0 is not supported")
+        if cycleType not in (CycleType.VCYCLE, CycleType.WCYCLE):
+            raise ValueError("cycleType must be CycleType.VCYCLE or CycleType.WCYCLE")
         self.storage = storage
         self.smoother, self.coarse = smoother, coarseSolver
         self.minLevel, self.maxLevel = minLevel, maxLevel
         self.nu1, self.nu2 = preSmoothSteps, postSmoothSteps
+        self.cycleType = cycleType
         self._h = {}
 
     def _handle(self, A, x):
@@ -341,6 +368,9 @@ class GeometricMultigridSolver:
             _lib.call("hymi_gmg_create", A.h, x.h, self.minLevel, self.maxLevel, self.nu1, self.nu2,
                       float(self.smoother.relax), int(cg_it), float(cg_tol), ctypes.byref(h))
             self._h[key] = (h, A)
+            if self.cycleType != CycleType.VCYCLE or getattr(self.smoother, "symmetric", False):
+                _lib.call("hymi_gmg_set_variant", h, int(self.cycleType),
+                          int(getattr(self.smoother, "symmetric", False)))
         return self._h[key][0]
 
     def solve(self, A, x, b, level):
@@ -349,6 +379,24 @@ class GeometricMultigridSolver:
     def solve_graph(self, A, x, b, level, count=1):
         """Replay a hipGraph-captured V-cycle `count` times."""
         _lib.call("hymi_gmg_vcycle_graph", self._handle(A, x), x.h, b.h, level, count)
+
+
+class FullMultigridSolver:
+    """FullMultigridSolver(storage, gmgSolver, fmgProlongation, minLevel, maxLevel, cyclesPerLevel)
+    (src/hyteg/solvers/FullMultigridSolver.hpp:31-68): cycles of gmgSolver on every level from minLevel up,
+    the iterate prolongated (Replace) to the next level in between.  b must hold the right-hand side on
+    every level; the cycles run from captured graphs."""
+
+    def __init__(self, storage, gmgSolver, fmgProlongation, minLevel, maxLevel, cyclesPerLevel=1):
+        if not isinstance(fmgProlongation, P1toP1LinearProlongation):
+            raise TypeError("only the linear prolongation is available")
+        if minLevel != gmgSolver.minLevel or maxLevel != gmgSolver.maxLevel:
+            raise ValueError("the FMG levels must be the multigrid solver's levels")
+        self.gmg, self.minLevel, self.maxLevel = gmgSolver, minLevel, maxLevel
+        self.cyclesPerLevel = cyclesPerLevel
+
+    def solve(self, A, x, b, level):
+        _lib.call("hymi_fmg_solve", self.gmg._handle(A, x), x.h, b.h, level, self.cyclesPerLevel)
 
     def __del__(self, _releasable=_releasable):
         if not _releasable():

@@ -83,6 +83,7 @@ SIGNATURES = {
     "hymi_apply": [_vp, _vp, _vp, _i, _i, _i],
     "hymi_residual": [_vp, _vp, _vp, _vp, _i, _i],
     "hymi_smooth_sor": [_vp, _vp, _vp, _d, _i, _i],
+    "hymi_smooth_sor_backwards": [_vp, _vp, _vp, _d, _i, _i],
     "hymi_restrict": [_vp, _i, _i],
     "hymi_prolongate_add": [_vp, _i, _i],
     "hymi_cg_solve": [_vp, _vp, _vp, _i, _i, _d],
@@ -90,6 +91,8 @@ SIGNATURES = {
     "hymi_gmg_destroy": [_vp],
     "hymi_gmg_vcycle": [_vp, _vp, _vp, _i],
     "hymi_gmg_vcycle_graph": [_vp, _vp, _vp, _i, _i],
+    "hymi_gmg_set_variant": [_vp, _i, _i],
+    "hymi_fmg_solve": [_vp, _vp, _vp, _i, _i],
     "hymi_time_kernel": [_vp, _vp, _vp, _vp, _vp, _i, _i, _i, _P(_d)],
     "hymi_mesh_set_schedule": [_vp, _i, _i, _i, _i, _i],
     "hymi_interpolate_modes": [_vp, _i, _i, _i, _P(_d)],

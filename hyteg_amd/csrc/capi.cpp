@@ -262,6 +262,7 @@ struct hymi_gmg_s
 {
    Operator*                   A = nullptr;
    int                         minL = 0, maxL = 0, nu1 = 3, nu2 = 3, prec = kF64, nrhs = 1;
+   int                         wcycle = 0, symmetric = 0; // cycle type W; symmetric SOR smoothing steps
    double                      omega = 1.0, cgTol = 1e-16;
    int                         cgIters = 0;
    std::unique_ptr< Function > tmp;
@@ -346,14 +347,32 @@ struct hymi_gmg_s
       // ghost copies of x are current after a sweep and after the coarse-level reset; only the caller's x on
       // the finest level, x after prolongation and x after the coarse solver need a refresh
       const bool finest = ( L == maxL_invoked );
-      launchSorSweeps( *A, x, b, omega, L, flag, nu1, finest );
+      smooth( x, b, L, flag, nu1, finest );
       launchApply( *A, x, *tmp, &b, L, flag, kResidual, nu1 == 0 && finest );
       // restriction written straight into b on the coarse level (== restrict + b.assign on flag DoFs)
       launchRestrict( *tmp, b, L, flag );
       launchLincomb( x, L - 1, kAll, 0, nullptr, nullptr, false, 0.0 );
       recurse( x, b, L - 1 );
+      // W-cycle (GeometricMultigridSolver.hpp:201-204): the coarse correction is visited again, from its
+      // result and with the same coarse right-hand side
+      if ( wcycle ) recurse( x, b, L - 1 );
       launchProlongateAdd( x, L - 1, flag, L - 1 == minL );
-      launchSorSweeps( *A, x, b, omega, L, flag, nu2, true );
+      smooth( x, b, L, flag, nu2, true );
+   }
+
+   // `count` smoothing steps: SOR sweeps, or (SymmetricSORSmoother.hpp:38-60) a forward and a backward sweep
+   void smooth( Function& x, Function& b, int L, int flag, int count, bool refreshFirst )
+   {
+      if ( !symmetric )
+      {
+         launchSorSweeps( *A, x, b, omega, L, flag, count, refreshFirst );
+         return;
+      }
+      for ( int k = 0; k < count; k++ )
+      {
+         launchSor( *A, x, b, omega, L, flag, refreshFirst && k == 0, false );
+         launchSor( *A, x, b, omega, L, flag, false, true );
+      }
    }
 };
 
@@ -1041,6 +1060,18 @@ int hymi_smooth_sor( hymi_operator op, hymi_function x, hymi_function b, double 
    } );
 }
 
+int hymi_smooth_sor_backwards( hymi_operator op, hymi_function x, hymi_function b, double omega, int level, int flag )
+{
+   return guarded( [&] {
+      checkOpFunc( op, x, level );
+      checkOpFunc( op, b, level );
+      require( x != b, "x and b must differ" );
+      require( x->f.prec == b->f.prec && x->f.nrhs == b->f.nrhs, "functions must share precision and nrhs" );
+      require( x->f.nrhs == 1, "backward SOR needs one right-hand side" );
+      launchSor( op->op, x->f, b->f, omega, level, flag, true, true );
+   } );
+}
+
 int hymi_restrict( hymi_function f, int srcLevel, int flag )
 {
    return guarded( [&] {
@@ -1148,6 +1179,47 @@ int hymi_gmg_vcycle( hymi_gmg g, hymi_function x, hymi_function b, int level )
    return guarded( [&] {
       checkGmg( g, x, b, level );
       g->cycle( x->f, b->f, level );
+   } );
+}
+
+int hymi_gmg_set_variant( hymi_gmg g, int cycle_type, int symmetric_sor )
+{
+   return guarded( [&] {
+      require( g, "null argument" );
+      require( cycle_type == 0 || cycle_type == 1, "cycle type must be 0 (V) or 1 (W)" );
+      require( symmetric_sor == 0 || symmetric_sor == 1, "symmetric_sor must be 0 or 1" );
+      require( !symmetric_sor || g->nrhs == 1, "symmetric SOR needs one right-hand side" );
+      if ( g->wcycle == cycle_type && g->symmetric == symmetric_sor ) return;
+      HYMI_HIP( hipStreamSynchronize( g->A->mesh->ctx->stream ) );
+      {
+         HYMI_LEGACY_LOCK;
+         for ( auto& kv : g->graphs ) // captured under the old variant
+            (void) hipGraphExecDestroy( kv.second.exec );
+      }
+      g->graphs.clear();
+      g->wcycle    = cycle_type;
+      g->symmetric = symmetric_sor;
+   } );
+}
+
+int hymi_fmg_solve( hymi_gmg g, hymi_function x, hymi_function b, int level, int cycles_per_level )
+{
+   return guarded( [&] {
+      checkGmg( g, x, b, level );
+      require( cycles_per_level >= 0, "negative cycle count" );
+      require( x->f.minL <= g->minL && b->f.minL <= g->minL, "x and b must hold every level of the solver" );
+      const int flag = kInner | kNeumann;
+      for ( int cur = g->minL; cur <= level; cur++ )
+      {
+         g->replay( x->f, b->f, cur, cycles_per_level );
+         if ( cur < g->maxL && cur + 1 <= x->f.maxL )
+         {
+            // prolongate with Replace: the next level's Inner | Neumann DoFs become P x (the others keep
+            // their values); the first cycle there refreshes every closure copy
+            launchLincomb( x->f, cur + 1, flag, 0, nullptr, nullptr, false, 0.0 );
+            launchProlongateAdd( x->f, cur, flag, true );
+         }
+      }
    } );
 }
 

@@ -706,7 +706,7 @@ struct ColourStarts
 template < typename T >
 __global__ __launch_bounds__( 256 ) void kCellSorSmall( LV g, const int* __restrict__ pts, ColourStarts cs,
                                                         const T* __restrict__ W8, const int* __restrict__ ctype, int flag,
-                                                        T* __restrict__ u, const T* __restrict__ b, T omega )
+                                                        T* __restrict__ u, const T* __restrict__ b, T omega, int reverse )
 {
    const int cell = blockIdx.x;
    if ( !( ctype[cell] & flag ) ) return;
@@ -719,8 +719,9 @@ __global__ __launch_bounds__( 256 ) void kCellSorSmall( LV g, const int* __restr
    const T cTC = c[dTC], cTW = c[dTW], cTS = c[dTS], cTSE = c[dTSE], cBC = c[dBC], cBN = c[dBN], cBE = c[dBE],
            cBNW = c[dBNW];
    const T relax = omega / c[dC];
-   for ( int colour = 0; colour < 4; colour++ )
+   for ( int cc = 0; cc < 4; cc++ )
    {
+      const int colour = reverse ? 3 - cc : cc; // a backward sweep takes the colours in reverse
       for ( int k = cs.s[colour] + (int) threadIdx.x; k < cs.s[colour + 1]; k += blockDim.x )
       {
          const int p = pts[k], x = p & 1023, y = ( p >> 10 ) & 1023, z = p >> 20;
@@ -1202,7 +1203,8 @@ __global__ __launch_bounds__( 64 ) void kVertexSorPush( LV g, const VertexGeom* 
 template < typename T >
 __global__ __launch_bounds__( 128 ) void kEdgeSorFused( LV g, const EdgeGeom* __restrict__ eg,
                                                         const EdgeTerm* __restrict__ terms, const T* __restrict__ EW,
-                                                        int flag, T* __restrict__ u, const T* __restrict__ b, T omega )
+                                                        int flag, T* __restrict__ u, const T* __restrict__ b, T omega,
+                                                        int reverse )
 {
    __shared__ int64_t sPrim[kEdgeTermsMax];                  // element offset of the neighbour primitive
    __shared__ int     sB[3][kEdgeTermsMax], sI[3][kEdgeTermsMax], sKind[kEdgeTermsMax];
@@ -1229,9 +1231,10 @@ __global__ __launch_bounds__( 128 ) void kEdgeSorFused( LV g, const EdgeGeom* __
    __syncthreads();
    const T   rl = omega / ew[1];
    const int w = g.w, t0 = tet32( w );
-   for ( int colour = 0; colour < 2; colour++ )
+   for ( int cc = 0; cc < 2; cc++ )
    {
-      // colour 0 = even edge points, colour 1 = odd ones
+      // colour 0 = even edge points, colour 1 = odd ones; a backward sweep takes them in reverse
+      const int colour = reverse ? 1 - cc : cc;
       for ( int x = 2 - colour + 2 * (int) threadIdx.x; x <= w - 2; x += 2 * blockDim.x )
       {
          T acc = ew[0] * u[eo + x - 1] + ew[2] * u[eo + x + 1];
@@ -1261,7 +1264,7 @@ __global__ __launch_bounds__( 128 ) void kEdgeSorFused( LV g, const EdgeGeom* __
 template < typename T >
 __global__ __launch_bounds__( 256 ) void kFaceSorFused( LV g, const FaceGeom* __restrict__ fg, const E2F* __restrict__ e2f,
                                                         const T* __restrict__ FW, int flag, T* __restrict__ u,
-                                                        const T* __restrict__ b, T omega )
+                                                        const T* __restrict__ b, T omega, int reverse )
 {
    const FaceGeom& G = fg[blockIdx.x];
    if ( !( G.type & flag ) ) return;
@@ -1279,8 +1282,9 @@ __global__ __launch_bounds__( 256 ) void kFaceSorFused( LV g, const FaceGeom* __
    const int64_t np = (int64_t) m * ( m + 1 ) / 2;
    const T*      fw = FW + (int64_t) G.face * kFaceW;
    const T       rl = omega / fw[0];
-   for ( int colour = 0; colour < 3; colour++ )
+   for ( int cc = 0; cc < 3; cc++ )
    {
+      const int colour = reverse ? 2 - cc : cc; // a backward sweep takes the colours in reverse
       for ( int64_t k = threadIdx.x; k < np; k += blockDim.x )
       {
          int a, bb;
@@ -2162,8 +2166,15 @@ void applyT( Operator& op, Function& src, Function& dst, const Function* rhs, in
 #undef LAUNCH_MODES
 }
 
+// One SOR sweep.  Forward (P1Operator.hpp:341-420): V -> E -> F -> C, each class reading the lower classes'
+// new values through closure copies refreshed in between and the higher classes' start-of-sweep values.
+// Backward (the same function with backwards = true): C -> F -> E -> V with the colours of every primitive
+// in reverse; each class reads the higher classes' new values straight from their arrays and lower-class
+// values from closure copies, which are refreshed only after the sweep.  Either way every closure copy is
+// current when the sweep ends.
 template < typename T, int NR >
-void sorT( Operator& op, Function& xf, const Function& bf, double omega, int level, int flag, bool refreshFirst )
+void sorT( Operator& op, Function& xf, const Function& bf, double omega, int level, int flag, bool refreshFirst,
+           bool backward = false )
 {
    Mesh&       M  = *op.mesh;
    LevelPlan&  P  = M.plan( level );
@@ -2177,45 +2188,74 @@ void sorT( Operator& op, Function& xf, const Function& bf, double omega, int lev
    if ( refreshFirst ) refreshT< T, NR >( xf, level, 7 );
    if ( NR == 1 && level >= 1 )
    {
-      hipLaunchKernelGGL( ( kVertexSorPush< T > ), dim3( nV ), dim3( 64 ), 0, s, g, P.dVertexGeom.p,
-                          P.dVertexTerms.p, P.dVertexEnds.p, nV, wsel< T >( Wt.vertex64, Wt.vertex32 ), flag, u, b, om );
-      hipLaunchKernelGGL( ( kEdgeSorFused< T > ), dim3( nE ), dim3( 128 ), 0, s, g, P.dEdgeGeom.p, P.dEdgeTerms.p,
-                          wsel< T >( Wt.edge64, Wt.edge32 ), flag, u, b, om );
-      if ( level >= 2 )
-      {
-         if ( nF > 0 && level <= kFaceFusedMaxLevel )
+      const int rev         = backward ? 1 : 0;
+      auto      vertexPhase = [&]() {
+         hipLaunchKernelGGL( ( kVertexSorPush< T > ), dim3( nV ), dim3( 64 ), 0, s, g, P.dVertexGeom.p,
+                             P.dVertexTerms.p, P.dVertexEnds.p, nV, wsel< T >( Wt.vertex64, Wt.vertex32 ), flag, u, b,
+                             om );
+      };
+      auto edgePhase = [&]() {
+         hipLaunchKernelGGL( ( kEdgeSorFused< T > ), dim3( nE ), dim3( 128 ), 0, s, g, P.dEdgeGeom.p, P.dEdgeTerms.p,
+                             wsel< T >( Wt.edge64, Wt.edge32 ), flag, u, b, om, rev );
+      };
+      auto facePhase = [&]() {
+         if ( nF == 0 ) return;
+         if ( level <= kFaceFusedMaxLevel )
             hipLaunchKernelGGL( ( kFaceSorFused< T > ), dim3( nF ), dim3( 256 ), 0, s, g, P.dFaceGeom.p, P.dE2F.p,
-                                wsel< T >( Wt.face64, Wt.face32 ), flag, u, b, om );
-         else if ( nF > 0 )
+                                wsel< T >( Wt.face64, Wt.face32 ), flag, u, b, om, rev );
+         else
          {
-            // large faces: one workgroup per face would leave most of the chip idle
-            refreshT< T, NR >( xf, level, 2 );
+            // large faces: one workgroup per face would leave most of the chip idle.  Forward, the faces'
+            // copies of the just-updated edges are refreshed first; backward, the edges are not updated yet.
+            if ( !backward ) refreshT< T, NR >( xf, level, 2 );
             T* gh = (T*) M.faceGhost( P );
             hipLaunchKernelGGL( ( kFaceGhost< T > ), dim3( P.w - 1, nF ), dim3( 128 ), 0, s, g, P.dFaceGeom.p, u, gh );
-            for ( int c = 0; c < 3; c++ )
+            for ( int cc = 0; cc < 3; cc++ )
                hipLaunchKernelGGL( ( kFaceSorRows< T > ), dim3( P.w - 3, nF ), dim3( 64 ), 0, s, g, P.dFaceGeom.p,
-                                   wsel< T >( Wt.face64, Wt.face32 ), flag, u, b, om, c, (const T*) gh );
+                                   wsel< T >( Wt.face64, Wt.face32 ), flag, u, b, om, backward ? 2 - cc : cc,
+                                   (const T*) gh );
          }
-         refreshT< T, NR >( xf, level, 4 );
+      };
+      auto cellPhase = [&]() {
          if ( level <= kCellSmallMaxLevel && P.colourStart[4] > 0 )
          {
             ColourStarts cs;
             for ( int k = 0; k < 5; k++ )
                cs.s[k] = P.colourStart[k];
             hipLaunchKernelGGL( ( kCellSorSmall< T > ), dim3( M.T.nC() ), dim3( 256 ), 0, s, g, P.dColourPts.p, cs,
-                                wsel< T >( Wt.cell64, Wt.cell32 ), P.dCellType.p, flag, u, b, om );
+                                wsel< T >( Wt.cell64, Wt.cell32 ), P.dCellType.p, flag, u, b, om, rev );
          }
          else
-            for ( int c = 0; c < 4; c++ )
-               cellSorLaunch< T, NR >( M, P, wsel< T >( Wt.cell64, Wt.cell32 ), flag, u, b, om, c );
+            for ( int cc = 0; cc < 4; cc++ )
+               cellSorLaunch< T, NR >( M, P, wsel< T >( Wt.cell64, Wt.cell32 ), flag, u, b, om, backward ? 3 - cc : cc );
+      };
+      if ( !backward )
+      {
+         vertexPhase();
+         edgePhase();
+         if ( level >= 2 )
+         {
+            facePhase();
+            refreshT< T, NR >( xf, level, 4 );
+            cellPhase();
+         }
+         else
+            refreshT< T, NR >( xf, level, 2 | 4 ); // level 1: no face or cell interior; keep face copies current
       }
       else
       {
-         // level 1: faces and cells have no interior points; keep the face closures current
-         refreshT< T, NR >( xf, level, 2 | 4 );
+         if ( level >= 2 )
+         {
+            cellPhase();
+            facePhase();
+         }
+         edgePhase();
+         vertexPhase();
+         refreshT< T, NR >( xf, level, 2 | 4 ); // faces' copies of edges, cells' copies of faces
       }
       return;
    }
+   if ( backward && NR != 1 ) throw std::invalid_argument( "backward SOR needs one right-hand side" );
    if ( !refreshFirst ) refreshT< T, NR >( xf, level, 7 );
    hipLaunchKernelGGL( ( kVertexSor< T, NR > ), dim3( blocksFor( (int64_t) nV * NR, 256 ) ), dim3( 256 ), 0, s, g,
                        P.dVertexGeom.p, P.dVertexTerms.p, nV, wsel< T >( Wt.vertex64, Wt.vertex32 ), flag, u, b, om );
@@ -2431,9 +2471,10 @@ struct ApplyF
 template < typename T, int NR >
 struct SorF
 {
-   static void run( Operator& op, Function& x, const Function& b, double omega, int L, int flag, bool refresh )
+   static void run( Operator& op, Function& x, const Function& b, double omega, int L, int flag, bool refresh,
+                    bool backward )
    {
-      sorT< T, NR >( op, x, b, omega, L, flag, refresh );
+      sorT< T, NR >( op, x, b, omega, L, flag, refresh, backward );
    }
 };
 template < typename T, int NR >
@@ -2491,9 +2532,10 @@ void launchApply( Operator& op, Function& src, Function& dst, const Function* rh
    dispatch< ApplyF >( src.prec, src.nrhs, op, src, dst, rhs, level, flag, mode, refreshSrc );
 }
 
-void launchSor( Operator& op, Function& x, const Function& b, double omega, int level, int flag, bool refreshFirst )
+void launchSor( Operator& op, Function& x, const Function& b, double omega, int level, int flag, bool refreshFirst,
+                bool backward )
 {
-   dispatch< SorF >( x.prec, x.nrhs, op, x, b, omega, level, flag, refreshFirst );
+   dispatch< SorF >( x.prec, x.nrhs, op, x, b, omega, level, flag, refreshFirst, backward );
 }
 
 void launchSorSweeps( Operator& op, Function& x, const Function& b, double omega, int level, int flag, int count,

@@ -20,8 +20,9 @@ void launchApply( Operator& op, Function& src, Function& dst, const Function* rh
                   bool refreshSrc = true );
 // one multicolour SOR sweep in the reference's V -> E -> F -> C order; refreshFirst = false skips the
 // initial ghost-copy refresh when the caller knows the copies of x are current (they are after a sweep)
+// backward = true: the reverse sweep C -> F -> E -> V with the colours in reverse (one right-hand side)
 void launchSor( Operator& op, Function& x, const Function& b, double omega, int level, int flag,
-                bool refreshFirst = true );
+                bool refreshFirst = true, bool backward = false );
 // `count` SOR sweeps (refreshFirst applies to the first)
 void launchSorSweeps( Operator& op, Function& x, const Function& b, double omega, int level, int flag, int count,
                       bool refreshFirst = true );

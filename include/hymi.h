@@ -128,6 +128,10 @@ int hymi_apply( hymi_operator op, hymi_function src, hymi_function dst, int leve
 int hymi_residual( hymi_operator op, hymi_function x, hymi_function b, hymi_function r, int level, int flag );
 /* P1Operator::smooth_sor (P1Operator.hpp:341-420), multicolour order inside each primitive */
 int hymi_smooth_sor( hymi_operator op, hymi_function x, hymi_function b, double omega, int level, int flag );
+/* P1Operator::smooth_sor_backwards (P1Operator.hpp:422-429, kernels sor_3D_macrocell_P1_backwards.cpp and
+ * sor_3D_macroface_P1_backwards.cpp): cells, faces, edges, vertices in turn, the colours of each primitive in
+ * reverse; one right-hand side. */
+int hymi_smooth_sor_backwards( hymi_operator op, hymi_function x, hymi_function b, double omega, int level, int flag );
 
 /* ---- grid transfer ---------------------------------------------------------------------------------
  * P1toP1LinearRestriction::restrict (src/hyteg/gridtransferoperators/P1toP1LinearRestriction.cpp:163-245)
@@ -148,6 +152,14 @@ int hymi_gmg_destroy( hymi_gmg g );
 int hymi_gmg_vcycle( hymi_gmg g, hymi_function x, hymi_function b, int level );
 /* capture one V-cycle for (x, b, level) into a hipGraph and replay it `count` times */
 int hymi_gmg_vcycle_graph( hymi_gmg g, hymi_function x, hymi_function b, int level, int count );
+/* cycle variants of GeometricMultigridSolver: cycle_type 0 = V, 1 = W (GeometricMultigridSolver.hpp:201-204,
+ * CycleType); symmetric_sor = 1 makes every smoothing step a forward and a backward sweep
+ * (SymmetricSORSmoother.hpp:38-60).  Drops the solver's captured graphs. */
+int hymi_gmg_set_variant( hymi_gmg g, int cycle_type, int symmetric_sor );
+/* FullMultigridSolver::solve (src/hyteg/solvers/FullMultigridSolver.hpp:41-68) with the linear prolongation:
+ * from the solver's coarsest level up to `level`, cycles_per_level cycles, then x on the next level = P x
+ * (Replace, Inner | Neumann DoFs).  b must hold the right-hand side on every level from the coarsest up. */
+int hymi_fmg_solve( hymi_gmg g, hymi_function x, hymi_function b, int level, int cycles_per_level );
 
 /* ---- mixed precision: FP32 V-cycles inside FP64 iterative refinement -------------------------------
  * No single reference counterpart; the outer loop is the reference's cycle-and-check driver
