@@ -18,11 +18,13 @@ import numpy as np
 from . import _lib
 from ._lib import (ADD, ALL, CELL, DIRICHLET, EDGE, F32, F64, FACE, FREESLIP, INNER, NEUMANN, REPLACE,
                    VERTEX, HymiError)
+from .dataexport import DataFormat, FixedSizeSQLDB, VTKOutput
 
 __all__ = ["Context", "context", "MeshInfo", "PrimitiveStorage", "P1Function", "P1ConstantLaplaceOperator",
            "P1ConstantMassOperator", "P1toP1LinearRestriction", "P1toP1LinearProlongation", "SORSmoother",
            "CGSolver", "GeometricMultigridSolver", "HymiError", "INNER", "DIRICHLET", "NEUMANN", "FREESLIP", "ALL",
-           "REPLACE", "ADD", "F64", "F32", "VERTEX", "EDGE", "FACE", "CELL"]
+           "REPLACE", "ADD", "F64", "F32", "VERTEX", "EDGE", "FACE", "CELL", "DataFormat", "VTKOutput",
+           "FixedSizeSQLDB"]
 
 
 def _dp(a):
@@ -125,6 +127,12 @@ class PrimitiveStorage:
         _lib.call("hymi_mesh_primitive_type", self.h, cls, idx, ctypes.byref(v))
         return v.value
 
+    def primitive_coordinates(self, cls, idx):
+        """Corner coordinates of one macro primitive in its local order (Primitive::getCoordinates)."""
+        out = np.zeros((cls + 1, 3))
+        _lib.call("hymi_mesh_primitive_coordinates", self.h, cls, idx, _dp(out))
+        return out
+
     def owned_count(self, level):
         v = ctypes.c_int64()
         _lib.call("hymi_mesh_owned_count", self.h, level, ctypes.byref(v))
@@ -194,6 +202,16 @@ class P1Function:
         out = np.zeros(n.value)
         _lib.call("hymi_function_download_primitive", self.h, level, cls, idx, _dp(out))
         return out
+
+    def download_micro_vertices(self, level, cls=CELL):
+        """Values of every macro-cell (or macro-face) at all of its micro-vertices, closure copies synced:
+        shape (primitives, points per primitive[, nrhs]), points in the primitive's index order."""
+        w = (1 << level) + 1
+        per = w * (w + 1) * (w + 2) // 6 if cls == CELL else w * (w + 1) // 2
+        n = self.storage.counts()[cls]
+        out = np.zeros(n * per * self.nrhs)
+        _lib.call("hymi_function_download_micro_vertices", self.h, level, cls, _dp(out))
+        return out.reshape((n, per) if self.nrhs == 1 else (n, per, self.nrhs))
 
     def upload_primitive(self, level, cls, idx, values):
         v = np.ascontiguousarray(values, np.float64)

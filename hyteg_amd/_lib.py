@@ -61,6 +61,7 @@ SIGNATURES = {
     "hymi_mesh_destroy": [_vp],
     "hymi_mesh_counts": [_vp, _P(_i)],
     "hymi_mesh_primitive_type": [_vp, _i, _i, _P(_i)],
+    "hymi_mesh_primitive_coordinates": [_vp, _i, _i, _P(_d)],
     "hymi_mesh_owned_count": [_vp, _i, _P(_i64)],
     "hymi_mesh_owned_coords": [_vp, _i, _P(_d)],
     "hymi_mesh_owned_types": [_vp, _i, _P(_i)],
@@ -70,6 +71,7 @@ SIGNATURES = {
     "hymi_function_download_owned": [_vp, _i, _P(_d)],
     "hymi_function_primitive_size": [_vp, _i, _i, _i, _P(_i64)],
     "hymi_function_download_primitive": [_vp, _i, _i, _i, _P(_d)],
+    "hymi_function_download_micro_vertices": [_vp, _i, _i, _P(_d)],
     "hymi_function_upload_primitive": [_vp, _i, _i, _i, _P(_d)],
     "hymi_interpolate_const": [_vp, _i, _i, _d],
     "hymi_assign": [_vp, _i, _i, _i, _P(_d), _P(_vp)],

@@ -535,6 +535,27 @@ int hymi_mesh_primitive_type( hymi_mesh mesh, int cls, int idx, int* out )
    } );
 }
 
+int hymi_mesh_primitive_coordinates( hymi_mesh mesh, int cls, int idx, double* xyz )
+{
+   return guarded( [&] {
+      require( mesh && xyz, "null argument" );
+      const Topology& T = mesh->m.T;
+      auto            put = [&]( const int* v, int n ) {
+         for ( int k = 0; k < n; k++ )
+            for ( int d = 0; d < 3; d++ )
+               xyz[3 * k + d] = T.vx.at( v[k] )[d];
+      };
+      switch ( cls )
+      {
+      case HYMI_VERTEX: put( &idx, 1 ); break;
+      case HYMI_EDGE: put( T.ev.at( idx ).data(), 2 ); break;
+      case HYMI_FACE: put( T.fv.at( idx ).data(), 3 ); break;
+      case HYMI_CELL: put( T.cv.at( idx ).data(), 4 ); break;
+      default: throw std::invalid_argument( "bad primitive class" );
+      }
+   } );
+}
+
 int hymi_mesh_owned_count( hymi_mesh mesh, int level, int64_t* out )
 {
    return guarded( [&] {
@@ -729,6 +750,29 @@ int hymi_function_primitive_size( hymi_function fh, int level, int cls, int idx,
       require( fh && out, "null argument" );
       requireLevel( fh->f, level );
       *out = primSize( fh->f.mesh->T, level, cls, idx );
+   } );
+}
+
+int hymi_function_download_micro_vertices( hymi_function fh, int level, int cls, double* host )
+{
+   return guarded( [&] {
+      require( fh && host, "null argument" );
+      Function& f = fh->f;
+      requireLevel( f, level );
+      require( cls == HYMI_FACE || cls == HYMI_CELL, "micro-vertex export covers macro-faces or macro-cells" );
+      launchRefresh( f, level, 7 );
+      Mesh&           M   = *f.mesh;
+      LevelPlan&      P   = M.plan( level );
+      const int       w   = P.w;
+      const int       r   = f.nrhs;
+      const int       n   = cls == HYMI_CELL ? M.T.nC() : M.T.nF();
+      const int64_t   per = cls == HYMI_CELL ? tetSize( w ) : triSize( w );
+      auto            buf = downloadLevel( f, level );
+      for ( int p = 0; p < n; p++ )
+      {
+         const int64_t off = cls == HYMI_CELL ? P.cellOff( p ) : P.faceOff( p );
+         std::copy( buf.begin() + off * r, buf.begin() + ( off + per ) * r, host + p * per * r );
+      }
    } );
 }
 

@@ -78,6 +78,10 @@ int hymi_mesh_destroy( hymi_mesh mesh );
 int hymi_mesh_counts( hymi_mesh mesh, int out[4] );
 /* DoF type of a macro primitive (PrimitiveStorage boundary flag -> DoFType) */
 int hymi_mesh_primitive_type( hymi_mesh mesh, int cls, int idx, int* out );
+/* corner coordinates of a macro primitive in its local order (the order micro-vertex indices run along):
+ * 1, 2, 3 or 4 points of 3 doubles (Primitive::getCoordinates, src/hyteg/primitives/Cell.hpp:99,
+ * Face.hpp:89, Edge.hpp:76) */
+int hymi_mesh_primitive_coordinates( hymi_mesh mesh, int cls, int idx, double* xyz );
 /* owned-DoF numbering of one level: vertices, then edge, face and cell interiors in primitive order
  * (numberOfGlobalDoFs, src/hyteg/p1functionspace/VertexDoFFunction.cpp:1410-1498) */
 int hymi_mesh_owned_count( hymi_mesh mesh, int level, int64_t* out );
@@ -98,6 +102,12 @@ int hymi_function_download_owned( hymi_function f, int level, double* host );
  * from the neighbours on download and ignored on upload.  nrhs must be 1. */
 int hymi_function_primitive_size( hymi_function f, int level, int cls, int idx, int64_t* out );
 int hymi_function_download_primitive( hymi_function f, int level, int cls, int idx, double* host );
+/* micro-vertex values of every macro-cell (cls = HYMI_CELL, tetSize(w) values each) or every macro-face
+ * (HYMI_FACE, w(w+1)/2 each), w = 2^level + 1, primitives in id order, each in its own index order (x fastest,
+ * then y, then z), closure copies refreshed first, nrhs values per micro-vertex: the data that
+ * VTKP1Writer::writeScalarFunction streams after VTKOutput::syncAllFunctions
+ * (src/hyteg/dataexport/VTKP1Writer.cpp:110-150, VTKOutput.cpp:252) */
+int hymi_function_download_micro_vertices( hymi_function f, int level, int cls, double* host );
 int hymi_function_upload_primitive( hymi_function f, int level, int cls, int idx, const double* host );
 /* interpolate(constant, level, flag)  (VertexDoFFunction.cpp:238-300) */
 int hymi_interpolate_const( hymi_function f, int level, int flag, double value );
