@@ -93,9 +93,13 @@ def kernels_only(H, n, L, reps):
     t_apply = time_kernel(H, A, None, x, None, y, L, 0, reps)
     t_sor = time_kernel(H, A, None, x, b, None, L, 1, reps)
     t_copy = time_kernel(H, A, None, x, None, y, L, 5, reps)
+    out = {"apply_ms": t_apply, "sor_pass_ms": t_sor, "copy_ms": t_copy}
+    if S.get_cell_sor(L):
+        out["sor_tile_ms"] = time_kernel(H, A, None, x, b, None, L, 7, reps)
+        out["sor_cell_phase_ms"] = time_kernel(H, A, None, x, b, None, L, 6, reps)
     copy_bytes = S.ctx.bytes_in_use() // 3       # one level of one function (x, b, y are alike)
-    return {"apply_ms": t_apply, "sor_pass_ms": t_sor, "copy_ms": t_copy, "copy_bytes_each_way": copy_bytes,
-            "copy_GBs": 2 * copy_bytes / (t_copy * 1e-3) / 1e9}
+    out.update({"copy_bytes_each_way": copy_bytes, "copy_GBs": 2 * copy_bytes / (t_copy * 1e-3) / 1e9})
+    return out
 
 
 def mixed_vs_fp64(H, n, L, tol, seed, cycles_per_outer):
@@ -311,10 +315,25 @@ def main():
     kname = KIND_NAMES[sched["kind"]]
     s = 8
     apply_bytes = 2 * s * Ncell                 # read x once, write y once
-    sor_bytes = 3 * s * Ncell / 4               # one colour pass = a quarter of a sweep's read x, b / write x
-    sor_share = 6 * 4 * t_sor / t_vc            # V(3,3): 6 sweeps x 4 colour passes on the finest level
-    roof_sor = {"bound": "hbm", "achieved": sor_bytes / (t_sor * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "traffic": traffic_from_profile("cell_sor", kname)}
+    pass_bytes = 3 * s * Ncell / 4              # one colour pass = a quarter of a sweep's read x, b / write x
+    roof_pass = {"bound": "hbm", "achieved": pass_bytes / (t_sor * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                 "unit": "GB/s", "traffic": traffic_from_profile("cell_sor", kname)}
+    roof_pass["frac"] = roof_pass["achieved"] / HBM_PEAK_GBS
+    roof_pass = dict(roof_pass, kernel=f"{kname}, SOR, one colour pass", schedule=sched,
+                     algorithmic_bytes_per_launch=pass_bytes, ms_per_launch=t_sor)
+    if S.get_cell_sor(L):
+        # the fused SOR cell phase: all four colours in one tile launch, then the band copy
+        t_tile = time_kernel(H, A, None, x, b, None, L, 7, reps)
+        t_phase = time_kernel(H, A, None, x, b, None, L, 6, reps)
+        sor_bytes = 3 * s * Ncell               # a whole sweep of the cell interiors: read x, b, write x
+        roof_sor = {"bound": "hbm", "achieved": sor_bytes / (t_tile * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "traffic": traffic_from_profile("cell_sor_tile", "kCellSorTile")}
+        roof_sor["frac"] = roof_sor["achieved"] / HBM_PEAK_GBS
+        roof_sor = dict(roof_sor, kernel="kCellSorTile<double>, SOR cell phase, all four colours",
+                        tile=S.get_cell_sor(L), algorithmic_bytes_per_launch=sor_bytes, ms_per_launch=t_tile,
+                        share_of_vcycle=6 * t_tile / t_vc, band_copy_ms=t_phase - t_tile)
+    else:
+        roof_sor = dict(roof_pass, share_of_vcycle=6 * 4 * t_sor / t_vc)
     roof_sor["frac"] = roof_sor["achieved"] / HBM_PEAK_GBS
     roof_apply = {"bound": "hbm", "achieved": apply_bytes / (t_apply * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                   "unit": "GB/s", "traffic": traffic_from_profile("cell_apply", kname)}
@@ -344,9 +363,8 @@ def main():
             "rel_residual_after_all_cycles": rel_res,
             "cycles_total": 1 + args.warmup + args.steps + reps * 0,
         },
-        "roofline": dict(roof_sor, kernel=f"{kname}, SOR, one colour pass", schedule=sched,
-                         algorithmic_bytes_per_launch=sor_bytes, ms_per_launch=t_sor,
-                         share_of_vcycle=sor_share),
+        "roofline": roof_sor,
+        "roofline_colour_pass": roof_pass,
         "roofline_apply": dict(roof_apply, kernel=f"{kname}, 15-point apply", schedule=sched,
                                algorithmic_bytes_per_launch=apply_bytes, ms_per_launch=t_apply),
         "vcycle_ms_events": t_vc,

@@ -154,13 +154,26 @@ class PrimitiveStorage:
         -1 / 0 = level default."""
         _lib.call("hymi_mesh_set_schedule", self.h, level, kind, rows, waves, block)
 
-    def use_column_schedules(self, maxLevel, minLevel=5):
+    def use_column_schedules(self, maxLevel, minLevel=5, fusedSorFrom=7):
         """Select the column march (kind 3) with its measured best parameters on levels minLevel..maxLevel:
         16 slabs per unit and 4 waves per workgroup from level 7 up, 8 and 4 on level 6, 4 and 2 below;
-        loads one slab ahead."""
+        loads one slab ahead.  From level fusedSorFrom up, the SOR cell phase runs as one fused launch
+        (set_cell_sor(level, 24)), faster than four colour passes there; below it the passes are faster."""
         for L in range(minLevel, maxLevel + 1):
             rows, waves = (16, 4) if L >= 7 else (8, 4) if L == 6 else (4, 2)
             self.set_schedule(L, rows, waves, 1, 3)
+            if L >= max(fusedSorFrom, 3):
+                self.set_cell_sor(L, 24)
+
+    def set_cell_sor(self, level, tile):
+        """SOR cell phase on `level`: tile 0 = four colour passes, 24 = all four colours in one launch over
+        24 x 24 tiles marched along z (FP64 functions with one right-hand side; same values bit for bit)."""
+        _lib.call("hymi_mesh_set_cell_sor", self.h, level, int(tile))
+
+    def get_cell_sor(self, level):
+        v = ctypes.c_int()
+        _lib.call("hymi_mesh_get_cell_sor", self.h, level, ctypes.byref(v))
+        return v.value
 
     def get_schedule(self, level):
         out = np.zeros(4, np.int32)

@@ -624,6 +624,22 @@ int hymi_mesh_set_schedule( hymi_mesh mesh, int level, int kind, int rows, int w
    } );
 }
 
+int hymi_mesh_set_cell_sor( hymi_mesh mesh, int level, int tile )
+{
+   return guarded( [&] {
+      require( mesh && level >= 0 && level <= 12, "bad argument" );
+      mesh->m.setCellSor( level, tile );
+   } );
+}
+
+int hymi_mesh_get_cell_sor( hymi_mesh mesh, int level, int* tile )
+{
+   return guarded( [&] {
+      require( mesh && tile && level >= 0 && level <= 12, "bad argument" );
+      *tile = mesh->m.plan( level ).sorTile;
+   } );
+}
+
 int hymi_mesh_get_schedule( hymi_mesh mesh, int level, int out[4] )
 {
    return guarded( [&] {
@@ -1358,7 +1374,7 @@ int hymi_time_kernel( hymi_operator op, hymi_gmg gmg, hymi_function x, hymi_func
    return guarded( [&] {
       require( op && x && ms && reps > 0, "bad argument" );
       checkOpFunc( op, x, level );
-      if ( which == 1 || which == 3 || which == 4 ) checkOpFunc( op, b, level );
+      if ( which == 1 || which == 3 || which == 4 || which == 6 || which == 7 ) checkOpFunc( op, b, level );
       if ( which == 0 || which == 2 || which == 5 ) checkOpFunc( op, y, level );
       if ( which == 4 ) checkGmg( gmg, x, b, level );
       Operator&   A = op->op;
@@ -1372,6 +1388,8 @@ int hymi_time_kernel( hymi_operator op, hymi_gmg gmg, hymi_function x, hymi_func
          case 2: launchApply( A, x->f, y->f, nullptr, level, flag, kReplace ); break;
          case 3: launchSor( A, x->f, b->f, 1.0, level, flag ); break;
          case 4: gmg->cycle( x->f, b->f, level ); break;
+         case 6: launchCellSorPhase( A, x->f, b->f, 1.0, level, flag, false ); break;
+         case 7: launchCellSorPhase( A, x->f, b->f, 1.0, level, flag, true ); break;
          case 5:
          {
             // plain copy y = x of the whole level (closures of all primitives): a bandwidth reference

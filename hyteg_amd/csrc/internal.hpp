@@ -132,6 +132,15 @@ struct LevelPlan
    // per-work-unit partial sums of the split refinement residual (never used inside a captured graph)
    DevBuf< double > normPartials;
 
+   // fused SOR cell phase (sorTile = 0: four colour passes): tile side, work units (cell, last slab with an
+   // interior point of the tile, x0, y0) and the scratch array of the tiles' band positions (level-sized, FP64
+   // so that either precision fits)
+   int                                    sorTile = 0;
+   std::unique_ptr< DevBuf< MarchUnit > > sorUnits;
+   int64_t                                sorUnitCount = 0;
+   DevBuf< double >                       sorBand;
+   DevBuf< int32_t >                      sorSlabStart; // index of (0, 0, s) in a macro-cell, s = 0 .. w
+
    MarchSchedule                          sched;
    // closure ranges of primitives matching a flag, keyed by flag
    std::map< int, std::unique_ptr< DevBuf< Range > > > ranges;
@@ -158,6 +167,9 @@ struct Mesh
    // replaces the launch schedule of the row-marching kernels on one level (zeros = level default);
    // bumps scheduleEpoch so that captured graphs holding the old work-unit table are captured again
    void     setSchedule( int level, const MarchSchedule& S );
+   // selects the fused SOR cell phase with square tiles of side `tile` (24) on one level, or the four
+   // colour passes (tile 0); bumps scheduleEpoch like setSchedule
+   void setCellSor( int level, int tile );
    uint64_t scheduleEpoch = 0;
    // affine frames of the primitives, level-independent: micro-vertex (x, y, z) of a primitive lies at
    // o + (x a + y b + z c) / (w - 1); 12 doubles (o, a, b, c) per primitive, vertices, edges, faces, cells

@@ -80,6 +80,16 @@ __device__ __forceinline__ int64_t xcdBlock( int64_t nLogical )
    return ( b % 8 ) * per + b / 8;
 }
 int64_t xcdGrid( int64_t nLogical ) { return ( ( nLogical + 7 ) / 8 ) * 8; }
+// calls f(std::integral_constant<int, I>) for I = B .. E-1 (compile-time loop index)
+template < int B, int E, typename F >
+__device__ __forceinline__ void static_for( F&& f )
+{
+   if constexpr ( B < E )
+   {
+      f( std::integral_constant< int, B >{} );
+      static_for< B + 1, E >( f );
+   }
+}
 
 enum
 {
@@ -318,6 +328,15 @@ struct CellBuf
          return __builtin_bit_cast( T, __builtin_amdgcn_raw_buffer_load_b64( r, idx * 8, 0, 0 ) );
       else
          return __builtin_bit_cast( T, __builtin_amdgcn_raw_buffer_load_b32( r, idx * 4, 0, 0 ) );
+   }
+   // value at idx if keep, else 0 without a branch (a dropped lane reads at a byte offset past the range)
+   __device__ __forceinline__ T load( int idx, bool keep ) const
+   {
+      const int off = keep ? idx * (int) sizeof( T ) : (int) 0x80000000u;
+      if constexpr ( sizeof( T ) == 8 )
+         return __builtin_bit_cast( T, __builtin_amdgcn_raw_buffer_load_b64( r, off, 0, 0 ) );
+      else
+         return __builtin_bit_cast( T, __builtin_amdgcn_raw_buffer_load_b32( r, off, 0, 0 ) );
    }
    // store v at idx if keep, else nothing: a dropped lane's byte offset 0x80000000 lies past the range, and
    // the hardware discards stores outside it, so the store needs no branch
@@ -743,6 +762,267 @@ __global__ __launch_bounds__( 256 ) void kCellSorSmall( LV g, const int* __restr
       }
       __syncthreads(); // the next colour reads this colour's values
    }
+}
+
+// The whole SOR cell phase of a level in one launch, for levels above the one-workgroup-per-cell phase.
+//
+// Colour c = (x + 2y + 3z) mod 4.  Every neighbour of a point differs from it in colour and lies in slabs
+// z-1 .. z+1, so if the k-th colour of the sweep (k = c forward, 3 - c backward) is updated on slab s at step
+// t = s + 2k, each update of step t reads neighbours of earlier colours updated at earlier steps and of later
+// colours not yet updated, and no update of a step reads a point written in the same step.  A workgroup owns
+// an xy tile of TS x TS positions (x0 .. x0+TS-1, y0 .. y0+TS-1) and marches it up through all slabs one step
+// at a time, with one barrier per step: in step t each position of the tile and its halo does the one update
+// whose slab t - 2k carries its colour k, so all lanes have work.  The values an update of colour k needs
+// reach 3 - k rings beyond the tile, so colour k is recomputed on the tile widened by 3 - k rings and the tile
+// widened by kSorTileHalo = 4 rings is loaded; these recomputed halo values equal the owning tile's, because
+// they are computed with the same expression from the same inputs.  Slabs t-7 .. t+2 are in LDS (a ring of
+// kSorTileRing slots); slab t+3 is loaded into registers during step t and stored into LDS at the end of
+// step t+1, the right-hand side of step t+1 is loaded during step t.  Slab t-7 is final after step t-1 and
+// is written back in step t.
+//
+// Workgroups are not synchronised with each other, so a tile may load its halo after the neighbouring tile
+// has finished.  Positions within kSorTileHalo of a side that another tile's halo may cover (the band) are
+// therefore written to the scratch array `band` (same layout as u), and kCellSorBand copies them into u after
+// the launch; the other positions go to u directly, which no other tile reads.  The update expression is that
+// of the colour-pass kernels, so the result equals the four colour passes bit for bit.
+// Units: cell, z = last slab with an interior point in the tile, x0, y0 (both 1 + multiples of TS).
+__device__ __forceinline__ bool sorTileBand( int lx, int ly, int i, int j, int TS )
+{
+   return ( lx < kSorTileHalo && i > 0 ) || lx >= TS - kSorTileHalo || ( ly < kSorTileHalo && j > 0 ) ||
+          ly >= TS - kSorTileHalo;
+}
+
+// Lanes: a wave updates positions of one colour offset c = (3x + 2y) mod 4 (plus 2 backward), so at every step
+// its colour, slab and LDS slot are uniform and live in scalar registers: wave v takes class v mod 4, 8 rows of
+// 8 positions 4 apart.  LDS rows are padded to R + 1 values, which keeps those reads free of
+// bank conflicts.  Loads and stores between memory and LDS walk the region and the core row by row instead.
+constexpr int kSorTile = 24; // tile side: 32 x 32 region with the halo, 4 classes x 256 positions = 16 waves x 64
+template < typename T, int DP, int PAD >
+__global__ __launch_bounds__( 1024 ) void kCellSorTile( LV g, const MarchUnit* __restrict__ units, int64_t nUnits,
+                                                      const int* __restrict__ slabTab, const T* __restrict__ W8,
+                                                      const int* __restrict__ ctype, int flag, T* __restrict__ u,
+                                                      T* __restrict__ band, const T* __restrict__ b, T omega, int reverse )
+{
+   constexpr int TS = kSorTile, H = kSorTileHalo, R = TS + 2 * H, RP = R + PAD, RS = R * RP, NT = 1024;
+   constexpr int NL = R * R / NT, RING = kSorTileRing;
+   static_assert( R == 32 && NL == 1, "the lane mapping assumes a 32 x 32 region and 1024 threads" );
+   __shared__ T lds[RING * RS];
+   const int64_t ui = xcdBlock( nUnits );
+   if ( ui >= nUnits ) return;
+   const MarchUnit un = units[ui];
+   if ( !( ctype[un.cell] & flag ) ) return;
+   const int     w = g.w, t0 = tet32( w ), X0 = un.x0, Y0 = un.y0, smax = un.z;
+   const int64_t base = g.offC + (int64_t) un.cell * g.cS;
+   // buffer views of the cell's arrays: loads and stores of dropped lanes need no branch, so the waits for
+   // loads issued steps earlier can be counted exactly
+   const CellBuf< T > cu( u + base, t0 ), cband( band + base, t0 ), cb( b + base, t0 );
+   const T*           c = W8 + (int64_t) un.cell * 15;
+   const T cE = c[dE], cW = c[dW], cN = c[dN], cS = c[dS], cNW = c[dNW], cSE = c[dSE];
+   const T cTC = c[dTC], cTW = c[dTW], cTS = c[dTS], cTSE = c[dTSE], cBC = c[dBC], cBN = c[dBN], cBE = c[dBE],
+           cBNW = c[dBNW];
+   const T   relax = omega / c[dC];
+   const int tid = (int) threadIdx.x, lane = tid & 63;
+   const int wave = __builtin_amdgcn_readfirstlane( tid >> 6 ), cls = wave & 3, sh = reverse ? 2 : 0;
+   const int x0r = X0 - H, y0r = Y0 - H;
+   // index of (x, y, s) in the cell = slabStart(s) + y (w + 1 - s) + x - y (y + 1) / 2; slabStart from the
+   // level's table (entries 0 .. w, a scalar load for a uniform s)
+   auto slabStart = [&]( int s ) { return slabTab[min( max( s, 0 ), w )]; };
+
+   // row-major region positions (memory <-> LDS): LDS place, highest slab containing the point (-1: none) and
+   // the slab-independent parts of its cell index
+   int lq[NL], lsx[NL], ly_[NL], lbo[NL];
+#pragma unroll
+   for ( int j = 0; j < NL; j++ )
+   {
+      const int q = tid + j * NT, lx = q % R, ly = q / R, x = x0r + lx, y = y0r + ly;
+      lq[j]       = ly * RP + lx;
+      lsx[j]      = ( x >= 0 && y >= 0 ) ? w - 1 - x - y : -1;
+      ly_[j]      = y;
+      lbo[j]      = x - ( ( y * ( y + 1 ) ) >> 1 );
+   }
+   // update positions of colour class cls: LDS place, largest sweep position k whose ring holds the point,
+   // highest slab in which it is interior (-1: none), slab-independent index parts
+   int uq[NL], kmax[NL], smx[NL], uy[NL], ubo[NL];
+#pragma unroll
+   for ( int j = 0; j < NL; j++ )
+   {
+      const int m = ( wave >> 2 ) * 64 + j * 64 + lane, ly = m >> 3, y = y0r + ly;
+      const int lx = 4 * ( m & 7 ) + ( ( 3 * ( cls - sh - 3 * x0r - 2 * y ) ) & 3 ), x = x0r + lx;
+      uq[j]        = ly * RP + lx;
+      kmax[j]      = min( min( lx - 1, R - 2 - lx ), min( ly - 1, R - 2 - ly ) );
+      smx[j]       = ( x >= 1 && y >= 1 ) ? w - 2 - x - y : -1;
+      uy[j]        = y;
+      ubo[j]       = x - ( ( y * ( y + 1 ) ) >> 1 );
+   }
+   // the class's sweep position k at step t (uniform over the wave; t mod 4 is known at compile time where the
+   // steps are unrolled, so k comes from this per-wave table)
+   int kTab[4];
+#pragma unroll
+   for ( int r = 0; r < 4; r++ )
+   {
+      const int cc = ( cls + r ) & 3;
+      kTab[r]      = reverse ? 3 - cc : cc;
+   }
+   static_assert( ( DP + 1 ) % 4 == 0, "steps are unrolled in groups of a multiple of 4: the step's k is kTab[t mod 4]" );
+   auto loadSlab = [&]( int s, T( &v )[NL] ) {
+      const int ss = slabStart( s ), sy = w + 1 - s;
+#pragma unroll
+      for ( int j = 0; j < NL; j++ )
+      {
+         const bool ok = s <= lsx[j];
+         v[j]          = cu.load( ok ? ss + ly_[j] * sy + lbo[j] : 0, ok );
+      }
+   };
+   auto storeSlab = [&]( int slot, const T( &v )[NL] ) {
+#pragma unroll
+      for ( int j = 0; j < NL; j++ )
+         lds[slot * RS + lq[j]] = v[j];
+   };
+   auto loadRhs = [&]( int t, int k, T( &v )[NL] ) {
+      const int s = t - 2 * k, ss = s >= 1 ? slabStart( s ) : 0, sy = w + 1 - s;
+#pragma unroll
+      for ( int j = 0; j < NL; j++ )
+      {
+         const bool ok = s >= 1 && k <= kmax[j] && s <= smx[j];
+         v[j]          = cb.load( ok ? ss + uy[j] * sy + ubo[j] : 0, ok );
+      }
+   };
+   // the core positions of this thread (row-major), whether they belong to the band, LDS places, index parts
+   constexpr int NC = ( TS * TS + NT - 1 ) / NT;
+   int           ox[NC], oy[NC], oq[NC], obo[NC];
+   bool          inBand[NC];
+#pragma unroll
+   for ( int jc = 0; jc < NC; jc++ )
+   {
+      const int qc = tid + jc * NT, cx = qc % TS, cy = qc < TS * TS ? qc / TS : 0;
+      ox[jc]       = qc < TS * TS ? X0 + cx : 1 << 20; // 1 << 20: never written back
+      oy[jc]       = Y0 + cy;
+      oq[jc]       = ( cy + H ) * RP + cx + H;
+      obo[jc]      = ox[jc] - ( ( oy[jc] * ( oy[jc] + 1 ) ) >> 1 );
+      inBand[jc]   = sorTileBand( cx, cy, ( X0 - 1 ) / TS, ( Y0 - 1 ) / TS, TS );
+   }
+
+   // Step t, with tm = t mod RING (slab s lives in slot s mod RING).  Loads run DP steps ahead: slab s is loaded
+   // into xbuf[s mod (DP+1)] in step s - 2 - DP and stored into LDS at the end of step s - 2; the right-hand
+   // side of step t is loaded into bbuf[t mod (DP+1)] in step t - DP.  P = t mod (DP+1), known at compile time.
+   T xbuf[DP + 1][NL], bbuf[DP + 1][NL];
+   auto step = [&]( int t, int tm, auto Pc ) {
+      constexpr int P = decltype( Pc )::value;
+      loadSlab( t + 2 + DP, xbuf[( P + 2 + DP ) % ( DP + 1 )] );
+      loadRhs( t + DP, kTab[( P + DP ) & 3], bbuf[( P + DP ) % ( DP + 1 )] );
+      const int sw     = t - 7; // final since step t - 1
+      const int swSlot = tm + 3 >= RING ? tm + 3 - RING : tm + 3;
+      const int swBase = sw >= 1 ? slabStart( sw ) : 0;
+#pragma unroll
+      for ( int jc = 0; jc < NC; jc++ )
+      {
+         const bool wb = sw >= 1 && sw <= smax && ox[jc] + oy[jc] + sw <= w - 2;
+         const T    v  = lds[swSlot * RS + ( wb ? oq[jc] : 0 )];
+         const int  o  = wb ? swBase + oy[jc] * ( w + 1 - sw ) + obo[jc] : 0;
+         cband.store( o, v, wb && inBand[jc] );
+         cu.store( o, v, wb && !inBand[jc] );
+      }
+      // the wave's update: sweep position k on slab s, slots of s and its neighbours (uniform)
+      const int k = kTab[P & 3], s = t - 2 * k;
+      int       sl = tm - 2 * k;
+      sl += sl < 0 ? RING : 0;
+      const int slT = sl == RING - 1 ? 0 : sl + 1, slB = sl == 0 ? RING - 1 : sl - 1;
+#pragma unroll
+      for ( int j = 0; j < NL; j++ )
+      {
+         if ( !( s >= 1 && k <= kmax[j] && s <= smx[j] ) ) continue;
+         const T* Zs = lds + sl * RS + uq[j];
+         const T* Ts = lds + slT * RS + uq[j];
+         const T* Bs = lds + slB * RS + uq[j];
+         const T  Z0 = Zs[0], Z0l = Zs[-1], Z0r = Zs[1], Zp = Zs[RP], Zpl = Zs[RP - 1], Zm = Zs[-RP], Zmr = Zs[1 - RP];
+         const T  T0 = Ts[0], T0l = Ts[-1], Tm = Ts[-RP], Tmr = Ts[1 - RP];
+         const T  B0 = Bs[0], B0r = Bs[1], Bp = Bs[RP], Bpl = Bs[RP - 1];
+         T        acc = cW * Z0l + cE * Z0r;
+         acc += cN * Zp + cNW * Zpl;
+         acc += cS * Zm + cSE * Zmr;
+         acc += cTC * T0 + cTW * T0l;
+         acc += cTS * Tm + cTSE * Tmr;
+         acc += cBC * B0 + cBE * B0r;
+         acc += cBN * Bp + cBNW * Bpl;
+         lds[sl * RS + uq[j]] = ( T( 1 ) - omega ) * Z0 + relax * ( bbuf[P][j] - acc );
+      }
+      storeSlab( tm + 2 >= RING ? tm + 2 - RING : tm + 2, xbuf[( P + 2 ) % ( DP + 1 )] );
+      __syncthreads();
+   };
+
+   for ( int s = 0; s < 3; s++ )
+   {
+      loadSlab( s, xbuf[0] );
+      storeSlab( s, xbuf[0] );
+   }
+   // slabs 3 .. 2 + DP and the right-hand sides of steps 1 .. DP
+#pragma unroll
+   for ( int d = 0; d < DP; d++ )
+   {
+      loadSlab( 3 + d, xbuf[( 3 + d ) % ( DP + 1 )] );
+      loadRhs( 1 + d, kTab[( 1 + d ) & 3], bbuf[( 1 + d ) % ( DP + 1 )] );
+   }
+   __syncthreads();
+   // steps 1 .. smax + 7 (the last writes back slab smax), in groups of DP + 1: a step past smax + 7 updates
+   // only slabs above smax + 1, which are never written back, so a group needs no condition
+   const int tEnd = smax + 7;
+   int       tm   = 1;
+   for ( int t = 1; t <= tEnd; t += DP + 1 )
+   {
+      static_for< 0, DP + 1 >( [&]( auto pc ) {
+         constexpr int p = decltype( pc )::value;
+         step( t + p, tm, std::integral_constant< int, ( 1 + p ) % ( DP + 1 ) >{} );
+         tm = tm + 1 == RING ? 0 : tm + 1;
+      } );
+   }
+}
+
+// Copies the band positions of the fused SOR sweep from the scratch array into u.  Block = (cell, slab), 16
+// waves; wave v takes rows v + 1, v + 17, ... in batches of kSorBandBatch rows, lanes along x.  All loads of a
+// batch are issued before its stores and need no branch, so a wave waits for memory about once per batch.
+constexpr int kSorBandBatch = 8;
+template < typename T, int TS >
+__global__ __launch_bounds__( 1024 ) void kCellSorBand( LV g, int nCells, const int* __restrict__ ctype, int flag,
+                                                       T* __restrict__ u, const T* __restrict__ band )
+{
+   constexpr int XC = 2, RB = kSorBandBatch, NW = 16; // x chunks of 64 per row and pass, rows per batch, waves
+   const int     nz = g.w - 4, nBlk = nCells * nz;
+   const int     blk = __builtin_amdgcn_readfirstlane( (int) xcdBlock( nBlk ) );
+   if ( blk >= nBlk ) return;
+   const int cell = blk / nz, z = 1 + blk - cell * nz;
+   if ( !( ctype[cell] & flag ) ) return;
+   const int          w = g.w, last = w - 3 - z, t0 = tet32( w ); // rows 1 .. last
+   const int64_t      base = g.offC + (int64_t) cell * g.cS;
+   const CellBuf< T > cu( u + base, t0 ), cbd( band + base, t0 );
+   const int          lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane( (int) ( threadIdx.x >> 6 ) );
+   const int          ss = t0 - tet32( w - z ), sy = w + 1 - z;
+   for ( int yb = 1 + wave; yb <= last; yb += NW * RB )
+      for ( int xb = 1; xb + yb + z <= w - 2; xb += 64 * XC )
+      {
+         T    v[RB][XC];
+         int  o[RB][XC];
+         bool in[RB][XC];
+#pragma unroll
+         for ( int c = 0; c < XC; c++ )
+         {
+            const int  xm = xb - 1 + 64 * c + lane, lx = xm % TS;
+            const bool colB = ( lx < kSorTileHalo && xm >= TS ) || lx >= TS - kSorTileHalo;
+#pragma unroll
+            for ( int r = 0; r < RB; r++ )
+            {
+               const int  y = yb + NW * r, ym = y - 1, ly = ym % TS, x = xm + 1;
+               const bool rowB = ( ly < kSorTileHalo && ym >= TS ) || ly >= TS - kSorTileHalo;
+               in[r][c]        = y <= last && x + y + z <= w - 2 && ( rowB || colB );
+               o[r][c]         = in[r][c] ? ss + y * sy - ( ( y * ( y + 1 ) ) >> 1 ) + x : 0;
+               v[r][c]         = cbd.load( o[r][c], in[r][c] );
+            }
+         }
+#pragma unroll
+         for ( int r = 0; r < RB; r++ )
+#pragma unroll
+            for ( int c = 0; c < XC; c++ )
+               cu.store( o[r][c], v[r][c], in[r][c] );
+      }
 }
 
 // Column march along z (schedule kind 3).  A wave owns a fixed set of 64 positions (x, y): the elements
@@ -2103,6 +2383,61 @@ void cellSorLaunch( Mesh& M, LevelPlan& P, const T* CW, int flag, T* u, const T*
                        nseg, CW, P.dCellType.p, flag, u, b, omega, colour );
 }
 
+// all four colours of the SOR cell phase in one launch (kCellSorTile), then (withBand) the band copy
+template < typename T >
+void cellSorFused( Mesh& M, LevelPlan& P, const T* CW, int flag, T* u, const T* b, T omega, int reverse,
+                   bool withBand = true )
+{
+   if ( P.sorUnitCount == 0 ) return;
+   hipStream_t s    = M.ctx->stream;
+   LV          g    = viewOf( P );
+   T*          band = reinterpret_cast< T* >( P.sorBand.p );
+   const int   nC   = M.T.nC();
+   const dim3  grid( (unsigned) xcdGrid( P.sorUnitCount ) ),
+       bandGrid( (unsigned) xcdGrid( (int64_t) nC * ( P.w - 4 ) ) );
+   const MarchUnit* un = P.sorUnits->p;
+   const int64_t    n  = P.sorUnitCount;
+   switch ( P.sorTile )
+   {
+   case kSorTile:
+      hipLaunchKernelGGL( ( kCellSorTile< T, 3, 1 > ), grid, dim3( 1024 ), 0, s, g, un, n, P.sorSlabStart.p, CW, P.dCellType.p,
+                          flag, u, band, b,
+                          omega, reverse );
+      if ( withBand )
+         hipLaunchKernelGGL( ( kCellSorBand< T, kSorTile > ), bandGrid, dim3( 1024 ), 0, s, g, nC, P.dCellType.p, flag, u,
+                             band );
+      break;
+   default: throw std::logic_error( "bad cell SOR tile" );
+   }
+}
+
+// The SOR cell phase of one level: the fused launch where it is selected (FP64, one right-hand side), the
+// one-workgroup-per-cell launch on small levels, else the four colour passes; backward = colours in reverse.
+// FP32 keeps the colour passes: the fused march is bound by its per-slab step latency, not by bytes, so it gains
+// nothing from FP32's halved traffic, and the passes are faster in FP32 (level 7: 151 us against 226 us).
+template < typename T, int NR >
+void cellSorPhase( Mesh& M, LevelPlan& P, const T* CW, int flag, T* u, const T* b, T om, bool backward )
+{
+   const int rev = backward ? 1 : 0;
+   if constexpr ( NR == 1 && std::is_same< T, double >::value )
+      if ( P.sorTile > 0 )
+      {
+         cellSorFused< T >( M, P, CW, flag, u, b, om, rev );
+         return;
+      }
+   if ( P.L <= kCellSmallMaxLevel && P.colourStart[4] > 0 )
+   {
+      ColourStarts cs;
+      for ( int k = 0; k < 5; k++ )
+         cs.s[k] = P.colourStart[k];
+      hipLaunchKernelGGL( ( kCellSorSmall< T > ), dim3( M.T.nC() ), dim3( 256 ), 0, M.ctx->stream, viewOf( P ),
+                          P.dColourPts.p, cs, CW, P.dCellType.p, flag, u, b, om, rev );
+   }
+   else
+      for ( int cc = 0; cc < 4; cc++ )
+         cellSorLaunch< T, NR >( M, P, CW, flag, u, b, om, backward ? 3 - cc : cc );
+}
+
 template < typename T, int NR >
 void refreshT( Function& f, int level, int mask )
 {
@@ -2216,19 +2551,7 @@ void sorT( Operator& op, Function& xf, const Function& bf, double omega, int lev
                                    (const T*) gh );
          }
       };
-      auto cellPhase = [&]() {
-         if ( level <= kCellSmallMaxLevel && P.colourStart[4] > 0 )
-         {
-            ColourStarts cs;
-            for ( int k = 0; k < 5; k++ )
-               cs.s[k] = P.colourStart[k];
-            hipLaunchKernelGGL( ( kCellSorSmall< T > ), dim3( M.T.nC() ), dim3( 256 ), 0, s, g, P.dColourPts.p, cs,
-                                wsel< T >( Wt.cell64, Wt.cell32 ), P.dCellType.p, flag, u, b, om, rev );
-         }
-         else
-            for ( int cc = 0; cc < 4; cc++ )
-               cellSorLaunch< T, NR >( M, P, wsel< T >( Wt.cell64, Wt.cell32 ), flag, u, b, om, backward ? 3 - cc : cc );
-      };
+      auto cellPhase = [&]() { cellSorPhase< T, NR >( M, P, wsel< T >( Wt.cell64, Wt.cell32 ), flag, u, b, om, backward ); };
       if ( !backward )
       {
          vertexPhase();
@@ -2638,6 +2961,31 @@ struct CellApplyOnlyF
    }
 };
 template < typename T, int NR >
+struct CellSorPhaseF
+{
+   static void run( Operator& op, Function& x, const Function& b, double omega, int level, int flag, bool tileOnly )
+   {
+      Mesh&      M  = *op.mesh;
+      LevelPlan& P  = M.plan( level );
+      auto&      Wt = op.weights( level );
+      T*         u  = (T*) x.ptr( level );
+      const T*   bb = (const T*) b.ptr( level );
+      const T*   CW = wsel< T >( Wt.cell64, Wt.cell32 );
+      if ( !tileOnly )
+      {
+         cellSorPhase< T, NR >( M, P, CW, flag, u, bb, (T) omega, false );
+         return;
+      }
+      if constexpr ( NR == 1 && std::is_same< T, double >::value )
+      {
+         if ( P.sorTile == 0 ) throw std::invalid_argument( "no fused cell SOR phase on this level" );
+         cellSorFused< T >( M, P, CW, flag, u, bb, (T) omega, 0, false );
+      }
+      else
+         throw std::invalid_argument( "the fused cell SOR phase runs FP64 functions with one right-hand side" );
+   }
+};
+template < typename T, int NR >
 struct CellSorOnlyF
 {
    static void run( Operator& op, Function& x, const Function& b, double omega, int level, int flag, int colour )
@@ -2654,6 +3002,11 @@ struct CellSorOnlyF
 void launchCellApplyOnly( Operator& op, Function& src, Function& dst, int level, int flag )
 {
    dispatch< CellApplyOnlyF >( src.prec, src.nrhs, op, src, dst, level, flag );
+}
+
+void launchCellSorPhase( Operator& op, Function& x, const Function& b, double omega, int level, int flag, bool tileOnly )
+{
+   dispatch< CellSorPhaseF >( x.prec, x.nrhs, op, x, b, omega, level, flag, tileOnly );
 }
 
 void launchCellSorOnly( Operator& op, Function& x, const Function& b, double omega, int level, int flag, int colour )

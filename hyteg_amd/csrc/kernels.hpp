@@ -57,5 +57,7 @@ void launchInterpolateModes( Function& f, int level, int flag, int nModes, const
 // single macro-cell kernels (no refresh, no lower-dimensional primitives) for measurement
 void launchCellApplyOnly( Operator& op, Function& src, Function& dst, int level, int flag );
 void launchCellSorOnly( Operator& op, Function& x, const Function& b, double omega, int level, int flag, int colour );
+// the whole SOR cell phase of a level as configured (fused or colour passes), or only the fused tile kernel
+void launchCellSorPhase( Operator& op, Function& x, const Function& b, double omega, int level, int flag, bool tileOnly );
 
 } // namespace hymi

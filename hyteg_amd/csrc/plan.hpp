@@ -84,6 +84,10 @@ constexpr int     kDotBlocks     = 2048;
 constexpr int64_t kPartialsCap   = ( kDotBlocks + 64 ) * 32;
 // levels whose cell SOR phase runs as one workgroup per macro-cell for all four colours
 constexpr int kCellSmallMaxLevel = 5;
+// fused macro-cell SOR sweep (one launch for the four colours): square xy tiles of 24 points per side, a halo of kSorTileHalo rings loaded around each tile and recomputed, kSorTileRing slab slots in LDS
+constexpr int kSorTileHalo = 4;
+constexpr int kSorTileRing = 10;
+inline bool sorTileSizeOk( int t ) { return t == 24; }
 
 struct EdgeGeom
 {

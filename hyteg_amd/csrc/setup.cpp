@@ -650,6 +650,49 @@ void Mesh::setSchedule( int level, const MarchSchedule& S )
    if ( !ctx->hostOnly ) marchUnits( P, &n );
 }
 
+void Mesh::setCellSor( int level, int tile )
+{
+   if ( tile != 0 && !sorTileSizeOk( tile ) ) throw std::invalid_argument( "cell SOR tile must be 0 or 24" );
+   if ( tile != 0 && level < 3 ) throw std::invalid_argument( "the fused cell SOR phase needs level >= 3" );
+   LevelPlan& P = plan( level );
+   // captured graphs hold the old unit table and scratch array: they are captured again on their next replay,
+   // and no queued work may still use the buffers replaced here
+   scheduleEpoch++;
+   if ( !ctx->hostOnly ) HYMI_HIP( hipStreamSynchronize( ctx->stream ) );
+   P.sorTile = 0;
+   P.sorUnits.reset();
+   P.sorUnitCount = 0;
+   P.sorBand.release();
+   P.sorSlabStart.release();
+   if ( tile == 0 ) return;
+   std::vector< MarchUnit > v;
+   const int                w = P.w;
+   for ( int c = 0; c < T.nC(); c++ )
+      for ( int x0 = 1; x0 + 1 <= w - 3; x0 += tile )
+         for ( int y0 = 1; x0 + y0 <= w - 3; y0 += tile )
+            v.push_back( { c, w - 2 - x0 - y0, x0, y0 } );
+   // the launch gives XCD k the units [k per, (k + 1) per), which hold consecutive cells (a tile's neighbours
+   // mostly share its L2); within each range the longest tiles go first and the short ones fill in behind them
+   const size_t per = ( v.size() + 7 ) / 8;
+   for ( size_t a = 0; a < v.size(); a += per )
+      std::stable_sort( v.begin() + a, v.begin() + std::min( v.size(), a + per ),
+                        []( const MarchUnit& p, const MarchUnit& q ) { return p.z > q.z; } );
+   if ( !ctx->hostOnly )
+   {
+      P.sorUnits = std::make_unique< DevBuf< MarchUnit > >();
+      P.sorUnits->upload( v, ctx->stream );
+      P.sorBand.alloc( (size_t) P.total );
+      std::vector< int32_t > ss( (size_t) w + 1 );
+      auto                   tet = []( int64_t n ) { return n * ( n + 1 ) * ( n + 2 ) / 6; };
+      for ( int z = 0; z <= w; z++ )
+         ss[z] = (int32_t) ( tet( w ) - tet( w - z ) );
+      P.sorSlabStart.upload( ss, ctx->stream );
+      HYMI_HIP( hipStreamSynchronize( ctx->stream ) );
+   }
+   P.sorUnitCount = (int64_t) v.size();
+   P.sorTile      = tile;
+}
+
 const double* Mesh::frames()
 {
    if ( !dFrames.p )

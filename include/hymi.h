@@ -188,7 +188,8 @@ int hymi_solve_refined( hymi_gmg g, hymi_function x, hymi_function b, int level,
  * 99-115).  Average duration in ms of `reps` back-to-back launches on the context stream, bracketed by
  * hipEvents recorded on that stream.  which: 0 macro-cell apply kernel (x -> y), 1 macro-cell SOR colour
  * pass (x, b; colour 0), 2 full apply (x -> y), 3 full SOR sweep (x, b), 4 one V-cycle of gmg (x, b),
- * 5 plain copy y = x of the whole level (a bandwidth reference). */
+ * 5 plain copy y = x of the whole level (a bandwidth reference), 6 the SOR cell phase as configured for the level
+ * (hymi_mesh_set_cell_sor; x, b), 7 the fused cell phase's tile kernel alone (x, b). */
 int hymi_time_kernel( hymi_operator op, hymi_gmg gmg, hymi_function x, hymi_function b, hymi_function y, int level,
                       int which, int reps, double* ms_per_launch );
 
@@ -205,6 +206,12 @@ int hymi_time_kernel( hymi_operator op, hymi_gmg gmg, hymi_function x, hymi_func
  * next replay.  out = {rows, waves, block, kind}. */
 int hymi_mesh_set_schedule( hymi_mesh mesh, int level, int kind, int rows, int waves, int block );
 int hymi_mesh_get_schedule( hymi_mesh mesh, int level, int out[4] );
+/* SOR cell phase of one level: tile = 0 runs the four colours as four passes; tile = 24 runs all four in
+ * one launch over 24 x 24 xy tiles marched along z (levels >= 3; FP64 functions with one right-hand side, others
+ * keep the passes; same values bit for bit).  No reference counterpart: the reference's smooth_sor runs the
+ * macro-cell interior point by point (src/hyteg/p1functionspace/P1Operator.hpp:341-420) */
+int hymi_mesh_set_cell_sor( hymi_mesh mesh, int level, int tile );
+int hymi_mesh_get_cell_sor( hymi_mesh mesh, int level, int* tile );
 
 #ifdef __cplusplus
 }

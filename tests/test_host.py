@@ -153,3 +153,16 @@ def test_serving_counts_and_stream(H, hctx):
     assert abs(gaps.mean() - 1 / 200.0) < 0.1 / 200.0          # exponential inter-arrival, mean 1/rate
     assert {r.n for r in a} == {1, 2, 3} and {r.level for r in a} == {4, 5, 6, 7}
     assert all(r.modes.shape == (4, 4) and 1 <= r.modes[:, 1:].min() and r.modes[:, 1:].max() <= 8 for r in a)
+
+
+def test_cell_sor_setting(H, hctx):
+    S = H.PrimitiveStorage(H.MeshInfo.meshCuboid((0, 0, 0), (1, 1, 1), 1, 1, 1), hctx)
+    assert S.get_cell_sor(6) == 0
+    S.set_cell_sor(6, 24)
+    S.set_cell_sor(3, 24)
+    assert (S.get_cell_sor(6), S.get_cell_sor(3), S.get_cell_sor(7)) == (24, 24, 0)
+    S.set_cell_sor(6, 0)
+    assert S.get_cell_sor(6) == 0
+    for level, tile in ((6, 8), (6, 16), (6, 32), (2, 24)):
+        with pytest.raises(H.HymiError):
+            S.set_cell_sor(level, tile)
