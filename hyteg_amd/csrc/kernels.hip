@@ -977,52 +977,55 @@ __global__ __launch_bounds__( 1024 ) void kCellSorTile( LV g, const MarchUnit* _
    }
 }
 
-// Copies the band positions of the fused SOR sweep from the scratch array into u.  Block = (cell, slab), 16
-// waves; wave v takes rows v + 1, v + 17, ... in batches of kSorBandBatch rows, lanes along x.  All loads of a
-// batch are issued before its stores and need no branch, so a wave waits for memory about once per batch.
+// Copies the band positions of the fused SOR sweep from the scratch array into u.  Block = (cell, slab, chunk
+// of kSorBandRows rows), 4 waves; wave v takes rows v + 1, v + 5, ... of the chunk, lanes along x.  All loads of
+// a wave are issued before its stores and need no branch; small blocks keep several resident per CU, so the
+// copy streams instead of waiting for one block's memory round trip at a time.
 constexpr int kSorBandBatch = 8;
+constexpr int kSorBandRows  = 4 * kSorBandBatch;
 template < typename T, int TS >
-__global__ __launch_bounds__( 1024 ) void kCellSorBand( LV g, int nCells, const int* __restrict__ ctype, int flag,
-                                                       T* __restrict__ u, const T* __restrict__ band )
+__global__ __launch_bounds__( 256 ) void kCellSorBand( LV g, int nCells, const int* __restrict__ ctype, int flag,
+                                                      T* __restrict__ u, const T* __restrict__ band )
 {
-   constexpr int XC = 2, RB = kSorBandBatch, NW = 16; // x chunks of 64 per row and pass, rows per batch, waves
-   const int     nz = g.w - 4, nBlk = nCells * nz;
+   constexpr int XC = 2, RB = kSorBandBatch, NW = 4; // x chunks of 64 per row and pass, rows per wave, waves
+   const int     nz = g.w - 4, nyc = ( g.w - 4 + kSorBandRows - 1 ) / kSorBandRows, nBlk = nCells * nz * nyc;
    const int     blk = __builtin_amdgcn_readfirstlane( (int) xcdBlock( nBlk ) );
    if ( blk >= nBlk ) return;
-   const int cell = blk / nz, z = 1 + blk - cell * nz;
-   if ( !( ctype[cell] & flag ) ) return;
-   const int          w = g.w, last = w - 3 - z, t0 = tet32( w ); // rows 1 .. last
+   const int cell = blk / ( nz * nyc ), rem = blk - cell * ( nz * nyc ), z = 1 + rem / nyc;
+   const int yc = rem - ( z - 1 ) * nyc;
+   const int w = g.w, last = w - 3 - z; // rows 1 .. last
+   if ( 1 + yc * kSorBandRows > last || !( ctype[cell] & flag ) ) return;
+   const int          t0   = tet32( w );
    const int64_t      base = g.offC + (int64_t) cell * g.cS;
    const CellBuf< T > cu( u + base, t0 ), cbd( band + base, t0 );
    const int          lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane( (int) ( threadIdx.x >> 6 ) );
-   const int          ss = t0 - tet32( w - z ), sy = w + 1 - z;
-   for ( int yb = 1 + wave; yb <= last; yb += NW * RB )
-      for ( int xb = 1; xb + yb + z <= w - 2; xb += 64 * XC )
+   const int          ss = t0 - tet32( w - z ), sy = w + 1 - z, yb = 1 + yc * kSorBandRows + wave;
+   for ( int xb = 1; xb + yb + z <= w - 2; xb += 64 * XC )
+   {
+      T    v[RB][XC];
+      int  o[RB][XC];
+      bool in[RB][XC];
+#pragma unroll
+      for ( int c = 0; c < XC; c++ )
       {
-         T    v[RB][XC];
-         int  o[RB][XC];
-         bool in[RB][XC];
-#pragma unroll
-         for ( int c = 0; c < XC; c++ )
-         {
-            const int  xm = xb - 1 + 64 * c + lane, lx = xm % TS;
-            const bool colB = ( lx < kSorTileHalo && xm >= TS ) || lx >= TS - kSorTileHalo;
-#pragma unroll
-            for ( int r = 0; r < RB; r++ )
-            {
-               const int  y = yb + NW * r, ym = y - 1, ly = ym % TS, x = xm + 1;
-               const bool rowB = ( ly < kSorTileHalo && ym >= TS ) || ly >= TS - kSorTileHalo;
-               in[r][c]        = y <= last && x + y + z <= w - 2 && ( rowB || colB );
-               o[r][c]         = in[r][c] ? ss + y * sy - ( ( y * ( y + 1 ) ) >> 1 ) + x : 0;
-               v[r][c]         = cbd.load( o[r][c], in[r][c] );
-            }
-         }
+         const int  xm = xb - 1 + 64 * c + lane, lx = xm % TS;
+         const bool colB = ( lx < kSorTileHalo && xm >= TS ) || lx >= TS - kSorTileHalo;
 #pragma unroll
          for ( int r = 0; r < RB; r++ )
-#pragma unroll
-            for ( int c = 0; c < XC; c++ )
-               cu.store( o[r][c], v[r][c], in[r][c] );
+         {
+            const int  y = yb + NW * r, ym = y - 1, ly = ym % TS, x = xm + 1;
+            const bool rowB = ( ly < kSorTileHalo && ym >= TS ) || ly >= TS - kSorTileHalo;
+            in[r][c]        = y <= last && x + y + z <= w - 2 && ( rowB || colB );
+            o[r][c]         = in[r][c] ? ss + y * sy - ( ( y * ( y + 1 ) ) >> 1 ) + x : 0;
+            v[r][c]         = cbd.load( o[r][c], in[r][c] );
+         }
       }
+#pragma unroll
+      for ( int r = 0; r < RB; r++ )
+#pragma unroll
+         for ( int c = 0; c < XC; c++ )
+            cu.store( o[r][c], v[r][c], in[r][c] );
+   }
 }
 
 // Column march along z (schedule kind 3).  A wave owns a fixed set of 64 positions (x, y): the elements
@@ -2394,7 +2397,7 @@ void cellSorFused( Mesh& M, LevelPlan& P, const T* CW, int flag, T* u, const T* 
    T*          band = reinterpret_cast< T* >( P.sorBand.p );
    const int   nC   = M.T.nC();
    const dim3  grid( (unsigned) xcdGrid( P.sorUnitCount ) ),
-       bandGrid( (unsigned) xcdGrid( (int64_t) nC * ( P.w - 4 ) ) );
+       bandGrid( (unsigned) xcdGrid( (int64_t) nC * ( P.w - 4 ) * ( ( P.w - 4 + kSorBandRows - 1 ) / kSorBandRows ) ) );
    const MarchUnit* un = P.sorUnits->p;
    const int64_t    n  = P.sorUnitCount;
    switch ( P.sorTile )
@@ -2404,7 +2407,7 @@ void cellSorFused( Mesh& M, LevelPlan& P, const T* CW, int flag, T* u, const T* 
                           flag, u, band, b,
                           omega, reverse );
       if ( withBand )
-         hipLaunchKernelGGL( ( kCellSorBand< T, kSorTile > ), bandGrid, dim3( 1024 ), 0, s, g, nC, P.dCellType.p, flag, u,
+         hipLaunchKernelGGL( ( kCellSorBand< T, kSorTile > ), bandGrid, dim3( 256 ), 0, s, g, nC, P.dCellType.p, flag, u,
                              band );
       break;
    default: throw std::logic_error( "bad cell SOR tile" );
