@@ -368,10 +368,10 @@ struct hymi_gmg_s
          launchSorSweeps( *A, x, b, omega, L, flag, count, refreshFirst );
          return;
       }
-      for ( int k = 0; k < count; k++ )
+      for ( int k = 0; k < count; k++ ) // one smoothing sequence: each sweep is chained to the one before
       {
-         launchSor( *A, x, b, omega, L, flag, refreshFirst && k == 0, false );
-         launchSor( *A, x, b, omega, L, flag, false, true );
+         launchSor( *A, x, b, omega, L, flag, refreshFirst && k == 0, false, 2 * k );
+         launchSor( *A, x, b, omega, L, flag, false, true, 2 * k + 1 );
       }
    }
 };

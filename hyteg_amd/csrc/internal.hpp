@@ -138,7 +138,7 @@ struct LevelPlan
    int                                    sorTile = 0;
    std::unique_ptr< DevBuf< MarchUnit > > sorUnits;
    int64_t                                sorUnitCount = 0;
-   DevBuf< double >                       sorBand;
+   DevBuf< double >                       sorBand, sorBand2; // alternate between chained sweeps
    DevBuf< int32_t >                      sorSlabStart; // index of (0, 0, s) in a macro-cell, s = 0 .. w
 
    MarchSchedule                          sched;

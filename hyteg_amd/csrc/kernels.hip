@@ -783,7 +783,11 @@ __global__ __launch_bounds__( 256 ) void kCellSorSmall( LV g, const int* __restr
 // Workgroups are not synchronised with each other, so a tile may load its halo after the neighbouring tile
 // has finished.  Positions within kSorTileHalo of a side that another tile's halo may cover (the band) are
 // therefore written to the scratch array `band` (same layout as u), and kCellSorBand copies them into u after
-// the launch; the other positions go to u directly, which no other tile reads.  The update expression is that
+// the launch; the other positions go to u directly, which no other tile reads.  In a chained sweep (the next
+// sweep of the same smoothing sequence: nothing but lower-dimensional phases ran in between, and those never
+// write macro-cell interiors) the previous sweep's band array `snapIn` holds exactly the halo values a tile
+// needs, so the tile reads its halo from there, writes its band both into u and into the other band array,
+// and no copy is needed.  The update expression is that
 // of the colour-pass kernels, so the result equals the four colour passes bit for bit.
 // Units: cell, z = last slab with an interior point in the tile, x0, y0 (both 1 + multiples of TS).
 __device__ __forceinline__ bool sorTileBand( int lx, int ly, int i, int j, int TS )
@@ -797,11 +801,12 @@ __device__ __forceinline__ bool sorTileBand( int lx, int ly, int i, int j, int T
 // 8 positions 4 apart.  LDS rows are padded to R + 1 values, which keeps those reads free of
 // bank conflicts.  Loads and stores between memory and LDS walk the region and the core row by row instead.
 constexpr int kSorTile = 24; // tile side: 32 x 32 region with the halo, 4 classes x 256 positions = 16 waves x 64
-template < typename T, int DP, int PAD >
+template < typename T, int DP, int PAD, bool CHAINED >
 __global__ __launch_bounds__( 1024 ) void kCellSorTile( LV g, const MarchUnit* __restrict__ units, int64_t nUnits,
                                                       const int* __restrict__ slabTab, const T* __restrict__ W8,
                                                       const int* __restrict__ ctype, int flag, T* __restrict__ u,
-                                                      T* __restrict__ band, const T* __restrict__ b, T omega, int reverse )
+                                                      T* __restrict__ band, const T* __restrict__ snapIn,
+                                                      const T* __restrict__ b, T omega, int reverse )
 {
    constexpr int TS = kSorTile, H = kSorTileHalo, R = TS + 2 * H, RP = R + PAD, RS = R * RP, NT = 1024;
    constexpr int NL = R * R / NT, RING = kSorTileRing;
@@ -816,6 +821,9 @@ __global__ __launch_bounds__( 1024 ) void kCellSorTile( LV g, const MarchUnit* _
    // buffer views of the cell's arrays: loads and stores of dropped lanes need no branch, so the waits for
    // loads issued steps earlier can be counted exactly
    const CellBuf< T > cu( u + base, t0 ), cband( band + base, t0 ), cb( b + base, t0 );
+   // chained sweep: the band values of the previous sweep (snapIn), read for the halo
+   constexpr bool     chained = CHAINED;
+   const CellBuf< T > csnap( chained ? snapIn + base : b + base, chained ? t0 : 0 );
    const T*           c = W8 + (int64_t) un.cell * 15;
    const T cE = c[dE], cW = c[dW], cN = c[dN], cS = c[dS], cNW = c[dNW], cSE = c[dSE];
    const T cTC = c[dTC], cTW = c[dTW], cTS = c[dTS], cTSE = c[dTSE], cBC = c[dBC], cBN = c[dBN], cBE = c[dBE],
@@ -830,15 +838,18 @@ __global__ __launch_bounds__( 1024 ) void kCellSorTile( LV g, const MarchUnit* _
 
    // row-major region positions (memory <-> LDS): LDS place, highest slab containing the point (-1: none) and
    // the slab-independent parts of its cell index
-   int lq[NL], lsx[NL], ly_[NL], lbo[NL];
+   // and, in a chained sweep, the highest slab in which a halo point is interior (read from snapIn there)
+   int lq[NL], lsx[NL], ly_[NL], lbo[NL], lsnap[NL];
 #pragma unroll
    for ( int j = 0; j < NL; j++ )
    {
-      const int q = tid + j * NT, lx = q % R, ly = q / R, x = x0r + lx, y = y0r + ly;
-      lq[j]       = ly * RP + lx;
-      lsx[j]      = ( x >= 0 && y >= 0 ) ? w - 1 - x - y : -1;
-      ly_[j]      = y;
-      lbo[j]      = x - ( ( y * ( y + 1 ) ) >> 1 );
+      const int  q = tid + j * NT, lx = q % R, ly = q / R, x = x0r + lx, y = y0r + ly;
+      const bool core = lx >= H && lx < H + TS && ly >= H && ly < H + TS;
+      lq[j]           = ly * RP + lx;
+      lsx[j]          = ( x >= 0 && y >= 0 ) ? w - 1 - x - y : -1;
+      ly_[j]          = y;
+      lbo[j]          = x - ( ( y * ( y + 1 ) ) >> 1 );
+      lsnap[j]        = ( chained && !core && x >= 1 && y >= 1 ) ? w - 2 - x - y : -1;
    }
    // update positions of colour class cls: LDS place, largest sweep position k whose ring holds the point,
    // highest slab in which it is interior (-1: none), slab-independent index parts
@@ -869,8 +880,16 @@ __global__ __launch_bounds__( 1024 ) void kCellSorTile( LV g, const MarchUnit* _
 #pragma unroll
       for ( int j = 0; j < NL; j++ )
       {
-         const bool ok = s <= lsx[j];
-         v[j]          = cu.load( ok ? ss + ly_[j] * sy + lbo[j] : 0, ok );
+         const bool ok = s <= lsx[j], fromSnap = chained && s >= 1 && s <= lsnap[j];
+         const int  o  = ok ? ss + ly_[j] * sy + lbo[j] : 0;
+         const T    vu = cu.load( o, ok && !fromSnap );
+         if constexpr ( chained )
+         {
+            const T vs = csnap.load( o, fromSnap ); // both loads issued: no branch around either
+            v[j]       = fromSnap ? vs : vu;
+         }
+         else
+            v[j] = vu;
       }
    };
    auto storeSlab = [&]( int slot, const T( &v )[NL] ) {
@@ -920,7 +939,7 @@ __global__ __launch_bounds__( 1024 ) void kCellSorTile( LV g, const MarchUnit* _
          const T    v  = lds[swSlot * RS + ( wb ? oq[jc] : 0 )];
          const int  o  = wb ? swBase + oy[jc] * ( w + 1 - sw ) + obo[jc] : 0;
          cband.store( o, v, wb && inBand[jc] );
-         cu.store( o, v, wb && !inBand[jc] );
+         cu.store( o, v, wb && ( chained || !inBand[jc] ) );
       }
       // the wave's update: sweep position k on slab s, slots of s and its neighbours (uniform)
       const int k = kTab[P & 3], s = t - 2 * k;
@@ -2387,14 +2406,20 @@ void cellSorLaunch( Mesh& M, LevelPlan& P, const T* CW, int flag, T* u, const T*
 }
 
 // all four colours of the SOR cell phase in one launch (kCellSorTile), then (withBand) the band copy
+// chain: index of the sweep within its smoothing sequence (0 = first: halo from u, band copy after the
+// launch; k > 0: halo from the band array written by sweep k - 1, no copy); the two band arrays alternate
 template < typename T >
 void cellSorFused( Mesh& M, LevelPlan& P, const T* CW, int flag, T* u, const T* b, T omega, int reverse,
-                   bool withBand = true )
+                   bool withBand = true, int chain = 0 )
 {
    if ( P.sorUnitCount == 0 ) return;
-   hipStream_t s    = M.ctx->stream;
-   LV          g    = viewOf( P );
-   T*          band = reinterpret_cast< T* >( P.sorBand.p );
+   hipStream_t s      = M.ctx->stream;
+   LV          g      = viewOf( P );
+   T*          bandA  = reinterpret_cast< T* >( P.sorBand.p );
+   T*          bandB  = reinterpret_cast< T* >( P.sorBand2.p );
+   T*          band   = chain % 2 == 0 ? bandA : bandB;
+   const T*    snapIn = chain == 0 ? nullptr : ( chain % 2 == 0 ? bandB : bandA );
+   withBand           = withBand && chain == 0;
    const int   nC   = M.T.nC();
    const dim3  grid( (unsigned) xcdGrid( P.sorUnitCount ) ),
        bandGrid( (unsigned) xcdGrid( (int64_t) nC * ( P.w - 4 ) * ( ( P.w - 4 + kSorBandRows - 1 ) / kSorBandRows ) ) );
@@ -2403,9 +2428,12 @@ void cellSorFused( Mesh& M, LevelPlan& P, const T* CW, int flag, T* u, const T* 
    switch ( P.sorTile )
    {
    case kSorTile:
-      hipLaunchKernelGGL( ( kCellSorTile< T, 3, 1 > ), grid, dim3( 1024 ), 0, s, g, un, n, P.sorSlabStart.p, CW, P.dCellType.p,
-                          flag, u, band, b,
-                          omega, reverse );
+      if ( snapIn )
+         hipLaunchKernelGGL( ( kCellSorTile< T, 3, 1, true > ), grid, dim3( 1024 ), 0, s, g, un, n, P.sorSlabStart.p, CW,
+                             P.dCellType.p, flag, u, band, snapIn, b, omega, reverse );
+      else
+         hipLaunchKernelGGL( ( kCellSorTile< T, 3, 1, false > ), grid, dim3( 1024 ), 0, s, g, un, n, P.sorSlabStart.p, CW,
+                             P.dCellType.p, flag, u, band, snapIn, b, omega, reverse );
       if ( withBand )
          hipLaunchKernelGGL( ( kCellSorBand< T, kSorTile > ), bandGrid, dim3( 256 ), 0, s, g, nC, P.dCellType.p, flag, u,
                              band );
@@ -2415,17 +2443,18 @@ void cellSorFused( Mesh& M, LevelPlan& P, const T* CW, int flag, T* u, const T* 
 }
 
 // The SOR cell phase of one level: the fused launch where it is selected (FP64, one right-hand side), the
-// one-workgroup-per-cell launch on small levels, else the four colour passes; backward = colours in reverse.
+// one-workgroup-per-cell launch on small levels, else the four colour passes; backward = colours in reverse;
+// chain = index of the sweep within its smoothing sequence (see cellSorFused).
 // FP32 keeps the colour passes: the fused march is bound by its per-slab step latency, not by bytes, so it gains
 // nothing from FP32's halved traffic, and the passes are faster in FP32 (level 7: 151 us against 226 us).
 template < typename T, int NR >
-void cellSorPhase( Mesh& M, LevelPlan& P, const T* CW, int flag, T* u, const T* b, T om, bool backward )
+void cellSorPhase( Mesh& M, LevelPlan& P, const T* CW, int flag, T* u, const T* b, T om, bool backward, int chain = 0 )
 {
    const int rev = backward ? 1 : 0;
    if constexpr ( NR == 1 && std::is_same< T, double >::value )
       if ( P.sorTile > 0 )
       {
-         cellSorFused< T >( M, P, CW, flag, u, b, om, rev );
+         cellSorFused< T >( M, P, CW, flag, u, b, om, rev, true, chain );
          return;
       }
    if ( P.L <= kCellSmallMaxLevel && P.colourStart[4] > 0 )
@@ -2512,7 +2541,7 @@ void applyT( Operator& op, Function& src, Function& dst, const Function* rhs, in
 // current when the sweep ends.
 template < typename T, int NR >
 void sorT( Operator& op, Function& xf, const Function& bf, double omega, int level, int flag, bool refreshFirst,
-           bool backward = false )
+           bool backward = false, int chain = 0 )
 {
    Mesh&       M  = *op.mesh;
    LevelPlan&  P  = M.plan( level );
@@ -2554,7 +2583,9 @@ void sorT( Operator& op, Function& xf, const Function& bf, double omega, int lev
                                    (const T*) gh );
          }
       };
-      auto cellPhase = [&]() { cellSorPhase< T, NR >( M, P, wsel< T >( Wt.cell64, Wt.cell32 ), flag, u, b, om, backward ); };
+      auto cellPhase = [&]() {
+         cellSorPhase< T, NR >( M, P, wsel< T >( Wt.cell64, Wt.cell32 ), flag, u, b, om, backward, chain );
+      };
       if ( !backward )
       {
          vertexPhase();
@@ -2608,8 +2639,8 @@ template < typename T, int NR >
 void sorSweepsT( Operator& op, Function& xf, const Function& bf, double omega, int level, int flag, int count,
                  bool refreshFirst )
 {
-   for ( int k = 0; k < count; k++ )
-      sorT< T, NR >( op, xf, bf, omega, level, flag, refreshFirst && k == 0 );
+   for ( int k = 0; k < count; k++ ) // one smoothing sequence: sweep k is chained to sweep k - 1
+      sorT< T, NR >( op, xf, bf, omega, level, flag, refreshFirst && k == 0, false, k );
 }
 
 template < typename T, int NR >
@@ -2798,9 +2829,9 @@ template < typename T, int NR >
 struct SorF
 {
    static void run( Operator& op, Function& x, const Function& b, double omega, int L, int flag, bool refresh,
-                    bool backward )
+                    bool backward, int chain )
    {
-      sorT< T, NR >( op, x, b, omega, L, flag, refresh, backward );
+      sorT< T, NR >( op, x, b, omega, L, flag, refresh, backward, chain );
    }
 };
 template < typename T, int NR >
@@ -2859,9 +2890,9 @@ void launchApply( Operator& op, Function& src, Function& dst, const Function* rh
 }
 
 void launchSor( Operator& op, Function& x, const Function& b, double omega, int level, int flag, bool refreshFirst,
-                bool backward )
+                bool backward, int chain )
 {
-   dispatch< SorF >( x.prec, x.nrhs, op, x, b, omega, level, flag, refreshFirst, backward );
+   dispatch< SorF >( x.prec, x.nrhs, op, x, b, omega, level, flag, refreshFirst, backward, chain );
 }
 
 void launchSorSweeps( Operator& op, Function& x, const Function& b, double omega, int level, int flag, int count,

@@ -21,8 +21,10 @@ void launchApply( Operator& op, Function& src, Function& dst, const Function* rh
 // one multicolour SOR sweep in the reference's V -> E -> F -> C order; refreshFirst = false skips the
 // initial ghost-copy refresh when the caller knows the copies of x are current (they are after a sweep)
 // backward = true: the reverse sweep C -> F -> E -> V with the colours in reverse (one right-hand side)
+// chain: index of the sweep within a smoothing sequence on this level (0 = first); a sweep with chain k > 0
+// must directly follow sweep k - 1 of the same function and level, with no other write to macro-cell interiors
 void launchSor( Operator& op, Function& x, const Function& b, double omega, int level, int flag,
-                bool refreshFirst = true, bool backward = false );
+                bool refreshFirst = true, bool backward = false, int chain = 0 );
 // `count` SOR sweeps (refreshFirst applies to the first)
 void launchSorSweeps( Operator& op, Function& x, const Function& b, double omega, int level, int flag, int count,
                       bool refreshFirst = true );

@@ -663,6 +663,7 @@ void Mesh::setCellSor( int level, int tile )
    P.sorUnits.reset();
    P.sorUnitCount = 0;
    P.sorBand.release();
+   P.sorBand2.release();
    P.sorSlabStart.release();
    if ( tile == 0 ) return;
    std::vector< MarchUnit > v;
@@ -682,6 +683,7 @@ void Mesh::setCellSor( int level, int tile )
       P.sorUnits = std::make_unique< DevBuf< MarchUnit > >();
       P.sorUnits->upload( v, ctx->stream );
       P.sorBand.alloc( (size_t) P.total );
+      P.sorBand2.alloc( (size_t) P.total );
       std::vector< int32_t > ss( (size_t) w + 1 );
       auto                   tet = []( int64_t n ) { return n * ( n + 1 ) * ( n + 2 ) / 6; };
       for ( int z = 0; z <= w; z++ )

@@ -105,3 +105,28 @@ def test_vcycle_graphs_with_fused_cell_phase(gpu_ctx, oracle_lib):
     a, b2 = runs[(0, 0)]
     assert np.array_equal(a, b2)
     assert np.array_equal(runs[(24, 24)][0], a)
+
+
+def test_chained_sweeps_symmetric_and_w_cycles(gpu_ctx, oracle_lib):
+    """Smoothing sequences chain their fused sweeps (the halo of sweep k comes from sweep k-1's band array, no
+    copy): symmetric SOR smoothing (forward and backward sweeps alternating) in W-cycles gives the colour
+    passes' iterates bit for bit, and so do 5 pre- and post-sweeps of a V-cycle."""
+    import hyteg_amd as H
+    L = 7
+    say(f"[sor fused] cube n=1 level={L} chained sweeps: symmetric W(2,2) and V(5,5)")
+    om, S = mesh_pair(("cube", 1), gpu_ctx)
+    rhs = sine_rhs(om.owned_coords(L))
+    A = H.P1ConstantLaplaceOperator(S, 0, L)
+    f = H.P1Function("f", S, 0, L)
+    f.upload(L, rhs)
+    for smoother, cyc, nu in ((H.SymmetricSORSmoother(1.0), H.CycleType.WCYCLE, 2), (H.SORSmoother(1.0), H.CycleType.VCYCLE, 5)):
+        gmg = H.GeometricMultigridSolver(S, smoother, H.CGSolver(S, 0, 0), H.P1toP1LinearRestriction(),
+                                         H.P1toP1LinearProlongation(), 0, L, nu, nu, cycleType=cyc)
+        out = []
+        for tile in (0, 24):
+            S.set_cell_sor(L, tile)
+            S.set_cell_sor(6, tile)
+            u = H.P1Function("u", S, 0, L)
+            gmg.solve_graph(A, u, f, L, count=2)
+            out.append(u.download(L))
+        assert np.array_equal(out[0], out[1]), (cyc, nu)
