@@ -154,26 +154,34 @@ class PrimitiveStorage:
         -1 / 0 = level default."""
         _lib.call("hymi_mesh_set_schedule", self.h, level, kind, rows, waves, block)
 
-    def use_column_schedules(self, maxLevel, minLevel=5, fusedSorFrom=7):
+    def use_column_schedules(self, maxLevel, minLevel=5, fusedSorFrom=7, fusedSorFp32From=8):
         """Select the column march (kind 3) with its measured best parameters on levels minLevel..maxLevel:
         16 slabs per unit and 4 waves per workgroup from level 7 up, 8 and 4 on level 6, 4 and 2 below;
-        loads one slab ahead.  From level fusedSorFrom up, the SOR cell phase runs as one fused launch
-        (set_cell_sor(level, 24)), faster than four colour passes there; below it the passes are faster."""
+        loads one slab ahead.  The SOR cell phase runs as one fused launch (set_cell_sor(level, 24)) for FP64
+        from level fusedSorFrom up and for FP32 from level fusedSorFp32From up: where it beats four colour
+        passes.  FP32 gains less from it, since the fused march is bound by step latency rather than bytes."""
         for L in range(minLevel, maxLevel + 1):
             rows, waves = (16, 4) if L >= 7 else (8, 4) if L == 6 else (4, 2)
             self.set_schedule(L, rows, waves, 1, 3)
-            if L >= max(fusedSorFrom, 3):
-                self.set_cell_sor(L, 24)
+            fp64, fp32 = L >= max(fusedSorFrom, 3), L >= max(fusedSorFp32From, 3)
+            if fp64 or fp32:
+                self.set_cell_sor(L, 24, fp64=fp64, fp32=fp32)
 
-    def set_cell_sor(self, level, tile):
+    def set_cell_sor(self, level, tile, fp64=True, fp32=False):
         """SOR cell phase on `level`: tile 0 = four colour passes, 24 = all four colours in one launch over
-        24 x 24 tiles marched along z (FP64 functions with one right-hand side; same values bit for bit)."""
-        _lib.call("hymi_mesh_set_cell_sor", self.h, level, int(tile))
+        24 x 24 tiles marched along z, for functions of the selected precisions with one right-hand side (same
+        values bit for bit)."""
+        _lib.call("hymi_mesh_set_cell_sor", self.h, level, int(tile), (1 if fp64 else 0) | (2 if fp32 else 0))
 
     def get_cell_sor(self, level):
-        v = ctypes.c_int()
-        _lib.call("hymi_mesh_get_cell_sor", self.h, level, ctypes.byref(v))
-        return v.value
+        """Tile of the fused cell phase on `level` (0: colour passes)."""
+        return self.get_cell_sor_setting(level)[0]
+
+    def get_cell_sor_setting(self, level):
+        """(tile, precisions): precisions bit 0 FP64, bit 1 FP32."""
+        out = np.zeros(2, np.int32)
+        _lib.call("hymi_mesh_get_cell_sor", self.h, level, _ip(out))
+        return int(out[0]), int(out[1])
 
     def get_schedule(self, level):
         out = np.zeros(4, np.int32)

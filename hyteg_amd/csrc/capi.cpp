@@ -624,19 +624,21 @@ int hymi_mesh_set_schedule( hymi_mesh mesh, int level, int kind, int rows, int w
    } );
 }
 
-int hymi_mesh_set_cell_sor( hymi_mesh mesh, int level, int tile )
+int hymi_mesh_set_cell_sor( hymi_mesh mesh, int level, int tile, int precisions )
 {
    return guarded( [&] {
       require( mesh && level >= 0 && level <= 12, "bad argument" );
-      mesh->m.setCellSor( level, tile );
+      mesh->m.setCellSor( level, tile, precisions );
    } );
 }
 
-int hymi_mesh_get_cell_sor( hymi_mesh mesh, int level, int* tile )
+int hymi_mesh_get_cell_sor( hymi_mesh mesh, int level, int out[2] )
 {
    return guarded( [&] {
-      require( mesh && tile && level >= 0 && level <= 12, "bad argument" );
-      *tile = mesh->m.plan( level ).sorTile;
+      require( mesh && out && level >= 0 && level <= 12, "bad argument" );
+      const LevelPlan& P = mesh->m.plan( level );
+      out[0]             = P.sorTile;
+      out[1]             = P.sorPrecisions;
    } );
 }
 

@@ -136,6 +136,7 @@ struct LevelPlan
    // interior point of the tile, x0, y0) and the scratch array of the tiles' band positions (level-sized, FP64
    // so that either precision fits)
    int                                    sorTile = 0;
+   int                                    sorPrecisions = 0; // bit 0: FP64 functions, bit 1: FP32 functions
    std::unique_ptr< DevBuf< MarchUnit > > sorUnits;
    int64_t                                sorUnitCount = 0;
    DevBuf< double >                       sorBand, sorBand2; // alternate between chained sweeps
@@ -167,9 +168,10 @@ struct Mesh
    // replaces the launch schedule of the row-marching kernels on one level (zeros = level default);
    // bumps scheduleEpoch so that captured graphs holding the old work-unit table are captured again
    void     setSchedule( int level, const MarchSchedule& S );
-   // selects the fused SOR cell phase with square tiles of side `tile` (24) on one level, or the four
-   // colour passes (tile 0); bumps scheduleEpoch like setSchedule
-   void setCellSor( int level, int tile );
+   // selects the fused SOR cell phase with square tiles of side `tile` (24) on one level for the precisions in
+   // `precisions` (bit 0 FP64, bit 1 FP32), or the four colour passes (tile 0); bumps scheduleEpoch like
+   // setSchedule
+   void setCellSor( int level, int tile, int precisions = 1 );
    uint64_t scheduleEpoch = 0;
    // affine frames of the primitives, level-independent: micro-vertex (x, y, z) of a primitive lies at
    // o + (x a + y b + z c) / (w - 1); 12 doubles (o, a, b, c) per primitive, vertices, edges, faces, cells

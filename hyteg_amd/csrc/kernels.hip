@@ -2442,17 +2442,15 @@ void cellSorFused( Mesh& M, LevelPlan& P, const T* CW, int flag, T* u, const T* 
    }
 }
 
-// The SOR cell phase of one level: the fused launch where it is selected (FP64, one right-hand side), the
-// one-workgroup-per-cell launch on small levels, else the four colour passes; backward = colours in reverse;
-// chain = index of the sweep within its smoothing sequence (see cellSorFused).
-// FP32 keeps the colour passes: the fused march is bound by its per-slab step latency, not by bytes, so it gains
-// nothing from FP32's halved traffic, and the passes are faster in FP32 (level 7: 151 us against 226 us).
+// The SOR cell phase of one level: the fused launch where it is selected for the function's precision (one
+// right-hand side), the one-workgroup-per-cell launch on small levels, else the four colour passes; backward =
+// colours in reverse; chain = index of the sweep within its smoothing sequence (see cellSorFused).
 template < typename T, int NR >
 void cellSorPhase( Mesh& M, LevelPlan& P, const T* CW, int flag, T* u, const T* b, T om, bool backward, int chain = 0 )
 {
    const int rev = backward ? 1 : 0;
-   if constexpr ( NR == 1 && std::is_same< T, double >::value )
-      if ( P.sorTile > 0 )
+   if constexpr ( NR == 1 )
+      if ( P.sorTile > 0 && ( P.sorPrecisions & ( std::is_same< T, double >::value ? 1 : 2 ) ) )
       {
          cellSorFused< T >( M, P, CW, flag, u, b, om, rev, true, chain );
          return;
@@ -3010,13 +3008,14 @@ struct CellSorPhaseF
          cellSorPhase< T, NR >( M, P, CW, flag, u, bb, (T) omega, false );
          return;
       }
-      if constexpr ( NR == 1 && std::is_same< T, double >::value )
+      if constexpr ( NR == 1 )
       {
-         if ( P.sorTile == 0 ) throw std::invalid_argument( "no fused cell SOR phase on this level" );
+         if ( P.sorTile == 0 || !( P.sorPrecisions & ( std::is_same< T, double >::value ? 1 : 2 ) ) )
+            throw std::invalid_argument( "no fused cell SOR phase on this level for this precision" );
          cellSorFused< T >( M, P, CW, flag, u, bb, (T) omega, 0, false );
       }
       else
-         throw std::invalid_argument( "the fused cell SOR phase runs FP64 functions with one right-hand side" );
+         throw std::invalid_argument( "the fused cell SOR phase runs functions with one right-hand side" );
    }
 };
 template < typename T, int NR >

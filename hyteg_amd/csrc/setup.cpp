@@ -650,16 +650,19 @@ void Mesh::setSchedule( int level, const MarchSchedule& S )
    if ( !ctx->hostOnly ) marchUnits( P, &n );
 }
 
-void Mesh::setCellSor( int level, int tile )
+void Mesh::setCellSor( int level, int tile, int precisions )
 {
    if ( tile != 0 && !sorTileSizeOk( tile ) ) throw std::invalid_argument( "cell SOR tile must be 0 or 24" );
+   if ( tile != 0 && ( precisions < 1 || precisions > 3 ) )
+      throw std::invalid_argument( "cell SOR precisions: bit 0 FP64, bit 1 FP32, at least one" );
    if ( tile != 0 && level < 3 ) throw std::invalid_argument( "the fused cell SOR phase needs level >= 3" );
    LevelPlan& P = plan( level );
    // captured graphs hold the old unit table and scratch array: they are captured again on their next replay,
    // and no queued work may still use the buffers replaced here
    scheduleEpoch++;
    if ( !ctx->hostOnly ) HYMI_HIP( hipStreamSynchronize( ctx->stream ) );
-   P.sorTile = 0;
+   P.sorTile       = 0;
+   P.sorPrecisions = 0;
    P.sorUnits.reset();
    P.sorUnitCount = 0;
    P.sorBand.release();
@@ -672,12 +675,15 @@ void Mesh::setCellSor( int level, int tile )
       for ( int x0 = 1; x0 + 1 <= w - 3; x0 += tile )
          for ( int y0 = 1; x0 + y0 <= w - 3; y0 += tile )
             v.push_back( { c, w - 2 - x0 - y0, x0, y0 } );
-   // the launch gives XCD k the units [k per, (k + 1) per), which hold consecutive cells (a tile's neighbours
-   // mostly share its L2); within each range the longest tiles go first and the short ones fill in behind them
-   const size_t per = ( v.size() + 7 ) / 8;
-   for ( size_t a = 0; a < v.size(); a += per )
-      std::stable_sort( v.begin() + a, v.begin() + std::min( v.size(), a + per ),
-                        []( const MarchUnit& p, const MarchUnit& q ) { return p.z > q.z; } );
+   // The launch gives XCD k the units [k per, (k + 1) per), which hold consecutive cells.  With few tiles per
+   // CU (level 7: about 4 per CU of an XCD) the longest tiles go first and the short ones fill in behind them
+   // (220 us against 294 us in cell order); with many (level 8: about 12) cell order is faster (1.60 ms
+   // against 1.79 ms), since neighbouring tiles then march together and share their halos in L2.
+   const size_t per = ( v.size() + 7 ) / 8, cusPerXcd = 32;
+   if ( per < 8 * cusPerXcd )
+      for ( size_t a = 0; a < v.size(); a += per )
+         std::stable_sort( v.begin() + a, v.begin() + std::min( v.size(), a + per ),
+                           []( const MarchUnit& p, const MarchUnit& q ) { return p.z > q.z; } );
    if ( !ctx->hostOnly )
    {
       P.sorUnits = std::make_unique< DevBuf< MarchUnit > >();
@@ -691,8 +697,9 @@ void Mesh::setCellSor( int level, int tile )
       P.sorSlabStart.upload( ss, ctx->stream );
       HYMI_HIP( hipStreamSynchronize( ctx->stream ) );
    }
-   P.sorUnitCount = (int64_t) v.size();
-   P.sorTile      = tile;
+   P.sorUnitCount  = (int64_t) v.size();
+   P.sorTile       = tile;
+   P.sorPrecisions = precisions;
 }
 
 const double* Mesh::frames()

@@ -40,21 +40,28 @@ def test_fused_equals_colour_passes(kind, level, tile, backward, gpu_ctx, oracle
     assert np.array_equal(got, ref), np.abs(got - ref).max()
 
 
-def test_fp32_keeps_colour_passes_on_fused_levels(gpu_ctx, oracle_lib):
-    """The fused cell phase runs FP64 functions; FP32 functions on a fused level sweep with the colour passes
-    (same values as without the setting), and timing the FP32 tile kernel alone is refused."""
+@pytest.mark.parametrize("level", [6, 7])
+def test_fused_fp32(level, gpu_ctx, oracle_lib):
+    """FP32 functions run the fused phase where it is selected for FP32 (same values as the colour passes, bit
+    for bit), and keep the passes where it is selected for FP64 only (timing the tile kernel is then refused)."""
     import ctypes
     import hyteg_amd as H
-    L = 6
-    say(f"[sor fused] cube n=1 level={L} FP32 on a fused level")
+    L = level
+    say(f"[sor fused] cube n=1 level={L} FP32")
     om, S = mesh_pair(("cube", 1), gpu_ctx)
     rng = np.random.default_rng(8)
     x, b = rng.standard_normal((2, om.owned_count(L)))
     ref = _sweeps(H, S, L, x, b, 0, False, H.F32)
-    got = _sweeps(H, S, L, x, b, 24, False, H.F32)
-    assert np.array_equal(got, ref)
+    S.set_cell_sor(L, 24, fp32=True)
     A = H.P1ConstantLaplaceOperator(S, L, L)
     u, f = H.P1Function("u", S, L, L, precision=H.F32), H.P1Function("f", S, L, L, precision=H.F32)
+    u.upload(L, x)
+    f.upload(L, b)
+    for _ in range(2):
+        A.smooth_sor(u, f, 0.9, L, H.INNER)
+    assert np.array_equal(u.download(L), ref)
+    got = _sweeps(H, S, L, x, b, 24, False, H.F32)          # FP64 only: FP32 keeps the passes
+    assert np.array_equal(got, ref)
     ms = ctypes.c_double()
     with pytest.raises(H.HymiError):
         H._lib.call("hymi_time_kernel", A.h, None, u.h, f.h, None, L, 7, 1, ctypes.byref(ms))

@@ -159,10 +159,13 @@ def test_cell_sor_setting(H, hctx):
     S = H.PrimitiveStorage(H.MeshInfo.meshCuboid((0, 0, 0), (1, 1, 1), 1, 1, 1), hctx)
     assert S.get_cell_sor(6) == 0
     S.set_cell_sor(6, 24)
-    S.set_cell_sor(3, 24)
-    assert (S.get_cell_sor(6), S.get_cell_sor(3), S.get_cell_sor(7)) == (24, 24, 0)
+    S.set_cell_sor(3, 24, fp32=True)
+    S.set_cell_sor(8, 24, fp64=False, fp32=True)
+    assert [S.get_cell_sor_setting(L) for L in (6, 3, 8, 7)] == [(24, 1), (24, 3), (24, 2), (0, 0)]
     S.set_cell_sor(6, 0)
     assert S.get_cell_sor(6) == 0
-    for level, tile in ((6, 8), (6, 16), (6, 32), (2, 24)):
+    for level, tile, kw in ((6, 8, {}), (6, 16, {}), (6, 32, {}), (2, 24, {}), (6, 24, {"fp64": False})):
         with pytest.raises(H.HymiError):
-            S.set_cell_sor(level, tile)
+            S.set_cell_sor(level, tile, **kw)
+    S.use_column_schedules(8)
+    assert [S.get_cell_sor_setting(L) for L in (5, 6, 7, 8)] == [(0, 0), (0, 0), (24, 1), (24, 3)]
