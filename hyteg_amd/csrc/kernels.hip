@@ -1531,35 +1531,43 @@ __global__ __launch_bounds__( 128 ) void kEdgeSorFused( LV g, const EdgeGeom* __
       sW[k] = ew[3 + k];
    }
    __syncthreads();
-   const T   rl = omega / ew[1];
+   const T   rl = omega / ew[1], eW = ew[0], eE = ew[2];
    const int w = g.w, t0 = tet32( w );
+   // The neighbour terms read face and cell points, which the edge phase does not change: the term sums of all
+   // points are formed first (each thread's loads in flight together) and kept in LDS, and the two colour
+   // updates then only wait for the edge neighbours.  Colour 0 = even edge points, colour 1 = odd ones; a
+   // backward sweep takes them in reverse.  Barriers sit outside the point loops (all threads reach them).
+   __shared__ T tsum[kEdgeMaxW];
+   for ( int x = 1 + (int) threadIdx.x; x <= w - 2; x += blockDim.x )
+   {
+      T acc = T( 0 );
+      for ( int k0 = 0; k0 < nT; k0 += 16 )
+      {
+         T v[16];
+#pragma unroll
+         for ( int q = 0; q < 16; q++ )
+         {
+            const int k = min( k0 + q, nT - 1 );
+            const int c0 = sB[0][k] + x * sI[0][k], c1 = sB[1][k] + x * sI[1][k];
+            const int rel = sKind[k] == 0 ? faceRel( w, c0, c1 ) : cellRel( w, t0, c0, c1, sB[2][k] + x * sI[2][k] );
+            v[q] = u[sPrim[k] + rel];
+         }
+#pragma unroll
+         for ( int q = 0; q < 16; q++ )
+            if ( k0 + q < nT ) acc += sW[k0 + q] * v[q];
+      }
+      tsum[x] = acc;
+   }
+   __syncthreads();
    for ( int cc = 0; cc < 2; cc++ )
    {
-      // colour 0 = even edge points, colour 1 = odd ones; a backward sweep takes them in reverse
       const int colour = reverse ? 1 - cc : cc;
       for ( int x = 2 - colour + 2 * (int) threadIdx.x; x <= w - 2; x += 2 * blockDim.x )
       {
-         T acc = ew[0] * u[eo + x - 1] + ew[2] * u[eo + x + 1];
-         // terms in groups of 8: the eight values are requested before the first is summed (32-bit index
-         // arithmetic keeps the address math short); the summation order is the term order
-         for ( int k0 = 0; k0 < nT; k0 += 8 )
-         {
-            T v[8];
-#pragma unroll
-            for ( int q = 0; q < 8; q++ )
-            {
-               const int k = min( k0 + q, nT - 1 );
-               const int c0 = sB[0][k] + x * sI[0][k], c1 = sB[1][k] + x * sI[1][k];
-               const int rel = sKind[k] == 0 ? faceRel( w, c0, c1 ) : cellRel( w, t0, c0, c1, sB[2][k] + x * sI[2][k] );
-               v[q] = u[sPrim[k] + rel];
-            }
-#pragma unroll
-            for ( int q = 0; q < 8; q++ )
-               if ( k0 + q < nT ) acc += sW[k0 + q] * v[q];
-         }
-         u[eo + x] = ( T( 1 ) - omega ) * u[eo + x] + rl * ( b[eo + x] - acc );
+         const T acc = tsum[x] + ( eW * u[eo + x - 1] + eE * u[eo + x + 1] );
+         u[eo + x]   = ( T( 1 ) - omega ) * u[eo + x] + rl * ( b[eo + x] - acc );
       }
-      __syncthreads();
+      __syncthreads(); // the second colour reads the first colour's new values
    }
 }
 
@@ -2560,6 +2568,7 @@ void sorT( Operator& op, Function& xf, const Function& bf, double omega, int lev
                              om );
       };
       auto edgePhase = [&]() {
+         if ( P.w > kEdgeMaxW ) throw std::invalid_argument( "the fused edge SOR supports levels up to 10" );
          hipLaunchKernelGGL( ( kEdgeSorFused< T > ), dim3( nE ), dim3( 128 ), 0, s, g, P.dEdgeGeom.p, P.dEdgeTerms.p,
                              wsel< T >( Wt.edge64, Wt.edge32 ), flag, u, b, om, rev );
       };

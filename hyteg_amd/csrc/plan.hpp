@@ -78,6 +78,8 @@ struct EdgeTerm
 
 // largest number of neighbour terms of one edge (the fused edge SOR stages them in LDS)
 constexpr int kEdgeTermsMax = 128;
+// largest edge width w = 2^level + 1 of the fused edge SOR (its per-point term sums sit in LDS): level 10
+constexpr int kEdgeMaxW = 1025;
 // reductions: the cell part of a dot product runs on at most kDotBlocks workgroups, so a context's partial-sum
 // buffer has a fixed size (it is allocated once, with the context: captured graphs hold its address)
 constexpr int     kDotBlocks     = 2048;
