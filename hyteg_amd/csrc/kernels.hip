@@ -723,7 +723,7 @@ struct ColourStarts
    int s[5];
 };
 template < typename T >
-__global__ __launch_bounds__( 256 ) void kCellSorSmall( LV g, const int* __restrict__ pts, ColourStarts cs,
+__global__ __launch_bounds__( 1024 ) void kCellSorSmall( LV g, const int* __restrict__ pts, ColourStarts cs,
                                                         const T* __restrict__ W8, const int* __restrict__ ctype, int flag,
                                                         T* __restrict__ u, const T* __restrict__ b, T omega, int reverse )
 {
@@ -2468,7 +2468,10 @@ void cellSorPhase( Mesh& M, LevelPlan& P, const T* CW, int flag, T* u, const T* 
       ColourStarts cs;
       for ( int k = 0; k < 5; k++ )
          cs.s[k] = P.colourStart[k];
-      hipLaunchKernelGGL( ( kCellSorSmall< T > ), dim3( M.T.nC() ), dim3( 256 ), 0, M.ctx->stream, viewOf( P ),
+      // threads: about one point of a colour each, from 64 (level 2) up to 1024 (level 5: 1124 per colour)
+      const int perColour = ( P.colourStart[4] + 3 ) / 4;
+      const int threads   = std::min( 1024, std::max( 64, ( perColour + 63 ) / 64 * 64 ) );
+      hipLaunchKernelGGL( ( kCellSorSmall< T > ), dim3( M.T.nC() ), dim3( threads ), 0, M.ctx->stream, viewOf( P ),
                           P.dColourPts.p, cs, CW, P.dCellType.p, flag, u, b, om, rev );
    }
    else
