@@ -776,9 +776,9 @@ __global__ __launch_bounds__( 1024 ) void kCellSorSmall( LV g, const int* __rest
 // reach 3 - k rings beyond the tile, so colour k is recomputed on the tile widened by 3 - k rings and the tile
 // widened by kSorTileHalo = 4 rings is loaded; these recomputed halo values equal the owning tile's, because
 // they are computed with the same expression from the same inputs.  Slabs t-7 .. t+2 are in LDS (a ring of
-// kSorTileRing slots); slab t+3 is loaded into registers during step t and stored into LDS at the end of
-// step t+1, the right-hand side of step t+1 is loaded during step t.  Slab t-7 is final after step t-1 and
-// is written back in step t.
+// kSorTileRing slots); loads run DP steps ahead in registers (slab s is loaded in step s - 2 - DP and stored
+// into LDS at the end of step s - 2, the right-hand side of step t is loaded in step t - DP).  Slab t-7 is
+// final after step t-1 and is written back in step t.
 //
 // Workgroups are not synchronised with each other, so a tile may load its halo after the neighbouring tile
 // has finished.  Positions within kSorTileHalo of a side that another tile's halo may cover (the band) are
@@ -787,8 +787,8 @@ __global__ __launch_bounds__( 1024 ) void kCellSorSmall( LV g, const int* __rest
 // sweep of the same smoothing sequence: nothing but lower-dimensional phases ran in between, and those never
 // write macro-cell interiors) the previous sweep's band array `snapIn` holds exactly the halo values a tile
 // needs, so the tile reads its halo from there, writes its band both into u and into the other band array,
-// and no copy is needed.  The update expression is that
-// of the colour-pass kernels, so the result equals the four colour passes bit for bit.
+// and no copy is needed.  The update expression is that of the colour-pass kernels, so the result equals the
+// four colour passes bit for bit.
 // Units: cell, z = last slab with an interior point in the tile, x0, y0 (both 1 + multiples of TS).
 __device__ __forceinline__ bool sorTileBand( int lx, int ly, int i, int j, int TS )
 {
@@ -798,8 +798,8 @@ __device__ __forceinline__ bool sorTileBand( int lx, int ly, int i, int j, int T
 
 // Lanes: a wave updates positions of one colour offset c = (3x + 2y) mod 4 (plus 2 backward), so at every step
 // its colour, slab and LDS slot are uniform and live in scalar registers: wave v takes class v mod 4, 8 rows of
-// 8 positions 4 apart.  LDS rows are padded to R + 1 values, which keeps those reads free of
-// bank conflicts.  Loads and stores between memory and LDS walk the region and the core row by row instead.
+// 8 positions 4 apart.  LDS rows are padded to R + PAD values (PAD = 1 keeps those reads free of bank
+// conflicts).  Loads and stores between memory and LDS walk the region and the core row by row instead.
 constexpr int kSorTile = 24; // tile side: 32 x 32 region with the halo, 4 classes x 256 positions = 16 waves x 64
 template < typename T, int DP, int PAD, bool CHAINED >
 __global__ __launch_bounds__( 1024 ) void kCellSorTile( LV g, const MarchUnit* __restrict__ units, int64_t nUnits,
