@@ -215,16 +215,17 @@ __global__ __launch_bounds__( 256 ) void kCellSor( LV g, const CellSeg* __restri
                                                    const T* __restrict__ W, const int* __restrict__ ctype, int flag,
                                                    T* __restrict__ u, const T* __restrict__ b, T omega, int colour )
 {
+   // segments hold 4 x (64 / NR) points, among them 64 / NR of the colour (every fourth x): lane -> (the
+   // segment's j-th point of the colour, right-hand side r), so no lane is launched for a point of another colour
    const int64_t s = xcdBlock( ( nseg + 3 ) / 4 ) * 4 + ( threadIdx.x >> 6 );
    if ( s >= nseg ) return;
    const CellSeg sg = segs[s];
    if ( !( ctype[sg.cell] & flag ) ) return;
    const int lane = threadIdx.x & 63;
-   const int x    = sg.x0 + lane / NR;
+   const int x    = sg.x0 + ( ( colour - sg.x0 - 2 * sg.y - 3 * sg.z ) & 3 ) + 4 * ( lane / NR );
    const int r    = lane % NR;
    const int w    = g.w;
-   if ( x > w - 2 - sg.y - sg.z ) return;
-   if ( ( ( x + 2 * sg.y + 3 * sg.z ) & 3 ) != colour ) return;
+   if ( x > w - 2 - sg.y - sg.z || x >= sg.x0 + 4 * ( 64 / NR ) ) return;
    const int64_t base = g.offC + (int64_t) sg.cell * g.cS;
    const int64_t t0   = tet( w );
    const int64_t r0   = base + crow( w, t0, sg.y, sg.z ) + x;
@@ -2407,7 +2408,7 @@ void cellSorLaunch( Mesh& M, LevelPlan& P, const T* CW, int flag, T* u, const T*
       return;
    }
    int64_t        nseg = 0;
-   const CellSeg* sg   = M.segments( P, 64 / NR, &nseg );
+   const CellSeg* sg   = M.segments( P, 4 * ( 64 / NR ), &nseg ); // built with the solver's tables (no capture)
    if ( nseg == 0 ) return;
    hipLaunchKernelGGL( ( kCellSor< T, NR > ), dim3( (unsigned) xcdGrid( ( nseg + 3 ) / 4 ) ), dim3( 256 ), 0, s, g, sg,
                        nseg, CW, P.dCellType.p, flag, u, b, omega, colour );
