@@ -1271,18 +1271,17 @@ template < typename T, int NR >
 __global__ void kFaceSor( LV g, const FaceGeom* __restrict__ fg, const T* __restrict__ FW, int flag, T* __restrict__ u,
                           const T* __restrict__ b, T omega, int colour )
 {
-   const FaceGeom& G = fg[blockIdx.y];
+   // block (chunk, row j - 1, face); lane -> (the chunk's p-th point of the colour in row j, right-hand side r):
+   // the colour's points of a row are i = i0, i0 + 3, ..., so no lane is launched for another colour
+   const FaceGeom& G = fg[blockIdx.z];
    if ( !( G.type & flag ) ) return;
-   const int     m  = g.w - 3;
-   const int64_t np = (int64_t) m * ( m + 1 ) / 2;
-   const int64_t t  = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
-   const int     r  = (int) ( t % NR );
-   const int64_t k  = t / NR;
-   if ( k >= np ) return;
-   int a, bb;
-   triPos( m, k, a, bb );
-   const int i = a + 1, j = bb + 1;
-   if ( ( i + 2 * j ) % 3 != colour ) return;
+   const int m  = g.w - 3;
+   const int j  = (int) blockIdx.y + 1;
+   const int r  = (int) ( threadIdx.x % NR );
+   const int p  = (int) blockIdx.x * (int) ( blockDim.x / NR ) + (int) ( threadIdx.x / NR );
+   const int i0 = 1 + ( ( colour - 1 - 2 * j ) % 3 + 3 ) % 3;
+   const int i  = i0 + 3 * p;
+   if ( i > m + 1 - j ) return;
    const int64_t fo  = g.offF + (int64_t) G.face * g.fS;
    const T*      fw  = FW + (int64_t) G.face * kFaceW;
    const T       acc = faceStencilSum< T, NR >( g, G, fw, u, fo, i, j, r, false );
@@ -1408,10 +1407,11 @@ __global__ void kEdgeSor( LV g, const EdgeGeom* __restrict__ eg, const EdgeTerm*
 {
    const EdgeGeom& G = eg[blockIdx.y];
    if ( !( G.type & flag ) ) return;
+   // lane -> (the p-th point of the colour: x = 2, 4, ... for colour 0, 1, 3, ... for colour 1; right-hand side r)
    const int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
    const int     r = (int) ( t % NR );
-   const int     x = (int) ( t / NR ) + 1;
-   if ( x > g.w - 2 || ( x & 1 ) != colour ) return;
+   const int     x = ( colour == 0 ? 2 : 1 ) + 2 * (int) ( t / NR );
+   if ( x > g.w - 2 ) return;
    const T*      ew  = EW + 3 * (int64_t) G.edge + G.termOff;
    const int64_t eo  = g.offE + (int64_t) G.edge * g.eS;
    T             acc = ew[0] * u[( eo + x - 1 ) * NR + r] + ew[2] * u[( eo + x + 1 ) * NR + r];
@@ -2630,7 +2630,7 @@ void sorT( Operator& op, Function& xf, const Function& bf, double omega, int lev
    refreshT< T, NR >( xf, level, 1 );
    if ( level >= 1 )
       for ( int c = 0; c < 2; c++ )
-         hipLaunchKernelGGL( ( kEdgeSor< T, NR > ), dim3( blocksFor( (int64_t) ( P.w - 2 ) * NR, 64 ), nE ), dim3( 64 ), 0,
+         hipLaunchKernelGGL( ( kEdgeSor< T, NR > ), dim3( blocksFor( (int64_t) ( P.w - 1 ) / 2 * NR, 64 ), nE ), dim3( 64 ), 0,
                              s, g, P.dEdgeGeom.p, P.dEdgeTerms.p, wsel< T >( Wt.edge64, Wt.edge32 ), flag, u, b, om, c );
    refreshT< T, NR >( xf, level, 2 );
    if ( level >= 2 )
@@ -2638,7 +2638,7 @@ void sorT( Operator& op, Function& xf, const Function& bf, double omega, int lev
       const int64_t m = P.w - 3;
       if ( m > 0 && nF > 0 )
          for ( int c = 0; c < 3; c++ )
-            hipLaunchKernelGGL( ( kFaceSor< T, NR > ), dim3( blocksFor( m * ( m + 1 ) / 2 * NR, 256 ), nF ), dim3( 256 ), 0,
+            hipLaunchKernelGGL( ( kFaceSor< T, NR > ), dim3( blocksFor( ( m + 2 ) / 3 * NR, 256 ), m, nF ), dim3( 256 ), 0,
                                 s, g, P.dFaceGeom.p, wsel< T >( Wt.face64, Wt.face32 ), flag, u, b, om, c );
       refreshT< T, NR >( xf, level, 4 );
       for ( int c = 0; c < 4; c++ )
