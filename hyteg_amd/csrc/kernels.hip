@@ -997,54 +997,50 @@ __global__ __launch_bounds__( 1024 ) void kCellSorTile( LV g, const MarchUnit* _
    }
 }
 
-// Copies the band positions of the fused SOR sweep from the scratch array into u.  Block = (cell, slab, chunk
-// of kSorBandRows rows), 4 waves; wave v takes rows v + 1, v + 5, ... of the chunk, lanes along x.  All loads of
-// a wave are issued before its stores and need no branch; small blocks keep several resident per CU, so the
-// copy streams instead of waiting for one block's memory round trip at a time.
+// Copies the band positions of the fused SOR sweep from the scratch array into u.  Block = (cell, slab z),
+// 256 threads walking the slab's interior rows 1 .. w-3-z as one flat index range (rows are contiguous in a
+// slab, row y of the slab of width W = w - z starting at R(y) = y (2W + 1 - y) / 2), kSorBandBatch positions
+// per thread and pass, all loads before the stores.  A position's row is the integer part of the smaller
+// root of R(y) = f, taken in float and corrected by one row either way.  Lanes stay dense on the short rows
+// of the slab's upper part, where one lane per x of a row would mostly idle.
 constexpr int kSorBandBatch = 8;
-constexpr int kSorBandRows  = 4 * kSorBandBatch;
 template < typename T, int TS >
 __global__ __launch_bounds__( 256 ) void kCellSorBand( LV g, int nCells, const int* __restrict__ ctype, int flag,
                                                       T* __restrict__ u, const T* __restrict__ band )
 {
-   constexpr int XC = 2, RB = kSorBandBatch, NW = 4; // x chunks of 64 per row and pass, rows per wave, waves
-   const int     nz = g.w - 4, nyc = ( g.w - 4 + kSorBandRows - 1 ) / kSorBandRows, nBlk = nCells * nz * nyc;
-   const int     blk = __builtin_amdgcn_readfirstlane( (int) xcdBlock( nBlk ) );
+   const int nz = g.w - 4, nBlk = nCells * nz;
+   const int blk = __builtin_amdgcn_readfirstlane( (int) xcdBlock( nBlk ) );
    if ( blk >= nBlk ) return;
-   const int cell = blk / ( nz * nyc ), rem = blk - cell * ( nz * nyc ), z = 1 + rem / nyc;
-   const int yc = rem - ( z - 1 ) * nyc;
-   const int w = g.w, last = w - 3 - z; // rows 1 .. last
-   if ( 1 + yc * kSorBandRows > last || !( ctype[cell] & flag ) ) return;
-   const int          t0   = tet32( w );
+   const int cell = blk / nz, z = 1 + blk - cell * nz;
+   if ( !( ctype[cell] & flag ) ) return;
+   const int          w = g.w, W = w - z, last = W - 3, t0 = tet32( w );
+   const int          f0 = W, fEnd = ( last + 1 ) * ( 2 * W - last ) / 2; // R(1), R(last + 1)
    const int64_t      base = g.offC + (int64_t) cell * g.cS;
    const CellBuf< T > cu( u + base, t0 ), cbd( band + base, t0 );
-   const int          lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane( (int) ( threadIdx.x >> 6 ) );
-   const int          ss = t0 - tet32( w - z ), sy = w + 1 - z, yb = 1 + yc * kSorBandRows + wave;
-   for ( int xb = 1; xb + yb + z <= w - 2; xb += 64 * XC )
+   const int          ss = t0 - tet32( W );
+   const float        q  = (float) ( 2 * W + 1 );
+   for ( int fb = f0 + (int) threadIdx.x; fb < fEnd; fb += 256 * kSorBandBatch )
    {
-      T    v[RB][XC];
-      int  o[RB][XC];
-      bool in[RB][XC];
+      T    v[kSorBandBatch];
+      int  o[kSorBandBatch];
+      bool in[kSorBandBatch];
 #pragma unroll
-      for ( int c = 0; c < XC; c++ )
+      for ( int r = 0; r < kSorBandBatch; r++ )
       {
-         const int  xm = xb - 1 + 64 * c + lane, lx = xm % TS;
+         const int f = fb + 256 * r, fc = f < fEnd ? f : fEnd - 1; // fc: in range, so the root is real
+         int       y = (int) ( 0.5f * ( q - sqrtf( q * q - 8.0f * (float) fc ) ) );
+         y += ( y + 1 ) * ( 2 * W - y ) / 2 <= fc ? 1 : 0;
+         y -= y * ( 2 * W + 1 - y ) / 2 > fc ? 1 : 0;
+         const int  x = fc - y * ( 2 * W + 1 - y ) / 2, xm = x - 1, ym = y - 1, lx = xm % TS, ly = ym % TS;
          const bool colB = ( lx < kSorTileHalo && xm >= TS ) || lx >= TS - kSorTileHalo;
-#pragma unroll
-         for ( int r = 0; r < RB; r++ )
-         {
-            const int  y = yb + NW * r, ym = y - 1, ly = ym % TS, x = xm + 1;
-            const bool rowB = ( ly < kSorTileHalo && ym >= TS ) || ly >= TS - kSorTileHalo;
-            in[r][c]        = y <= last && x + y + z <= w - 2 && ( rowB || colB );
-            o[r][c]         = in[r][c] ? ss + y * sy - ( ( y * ( y + 1 ) ) >> 1 ) + x : 0;
-            v[r][c]         = cbd.load( o[r][c], in[r][c] );
-         }
+         const bool rowB = ( ly < kSorTileHalo && ym >= TS ) || ly >= TS - kSorTileHalo;
+         in[r]           = f < fEnd && x >= 1 && x <= W - 2 - y && ( rowB || colB );
+         o[r]            = ss + f;
+         v[r]            = cbd.load( o[r], in[r] );
       }
 #pragma unroll
-      for ( int r = 0; r < RB; r++ )
-#pragma unroll
-         for ( int c = 0; c < XC; c++ )
-            cu.store( o[r][c], v[r][c], in[r][c] );
+      for ( int r = 0; r < kSorBandBatch; r++ )
+         cu.store( o[r], v[r], in[r] );
    }
 }
 
@@ -2431,7 +2427,7 @@ void cellSorFused( Mesh& M, LevelPlan& P, const T* CW, int flag, T* u, const T* 
    withBand           = withBand && chain == 0;
    const int   nC   = M.T.nC();
    const dim3  grid( (unsigned) xcdGrid( P.sorUnitCount ) ),
-       bandGrid( (unsigned) xcdGrid( (int64_t) nC * ( P.w - 4 ) * ( ( P.w - 4 + kSorBandRows - 1 ) / kSorBandRows ) ) );
+       bandGrid( (unsigned) xcdGrid( (int64_t) nC * ( P.w - 4 ) ) );
    const MarchUnit* un = P.sorUnits->p;
    const int64_t    n  = P.sorUnitCount;
    switch ( P.sorTile )
