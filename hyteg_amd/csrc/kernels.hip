@@ -142,25 +142,48 @@ __global__ void kE2F( LV g, const E2F* __restrict__ tab, int n, T* __restrict__ 
    u[dst * NR + r]   = u[src * NR + r];
 }
 
-// one workgroup per (face row j, cell-face pair): face point (i, j), i = 0 .. w-1-j, is copied to the cell
-// point base + i a + j b of the pair's face-to-cell map
+// Row of flat index f in a triangle of width W stored row by row (row y has W - y points and starts at
+// R(y) = y (2W + 1 - y) / 2): the integer part of the smaller root of R(y) = f, taken in float, corrected by
+// one row either way (the float root is off by less than one row for W <= 1025).  0 <= f < W (W + 1) / 2.
+__device__ __forceinline__ int triRow32( int W, int f )
+{
+   const float q = (float) ( 2 * W + 1 );
+   int         y = (int) ( 0.5f * ( q - sqrtf( q * q - 8.0f * (float) f ) ) );
+   y += ( y + 1 ) * ( 2 * W - y ) / 2 <= f ? 1 : 0;
+   y -= y * ( 2 * W + 1 - y ) / 2 > f ? 1 : 0;
+   return y;
+}
+
+// Face-to-cell closure copy.  Block = (chunk of 256 * U values of one face, cell-face pair); thread t takes
+// the face's flat values chunk + t, chunk + t + 256, ... (face point p = flat / NR, right-hand side flat % NR).
+// Face point (i, j) is copied to the cell point base + i a + j b of the pair's face-to-cell map.  Reads are
+// contiguous; all loads of a thread are issued before its stores.
+constexpr int kF2CBatch = 4;
 template < typename T, int NR >
-__global__ void kF2C( LV g, const F2C* __restrict__ tab, T* __restrict__ u )
+__global__ __launch_bounds__( 256 ) void kF2C( LV g, const F2C* __restrict__ tab, T* __restrict__ u )
 {
    const F2C m  = tab[blockIdx.y];
-   const int j  = blockIdx.x;
    const int w  = g.w;
-   const int t0 = tet32( w );
-   const T*  fs = u + ( g.offF + (int64_t) m.face * g.fS + (int64_t) ( j * ( w + 1 ) - ( j * ( j + 1 ) >> 1 ) ) ) * NR;
+   const int t0 = tet32( w ), nV = ( w * ( w + 1 ) >> 1 ) * NR;
+   const T*  fs = u + ( g.offF + (int64_t) m.face * g.fS ) * NR;
    T*        cu = u + ( g.offC + (int64_t) m.m.cell * g.cS ) * NR;
-   for ( int t = threadIdx.x; t < ( w - j ) * NR; t += blockDim.x )
+   const int e0 = (int) blockIdx.x * 256 * kF2CBatch + (int) threadIdx.x;
+   T         v[kF2CBatch];
+   int       o[kF2CBatch];
+#pragma unroll
+   for ( int k = 0; k < kF2CBatch; k++ )
    {
-      const int i = t / NR, r = t % NR;
+      const int e = e0 + 256 * k, ec = e < nV ? e : 0, p = ec / NR, r = ec % NR;
+      const int j = triRow32( w, p ), i = p - j * ( 2 * w + 1 - j ) / 2;
       const int x = m.m.base[0] + i * m.m.a[0] + j * m.m.b[0];
       const int y = m.m.base[1] + i * m.m.a[1] + j * m.m.b[1];
       const int z = m.m.base[2] + i * m.m.a[2] + j * m.m.b[2];
-      cu[(int64_t) ( crow32( w, t0, y, z ) + x ) * NR + r] = fs[t];
+      o[k]        = ( crow32( w, t0, y, z ) + x ) * NR + r;
+      v[k]        = fs[ec];
    }
+#pragma unroll
+   for ( int k = 0; k < kF2CBatch; k++ )
+      if ( e0 + 256 * k < nV ) cu[o[k]] = v[k];
 }
 
 // --------------------------------------------------------------------------------------------------------
@@ -263,6 +286,7 @@ constexpr int kMarchSor = 3; // MODE value of the in-place multicolour SOR pass
 constexpr int kResidualSplit = 4;
 // finest level whose faces are small enough for one workgroup per face in the fused SOR phase
 constexpr int kFaceFusedMaxLevel = 4;
+constexpr int kFaceFusedMaxNp    = ( ( 1 << kFaceFusedMaxLevel ) - 2 ) * ( ( 1 << kFaceFusedMaxLevel ) - 1 ) / 2; // interior points
 
 // MODE kReplace / kAdd / kResidual: y = A u, y += A u, y = b - A u.
 // MODE kMarchSor: in-place SOR update of the points of `colour` (y == u).
@@ -1000,9 +1024,8 @@ __global__ __launch_bounds__( 1024 ) void kCellSorTile( LV g, const MarchUnit* _
 // Copies the band positions of the fused SOR sweep from the scratch array into u.  Block = (cell, slab z),
 // 256 threads walking the slab's interior rows 1 .. w-3-z as one flat index range (rows are contiguous in a
 // slab, row y of the slab of width W = w - z starting at R(y) = y (2W + 1 - y) / 2), kSorBandBatch positions
-// per thread and pass, all loads before the stores.  A position's row is the integer part of the smaller
-// root of R(y) = f, taken in float and corrected by one row either way.  Lanes stay dense on the short rows
-// of the slab's upper part, where one lane per x of a row would mostly idle.
+// per thread and pass, all loads before the stores; a position's row comes from triRow32.  Lanes stay dense
+// on the short rows of the slab's upper part, where one lane per x of a row would mostly idle.
 constexpr int kSorBandBatch = 8;
 template < typename T, int TS >
 __global__ __launch_bounds__( 256 ) void kCellSorBand( LV g, int nCells, const int* __restrict__ ctype, int flag,
@@ -1018,7 +1041,6 @@ __global__ __launch_bounds__( 256 ) void kCellSorBand( LV g, int nCells, const i
    const int64_t      base = g.offC + (int64_t) cell * g.cS;
    const CellBuf< T > cu( u + base, t0 ), cbd( band + base, t0 );
    const int          ss = t0 - tet32( W );
-   const float        q  = (float) ( 2 * W + 1 );
    for ( int fb = f0 + (int) threadIdx.x; fb < fEnd; fb += 256 * kSorBandBatch )
    {
       T    v[kSorBandBatch];
@@ -1027,10 +1049,7 @@ __global__ __launch_bounds__( 256 ) void kCellSorBand( LV g, int nCells, const i
 #pragma unroll
       for ( int r = 0; r < kSorBandBatch; r++ )
       {
-         const int f = fb + 256 * r, fc = f < fEnd ? f : fEnd - 1; // fc: in range, so the root is real
-         int       y = (int) ( 0.5f * ( q - sqrtf( q * q - 8.0f * (float) fc ) ) );
-         y += ( y + 1 ) * ( 2 * W - y ) / 2 <= fc ? 1 : 0;
-         y -= y * ( 2 * W + 1 - y ) / 2 > fc ? 1 : 0;
+         const int  f = fb + 256 * r, fc = f < fEnd ? f : fEnd - 1, y = triRow32( W, fc );
          const int  x = fc - y * ( 2 * W + 1 - y ) / 2, xm = x - 1, ym = y - 1, lx = xm % TS, ly = ym % TS;
          const bool colB = ( lx < kSorTileHalo && xm >= TS ) || lx >= TS - kSorTileHalo;
          const bool rowB = ( ly < kSorTileHalo && ym >= TS ) || ly >= TS - kSorTileHalo;
@@ -1208,18 +1227,12 @@ __global__ __launch_bounds__( 1024 ) void kCellColumn( LV g, const MarchUnit* __
 // --------------------------------------------------------------------------------------------------------
 // macro-face interior
 
+// acc plus the off-plane terms of face point (i, j) (the cells' points next to the face), in side order
 template < typename T, int NR >
-__device__ __forceinline__ T faceStencilSum( const LV& g, const FaceGeom& G, const T* __restrict__ fw,
-                                             const T* __restrict__ u, int64_t fo, int i, int j, int r, bool withCentre )
+__device__ __forceinline__ T faceOffPlaneAdd( const LV& g, const FaceGeom& G, const T* __restrict__ fw,
+                                              const T* __restrict__ u, int i, int j, int r, T acc )
 {
-   const int w = g.w;
-#define UF( ii, jj ) u[( fo + fidx( w, ii, jj ) ) * NR + r]
-   T acc = fw[1] * UF( i - 1, j ) + fw[2] * UF( i + 1, j );
-   acc += fw[3] * UF( i, j + 1 ) + fw[4] * UF( i, j - 1 );
-   acc += fw[5] * UF( i - 1, j + 1 ) + fw[6] * UF( i + 1, j - 1 );
-   if ( withCentre ) acc += fw[0] * UF( i, j );
-#undef UF
-   const int t0 = tet32( w );
+   const int w = g.w, t0 = tet32( w );
    for ( int s = 0; s < G.nSides; s++ )
    {
       const FaceCellMap& M  = G.side[s];
@@ -1235,6 +1248,20 @@ __device__ __forceinline__ T faceStencilSum( const LV& g, const FaceGeom& G, con
       }
    }
    return acc;
+}
+
+template < typename T, int NR >
+__device__ __forceinline__ T faceStencilSum( const LV& g, const FaceGeom& G, const T* __restrict__ fw,
+                                             const T* __restrict__ u, int64_t fo, int i, int j, int r, bool withCentre )
+{
+   const int w = g.w;
+#define UF( ii, jj ) u[( fo + fidx( w, ii, jj ) ) * NR + r]
+   T acc = fw[1] * UF( i - 1, j ) + fw[2] * UF( i + 1, j );
+   acc += fw[3] * UF( i, j + 1 ) + fw[4] * UF( i, j - 1 );
+   acc += fw[5] * UF( i - 1, j + 1 ) + fw[6] * UF( i + 1, j - 1 );
+   if ( withCentre ) acc += fw[0] * UF( i, j );
+#undef UF
+   return faceOffPlaneAdd< T, NR >( g, G, fw, u, i, j, r, acc );
 }
 
 template < typename T, int NR, int MODE >
@@ -1497,8 +1524,10 @@ __global__ __launch_bounds__( 64 ) void kVertexSorPush( LV g, const VertexGeom* 
       u[ends[G.termOff + k]] = nv;
 }
 
-// one workgroup per edge; the edge's neighbour terms and weights are staged in LDS once, so the per-point
-// sums issue independent value loads
+// one workgroup per edge; the edge's neighbour terms and weights, its own values and its right-hand side are
+// staged in LDS once, so the per-point sums issue independent value loads and the colour updates issue none
+constexpr int kEdgeTermBatch = 16;
+constexpr int kEdgeSliceMax  = 128; // slices x points of a short edge (<= blockDim.x)
 template < typename T >
 __global__ __launch_bounds__( 128 ) void kEdgeSorFused( LV g, const EdgeGeom* __restrict__ eg,
                                                         const EdgeTerm* __restrict__ terms, const T* __restrict__ EW,
@@ -1508,11 +1537,13 @@ __global__ __launch_bounds__( 128 ) void kEdgeSorFused( LV g, const EdgeGeom* __
    __shared__ int64_t sPrim[kEdgeTermsMax];                  // element offset of the neighbour primitive
    __shared__ int     sB[3][kEdgeTermsMax], sI[3][kEdgeTermsMax], sKind[kEdgeTermsMax];
    __shared__ T       sW[kEdgeTermsMax];
+   __shared__ T       tsum[kEdgeMaxW], su[kEdgeMaxW], sb[kEdgeMaxW];
    const EdgeGeom& G = eg[blockIdx.x];
    if ( !( G.type & flag ) ) return;
    const T*      ew = EW + 3 * (int64_t) G.edge + G.termOff;
    const int64_t eo = g.offE + (int64_t) G.edge * g.eS;
    const int     nT = G.nTerms; // <= kEdgeTermsMax (checked when the plan is built)
+   const int     w = g.w, t0 = tet32( w );
    // term k at edge point x: face element fidx(w, B0 + x I0, B1 + x I1) or cell element
    // cidx(w, B0 + x I0, B1 + x I1, B2 + x I2) of the neighbour primitive
    for ( int k = threadIdx.x; k < nT; k += blockDim.x )
@@ -1527,22 +1558,34 @@ __global__ __launch_bounds__( 128 ) void kEdgeSorFused( LV g, const EdgeGeom* __
       }
       sW[k] = ew[3 + k];
    }
+   for ( int x = threadIdx.x; x < w; x += blockDim.x ) // the edge, its end copies included, and its right-hand side
+   {
+      su[x] = u[eo + x];
+      sb[x] = b[eo + x];
+   }
+   const T rl = omega / ew[1], eW = ew[0], eE = ew[2];
    __syncthreads();
-   const T   rl = omega / ew[1], eW = ew[0], eE = ew[2];
-   const int w = g.w, t0 = tet32( w );
    // The neighbour terms read face and cell points, which the edge phase does not change: the term sums of all
    // points are formed first (each thread's loads in flight together) and kept in LDS, and the two colour
-   // updates then only wait for the edge neighbours.  Colour 0 = even edge points, colour 1 = odd ones; a
-   // backward sweep takes them in reverse.  Barriers sit outside the point loops (all threads reach them).
-   __shared__ T tsum[kEdgeMaxW];
-   for ( int x = 1 + (int) threadIdx.x; x <= w - 2; x += blockDim.x )
+   // updates then read only LDS.  Colour 0 = even edge points, colour 1 = odd ones; a backward sweep takes them
+   // in reverse.  Barriers sit outside the point loops (all threads reach them).
+   // On short edges the terms of a point are split into S contiguous slices summed by S threads (S = the
+   // largest power of two <= 8 with S (w - 2) <= blockDim.x); the slices' sums are then added in slice order.
+   const int np = w - 2;
+   int       S  = 1;
+   while ( S < 8 && 2 * S * np <= (int) blockDim.x )
+      S *= 2;
+   const int per = ( nT + S - 1 ) / S;
+   __shared__ T part[kEdgeSliceMax];
+   for ( int t = (int) threadIdx.x; t < np * S; t += blockDim.x )
    {
-      T acc = T( 0 );
-      for ( int k0 = 0; k0 < nT; k0 += 16 )
+      const int x = 1 + t % np, sl = t / np, kb = sl * per, ke = min( nT, kb + per );
+      T         acc = T( 0 );
+      for ( int k0 = kb; k0 < ke; k0 += kEdgeTermBatch )
       {
-         T v[16];
+         T v[kEdgeTermBatch];
 #pragma unroll
-         for ( int q = 0; q < 16; q++ )
+         for ( int q = 0; q < kEdgeTermBatch; q++ )
          {
             const int k = min( k0 + q, nT - 1 );
             const int c0 = sB[0][k] + x * sI[0][k], c1 = sB[1][k] + x * sI[1][k];
@@ -1550,10 +1593,24 @@ __global__ __launch_bounds__( 128 ) void kEdgeSorFused( LV g, const EdgeGeom* __
             v[q] = u[sPrim[k] + rel];
          }
 #pragma unroll
-         for ( int q = 0; q < 16; q++ )
-            if ( k0 + q < nT ) acc += sW[k0 + q] * v[q];
+         for ( int q = 0; q < kEdgeTermBatch; q++ )
+            if ( k0 + q < ke ) acc += sW[k0 + q] * v[q];
       }
-      tsum[x] = acc;
+      if ( S == 1 )
+         tsum[x] = acc;
+      else
+         part[t] = acc;
+   }
+   if ( S > 1 )
+   {
+      __syncthreads();
+      for ( int x = 1 + (int) threadIdx.x; x <= np; x += blockDim.x )
+      {
+         T acc = part[x - 1];
+         for ( int sl = 1; sl < S; sl++ )
+            acc += part[sl * np + x - 1];
+         tsum[x] = acc;
+      }
    }
    __syncthreads();
    for ( int cc = 0; cc < 2; cc++ )
@@ -1561,8 +1618,10 @@ __global__ __launch_bounds__( 128 ) void kEdgeSorFused( LV g, const EdgeGeom* __
       const int colour = reverse ? 1 - cc : cc;
       for ( int x = 2 - colour + 2 * (int) threadIdx.x; x <= w - 2; x += 2 * blockDim.x )
       {
-         const T acc = tsum[x] + ( eW * u[eo + x - 1] + eE * u[eo + x + 1] );
-         u[eo + x]   = ( T( 1 ) - omega ) * u[eo + x] + rl * ( b[eo + x] - acc );
+         const T acc = tsum[x] + ( eW * su[x - 1] + eE * su[x + 1] );
+         const T nv  = ( T( 1 ) - omega ) * su[x] + rl * ( sb[x] - acc );
+         su[x]       = nv;
+         u[eo + x]   = nv;
       }
       __syncthreads(); // the second colour reads the first colour's new values
    }
@@ -1576,31 +1635,46 @@ __global__ __launch_bounds__( 256 ) void kFaceSorFused( LV g, const FaceGeom* __
    const FaceGeom& G = fg[blockIdx.x];
    if ( !( G.type & flag ) ) return;
    const int64_t fo = g.offF + (int64_t) G.face * g.fS;
-   // own boundary <- final edge values (the E2F table holds the three edges of face f at 3f .. 3f+2)
+   // own boundary <- final edge values (the E2F table holds the three edges of face f at 3f .. 3f+2); the
+   // three edges' values are loaded before any is stored, so their loads are in flight together
+   E2F m3[3];
    for ( int le = 0; le < 3; le++ )
+      m3[le] = e2f[3 * G.face + le];
+   for ( int p = threadIdx.x; p < g.w; p += blockDim.x )
    {
-      const E2F m = e2f[3 * G.face + le];
-      for ( int p = threadIdx.x; p < g.w; p += blockDim.x )
-         u[fo + fidx( g.w, m.base[0] + p * m.inc[0], m.base[1] + p * m.inc[1] )] =
-             u[g.offE + (int64_t) m.edge * g.eS + p];
+      T ev[3];
+      for ( int le = 0; le < 3; le++ )
+         ev[le] = u[g.offE + (int64_t) m3[le].edge * g.eS + p];
+      for ( int le = 0; le < 3; le++ )
+         u[fo + fidx( g.w, m3[le].base[0] + p * m3[le].inc[0], m3[le].base[1] + p * m3[le].inc[1] )] = ev[le];
    }
-   __syncthreads();
-   const int     m  = g.w - 3;
-   const int64_t np = (int64_t) m * ( m + 1 ) / 2;
+   // The off-plane terms read cell points, which the face phase does not change: their sums are formed once
+   // for all points (LDS), next to the edge copies above, and each colour pass adds only the in-plane terms.
+   __shared__ T  sOff[kFaceFusedMaxNp];
+   const int     w  = g.w, m = w - 3, np = m * ( m + 1 ) / 2; // np <= kFaceFusedMaxNp (levels <= kFaceFusedMaxLevel)
    const T*      fw = FW + (int64_t) G.face * kFaceW;
    const T       rl = omega / fw[0];
+   for ( int k = threadIdx.x; k < np; k += blockDim.x )
+   {
+      const int bb = triRow32( m, k ), i = k - bb * ( 2 * m + 1 - bb ) / 2 + 1, j = bb + 1;
+      sOff[k]      = faceOffPlaneAdd< T, 1 >( g, G, fw, u, i, j, 0, T( 0 ) );
+   }
+   __syncthreads();
    for ( int cc = 0; cc < 3; cc++ )
    {
       const int colour = reverse ? 2 - cc : cc; // a backward sweep takes the colours in reverse
-      for ( int64_t k = threadIdx.x; k < np; k += blockDim.x )
+      for ( int k = threadIdx.x; k < np; k += blockDim.x )
       {
-         int a, bb;
-         triPos( m, k, a, bb );
-         const int i = a + 1, j = bb + 1;
+         const int bb = triRow32( m, k ), i = k - bb * ( 2 * m + 1 - bb ) / 2 + 1, j = bb + 1;
          if ( ( i + 2 * j ) % 3 != colour ) continue;
-         const T       acc = faceStencilSum< T, 1 >( g, G, fw, u, fo, i, j, 0, false );
-         const int64_t o   = fo + fidx( g.w, i, j );
-         u[o]              = ( T( 1 ) - omega ) * u[o] + rl * ( b[o] - acc );
+#define UF( ii, jj ) u[fo + fidx( w, ii, jj )]
+         T acc = fw[1] * UF( i - 1, j ) + fw[2] * UF( i + 1, j );
+         acc += fw[3] * UF( i, j + 1 ) + fw[4] * UF( i, j - 1 );
+         acc += fw[5] * UF( i - 1, j + 1 ) + fw[6] * UF( i + 1, j - 1 );
+#undef UF
+         acc += sOff[k];
+         const int64_t o = fo + fidx( w, i, j );
+         u[o]            = ( T( 1 ) - omega ) * u[o] + rl * ( b[o] - acc );
       }
       __syncthreads();
    }
@@ -2491,8 +2565,10 @@ void refreshT( Function& f, int level, int mask )
       hipLaunchKernelGGL( ( kE2F< T, NR > ), dim3( blocksFor( 3 * (int64_t) M.T.nF() * P.w * NR, 256 ) ), dim3( 256 ), 0,
                           s, g, P.dE2F.p, 3 * M.T.nF(), u );
    if ( ( mask & 4 ) && M.T.nC() > 0 )
-      hipLaunchKernelGGL( ( kF2C< T, NR > ), dim3( P.w, 4 * M.T.nC() ), dim3( P.w * NR <= 64 ? 64 : 128 ), 0, s, g,
-                          P.dF2C.p, u );
+      hipLaunchKernelGGL( ( kF2C< T, NR > ),
+                          dim3( (unsigned) ( ( (int64_t) P.w * ( P.w + 1 ) / 2 * NR + 256 * kF2CBatch - 1 ) / ( 256 * kF2CBatch ) ),
+                                4 * M.T.nC() ),
+                          dim3( 256 ), 0, s, g, P.dF2C.p, u );
 }
 
 template < typename T, int NR >
