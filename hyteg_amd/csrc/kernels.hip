@@ -186,6 +186,86 @@ __global__ __launch_bounds__( 256 ) void kF2C( LV g, const F2C* __restrict__ tab
       if ( e0 + 256 * k < nV ) cu[o[k]] = v[k];
 }
 
+// Closure refresh of several steps (V->E, E->F, F->C) in one launch.  Run one after another, each step copies
+// what the previous one refreshed; here every copy reads the primitive that owns the value instead: an edge
+// end copy the vertex (when V->E is part of the refresh, else the edge's end copy as it stands), a face
+// boundary point the edge (when E->F is part of it, else the face's copy).  No part then reads what another
+// part writes, and the result equals the steps in order.  Blocks [0, nA) copy vertex -> edge ends (as kV2E),
+// [nA, nA + nB) edge -> face boundaries (as kE2F), the rest face -> cell (as kF2C, chunks per pair).
+template < typename T, int NR >
+__device__ __forceinline__ int64_t refreshEdgeSrc( const LV& g, const V2E* __restrict__ v2e, int edge, int t, bool fromVertex )
+{
+   if ( fromVertex && t == 0 ) return v2e[edge].v0;
+   if ( fromVertex && t == g.w - 1 ) return v2e[edge].v1;
+   return g.offE + (int64_t) edge * g.eS + t;
+}
+
+template < typename T, int NR >
+__global__ __launch_bounds__( 256 ) void kRefresh( LV g, const V2E* __restrict__ v2e, int nE, const E2F* __restrict__ e2f,
+                                                  int nF, const F2C* __restrict__ f2c, int nPairs, int mask, int nA,
+                                                  int nB, int chunks, T* __restrict__ u )
+{
+   const int  w = g.w, blk = (int) blockIdx.x;
+   const bool fromVertex = ( mask & 1 ) != 0, fromEdge = ( mask & 2 ) != 0;
+   if ( blk < nA )
+   {
+      const int64_t t = (int64_t) blk * 256 + threadIdx.x, k = t / NR;
+      const int     r = (int) ( t % NR );
+      if ( k >= 2 * (int64_t) nE ) return;
+      const V2E     e   = v2e[k >> 1];
+      const int     end = (int) ( k & 1 );
+      const int64_t dst = g.offE + (int64_t) e.edge * g.eS + ( end ? w - 1 : 0 );
+      u[dst * NR + r]   = u[(int64_t) ( end ? e.v1 : e.v0 ) * NR + r];
+      return;
+   }
+   if ( blk < nA + nB )
+   {
+      const int64_t t = (int64_t) ( blk - nA ) * 256 + threadIdx.x, k = t / NR;
+      const int     r = (int) ( t % NR );
+      if ( k >= 3 * (int64_t) nF * w ) return;
+      const E2F     m   = e2f[k / w];
+      const int     p   = (int) ( k % w );
+      const int64_t dst = g.offF + (int64_t) m.face * g.fS + faceRel( w, m.base[0] + p * m.inc[0], m.base[1] + p * m.inc[1] );
+      u[dst * NR + r]   = u[refreshEdgeSrc< T, NR >( g, v2e, m.edge, p, fromVertex ) * NR + r];
+      return;
+   }
+   const int q = blk - nA - nB, pair = q / chunks;
+   if ( pair >= nPairs ) return;
+   const F2C     m  = f2c[pair];
+   const int     t0 = tet32( w ), nV = ( w * ( w + 1 ) >> 1 ) * NR;
+   const int64_t fo = g.offF + (int64_t) m.face * g.fS;
+   T*            cu = u + ( g.offC + (int64_t) m.m.cell * g.cS ) * NR;
+   const int     e0 = ( q - pair * chunks ) * 256 * kF2CBatch + (int) threadIdx.x;
+   T             v[kF2CBatch];
+   int           o[kF2CBatch];
+#pragma unroll
+   for ( int k = 0; k < kF2CBatch; k++ )
+   {
+      const int e = e0 + 256 * k, ec = e < nV ? e : 0, p = ec / NR, r = ec % NR;
+      const int j = triRow32( w, p ), i = p - j * ( 2 * w + 1 - j ) / 2;
+      const int x = m.m.base[0] + i * m.m.a[0] + j * m.m.b[0];
+      const int y = m.m.base[1] + i * m.m.a[1] + j * m.m.b[1];
+      const int z = m.m.base[2] + i * m.m.a[2] + j * m.m.b[2];
+      o[k]        = ( crow32( w, t0, y, z ) + x ) * NR + r;
+      int64_t src = fo + p;                     // face interior, or the face's own copy of its boundary
+      if ( fromEdge && ( i == 0 || j == 0 || i + j == w - 1 ) )
+         for ( int le = 0; le < 3; le++ ) // the face edge through (i, j); corners lie on two, same owner value
+         {
+            const E2F me = e2f[3 * m.face + le];
+            const int te = me.inc[0] != 0 ? ( i - me.base[0] ) * me.inc[0] : ( j - me.base[1] ) * me.inc[1];
+            if ( te >= 0 && te <= w - 1 && me.base[0] + te * me.inc[0] == i && me.base[1] + te * me.inc[1] == j )
+            {
+               src = refreshEdgeSrc< T, NR >( g, v2e, me.edge, te, fromVertex );
+               break;
+            }
+         }
+      v[k] = u[src * NR + r];
+   }
+#pragma unroll
+   for ( int k = 0; k < kF2CBatch; k++ )
+      if ( e0 + 256 * k < nV ) cu[o[k]] = v[k];
+}
+
 // --------------------------------------------------------------------------------------------------------
 // macro-cell interior: apply / residual, multicolour SOR
 
@@ -2558,6 +2638,19 @@ void refreshT( Function& f, int level, int mask )
    LV          g = viewOf( P );
    hipStream_t s = M.ctx->stream;
    T*          u = (T*) f.ptr( level );
+   if ( M.T.nC() == 0 ) mask &= 3;
+   if ( mask == 3 || mask == 5 || mask == 6 || mask == 7 ) // several steps: one launch reading the owners
+   {
+      const int     nA = ( mask & 1 ) ? (int) blocksFor( 2 * (int64_t) M.T.nE() * NR, 256 ) : 0;
+      const int     nB = ( mask & 2 ) ? (int) blocksFor( 3 * (int64_t) M.T.nF() * P.w * NR, 256 ) : 0;
+      const int     chunks = (int) ( ( (int64_t) P.w * ( P.w + 1 ) / 2 * NR + 256 * kF2CBatch - 1 ) / ( 256 * kF2CBatch ) );
+      const int     nPairs = ( mask & 4 ) ? 4 * M.T.nC() : 0;
+      const int64_t nBlk   = (int64_t) nA + nB + (int64_t) nPairs * chunks;
+      if ( nBlk > 0 )
+         hipLaunchKernelGGL( ( kRefresh< T, NR > ), dim3( (unsigned) nBlk ), dim3( 256 ), 0, s, g, P.dV2E.p, M.T.nE(),
+                             P.dE2F.p, M.T.nF(), P.dF2C.p, nPairs, mask, nA, nB, chunks, u );
+      return;
+   }
    if ( mask & 1 )
       hipLaunchKernelGGL( ( kV2E< T, NR > ), dim3( blocksFor( 2 * (int64_t) M.T.nE() * NR, 256 ) ), dim3( 256 ), 0, s, g,
                           P.dV2E.p, M.T.nE(), u );
