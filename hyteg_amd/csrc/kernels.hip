@@ -1398,27 +1398,41 @@ __global__ void kFaceSor( LV g, const FaceGeom* __restrict__ fg, const T* __rest
 // Face ghost layers for the SOR sweep: for face f and side s, gh[(2 f + s) fS + fidx(col, row)] = the cell
 // value at base + col a + row b + o of the side's face-to-cell map, for every (col, row) of the face
 // triangle whose image lies in the cell (a superset of the off-plane neighbours the face stencil reads).
-// One workgroup per (row, face).  The cell reads are scattered for two of the four cell face planes (one
-// point per cell row); gathering them once per sweep replaces scattered reads in each colour pass.
+// Block = (chunk of 256 * kFaceGhostBatch points of the face triangle, face); a point's row comes from
+// triRow32, and all loads of a thread (both sides) are issued before its stores.  The cell reads are
+// scattered for two of the four cell face planes (one point per cell row); gathering them once per sweep
+// replaces scattered reads in each colour pass.
+constexpr int kFaceGhostBatch = 2;
 template < typename T >
-__global__ __launch_bounds__( 128 ) void kFaceGhost( LV g, const FaceGeom* __restrict__ fg, const T* __restrict__ u,
+__global__ __launch_bounds__( 256 ) void kFaceGhost( LV g, const FaceGeom* __restrict__ fg, const T* __restrict__ u,
                                                      T* __restrict__ gh )
 {
-   const FaceGeom& G   = fg[blockIdx.y];
-   const int       w   = g.w, row = blockIdx.x, t0 = tet32( w );
-   for ( int sd = 0; sd < G.nSides; sd++ )
+   const FaceGeom& G  = fg[blockIdx.y];
+   const int       w  = g.w, t0 = tet32( w ), nP = ( w * ( w + 1 ) >> 1 ) - 1, nS = G.nSides; // rows 0 .. w-2
+   const int       p0 = (int) blockIdx.x * 256 * kFaceGhostBatch + (int) threadIdx.x;
+   T               v[kFaceGhostBatch][2];
+   bool            in[kFaceGhostBatch][2];
+#pragma unroll
+   for ( int k = 0; k < kFaceGhostBatch; k++ )
    {
-      const FaceCellMap& M  = G.side[sd];
-      const T*           cu = u + g.offC + (int64_t) M.cell * g.cS;
-      T*                 go = gh + ( 2 * (int64_t) G.face + sd ) * g.fS;
-      for ( int col = threadIdx.x; col + row <= w - 1; col += blockDim.x )
+      const int p = p0 + 256 * k, pc = p < nP ? p : 0;
+      const int row = triRow32( w, pc ), col = pc - row * ( 2 * w + 1 - row ) / 2;
+#pragma unroll
+      for ( int sd = 0; sd < 2; sd++ )
       {
-         const int x = M.base[0] + col * M.a[0] + row * M.b[0] + M.o[0];
-         const int y = M.base[1] + col * M.a[1] + row * M.b[1] + M.o[1];
-         const int z = M.base[2] + col * M.a[2] + row * M.b[2] + M.o[2];
-         if ( x >= 0 && y >= 0 && z >= 0 && x + y + z <= w - 1 ) go[faceRel( w, col, row )] = cu[cellRel( w, t0, x, y, z )];
+         const FaceCellMap& M = G.side[sd < nS ? sd : 0];
+         const int          x = M.base[0] + col * M.a[0] + row * M.b[0] + M.o[0];
+         const int          y = M.base[1] + col * M.a[1] + row * M.b[1] + M.o[1];
+         const int          z = M.base[2] + col * M.a[2] + row * M.b[2] + M.o[2];
+         in[k][sd] = sd < nS && p < nP && x >= 0 && y >= 0 && z >= 0 && x + y + z <= w - 1;
+         v[k][sd]  = in[k][sd] ? u[g.offC + (int64_t) M.cell * g.cS + cellRel( w, t0, x, y, z )] : T( 0 );
       }
    }
+#pragma unroll
+   for ( int k = 0; k < kFaceGhostBatch; k++ )
+#pragma unroll
+      for ( int sd = 0; sd < 2; sd++ )
+         if ( in[k][sd] ) gh[( 2 * (int64_t) G.face + sd ) * g.fS + p0 + 256 * k] = v[k][sd];
 }
 
 template < typename T >
@@ -2752,7 +2766,9 @@ void sorT( Operator& op, Function& xf, const Function& bf, double omega, int lev
             // copies of the just-updated edges are refreshed first; backward, the edges are not updated yet.
             if ( !backward ) refreshT< T, NR >( xf, level, 2 );
             T* gh = (T*) M.faceGhost( P );
-            hipLaunchKernelGGL( ( kFaceGhost< T > ), dim3( P.w - 1, nF ), dim3( 128 ), 0, s, g, P.dFaceGeom.p, u, gh );
+            hipLaunchKernelGGL( ( kFaceGhost< T > ),
+                                dim3( ( P.w * ( P.w + 1 ) / 2 + 256 * kFaceGhostBatch - 1 ) / ( 256 * kFaceGhostBatch ), nF ),
+                                dim3( 256 ), 0, s, g, P.dFaceGeom.p, u, gh );
             for ( int cc = 0; cc < 3; cc++ )
                hipLaunchKernelGGL( ( kFaceSorRows< T > ), dim3( P.w - 3, nF ), dim3( 64 ), 0, s, g, P.dFaceGeom.p,
                                    wsel< T >( Wt.face64, Wt.face32 ), flag, u, b, om, backward ? 2 - cc : cc,
