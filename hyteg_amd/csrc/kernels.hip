@@ -1621,9 +1621,9 @@ __global__ __launch_bounds__( 64 ) void kVertexSorPush( LV g, const VertexGeom* 
 // one workgroup per edge; the edge's neighbour terms and weights, its own values and its right-hand side are
 // staged in LDS once, so the per-point sums issue independent value loads and the colour updates issue none
 constexpr int kEdgeTermBatch = 16;
-constexpr int kEdgeSliceMax  = 128; // slices x points of a short edge (<= blockDim.x)
+constexpr int kEdgeThreads   = 256; // threads per edge workgroup; also the slices x points of a short edge
 template < typename T >
-__global__ __launch_bounds__( 128 ) void kEdgeSorFused( LV g, const EdgeGeom* __restrict__ eg,
+__global__ __launch_bounds__( kEdgeThreads ) void kEdgeSorFused( LV g, const EdgeGeom* __restrict__ eg,
                                                         const EdgeTerm* __restrict__ terms, const T* __restrict__ EW,
                                                         int flag, T* __restrict__ u, const T* __restrict__ b, T omega,
                                                         int reverse )
@@ -1670,7 +1670,7 @@ __global__ __launch_bounds__( 128 ) void kEdgeSorFused( LV g, const EdgeGeom* __
    while ( S < 8 && 2 * S * np <= (int) blockDim.x )
       S *= 2;
    const int per = ( nT + S - 1 ) / S;
-   __shared__ T part[kEdgeSliceMax];
+   __shared__ T part[kEdgeThreads];
    for ( int t = (int) threadIdx.x; t < np * S; t += blockDim.x )
    {
       const int x = 1 + t % np, sl = t / np, kb = sl * per, ke = min( nT, kb + per );
@@ -2752,7 +2752,7 @@ void sorT( Operator& op, Function& xf, const Function& bf, double omega, int lev
       };
       auto edgePhase = [&]() {
          if ( P.w > kEdgeMaxW ) throw std::invalid_argument( "the fused edge SOR supports levels up to 10" );
-         hipLaunchKernelGGL( ( kEdgeSorFused< T > ), dim3( nE ), dim3( 128 ), 0, s, g, P.dEdgeGeom.p, P.dEdgeTerms.p,
+         hipLaunchKernelGGL( ( kEdgeSorFused< T > ), dim3( nE ), dim3( kEdgeThreads ), 0, s, g, P.dEdgeGeom.p, P.dEdgeTerms.p,
                              wsel< T >( Wt.edge64, Wt.edge32 ), flag, u, b, om, rev );
       };
       auto facePhase = [&]() {
