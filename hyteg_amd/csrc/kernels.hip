@@ -1402,12 +1402,25 @@ __global__ void kFaceSor( LV g, const FaceGeom* __restrict__ fg, const T* __rest
 // triRow32, and all loads of a thread (both sides) are issued before its stores.  The cell reads are
 // scattered for two of the four cell face planes (one point per cell row); gathering them once per sweep
 // replaces scattered reads in each colour pass.
+// With WITH_E2F, the last block of each face instead copies the face's three edges into its boundary (as
+// kE2F; the two parts touch disjoint data: cell points are read, face boundary points written).
 constexpr int kFaceGhostBatch = 2;
-template < typename T >
-__global__ __launch_bounds__( 256 ) void kFaceGhost( LV g, const FaceGeom* __restrict__ fg, const T* __restrict__ u,
-                                                     T* __restrict__ gh )
+template < typename T, bool WITH_E2F >
+__global__ __launch_bounds__( 256 ) void kFaceGhost( LV g, const FaceGeom* __restrict__ fg, const E2F* __restrict__ e2f,
+                                                     T* __restrict__ u, T* __restrict__ gh )
 {
    const FaceGeom& G  = fg[blockIdx.y];
+   if ( WITH_E2F && blockIdx.x == gridDim.x - 1 )
+   {
+      const int64_t fo = g.offF + (int64_t) G.face * g.fS;
+      for ( int le = 0; le < 3; le++ )
+      {
+         const E2F m = e2f[3 * G.face + le];
+         for ( int p = threadIdx.x; p < g.w; p += blockDim.x )
+            u[fo + faceRel( g.w, m.base[0] + p * m.inc[0], m.base[1] + p * m.inc[1] )] = u[g.offE + (int64_t) m.edge * g.eS + p];
+      }
+      return;
+   }
    const int       w  = g.w, t0 = tet32( w ), nP = ( w * ( w + 1 ) >> 1 ) - 1, nS = G.nSides; // rows 0 .. w-2
    const int       p0 = (int) blockIdx.x * 256 * kFaceGhostBatch + (int) threadIdx.x;
    T               v[kFaceGhostBatch][2];
@@ -2764,11 +2777,15 @@ void sorT( Operator& op, Function& xf, const Function& bf, double omega, int lev
          {
             // large faces: one workgroup per face would leave most of the chip idle.  Forward, the faces'
             // copies of the just-updated edges are refreshed first; backward, the edges are not updated yet.
-            if ( !backward ) refreshT< T, NR >( xf, level, 2 );
-            T* gh = (T*) M.faceGhost( P );
-            hipLaunchKernelGGL( ( kFaceGhost< T > ),
-                                dim3( ( P.w * ( P.w + 1 ) / 2 + 256 * kFaceGhostBatch - 1 ) / ( 256 * kFaceGhostBatch ), nF ),
-                                dim3( 256 ), 0, s, g, P.dFaceGeom.p, u, gh );
+            // (forward: the edge-to-face copy runs as one more block per face of the ghost launch)
+            T*             gh     = (T*) M.faceGhost( P );
+            const unsigned chunks = ( P.w * ( P.w + 1 ) / 2 + 256 * kFaceGhostBatch - 1 ) / ( 256 * kFaceGhostBatch );
+            if ( !backward )
+               hipLaunchKernelGGL( ( kFaceGhost< T, true > ), dim3( chunks + 1, nF ), dim3( 256 ), 0, s, g, P.dFaceGeom.p,
+                                   P.dE2F.p, u, gh );
+            else
+               hipLaunchKernelGGL( ( kFaceGhost< T, false > ), dim3( chunks, nF ), dim3( 256 ), 0, s, g, P.dFaceGeom.p,
+                                   P.dE2F.p, u, gh );
             for ( int cc = 0; cc < 3; cc++ )
                hipLaunchKernelGGL( ( kFaceSorRows< T > ), dim3( P.w - 3, nF ), dim3( 64 ), 0, s, g, P.dFaceGeom.p,
                                    wsel< T >( Wt.face64, Wt.face32 ), flag, u, b, om, backward ? 2 - cc : cc,
