@@ -22,7 +22,8 @@ from .dataexport import DataFormat, FixedSizeSQLDB, VTKOutput
 
 __all__ = ["Context", "context", "MeshInfo", "PrimitiveStorage", "P1Function", "P1ConstantLaplaceOperator",
            "P1ConstantMassOperator", "P1toP1LinearRestriction", "P1toP1LinearProlongation", "SORSmoother",
-           "CGSolver", "GeometricMultigridSolver", "HymiError", "INNER", "DIRICHLET", "NEUMANN", "FREESLIP", "ALL",
+           "CGSolver", "GeometricMultigridSolver", "GaussSeidelSmoother", "SymmetricSORSmoother", "CycleType",
+           "FullMultigridSolver", "MixedPrecisionRefinementSolver", "HymiError", "INNER", "DIRICHLET", "NEUMANN", "FREESLIP", "ALL",
            "REPLACE", "ADD", "F64", "F32", "VERTEX", "EDGE", "FACE", "CELL", "DataFormat", "VTKOutput",
            "FixedSizeSQLDB"]
 
@@ -321,7 +322,10 @@ class _P1ConstantOperator:
         _lib.call("hymi_smooth_sor_backwards", self.h, dst.h, rhs.h, float(relax), level, flag)
 
     def smooth_gs(self, dst, rhs, level, flag):
-        self.smooth_sor(dst, rhs, 1.0, level, flag)
+        _lib.call("hymi_smooth_gs", self.h, dst.h, rhs.h, level, flag)
+
+    def smooth_gs_backwards(self, dst, rhs, level, flag):
+        _lib.call("hymi_smooth_gs_backwards", self.h, dst.h, rhs.h, level, flag)
 
 
 class P1ConstantLaplaceOperator(_P1ConstantOperator):
@@ -338,6 +342,10 @@ class P1toP1LinearRestriction:
 
 
 class P1toP1LinearProlongation:
+    def prolongate(self, function, sourceLevel, flag):
+        """UpdateType Replace: the flag DoFs of sourceLevel + 1 become the prolongation of sourceLevel."""
+        _lib.call("hymi_prolongate", function.h, sourceLevel, flag)
+
     def prolongateAndAdd(self, function, sourceLevel, flag):
         _lib.call("hymi_prolongate_add", function.h, sourceLevel, flag)
 
@@ -350,6 +358,16 @@ class SORSmoother:
 
     def solve(self, A, x, b, level):
         A.smooth_sor(x, b, self.relax, level, INNER | NEUMANN)
+
+
+class GaussSeidelSmoother(SORSmoother):
+    """GaussSeidelSmoother (src/hyteg/solvers/GaussSeidelSmoother.hpp:38-50): smooth_gs, i.e. SOR with relax 1."""
+
+    def __init__(self):
+        super().__init__(1.0)
+
+    def solve(self, A, x, b, level):
+        A.smooth_gs(x, b, level, INNER | NEUMANN)
 
 
 class SymmetricSORSmoother(SORSmoother):
@@ -382,21 +400,28 @@ class CGSolver:
 
 class GeometricMultigridSolver:
     """GeometricMultigridSolver(storage, smoother, coarseSolver, restriction, prolongation, minLevel, maxLevel,
-    preSmoothSteps, postSmoothSteps, smoothIncrement, cycleType) -- V- or W-cycles with SOR or symmetric SOR
-    smoothing and a device CG coarse solver (GeometricMultigridSolver.hpp:42-235; smoothIncrement must be 0)."""
+    preSmoothSteps, postSmoothSteps, smoothIncrementOnCoarserGrids, cycleType) -- V- or W-cycles with SOR,
+    Gauss-Seidel or symmetric SOR smoothing and a device CG coarse solver (GeometricMultigridSolver.hpp:42-235)."""
 
     def __init__(self, storage, smoother, coarseSolver, restrictionOperator, prolongationOperator, minLevel,
                  maxLevel, preSmoothSteps=3, postSmoothSteps=3, smoothIncrement=0, cycleType=CycleType.VCYCLE):
-        if smoothIncrement != 0:
-            raise ValueError("smoothIncrement != 0 is not supported")
+        if smoothIncrement < 0:
+            raise ValueError("smoothIncrement must be >= 0")
         if cycleType not in (CycleType.VCYCLE, CycleType.WCYCLE):
             raise ValueError("cycleType must be CycleType.VCYCLE or CycleType.WCYCLE")
         self.storage = storage
         self.smoother, self.coarse = smoother, coarseSolver
         self.minLevel, self.maxLevel = minLevel, maxLevel
         self.nu1, self.nu2 = preSmoothSteps, postSmoothSteps
+        self.smoothIncrement = smoothIncrement
         self.cycleType = cycleType
         self._h = {}
+
+    def setSmoothingSteps(self, preSmoothingSteps, postSmoothingSteps, smoothIncrement=0):
+        """GeometricMultigridSolver::setSmoothingSteps (GeometricMultigridSolver.hpp:120-125)."""
+        self.nu1, self.nu2, self.smoothIncrement = preSmoothingSteps, postSmoothingSteps, smoothIncrement
+        for h, _ in self._h.values():
+            _lib.call("hymi_gmg_set_smoothing", h, self.nu1, self.nu2, self.smoothIncrement)
 
     def _handle(self, A, x):
         key = (id(A), x.precision, x.nrhs)
@@ -407,6 +432,8 @@ class GeometricMultigridSolver:
             _lib.call("hymi_gmg_create", A.h, x.h, self.minLevel, self.maxLevel, self.nu1, self.nu2,
                       float(self.smoother.relax), int(cg_it), float(cg_tol), ctypes.byref(h))
             self._h[key] = (h, A)
+            if self.smoothIncrement:
+                _lib.call("hymi_gmg_set_smoothing", h, self.nu1, self.nu2, self.smoothIncrement)
             if self.cycleType != CycleType.VCYCLE or getattr(self.smoother, "symmetric", False):
                 _lib.call("hymi_gmg_set_variant", h, int(self.cycleType),
                           int(getattr(self.smoother, "symmetric", False)))
@@ -418,6 +445,13 @@ class GeometricMultigridSolver:
     def solve_graph(self, A, x, b, level, count=1):
         """Replay a hipGraph-captured V-cycle `count` times."""
         _lib.call("hymi_gmg_vcycle_graph", self._handle(A, x), x.h, b.h, level, count)
+
+    def __del__(self, _releasable=_releasable):
+        if not _releasable():
+            return
+        for h, _ in getattr(self, "_h", {}).values():
+            _lib._lib.hymi_gmg_destroy(h)
+        self._h = {}
 
 
 class FullMultigridSolver:
@@ -436,13 +470,6 @@ class FullMultigridSolver:
 
     def solve(self, A, x, b, level):
         _lib.call("hymi_fmg_solve", self.gmg._handle(A, x), x.h, b.h, level, self.cyclesPerLevel)
-
-    def __del__(self, _releasable=_releasable):
-        if not _releasable():
-            return
-        for h, _ in getattr(self, "_h", {}).values():
-            _lib._lib.hymi_gmg_destroy(h)
-        self._h = {}
 
 
 class MixedPrecisionRefinementSolver:

@@ -88,6 +88,10 @@ SIGNATURES = {
     "hymi_smooth_sor_backwards": [_vp, _vp, _vp, _d, _i, _i],
     "hymi_restrict": [_vp, _i, _i],
     "hymi_prolongate_add": [_vp, _i, _i],
+    "hymi_prolongate": [_vp, _i, _i],
+    "hymi_smooth_gs": [_vp, _vp, _vp, _i, _i],
+    "hymi_smooth_gs_backwards": [_vp, _vp, _vp, _i, _i],
+    "hymi_gmg_set_smoothing": [_vp, _i, _i, _i],
     "hymi_cg_solve": [_vp, _vp, _vp, _i, _i, _d],
     "hymi_gmg_create": [_vp, _vp, _i, _i, _i, _i, _d, _i, _d, _P(_vp)],
     "hymi_gmg_destroy": [_vp],

@@ -118,40 +118,46 @@ std::vector< int64_t > ownedIndex( Mesh& M, int level )
    return idx;
 }
 
-std::vector< double > downloadLevel( Function& f, int level )
+// elements [off, off + len) of one level (every element holds nrhs values), as doubles
+std::vector< double > downloadRange( Function& f, int level, int64_t off, int64_t len )
 {
    Mesh&                 M = *f.mesh;
-   LevelPlan&            P = M.plan( level );
-   const int64_t         n = P.total * f.nrhs;
+   const int64_t         n = len * f.nrhs, o = off * f.nrhs;
    std::vector< double > out( (size_t) n );
    HYMI_HIP( hipStreamSynchronize( M.ctx->stream ) );
    HYMI_LEGACY_LOCK;
    if ( f.prec == kF64 )
-      HYMI_HIP( hipMemcpy( out.data(), f.ptr( level ), (size_t) n * 8, hipMemcpyDeviceToHost ) );
+      HYMI_HIP( hipMemcpy( out.data(), (double*) f.ptr( level ) + o, (size_t) n * 8, hipMemcpyDeviceToHost ) );
    else
    {
       std::vector< float > tmp( (size_t) n );
-      HYMI_HIP( hipMemcpy( tmp.data(), f.ptr( level ), (size_t) n * 4, hipMemcpyDeviceToHost ) );
+      HYMI_HIP( hipMemcpy( tmp.data(), (float*) f.ptr( level ) + o, (size_t) n * 4, hipMemcpyDeviceToHost ) );
       std::copy( tmp.begin(), tmp.end(), out.begin() );
    }
    return out;
 }
 
-void uploadLevel( Function& f, int level, const std::vector< double >& in )
+void uploadRange( Function& f, int level, int64_t off, const std::vector< double >& in )
 {
    Mesh&         M = *f.mesh;
-   LevelPlan&    P = M.plan( level );
-   const int64_t n = P.total * f.nrhs;
+   const int64_t n = (int64_t) in.size(), o = off * f.nrhs;
    HYMI_HIP( hipStreamSynchronize( M.ctx->stream ) );
    HYMI_LEGACY_LOCK;
    if ( f.prec == kF64 )
-      HYMI_HIP( hipMemcpy( f.ptr( level ), in.data(), (size_t) n * 8, hipMemcpyHostToDevice ) );
+      HYMI_HIP( hipMemcpy( (double*) f.ptr( level ) + o, in.data(), (size_t) n * 8, hipMemcpyHostToDevice ) );
    else
    {
       std::vector< float > tmp( in.begin(), in.end() );
-      HYMI_HIP( hipMemcpy( f.ptr( level ), tmp.data(), (size_t) n * 4, hipMemcpyHostToDevice ) );
+      HYMI_HIP( hipMemcpy( (float*) f.ptr( level ) + o, tmp.data(), (size_t) n * 4, hipMemcpyHostToDevice ) );
    }
 }
+
+std::vector< double > downloadLevel( Function& f, int level )
+{
+   return downloadRange( f, level, 0, f.mesh->plan( level ).total );
+}
+
+void uploadLevel( Function& f, int level, const std::vector< double >& in ) { uploadRange( f, level, 0, in ); }
 
 int64_t numCoarseUnknowns( Mesh& M, int level, int flag )
 {
@@ -262,6 +268,7 @@ struct hymi_gmg_s
 {
    Operator*                   A = nullptr;
    int                         minL = 0, maxL = 0, nu1 = 3, nu2 = 3, prec = kF64, nrhs = 1;
+   int                         smoothInc = 0; // extra pre- and post-smoothing steps per level below the invoked one
    int                         wcycle = 0, symmetric = 0; // cycle type W; symmetric SOR smoothing steps
    double                      omega = 1.0, cgTol = 1e-16;
    int                         cgIters = 0;
@@ -347,8 +354,11 @@ struct hymi_gmg_s
       // ghost copies of x are current after a sweep and after the coarse-level reset; only the caller's x on
       // the finest level, x after prolongation and x after the coarse solver need a refresh
       const bool finest = ( L == maxL_invoked );
-      smooth( x, b, L, flag, nu1, finest );
-      launchApply( *A, x, *tmp, &b, L, flag, kResidual, nu1 == 0 && finest );
+      // smoothIncrementOnCoarserGrids (GeometricMultigridSolver.hpp:156,218): smoothInc more steps per level
+      // below the invoked one
+      const int  extra  = smoothInc * ( maxL_invoked - L );
+      smooth( x, b, L, flag, nu1 + extra, finest );
+      launchApply( *A, x, *tmp, &b, L, flag, kResidual, nu1 + extra == 0 && finest );
       // restriction written straight into b on the coarse level (== restrict + b.assign on flag DoFs)
       launchRestrict( *tmp, b, L, flag );
       launchLincomb( x, L - 1, kAll, 0, nullptr, nullptr, false, 0.0 );
@@ -357,7 +367,7 @@ struct hymi_gmg_s
       // result and with the same coarse right-hand side
       if ( wcycle ) recurse( x, b, L - 1 );
       launchProlongateAdd( x, L - 1, flag, L - 1 == minL );
-      smooth( x, b, L, flag, nu2, true );
+      smooth( x, b, L, flag, nu2 + extra, true );
    }
 
    // `count` smoothing steps: SOR sweeps, or (SymmetricSORSmoother.hpp:38-60) a forward and a backward sweep
@@ -882,34 +892,40 @@ int hymi_function_upload_primitive( hymi_function fh, int level, int cls, int id
       Function& f = fh->f;
       require( f.nrhs == 1, "per-primitive transfer needs nrhs == 1" );
       requireLevel( f, level );
-      Mesh&      M   = *f.mesh;
-      LevelPlan& P   = M.plan( level );
-      const int  w   = P.w;
-      auto       buf = downloadLevel( f, level );
+      Mesh&      M = *f.mesh;
+      LevelPlan& P = M.plan( level );
+      const int  w = P.w;
+      // only the primitive's own closure array is read back, patched at its owned points and written again
+      int64_t off = 0, len = 0;
       switch ( cls )
       {
-      case HYMI_VERTEX: buf[idx] = host[0]; break;
+      case HYMI_VERTEX: M.T.vx.at( idx ); off = P.vertexOff( idx ), len = 1; break;
+      case HYMI_EDGE: M.T.ev.at( idx ); off = P.edgeOff( idx ), len = w; break;
+      case HYMI_FACE: M.T.fv.at( idx ); off = P.faceOff( idx ), len = triSize( w ); break;
+      case HYMI_CELL: M.T.cv.at( idx ); off = P.cellOff( idx ), len = tetSize( w ); break;
+      default: throw std::invalid_argument( "bad primitive class" );
+      }
+      auto buf = downloadRange( f, level, off, len );
+      switch ( cls )
+      {
+      case HYMI_VERTEX: buf[0] = host[0]; break;
       case HYMI_EDGE:
-         M.T.ev.at( idx );
          for ( int x = 1; x <= w - 2; x++ )
-            buf[P.edgeOff( idx ) + x] = host[x];
+            buf[x] = host[x];
          break;
       case HYMI_FACE:
-         M.T.fv.at( idx );
          for ( int j = 1; j <= w - 3; j++ )
             for ( int i = 1; i + j <= w - 2; i++ )
-               buf[P.faceOff( idx ) + faceIndex( w, i, j )] = host[faceIndex( w, i, j )];
+               buf[faceIndex( w, i, j )] = host[faceIndex( w, i, j )];
          break;
       case HYMI_CELL:
-         M.T.cv.at( idx );
          for ( int z = 1; z <= w - 4; z++ )
             for ( int y = 1; y + z <= w - 3; y++ )
                for ( int x = 1; x + y + z <= w - 2; x++ )
-                  buf[P.cellOff( idx ) + cellIndex( w, x, y, z )] = host[cellIndex( w, x, y, z )];
+                  buf[cellIndex( w, x, y, z )] = host[cellIndex( w, x, y, z )];
          break;
-      default: throw std::invalid_argument( "bad primitive class" );
       }
-      uploadLevel( f, level, buf );
+      uploadRange( f, level, off, buf );
    } );
 }
 
@@ -1118,8 +1134,19 @@ int hymi_smooth_sor( hymi_operator op, hymi_function x, hymi_function b, double 
       checkOpFunc( op, b, level );
       require( x != b, "x and b must differ" );
       require( x->f.prec == b->f.prec && x->f.nrhs == b->f.nrhs, "functions must share precision and nrhs" );
+      op->op.mesh->prepareCellSor( op->op.mesh->plan( level ) );
       launchSor( op->op, x->f, b->f, omega, level, flag );
    } );
+}
+
+int hymi_smooth_gs( hymi_operator op, hymi_function x, hymi_function b, int level, int flag )
+{
+   return hymi_smooth_sor( op, x, b, 1.0, level, flag );
+}
+
+int hymi_smooth_gs_backwards( hymi_operator op, hymi_function x, hymi_function b, int level, int flag )
+{
+   return hymi_smooth_sor_backwards( op, x, b, 1.0, level, flag );
 }
 
 int hymi_smooth_sor_backwards( hymi_operator op, hymi_function x, hymi_function b, double omega, int level, int flag )
@@ -1130,6 +1157,7 @@ int hymi_smooth_sor_backwards( hymi_operator op, hymi_function x, hymi_function 
       require( x != b, "x and b must differ" );
       require( x->f.prec == b->f.prec && x->f.nrhs == b->f.nrhs, "functions must share precision and nrhs" );
       require( x->f.nrhs == 1, "backward SOR needs one right-hand side" );
+      op->op.mesh->prepareCellSor( op->op.mesh->plan( level ) );
       launchSor( op->op, x->f, b->f, omega, level, flag, true, true );
    } );
 }
@@ -1141,6 +1169,19 @@ int hymi_restrict( hymi_function f, int srcLevel, int flag )
       requireLevel( f->f, srcLevel );
       requireLevel( f->f, srcLevel - 1 );
       launchRestrict( f->f, f->f, srcLevel, flag );
+   } );
+}
+
+int hymi_prolongate( hymi_function f, int srcLevel, int flag )
+{
+   return guarded( [&] {
+      require( f != nullptr, "null function" );
+      requireLevel( f->f, srcLevel );
+      requireLevel( f->f, srcLevel + 1 );
+      // Replace: the fine flag DoFs become P c, all other fine values keep theirs (the gather form writes every
+      // flag DoF of the fine level once, so zeroing them first and adding is the replacement)
+      launchLincomb( f->f, srcLevel + 1, flag, 0, nullptr, nullptr, false, 0.0 );
+      launchProlongateAdd( f->f, srcLevel, flag );
    } );
 }
 
@@ -1206,6 +1247,7 @@ int hymi_gmg_create( hymi_operator op, hymi_function xt, int minL, int maxL, int
          M.closureRanges( P, kInner | kNeumann, &n );
          M.closureRanges( P, kInner | kNeumann | kFreeslip, &n );
          M.closureRanges( P, kAll, &n );
+         if ( g->nrhs == 1 ) M.prepareCellSor( P ); // the fused cell phase's buffers, if the level selects it
       }
       // the context's scalar slots and partial-sum buffer were sized for 32 right-hand sides when it was
       // created and are never reallocated (graphs of other solvers on this context hold their addresses)
@@ -1242,6 +1284,25 @@ int hymi_gmg_vcycle( hymi_gmg g, hymi_function x, hymi_function b, int level )
    return guarded( [&] {
       checkGmg( g, x, b, level );
       g->cycle( x->f, b->f, level );
+   } );
+}
+
+int hymi_gmg_set_smoothing( hymi_gmg g, int pre, int post, int increment )
+{
+   return guarded( [&] {
+      require( g, "null argument" );
+      require( pre >= 0 && post >= 0 && increment >= 0, "negative smoothing steps" );
+      if ( g->nu1 == pre && g->nu2 == post && g->smoothInc == increment ) return;
+      HYMI_HIP( hipStreamSynchronize( g->A->mesh->ctx->stream ) );
+      {
+         HYMI_LEGACY_LOCK;
+         for ( auto& kv : g->graphs ) // captured with the old step counts
+            (void) hipGraphExecDestroy( kv.second.exec );
+      }
+      g->graphs.clear();
+      g->nu1       = pre;
+      g->nu2       = post;
+      g->smoothInc = increment;
    } );
 }
 
@@ -1383,6 +1444,7 @@ int hymi_time_kernel( hymi_operator op, hymi_gmg gmg, hymi_function x, hymi_func
       Operator&   A = op->op;
       hipStream_t s = A.mesh->ctx->stream;
       const int   flag = kInner | kNeumann;
+      A.mesh->prepareCellSor( A.mesh->plan( level ) );
       auto        once = [&]() {
          switch ( which )
          {

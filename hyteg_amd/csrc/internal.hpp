@@ -172,6 +172,10 @@ struct Mesh
    // `precisions` (bit 0 FP64, bit 1 FP32), or the four colour passes (tile 0); bumps scheduleEpoch like
    // setSchedule
    void setCellSor( int level, int tile, int precisions = 1 );
+   // the selection alone (no allocation), and the allocation of what the selection needs (outside capture)
+   void selectCellSor( int level, int tile, int precisions );
+   void prepareCellSor( LevelPlan& P );
+   bool cellSorReady( const LevelPlan& P ) const;
    uint64_t scheduleEpoch = 0;
    // affine frames of the primitives, level-independent: micro-vertex (x, y, z) of a primitive lies at
    // o + (x a + y b + z c) / (w - 1); 12 doubles (o, a, b, c) per primitive, vertices, edges, faces, cells
@@ -216,5 +220,6 @@ struct Operator
 void buildPlan( Mesh& M, LevelPlan& P, int level );
 void buildOperatorWeights( Operator& op, int level );
 MarchSchedule defaultSchedule( int level );
+int           defaultCellSorPrecisions( int level );
 
 } // namespace hymi

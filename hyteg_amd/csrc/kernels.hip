@@ -1739,11 +1739,11 @@ __global__ __launch_bounds__( 256 ) void kFaceSorFused( LV g, const FaceGeom* __
                                                         const T* __restrict__ FW, int flag, T* __restrict__ u,
                                                         const T* __restrict__ b, T omega, int reverse )
 {
-   const FaceGeom& G = fg[blockIdx.x];
-   if ( !( G.type & flag ) ) return;
-   const int64_t fo = g.offF + (int64_t) G.face * g.fS;
+   const FaceGeom& G  = fg[blockIdx.x];
+   const int64_t   fo = g.offF + (int64_t) G.face * g.fS;
    // own boundary <- final edge values (the E2F table holds the three edges of face f at 3f .. 3f+2); the
-   // three edges' values are loaded before any is stored, so their loads are in flight together
+   // three edges' values are loaded before any is stored, so their loads are in flight together.  Every face
+   // copies, also one whose own points the flag leaves alone: its edges may have been updated.
    E2F m3[3];
    for ( int le = 0; le < 3; le++ )
       m3[le] = e2f[3 * G.face + le];
@@ -1755,6 +1755,7 @@ __global__ __launch_bounds__( 256 ) void kFaceSorFused( LV g, const FaceGeom* __
       for ( int le = 0; le < 3; le++ )
          u[fo + fidx( g.w, m3[le].base[0] + p * m3[le].inc[0], m3[le].base[1] + p * m3[le].inc[1] )] = ev[le];
    }
+   if ( !( G.type & flag ) ) return; // uniform over the workgroup: no barrier is skipped by part of it
    // The off-plane terms read cell points, which the face phase does not change: their sums are formed once
    // for all points (LDS), next to the edge copies above, and each colour pass adds only the in-plane terms.
    __shared__ T  sOff[kFaceFusedMaxNp];
@@ -2598,6 +2599,7 @@ template < typename T >
 void cellSorFused( Mesh& M, LevelPlan& P, const T* CW, int flag, T* u, const T* b, T omega, int reverse,
                    bool withBand = true, int chain = 0 )
 {
+   if ( !P.sorUnits ) throw std::logic_error( "fused cell SOR phase launched before its buffers were prepared" );
    if ( P.sorUnitCount == 0 ) return;
    hipStream_t s      = M.ctx->stream;
    LV          g      = viewOf( P );

@@ -430,6 +430,13 @@ LevelPlan& Mesh::plan( int level )
    buildPlan( *this, *P, level );
    auto& ref = *P;
    plans[level] = std::move( P );
+   // the fused cell phase is selected by default where it was measured faster; its buffers are allocated when
+   // a sweep or a solver first needs them (prepareCellSor)
+   if ( const int bits = defaultCellSorPrecisions( level ) )
+   {
+      ref.sorTile       = 24;
+      ref.sorPrecisions = bits;
+   }
    return ref;
 }
 
@@ -467,15 +474,30 @@ const CellSeg* Mesh::segments( LevelPlan& P, int len, int64_t* count )
 // waves are needed to fill the GPU.  The column march (kind 3) is faster from level 5 up (level 7 FP64
 // apply 81 us against 105 us) but is selected explicitly, not by default: a serving run of many concurrent
 // hierarchies with it as the default ended in a GPU fault whose cause has not been found.
+// Measured per-level defaults (MI355X, FP64 and FP32 alike): the column march from level 5 up, 16 slabs per
+// unit and 4 waves per workgroup from level 7, 8 and 4 on level 6, 4 and 2 on level 5, loads one slab ahead;
+// flat slab windows below, where a cell is too thin for columns to pay.
 MarchSchedule defaultSchedule( int level )
 {
    MarchSchedule S;
+   if ( level >= 5 )
+   {
+      S.kind  = 3;
+      S.rows  = level >= 7 ? 16 : level == 6 ? 8 : 4;
+      S.waves = level >= 6 ? 4 : 2;
+      S.block = 1;
+      return S;
+   }
    S.kind  = 1;
-   S.rows  = level >= 6 ? 2 : 1;
-   S.waves = level >= 6 ? 2 : 4;
-   S.block = level >= 6 ? 2 : 1;
+   S.rows  = 1;
+   S.waves = 4;
+   S.block = 1;
    return S;
 }
+
+// Measured default of the fused SOR cell phase (tile 24): FP64 from level 7 up, FP32 from level 8 up, where it
+// beats the four colour passes; returns the precision bits (0: colour passes)
+int defaultCellSorPrecisions( int level ) { return ( level >= 7 ? 1 : 0 ) | ( level >= 8 ? 2 : 0 ); }
 
 void checkSchedule( const MarchSchedule& S, int level )
 {
@@ -652,11 +674,18 @@ void Mesh::setSchedule( int level, const MarchSchedule& S )
 
 void Mesh::setCellSor( int level, int tile, int precisions )
 {
+   selectCellSor( level, tile, precisions );
+   prepareCellSor( plan( level ) );
+}
+
+void Mesh::selectCellSor( int level, int tile, int precisions )
+{
    if ( tile != 0 && !sorTileSizeOk( tile ) ) throw std::invalid_argument( "cell SOR tile must be 0 or 24" );
    if ( tile != 0 && ( precisions < 1 || precisions > 3 ) )
       throw std::invalid_argument( "cell SOR precisions: bit 0 FP64, bit 1 FP32, at least one" );
    if ( tile != 0 && level < 3 ) throw std::invalid_argument( "the fused cell SOR phase needs level >= 3" );
    LevelPlan& P = plan( level );
+   if ( P.sorTile == tile && P.sorPrecisions == ( tile ? precisions : 0 ) ) return;
    // captured graphs hold the old unit table and scratch array: they are captured again on their next replay,
    // and no queued work may still use the buffers replaced here
    scheduleEpoch++;
@@ -669,6 +698,18 @@ void Mesh::setCellSor( int level, int tile, int precisions )
    P.sorBand2.release();
    P.sorSlabStart.release();
    if ( tile == 0 ) return;
+   P.sorTile       = tile;
+   P.sorPrecisions = precisions;
+}
+
+bool Mesh::cellSorReady( const LevelPlan& P ) const { return P.sorTile == 0 || ctx->hostOnly || P.sorUnits != nullptr; }
+
+// Work units and scratch arrays of the fused SOR cell phase selected on P (nothing if the passes are selected or
+// they exist already).  Called outside graph capture: by the entry points that sweep and by solver creation.
+void Mesh::prepareCellSor( LevelPlan& P )
+{
+   if ( P.sorTile == 0 || P.sorUnits ) return;
+   const int tile = P.sorTile;
    std::vector< MarchUnit > v;
    const int                w = P.w;
    for ( int c = 0; c < T.nC(); c++ )
@@ -697,9 +738,7 @@ void Mesh::setCellSor( int level, int tile, int precisions )
       P.sorSlabStart.upload( ss, ctx->stream );
       HYMI_HIP( hipStreamSynchronize( ctx->stream ) );
    }
-   P.sorUnitCount  = (int64_t) v.size();
-   P.sorTile       = tile;
-   P.sorPrecisions = precisions;
+   P.sorUnitCount = (int64_t) v.size();
 }
 
 const double* Mesh::frames()

@@ -142,12 +142,20 @@ int hymi_smooth_sor( hymi_operator op, hymi_function x, hymi_function b, double 
  * sor_3D_macroface_P1_backwards.cpp): cells, faces, edges, vertices in turn, the colours of each primitive in
  * reverse; one right-hand side. */
 int hymi_smooth_sor_backwards( hymi_operator op, hymi_function x, hymi_function b, double omega, int level, int flag );
+/* GSSmoothable::smooth_gs / smooth_gs_backwards (src/hyteg/solvers/Smoothables.hpp:63,72), which P1Operator
+ * implements as smooth_sor / smooth_sor_backwards with relax = 1 (P1Operator.hpp:327-339) */
+int hymi_smooth_gs( hymi_operator op, hymi_function x, hymi_function b, int level, int flag );
+int hymi_smooth_gs_backwards( hymi_operator op, hymi_function x, hymi_function b, int level, int flag );
 
 /* ---- grid transfer ---------------------------------------------------------------------------------
  * P1toP1LinearRestriction::restrict (src/hyteg/gridtransferoperators/P1toP1LinearRestriction.cpp:163-245)
  * P1toP1LinearProlongation::prolongateAndAdd (P1toP1LinearProlongation.cpp:189-397, UpdateType Add) */
 int hymi_restrict( hymi_function f, int src_level, int flag );
 int hymi_prolongate_add( hymi_function f, int src_level, int flag );
+/* P1toP1LinearProlongation::prolongate (ProlongationOperator.hpp:33, P1toP1LinearProlongation.hpp:31-43,
+ * UpdateType Replace): the flag DoFs of level src_level + 1 become P times level src_level; other DoFs keep
+ * their values */
+int hymi_prolongate( hymi_function f, int src_level, int flag );
 
 /* ---- solvers ---------------------------------------------------------------------------------------
  * CGSolver::solve (src/hyteg/solvers/CGSolver.hpp:69-150) run entirely on the device: max_iter iterations
@@ -158,6 +166,11 @@ int hymi_cg_solve( hymi_operator op, hymi_function x, hymi_function b, int level
 int hymi_gmg_create( hymi_operator op, hymi_function x_template, int min_level, int max_level, int nu1, int nu2,
                      double omega, int cg_max_iter, double cg_tol, hymi_gmg* out );
 int hymi_gmg_destroy( hymi_gmg g );
+/* GeometricMultigridSolver::setSmoothingSteps(pre, post, smoothIncrement) (GeometricMultigridSolver.hpp:120-125;
+ * constructor argument smoothIncrementOnCoarserGrids, :51): a level k levels below the invoked one smooths
+ * pre + k * increment times before and post + k * increment times after its coarse correction.  Drops the
+ * solver's captured graphs. */
+int hymi_gmg_set_smoothing( hymi_gmg g, int pre, int post, int increment );
 /* one V-cycle: solveRecursively(A, x, b, level) */
 int hymi_gmg_vcycle( hymi_gmg g, hymi_function x, hymi_function b, int level );
 /* capture one V-cycle for (x, b, level) into a hipGraph and replay it `count` times */
@@ -200,7 +213,8 @@ int hymi_time_kernel( hymi_operator op, hymi_gmg gmg, hymi_function x, hymi_func
  * windows of 64 elements, loaded `block` windows at a time, block 1, 2, 4 or 8), kind 2 stages slab bands
  * in LDS (a workgroup marches a band of `rows` rows of one cell over `block` slabs, 1..4096), kind 3
  * marches a wave's 64 (x, y) positions up through `rows` slabs with loads issued `block` slabs ahead (block
- * 1, 2 or 4); waves per workgroup 1, 2 or 4; rows 1..4096.  kind -1 and zeros select the level default.  A schedule the level
+ * 1, 2 or 4); waves per workgroup 1, 2 or 4; rows 1..4096.  kind -1 and zeros select the level default (the
+ * column march with 16 / 8 / 4 slabs per unit on levels >= 7 / 6 / 5, flat windows below).  A schedule the level
  * cannot run (a kind-2 band too large for LDS) is refused.  Results do not depend on the schedule.  Setting a
  * schedule waits for the context stream; V-cycle graphs captured before it are captured again on their
  * next replay.  out = {rows, waves, block, kind}. */
@@ -209,6 +223,7 @@ int hymi_mesh_get_schedule( hymi_mesh mesh, int level, int out[4] );
 /* SOR cell phase of one level: tile = 0 runs the four colours as four passes; tile = 24 runs all four in
  * one launch over 24 x 24 xy tiles marched along z (levels >= 3, functions with one right-hand side whose
  * precision is in `precisions`: bit 0 FP64, bit 1 FP32; others keep the passes; same values bit for bit).
+ * The default is tile 24 for FP64 from level 7 up and for FP32 from level 8 up, the four passes elsewhere.
  * get returns { tile, precisions }.  No reference counterpart: the reference's smooth_sor runs the macro-cell
  * interior point by point (src/hyteg/p1functionspace/P1Operator.hpp:341-420) */
 int hymi_mesh_set_cell_sor( hymi_mesh mesh, int level, int tile, int precisions );

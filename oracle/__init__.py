@@ -76,6 +76,7 @@ def lib():
             "om_smooth_sor": (None, [vp, vp, vp, d, i, i, i]),
             "om_smooth_sor_backward": (None, [vp, vp, vp, d, i, i, i]),
             "om_gmg_set_variant": (None, [vp, i, i]),
+            "om_gmg_set_smoothing": (None, [vp, i, i, i]),
             "om_fmg_solve": (None, [vp, vp, vp, vp, i, i]),
             "om_restrict": (None, [vp, i, i]),
             "om_prolongate": (None, [vp, i, i, i]),
@@ -300,6 +301,10 @@ class GMG:
                  wcycle=False, symmetric=False):
         self.h = lib().om_gmg_create(mesh.h, minL, maxL, nu1, nu2, float(omega), order, cg_max_iter, cg_tol)
         lib().om_gmg_set_variant(self.h, int(wcycle), int(symmetric))
+
+    def set_smoothing(self, pre, post, increment=0):
+        """setSmoothingSteps(pre, post, smoothIncrement)."""
+        lib().om_gmg_set_smoothing(self.h, pre, post, increment)
 
     def fmg(self, A, x, b, level, cycles_per_level=1):
         """FullMultigridSolver: cycles on every level from the coarsest up, prolongation (Replace) between."""

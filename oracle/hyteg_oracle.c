@@ -1860,6 +1860,8 @@ struct om_gmg
    int      minL, maxL, nu1, nu2, order;
    int      wcycle;    /* GeometricMultigridSolver.hpp:201-204: the coarse correction is visited twice */
    int      symmetric; /* SymmetricSORSmoother.hpp:38-60: a forward then a backward sweep per step */
+   int      inc;       /* smoothIncrementOnCoarserGrids (GeometricMultigridSolver.hpp:51,156,218) */
+   int      invoked;   /* invokedLevel_ of the running solve (GeometricMultigridSolver.hpp:131) */
    double   omega;
 };
 om_gmg* om_gmg_create(om_mesh* m, int minL, int maxL, int nu1, int nu2, double omega, int order, int cgMaxIter,
@@ -1888,6 +1890,13 @@ void om_gmg_set_variant(om_gmg* g, int wcycle, int symmetric)
    g->wcycle    = wcycle;
    g->symmetric = symmetric;
 }
+/* GeometricMultigridSolver::setSmoothingSteps (GeometricMultigridSolver.hpp:120-125) */
+void om_gmg_set_smoothing(om_gmg* g, int pre, int post, int inc)
+{
+   g->nu1 = pre;
+   g->nu2 = post;
+   g->inc = inc;
+}
 static void gmg_smooth(om_gmg* g, const om_op* A, om_func* x, om_func* b, int L, int flag)
 {
    om_smooth_sor( A, x, b, g->omega, L, flag, g->order );
@@ -1901,7 +1910,9 @@ static void gmg_rec(om_gmg* g, const om_op* A, om_func* x, om_func* b, int L)
       om_cg_solve( g->cg, A, x, b, L );
       return;
    }
-   for ( int i = 0; i < g->nu1; i++ )
+   /* GeometricMultigridSolver.hpp:156: preSmoothSteps_ + smoothIncrement_ * (invokedLevel_ - level) */
+   const int extra = g->inc * ( g->invoked - L );
+   for ( int i = 0; i < g->nu1 + extra; i++ )
       gmg_smooth( g, A, x, b, L, flag );
    om_apply( A, x, g->tmp, L, flag, 0 );
    double   c2[2] = {1.0, -1.0};
@@ -1915,10 +1926,14 @@ static void gmg_rec(om_gmg* g, const om_op* A, om_func* x, om_func* b, int L)
    gmg_rec( g, A, x, b, L - 1 );
    if ( g->wcycle ) gmg_rec( g, A, x, b, L - 1 );
    om_prolongate( x, L - 1, flag, 1 );
-   for ( int i = 0; i < g->nu2; i++ )
+   for ( int i = 0; i < g->nu2 + extra; i++ ) /* :218 */
       gmg_smooth( g, A, x, b, L, flag );
 }
-void om_gmg_solve(om_gmg* g, const om_op* A, om_func* x, om_func* b, int L) { gmg_rec( g, A, x, b, L ); }
+void om_gmg_solve(om_gmg* g, const om_op* A, om_func* x, om_func* b, int L)
+{
+   g->invoked = L; /* GeometricMultigridSolver.hpp:131 */
+   gmg_rec( g, A, x, b, L );
+}
 
 /* FullMultigridSolver.hpp:41-68: from the coarsest level up, cyclesPerLevel cycles of g, then x on the next
  * level = P x (prolongate with Replace on Inner | Neumann DoFs); b must hold the right-hand side on every
@@ -1928,6 +1943,7 @@ void om_fmg_solve(om_gmg* g, const om_op* A, om_func* x, om_func* b, int L, int 
    const int flag = OM_INNER | OM_NEUMANN;
    for ( int cur = g->minL; cur <= L; cur++ )
    {
+      g->invoked = cur;
       for ( int k = 0; k < cyclesPerLevel; k++ )
          gmg_rec( g, A, x, b, cur );
       if ( cur < g->maxL ) om_prolongate( x, cur, flag, 0 );

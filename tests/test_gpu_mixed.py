@@ -144,7 +144,8 @@ def test_refined_split_residual_matches_two_pass(gpu_ctx):
         rel = R.solve(A, x, b, L)
         return x.download(L), R.outerSteps, np.array(R.history), float(np.max(rel))
 
-    x1, n1, h1, r1 = solve()                      # level default: two-pass residual
+    S.set_schedule(L, 1, 4, 1, 1)                 # flat windows on the finest level: two-pass residual
+    x1, n1, h1, r1 = solve()
     S.set_schedule(L, 4, 2, 1, 3)                 # column march on the finest level: split residual
     x2, n2, h2, r2 = solve()
     S.set_schedule(L)

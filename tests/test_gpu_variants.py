@@ -54,7 +54,7 @@ def test_backward_sor_column_schedule_level6(gpu_ctx, oracle_lib):
     ou.set(L, x)
     of.set(L, b)
     OA.smooth_sor_backward(ou, of, L, omega=1.0, flag=O.INNER, order=1)
-    for sched in ((0, 0, 0, -1), (8, 4, 1, 3)):
+    for sched in ((1, 4, 1, 1), (8, 4, 1, 3)):
         S.set_schedule(L, *sched)
         u, f = H.P1Function("u", S, 0, L), H.P1Function("f", S, 0, L)
         u.upload(L, x)

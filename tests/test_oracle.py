@@ -522,3 +522,44 @@ def test_fmg_reaches_discretisation_error(O):
         if lev < L:
             O.prolongate(z, lev, flag=O.INNER | O.NEUMANN, add=False)
     assert np.array_equal(z.get(L), x.get(L))
+
+
+def test_smooth_increment_composes_with_the_invoked_level(O):
+    """GeometricMultigridSolver.hpp:156,218: a level k levels below the invoked one smooths pre + k * increment
+    times before and post + k * increment after its coarse correction.  So a cycle invoked on L with (1, 1, +1)
+    is one level-L step (1 sweep, residual, restriction, coarse solve, prolongation, 1 sweep) around a cycle
+    invoked on L - 1 with (2, 2, +1).  The first solver is built one level deeper than it is invoked, so counting
+    from maxLevel instead of the invoked level fails this, as does any other count per level; increment 0 must
+    give the plain cycle."""
+    om, _ = mesh_pair(("cube", 2), host_ctx=_host())
+    L = 3
+    A, b, _ = _poisson(O, om, L + 1)
+    flag = O.INNER | O.NEUMANN
+    g1 = O.GMG(om, 0, L + 1, 1, 1, 1.0, 1)
+    g1.set_smoothing(1, 1, 1)
+    x = O.Function(om, 0, L + 1)
+    g1.solve(A, x, b, L)
+    # the same by hand
+    A2, b2, _ = _poisson(O, om, L + 1)
+    y, t = O.Function(om, 0, L + 1), O.Function(om, 0, L + 1)
+    A2.smooth_sor(y, b2, L, omega=1.0, flag=flag, order=1)
+    A2.apply(y, t, L, flag=flag)
+    t.assign(L, [1, -1], [b2, t], flag=flag)
+    O.restrict(t, L, flag=flag)
+    b2.assign(L - 1, [1.0], [t], flag=flag)
+    y.interpolate(L - 1, 0.0, O.ALL)
+    g2 = O.GMG(om, 0, L, 2, 2, 1.0, 1)
+    g2.set_smoothing(2, 2, 1)
+    g2.solve(A2, y, b2, L - 1)
+    O.prolongate(y, L - 1, flag=flag, add=True)
+    A2.smooth_sor(y, b2, L, omega=1.0, flag=flag, order=1)
+    assert np.array_equal(x.get(L), y.get(L))
+    # increment 0 is the plain cycle
+    A3, b3, _ = _poisson(O, om, L + 1)
+    g3, g4 = O.GMG(om, 0, L + 1, 2, 1, 1.0, 1), O.GMG(om, 0, L + 1, 5, 5, 1.0, 1)
+    g4.set_smoothing(2, 1, 0)
+    u, v = O.Function(om, 0, L + 1), O.Function(om, 0, L + 1)
+    g3.solve(A3, u, b3, L)
+    A4, b4, _ = _poisson(O, om, L + 1)
+    g4.solve(A4, v, b4, L)
+    assert np.array_equal(u.get(L), v.get(L))
