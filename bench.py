@@ -1,13 +1,19 @@
-"""Benchmark: GDoF/s per V-cycle of the matrix-free P1 geometric multigrid on one MI355X.
+"""Benchmark: GDoF/s per V-cycle of the matrix-free P1 geometric multigrid on one MI355X, FP32 V-cycles inside
+FP64 iterative refinement against plain FP64 V-cycles.
 
-Workload (default): unit cube meshed as 2x2x2 sub-cubes x 6 tetrahedra (48 macro-tets), refinement level 7
-(16,581,375 unknowns), 3D Poisson with f = 3 pi^2 sin(pi x) sin(pi y) sin(pi z) and zero Dirichlet
-boundary, FP64 V(3,3) cycles with multicolour SOR (omega = 1) and a device CG at level 0.  A "step" is one
-V-cycle, replayed from a captured hipGraph, with all data resident in HBM.
+Headline workload: unit cube meshed as 2x2x2 sub-cubes x 6 tetrahedra (48 macro-tets), refinement level 8
+(133,432,831 unknowns), 3D Poisson with b = M I_h f + U[-1, 1] * 1e-3 per DoF (f = 3 pi^2 sin(pi x) sin(pi y)
+sin(pi z), seeded noise), zero Dirichlet boundary, solved from x = 0 to relative residual 1e-10.  A "step" is
+one whole solve: FP32 V(2,2) cycles (multicolour SOR, omega = 1, device CG at level 0), several per FP64
+refinement step, all data resident in HBM and the cycles replayed from captured hipGraphs.  `value` = unknowns
+x FP32 V-cycles of the solve / solve time (GDoF/s per V-cycle); the same line gives the FP64 V(2,2) solve of
+the same system (FP64 GDoF/s per V-cycle) and the ratio of the two solve times.
 
-Prints one JSON line.  `roofline` covers the dominant kernel of the V-cycle, timed live with hipEvents on
-the library's stream; `roofline_apply` the 15-point stencil apply kernel; `cpu_baseline` times the CPU
-oracle (the reference's algorithm restated in C, OpenMP over macro-primitives) on a bounded sample.
+Prints one JSON line.  `roofline` covers the dominant kernel of the solve (the fused SOR cell phase on the
+finest level, weighted over its launches as a smoothing sequence runs them), timed live with hipEvents on the
+library's stream; `roofline_apply` the 15-point stencil apply.  `config2_vcycle` is the FP64 V(3,3) cycle at
+level 7 (16,581,375 unknowns).  `cpu_baseline` times the CPU oracle (the reference's algorithm restated in C,
+lexicographic SOR, OpenMP over macro-primitives) on a bounded sample of the headline workload.
 """
 import argparse
 import ctypes
@@ -23,6 +29,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "GDoF/s per V-cycle, mixed vs FP64 at rel. res 1e-10; serving p50/p99 latency"
 HBM_PEAK_GBS = 8000.0
+SEED = 20251202
 
 
 def interior_count(n, level):
@@ -46,34 +53,16 @@ KIND_NAMES = {0: "kCellMarch (row windows)", 1: "kCellFlat (flat slab windows)",
               3: "kCellColumn (columns marched along z)"}
 
 
-def traffic_from_profile(kernel, kernel_name):
-    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE), if a summary for the kernel
-    that runs exists (profiles/traffic_<kernel>.json names the kernel it was measured on)."""
-    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}.json")
+def traffic_from_profile(name, kernel_name):
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE, scaled by a plain copy), if a
+    summary profiles/traffic_<name>.json exists and names the kernel that runs."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
     if os.path.exists(path):
         with open(path) as fh:
             d = json.load(fh)
-        if d.get("kernel", "").split("<")[0] == kernel_name.split(" ")[0]:
+        if d.get("kernel", "").split("<")[0] == kernel_name.split(" ")[0].split("<")[0]:
             return d.get("bytes_per_launch")
     return None
-
-
-def cpu_baseline(n, level, cycles):
-    """The CPU oracle (reference algorithm, lexicographic SOR) on the same workload: `cycles` V(3,3)."""
-    import oracle as O
-    m = O.Mesh.cuboid(n)
-    A = O.Operator(m, 0, level)
-    M = O.Operator(m, 0, level, 1)
-    x, b, t = O.Function(m, 0, level), O.Function(m, 0, level), O.Function(m, 0, level)
-    X = m.owned_coords(level)
-    t.set(level, 3 * np.pi ** 2 * np.prod(np.sin(np.pi * X), axis=1))
-    M.apply(t, b, level, flag=O.ALL)
-    g = O.GMG(m, 0, level, 3, 3, 1.0, 0)
-    t0 = O.wtime()
-    for _ in range(cycles):
-        g.solve(A, x, b, level)
-    dt = (O.wtime() - t0) / cycles
-    return interior_count(n, level) / dt / 1e9, dt, O.num_threads()
 
 
 def sine_modes():
@@ -81,46 +70,45 @@ def sine_modes():
     return np.array([[3 * np.pi ** 2, 1, 1, 1]])
 
 
-def kernels_only(H, n, L, reps):
-    """Each measured kernel alone, `reps` launches: the macro-cell apply, one SOR colour pass, and a plain
-    copy of a whole level (the reference for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)."""
+def storage(H, n, tune_to=None):
+    """The n x n x n x 6 cube; with tune_to = L, launch schedules and the fused cell phase of levels 2..L picked
+    by the library's autotuner (hymi_mesh_autotune) for FP64 and FP32."""
     S = H.PrimitiveStorage(H.MeshInfo.meshCuboid((0, 0, 0), (1, 1, 1), n, n, n))
-    S.use_column_schedules(L)
-    A = H.P1ConstantLaplaceOperator(S, L, L)
-    x, b, y = (H.P1Function(k, S, L, L) for k in "xby")
-    x.interpolate_modes(sine_modes(), L)
-    b.interpolate(1.0, L, H.ALL)
-    t_apply = time_kernel(H, A, None, x, None, y, L, 0, reps)
-    t_sor = time_kernel(H, A, None, x, b, None, L, 1, reps)
-    t_copy = time_kernel(H, A, None, x, None, y, L, 5, reps)
-    out = {"apply_ms": t_apply, "sor_pass_ms": t_sor, "copy_ms": t_copy}
-    if S.get_cell_sor(L):
-        out["sor_tile_ms"] = time_kernel(H, A, None, x, b, None, L, 7, reps)
-        out["sor_cell_phase_ms"] = time_kernel(H, A, None, x, b, None, L, 6, reps)
-    copy_bytes = S.ctx.bytes_in_use() // 3       # one level of one function (x, b, y are alike)
-    out.update({"copy_bytes_each_way": copy_bytes, "copy_GBs": 2 * copy_bytes / (t_copy * 1e-3) / 1e9})
-    return out
+    if tune_to is not None:
+        S.tuning = {lv: S.autotune(lv, reps=5) for lv in range(2, tune_to + 1)}
+    return S
 
 
-def mixed_vs_fp64(H, n, L, tol, seed, cycles_per_outer):
-    """FP32 V(2,2) cycles inside FP64 iterative refinement against plain FP64 V(2,2) cycles, both from x = 0
-    to relative residual tol, on b = M I_h f + U[-1, 1] * 1e-3 per DoF."""
-    S = H.PrimitiveStorage(H.MeshInfo.meshCuboid((0, 0, 0), (1, 1, 1), n, n, n))
-    S.use_column_schedules(L)
-    A = H.P1ConstantLaplaceOperator(S, 0, L)
+def tuned_choices(S, L):
+    return {lv: dict(S.get_schedule(lv), cell_sor=list(S.get_cell_sor_setting(lv))) for lv in range(2, L + 1)}
+
+
+def gmg_solver(H, S, L, nu):
+    return H.GeometricMultigridSolver(S, H.SORSmoother(1.0), H.CGSolver(S, 0, 0), H.P1toP1LinearRestriction(),
+                                      H.P1toP1LinearProlongation(), 0, L, nu, nu)
+
+
+def noisy_sine_rhs(H, S, L, seed):
+    """b = M I_h f + U[-1, 1] * 1e-3 per DoF (the headline's right-hand side)."""
     M = H.P1ConstantMassOperator(S, 0, L)
-    x, b, t = (H.P1Function(k, S, 0, L) for k in ("x", "b", "t"))
+    b, t = H.P1Function("b", S, 0, L), H.P1Function("t", S, 0, L)
     t.interpolate_modes(sine_modes(), L)
     M.apply(t, b, L, H.ALL)
     bv = b.download(L)
     bv += 1e-3 * np.random.default_rng(seed).uniform(-1.0, 1.0, bv.shape)
     b.upload(L, bv)
-    del bv
-    N = interior_count(n, L)
+    return b, t
 
-    def gmg():
-        return H.GeometricMultigridSolver(S, H.SORSmoother(1.0), H.CGSolver(S, 0, 0), H.P1toP1LinearRestriction(),
-                                          H.P1toP1LinearProlongation(), 0, L, 2, 2)
+
+def headline(H, n, L, tol, seed, cpo, steps, warmup):
+    """Config 3: `steps` timed solves (after `warmup` untimed ones) from x = 0 to relative residual tol, FP32
+    V(2,2) cycles (cpo per FP64 refinement step) against FP64 V(2,2) cycles on the same system.  The FP64
+    solve replays its cycle count (counted once, untimed) as one graph launch sequence."""
+    S = storage(H, n, tune_to=L)
+    A = H.P1ConstantLaplaceOperator(S, 0, L)
+    b, t = noisy_sine_rhs(H, S, L, seed)
+    x = H.P1Function("x", S, 0, L)
+    N = interior_count(n, L)
 
     def rel_residual():
         A.residual(x, b, t, L, H.INNER)
@@ -128,51 +116,152 @@ def mixed_vs_fp64(H, n, L, tol, seed, cycles_per_outer):
 
     A.residual(x, b, t, L, H.INNER)
     r0 = float(np.sqrt(t.dotGlobal(t, L, H.INNER)))
-    # FP64: count the cycles to tol (untimed), then time that many from x = 0 as one graph replay
-    g64 = gmg()
+    g64 = gmg_solver(H, S, L, 2)
     cycles64 = 0
-    while cycles64 < 60:
+    while cycles64 < 80:
         g64.solve_graph(A, x, b, L, 1)
         cycles64 += 1
         if rel_residual() <= tol:
             break
     x64 = x.download(L)
-    x.interpolate(0.0, L, H.ALL)
-    S.ctx.synchronize()
-    t0 = time.perf_counter()
-    g64.solve_graph(A, x, b, L, cycles64)
-    S.ctx.synchronize()
-    t64 = time.perf_counter() - t0
-    # mixed: one untimed solve (captures graphs, allocates refinement work), then a timed one from x = 0
-    R = H.MixedPrecisionRefinementSolver(S, gmg(), tolerance=tol, maxOuter=60, cyclesPerOuter=cycles_per_outer)
-    x.interpolate(0.0, L, H.ALL)
-    R.solve(A, x, b, L)
-    x.interpolate(0.0, L, H.ALL)
-    S.ctx.synchronize()
-    t0 = time.perf_counter()
-    rel = R.solve(A, x, b, L)
-    S.ctx.synchronize()
-    tmix = time.perf_counter() - t0
-    cycles32 = R.outerSteps * cycles_per_outer
+    rel64 = rel_residual()
+
+    def fp64_solve():
+        x.interpolate(0.0, L, H.ALL)
+        g64.solve_graph(A, x, b, L, cycles64)
+
+    R = H.MixedPrecisionRefinementSolver(S, gmg_solver(H, S, L, 2), tolerance=tol, maxOuter=80, cyclesPerOuter=cpo)
+    rel = None
+
+    def mixed_solve():
+        nonlocal rel
+        x.interpolate(0.0, L, H.ALL)
+        rel = R.solve(A, x, b, L)
+
+    def timed(fn, k):
+        S.ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            fn()
+        S.ctx.synchronize()
+        return (time.perf_counter() - t0) / k
+
+    for _ in range(max(1, warmup)):
+        mixed_solve()
+        fp64_solve()
+    t64 = timed(fp64_solve, steps)
+    tmix = timed(mixed_solve, steps)
     xm = x.download(L)
+    rel_mixed_check = rel_residual()
+    cycles32 = R.outerSteps * cpo
     err = float(np.abs(xm - x64).max() / np.abs(x64).max())
-    return {
-        "workload": f"unit cube {n}x{n}x{n} x 6 tets, level {L}, {N} unknowns, b = M I_h f + U[-1,1]*1e-3 "
-                    f"(seed {seed}), from x = 0 to relative residual {tol:g}",
-        "fp64": {"cycle": "V(2,2)", "cycles": cycles64, "seconds": t64, "GDoF_per_s_per_vcycle": N * cycles64 / t64 / 1e9},
-        "mixed": {"cycle": f"FP32 V(2,2), {cycles_per_outer} per FP64 refinement step", "outer_steps": R.outerSteps,
-                  "fp32_cycles": cycles32, "seconds": tmix, "GDoF_per_s_per_vcycle": N * cycles32 / tmix / 1e9,
-                  "final_rel_residual": float(np.max(rel)), "history": R.history},
+    out = {
+        "N": N, "S": S, "A": A, "x": x, "b": b, "t": t,
+        "fp64": {"cycle": "V(2,2)", "cycles": cycles64, "seconds_per_solve": t64,
+                 "GDoF_per_s_per_vcycle": N * cycles64 / t64 / 1e9, "final_rel_residual": rel64},
+        "mixed": {"cycle": f"FP32 V(2,2), {cpo} per FP64 refinement step", "outer_steps": R.outerSteps,
+                  "fp32_cycles": cycles32, "seconds_per_solve": tmix,
+                  "GDoF_per_s_per_vcycle": N * cycles32 / tmix / 1e9,
+                  "final_rel_residual": float(np.max(rel)), "final_rel_residual_fp64_check": rel_mixed_check,
+                  "history": R.history},
         "speedup_time_to_tol": t64 / tmix,
         "error_vs_fp64_solution": err,
+        "tuned": tuned_choices(S, L),
     }
+    return out
+
+
+def cell_phase_roofline(H, S, A, L, prec, Ncell, reps):
+    """The SOR cell phase on level L in precision prec as the level runs it (fused launch or colour passes, as
+    selected), measured as two chained sweeps the way a V(2,2) smoothing sequence runs them; for the fused
+    phase also the tile kernel alone (unchained) and with its band copy (the first sweep of a sequence).
+    `achieved` = algorithmic bytes of the sequence (2 sweeps x 3 s per cell DoF: read x, b, write x) / its
+    measured time."""
+    s = 8 if prec == H.F64 else 4
+    tn = "double" if s == 8 else "float"
+    x, b = H.P1Function("rx", S, L, L, precision=prec), H.P1Function("rb", S, L, L, precision=prec)
+    x.interpolate_modes(sine_modes(), L)
+    b.interpolate(1e-3, L, H.ALL)
+    tile, bits = S.get_cell_sor_setting(L)
+    fused = bool(tile) and bool(bits & (1 if s == 8 else 2))
+    t_seq = time_kernel(H, A, None, x, b, None, L, 8, reps)
+    seq_bytes = 2 * 3 * s * Ncell
+    if fused:
+        kname = f"kCellSorTile<{tn}>"
+        what = f"{kname} + kCellSorBand<{tn}>: SOR cell phase, all four colours in one launch"
+    else:
+        kname = KIND_NAMES[S.get_schedule(L)["kind"]].split(" ")[0] + f"<{tn}>"
+        what = f"{kname}: SOR cell phase as four colour passes"
+    tag = f"cell_sor_{'tile' if fused else 'passes'}_{'f64' if s == 8 else 'f32'}_L{L}"
+    r = {"bound": "hbm", "achieved": seq_bytes / (t_seq * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+         "traffic": traffic_from_profile(tag, kname)}
+    r["frac"] = r["achieved"] / HBM_PEAK_GBS
+    r.update({"kernel": f"{what}, level {L}, two chained sweeps (one smoothing sequence of V(2,2))",
+              "algorithmic_bytes_per_sequence": seq_bytes, "ms_per_sequence": t_seq})
+    if fused:
+        t_tile = time_kernel(H, A, None, x, b, None, L, 7, reps)
+        t_phase = time_kernel(H, A, None, x, b, None, L, 6, reps)
+        r.update({"ms_tile_unchained": t_tile, "ms_band_copy": t_phase - t_tile, "ms_tile_chained": t_seq - t_phase,
+                  "tile_unchained_frac": seq_bytes / 2 / (t_tile * 1e-3) / 1e9 / HBM_PEAK_GBS})
+    del x, b
+    return r
+
+
+def apply_roofline(H, S, A, L, Ncell, reps):
+    """The macro-cell 15-point apply kernel (FP64) and a plain copy of the same level, both timed live."""
+    x, y = H.P1Function("ax", S, L, L), H.P1Function("ay", S, L, L)
+    x.interpolate_modes(sine_modes(), L)
+    t_apply = time_kernel(H, A, None, x, None, y, L, 0, reps)
+    t_copy = time_kernel(H, A, None, x, None, y, L, 5, reps)
+    sched = S.get_schedule(L)
+    kname = KIND_NAMES[sched["kind"]]
+    apply_bytes = 2 * 8 * Ncell
+    r = {"bound": "hbm", "achieved": apply_bytes / (t_apply * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+         "traffic": traffic_from_profile(f"cell_apply_L{L}", kname)}
+    r["frac"] = r["achieved"] / HBM_PEAK_GBS
+    r.update({"kernel": f"{kname}, 15-point apply, FP64, level {L}", "schedule": sched,
+              "algorithmic_bytes_per_launch": apply_bytes, "ms_per_launch": t_apply,
+              "plain_copy_ms": t_copy})
+    del x, y
+    return r
+
+
+def config2_vcycle(H, n, L, steps, warmup):
+    """Config 2: FP64 V(3,3) cycles at level L on the sine problem, replayed from a captured graph."""
+    S = storage(H, n, tune_to=L)
+    A = H.P1ConstantLaplaceOperator(S, 0, L)
+    M = H.P1ConstantMassOperator(S, 0, L)
+    x, b, t = (H.P1Function(k, S, 0, L) for k in ("u", "f", "tmp"))
+    t.interpolate_modes(sine_modes(), L)
+    M.apply(t, b, L, H.ALL)
+    gmg = gmg_solver(H, S, L, 3)
+    A.residual(x, b, t, L, H.INNER)
+    r0 = np.sqrt(t.dotGlobal(t, L, H.INNER))
+    gmg.solve_graph(A, x, b, L, max(1, warmup))
+    S.ctx.synchronize()
+    t0 = time.perf_counter()
+    gmg.solve_graph(A, x, b, L, steps)
+    S.ctx.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    A.residual(x, b, t, L, H.INNER)
+    N = interior_count(n, L)
+    Ncell = cell_interior_count(n, L)
+    t_vc = time_kernel(H, A, gmg._handle(A, x), x, b, None, L, 4, 3)
+    t_seq = time_kernel(H, A, None, x, b, None, L, 8, 5) if S.get_cell_sor(L) else None
+    return {"workload": f"unit cube {n}x{n}x{n} x 6 tets, level {L}, {N} unknowns, FP64 V(3,3) multicolour SOR "
+                        f"omega=1, CG at level 0, sine RHS, zero Dirichlet",
+            "ms_per_vcycle": ms, "GDoF_per_s_per_vcycle": N / (ms * 1e-3) / 1e9, "vcycle_ms_events": t_vc,
+            "rel_residual_after_cycles": float(np.sqrt(t.dotGlobal(t, L, H.INNER)) / r0),
+            "cycles": 1 + max(1, warmup) + steps, "tuned": tuned_choices(S, L),
+            "cell_phase_two_chained_sweeps_ms": t_seq,
+            "cell_phase_GBs": (2 * 3 * 8 * Ncell / (t_seq * 1e-3) / 1e9) if t_seq else None}
 
 
 def batched_rhs(H, n, L, nrhs, tol, seed, cycles_per_outer):
     """Config-4 batch: nrhs right-hand sides, each f_r a sum of 8 seeded modes (amplitude N(0,1), k, l, m in
     1..8), b_r = M I_h f_r, solved together in the interleaved [dof][rhs] layout with FP32 V(2,2) cycles in
     FP64 refinement to tol; against the same solves one right-hand side at a time."""
-    S = H.PrimitiveStorage(H.MeshInfo.meshCuboid((0, 0, 0), (1, 1, 1), n, n, n))
+    S = storage(H, n)
     A = H.P1ConstantLaplaceOperator(S, 0, L)
     M = H.P1ConstantMassOperator(S, 0, L)
     rng = np.random.default_rng(seed)
@@ -185,9 +274,8 @@ def batched_rhs(H, n, L, nrhs, tol, seed, cycles_per_outer):
     N = interior_count(n, L)
 
     def refine():
-        g = H.GeometricMultigridSolver(S, H.SORSmoother(1.0), H.CGSolver(S, 0, 0), H.P1toP1LinearRestriction(),
-                                       H.P1toP1LinearProlongation(), 0, L, 2, 2)
-        return H.MixedPrecisionRefinementSolver(S, g, tolerance=tol, maxOuter=60, cyclesPerOuter=cycles_per_outer)
+        return H.MixedPrecisionRefinementSolver(S, gmg_solver(H, S, L, 2), tolerance=tol, maxOuter=60,
+                                                cyclesPerOuter=cycles_per_outer)
 
     Rb = refine()
     for timed in (False, True):                      # the first solve captures graphs and allocates
@@ -198,7 +286,6 @@ def batched_rhs(H, n, L, nrhs, tol, seed, cycles_per_outer):
         S.ctx.synchronize()
         tb = time.perf_counter() - t0
     cycles_b = Rb.outerSteps * cycles_per_outer
-    # one right-hand side at a time
     x1, b1 = H.P1Function("x1", S, 0, L), H.P1Function("b1", S, 0, L)
     R1 = refine()
     t1, cycles_1 = 0.0, 0
@@ -244,24 +331,112 @@ def serving_run(count, rate, slots, seed):
     return out
 
 
+def cpu_baseline(n, L, cycles, apply_runs, seed):
+    """The CPU oracle (the reference's algorithm, lexicographic SOR, OpenMP over macro-primitives) on a bounded
+    sample of the headline workload: the stencil apply (best of apply_runs runs, each at least 1 s of
+    back-to-back applies after a warm-up, as ApplyBenchmark.cpp:78-94 times it) and `cycles` FP64 V(2,2)
+    cycles from x = 0 on the same right-hand side, the mean of cycles 2.. (MultigridStudies.cpp:243-255 logs
+    the per-cycle residual ratio, recorded here too)."""
+    import oracle as O
+    m = O.Mesh.cuboid(n)
+    A = O.Operator(m, 0, L)
+    M = O.Operator(m, 0, L, 1)
+    x, b, t = O.Function(m, 0, L), O.Function(m, 0, L), O.Function(m, 0, L)
+    X = m.owned_coords(L)
+    t.set(L, 3 * np.pi ** 2 * np.prod(np.sin(np.pi * X), axis=1))
+    M.apply(t, b, L, flag=O.ALL)
+    bv = b.get(L)
+    bv += 1e-3 * np.random.default_rng(seed).uniform(-1.0, 1.0, bv.shape)
+    b.set(L, bv)
+    del X, bv
+    # apply: y = A t on Inner DoFs
+    A.apply(t, x, L, flag=O.INNER)
+    best = None
+    for _ in range(apply_runs):
+        k, t0 = 0, O.wtime()
+        while True:
+            A.apply(t, x, L, flag=O.INNER)
+            k += 1
+            dt = O.wtime() - t0
+            if dt >= 1.0:
+                break
+        best = dt / k if best is None else min(best, dt / k)
+    x.set(L, np.zeros(len(t.get(L))))
+
+    def resnorm():
+        A.apply(x, t, L, flag=O.INNER)
+        r = b.get(L) - t.get(L)
+        inner = owned_inner
+        return float(np.sqrt(np.sum(r[inner] ** 2)))
+
+    owned_inner = m.owned_types(L) == O.INNER
+    bkeep = b.get(L)
+    g = O.GMG(m, 0, L, 2, 2, 1.0, 0)
+    res = [resnorm()]
+    times = []
+    for _ in range(cycles):
+        t0 = O.wtime()
+        g.solve(A, x, b, L)
+        times.append(O.wtime() - t0)
+        b.set(L, bkeep)
+        res.append(resnorm())
+    N = interior_count(n, L)
+    dt = float(np.mean(times[1:])) if cycles > 1 else times[0]
+    Ncell = cell_interior_count(n, L)
+    return {"value": N / dt / 1e9, "unit": "GDoF/s", "cores": O.num_threads(), "kind": "port",
+            "sample": f"the C oracle (lexicographic SOR) on the headline system: {cycles} FP64 V(2,2) cycles from "
+                      f"x = 0, value = unknowns / mean time of cycles 2..{cycles} ({dt:.2f} s per cycle); apply "
+                      f"best of {apply_runs} runs of >= 1 s",
+            "vcycle_seconds": times, "residual_ratio_per_cycle": [res[k + 1] / res[k] for k in range(cycles)],
+            "apply_ms": best * 1e3, "apply_GDoF_per_s": N / best / 1e9,
+            "apply_cell_GBs": 2 * 8 * Ncell / best / 1e9}
+
+
+def kernels_only(H, n, L, reps):
+    """Every measured kernel of the headline workload alone, launched eagerly (no graphs): for rocprofv3 kernel
+    traces and counter passes.  One V(2,2) cycle in FP32 and in FP64 (x = 0, b = the headline's right-hand
+    side), then `reps` launches each of the fused cell phase (two chained sweeps, FP32 and FP64), the FP64
+    macro-cell apply and a plain copy of the level (the reference for scaling FETCH_SIZE / WRITE_SIZE)."""
+    S = storage(H, n)
+    A = H.P1ConstantLaplaceOperator(S, 0, L)
+    b, t = noisy_sine_rhs(H, S, L, SEED)
+    out = {}
+    for prec, name in ((H.F32, "fp32"), (H.F64, "fp64")):
+        x, bb = H.P1Function("x", S, 0, L, precision=prec), H.P1Function("bb", S, 0, L, precision=prec)
+        H._lib.call("hymi_convert", bb.h, b.h, L, H.ALL)
+        g = gmg_solver(H, S, L, 2)
+        out[f"vcycle_{name}_ms"] = time_kernel(H, A, g._handle(A, x), x, bb, None, L, 4, 1)
+        del x, bb, g
+    Ncell = cell_interior_count(n, L)
+    out["cell_phase_fp32"] = cell_phase_roofline(H, S, A, L, H.F32, Ncell, reps)["ms_per_sequence"]
+    out["cell_phase_fp64"] = cell_phase_roofline(H, S, A, L, H.F64, Ncell, reps)["ms_per_sequence"]
+    ra = apply_roofline(H, S, A, L, Ncell, reps)
+    out.update({"apply_ms": ra["ms_per_launch"], "copy_ms": ra["plain_copy_ms"],
+                "copy_bytes_each_way": 8 * S.owned_count(L)})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--level", type=int, default=7)
+    ap.add_argument("--steps", type=int, default=5, help="timed solves of the headline workload")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--level", type=int, default=8)
     ap.add_argument("--n", type=int, default=2, help="sub-cubes per side")
-    ap.add_argument("--cpu-cycles", type=int, default=1)
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--cycles-per-step", type=int, default=4, help="FP32 V-cycles per FP64 refinement step")
+    ap.add_argument("--config2-level", type=int, default=7)
+    ap.add_argument("--config2-steps", type=int, default=20)
+    ap.add_argument("--cpu-cycles", type=int, default=4)
+    ap.add_argument("--cpu-apply-runs", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-mixed", action="store_true")
-    ap.add_argument("--mixed-level", type=int, default=8)
-    ap.add_argument("--mixed-cycles-per-step", type=int, default=4)
+    ap.add_argument("--no-config2", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-requests", type=int, default=200)
     ap.add_argument("--serving-slots", type=int, default=4)
     ap.add_argument("--kernels-only", action="store_true",
-                    help="run only the measured kernels (for rocprofv3 counter passes) and print their timings")
+                    help="launch the measured kernels eagerly (for rocprofv3) and print their timings")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -273,114 +448,62 @@ def main():
 
     n, L = args.n, args.level
     if args.kernels_only:
-        print(json.dumps(kernels_only(H, n, L, max(5, args.steps))))
+        print(json.dumps(kernels_only(H, n, L, max(3, args.steps))))
         return
-    S = H.PrimitiveStorage(H.MeshInfo.meshCuboid((0, 0, 0), (1, 1, 1), n, n, n))
-    S.use_column_schedules(L)
-    A = H.P1ConstantLaplaceOperator(S, 0, L)
-    M = H.P1ConstantMassOperator(S, 0, L)
-    x = H.P1Function("u", S, 0, L)
-    b = H.P1Function("f", S, 0, L)
-    t = H.P1Function("tmp", S, 0, L)
-    t.interpolate(lambda X: 3 * np.pi ** 2 * np.prod(np.sin(np.pi * X), axis=1), L)
-    M.apply(t, b, L, H.ALL)
-    gmg = H.GeometricMultigridSolver(S, H.SORSmoother(1.0), H.CGSolver(S, 0, 0), H.P1toP1LinearRestriction(),
-                                     H.P1toP1LinearProlongation(), 0, L, 3, 3)
-    gh = gmg._handle(A, x)
-
-    def resnorm():
-        A.residual(x, b, t, L, H.INNER)
-        return np.sqrt(t.dotGlobal(t, L, H.INNER))
-
-    r0 = resnorm()
-    gmg.solve_graph(A, x, b, L, args.warmup)
-    S.ctx.synchronize()
-    t0 = time.perf_counter()
-    gmg.solve_graph(A, x, b, L, args.steps)
-    S.ctx.synchronize()
-    t1 = time.perf_counter()
-    ms = (t1 - t0) * 1e3 / args.steps
-    rel_res = resnorm() / r0
-    N = interior_count(n, L)
-    gdofs = N / (ms / 1e3) / 1e9
-
-    # per-kernel timing (hipEvents on the library stream)
-    y = H.P1Function("y", S, 0, L)
+    hl = headline(H, n, L, args.tol, SEED, args.cycles_per_step, args.steps, args.warmup)
+    N = hl["N"]
     Ncell = cell_interior_count(n, L)
-    reps = max(5, min(50, args.steps))
-    t_apply = time_kernel(H, A, None, x, None, y, L, 0, reps)
-    t_sor = time_kernel(H, A, None, x, b, None, L, 1, reps)
-    t_vc = time_kernel(H, A, gh, x, b, None, L, 4, 3)
-    sched = S.get_schedule(L)
-    kname = KIND_NAMES[sched["kind"]]
-    s = 8
-    apply_bytes = 2 * s * Ncell                 # read x once, write y once
-    pass_bytes = 3 * s * Ncell / 4              # one colour pass = a quarter of a sweep's read x, b / write x
-    roof_pass = {"bound": "hbm", "achieved": pass_bytes / (t_sor * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
-                 "unit": "GB/s", "traffic": traffic_from_profile("cell_sor", kname)}
-    roof_pass["frac"] = roof_pass["achieved"] / HBM_PEAK_GBS
-    roof_pass = dict(roof_pass, kernel=f"{kname}, SOR, one colour pass", schedule=sched,
-                     algorithmic_bytes_per_launch=pass_bytes, ms_per_launch=t_sor)
-    if S.get_cell_sor(L):
-        # the fused SOR cell phase: all four colours in one tile launch, then the band copy
-        t_tile = time_kernel(H, A, None, x, b, None, L, 7, reps)
-        t_phase = time_kernel(H, A, None, x, b, None, L, 6, reps)
-        sor_bytes = 3 * s * Ncell               # a whole sweep of the cell interiors: read x, b, write x
-        roof_sor = {"bound": "hbm", "achieved": sor_bytes / (t_tile * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "traffic": traffic_from_profile("cell_sor_tile", "kCellSorTile")}
-        roof_sor["frac"] = roof_sor["achieved"] / HBM_PEAK_GBS
-        roof_sor = dict(roof_sor, kernel="kCellSorTile<double>, SOR cell phase, all four colours",
-                        tile=S.get_cell_sor(L), algorithmic_bytes_per_launch=sor_bytes, ms_per_launch=t_tile,
-                        share_of_vcycle=6 * t_tile / t_vc, band_copy_ms=t_phase - t_tile)
-    else:
-        roof_sor = dict(roof_pass, share_of_vcycle=6 * 4 * t_sor / t_vc)
-    roof_sor["frac"] = roof_sor["achieved"] / HBM_PEAK_GBS
-    roof_apply = {"bound": "hbm", "achieved": apply_bytes / (t_apply * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
-                  "unit": "GB/s", "traffic": traffic_from_profile("cell_apply", kname)}
-    roof_apply["frac"] = roof_apply["achieved"] / HBM_PEAK_GBS
+    S, A = hl["S"], hl["A"]
+    reps = 10
+    roof32 = cell_phase_roofline(H, S, A, L, H.F32, Ncell, reps)
+    roof64 = cell_phase_roofline(H, S, A, L, H.F64, Ncell, reps)
+    roof_apply = apply_roofline(H, S, A, L, Ncell, reps)
+    for k in ("x", "b", "t", "S", "A"):
+        hl.pop(k)
+    del S, A
 
+    mixed, fp64 = hl["mixed"], hl["fp64"]
     out = {
         "metric": METRIC,
-        "value": gdofs,
+        "value": mixed["GDoF_per_s_per_vcycle"],
         "unit": "GDoF/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms,
+        "ms_per_step": mixed["seconds_per_solve"] * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32 V-cycles in f64 refinement",
         "data": "synthetic",
         "config": {
-            "workload": f"unit cube {n}x{n}x{n} sub-cubes x 6 tets ({6 * n ** 3} macro-tets), level {L}, "
-                        f"{N} unknowns, FP64 V(3,3) multicolour SOR omega=1, CG at level 0, sine RHS, zero Dirichlet",
-            "unknowns": N,
-            "level": L,
-            "macro_tets": 6 * n ** 3,
-            "precision": "fp64",
-            "cycle": "V(3,3)",
-            "rel_residual_after_all_cycles": rel_res,
-            "cycles_total": 1 + args.warmup + args.steps + reps * 0,
+            "workload": f"unit cube {n}x{n}x{n} sub-cubes x 6 tets ({6 * n ** 3} macro-tets), level {L}, {N} "
+                        f"unknowns, b = M I_h f + U[-1,1]*1e-3 (seed {SEED}), zero Dirichlet, from x = 0 to relative "
+                        f"residual {args.tol:g}; one step = one solve: FP32 V(2,2) multicolour SOR omega=1 cycles, "
+                        f"{args.cycles_per_step} per FP64 refinement step, CG at level 0",
+            "unknowns": N, "level": L, "macro_tets": 6 * n ** 3,
+            "mixed_GDoF_per_s_per_vcycle": mixed["GDoF_per_s_per_vcycle"],
+            "fp64_GDoF_per_s_per_vcycle": fp64["GDoF_per_s_per_vcycle"],
+            "mixed_over_fp64_time_to_tol": hl["speedup_time_to_tol"],
+            "mixed_ms_per_solve": mixed["seconds_per_solve"] * 1e3,
+            "fp64_ms_per_solve": fp64["seconds_per_solve"] * 1e3,
+            "fp32_cycles": mixed["fp32_cycles"], "fp64_cycles": fp64["cycles"],
+            "error_vs_fp64_solution": hl["error_vs_fp64_solution"],
+            "final_rel_residual": mixed["final_rel_residual"],
         },
-        "roofline": roof_sor,
-        "roofline_colour_pass": roof_pass,
-        "roofline_apply": dict(roof_apply, kernel=f"{kname}, 15-point apply", schedule=sched,
-                               algorithmic_bytes_per_launch=apply_bytes, ms_per_launch=t_apply),
-        "vcycle_ms_events": t_vc,
+        "roofline": roof32,
+        "roofline_fp64": roof64,
+        "roofline_apply": roof_apply,
+        "mixed_precision": {k: v for k, v in hl.items() if k != "N"},
     }
-    if not args.no_mixed:
-        out["mixed_precision"] = mixed_vs_fp64(H, n, args.mixed_level, 1e-10, 20251202, args.mixed_cycles_per_step)
+    if not args.no_config2:
+        out["config2_vcycle"] = config2_vcycle(H, n, args.config2_level, args.config2_steps, 3)
     if not args.no_batched:
-        out["batched_rhs"] = batched_rhs(H, n, 6, 32, 1e-8, 20251202, args.mixed_cycles_per_step)
+        out["batched_rhs"] = batched_rhs(H, n, 6, 32, 1e-8, SEED, args.cycles_per_step)
     if not args.no_serving:
-        out["serving"] = serving_run(args.serving_requests, 200.0, args.serving_slots, 20251202)
+        out["serving"] = serving_run(args.serving_requests, 200.0, args.serving_slots, SEED)
     if not args.no_cpu:
-        cpu_level = min(L, 7)
-        v, dt, cores = cpu_baseline(n, cpu_level, args.cpu_cycles)
-        out["cpu_baseline"] = {"value": v, "unit": "GDoF/s", "cores": cores, "kind": "port",
-                               "sample": f"{args.cpu_cycles} V(3,3) cycle(s) of the C oracle (lexicographic SOR) on "
-                                         f"the same mesh at level {cpu_level}, {dt:.2f} s per cycle"}
+        out["cpu_baseline"] = cpu_baseline(n, L, args.cpu_cycles, args.cpu_apply_runs, SEED)
     print(json.dumps(out))
 
 

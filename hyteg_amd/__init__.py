@@ -168,6 +168,16 @@ class PrimitiveStorage:
             if fp64 or fp32:
                 self.set_cell_sor(L, 24, fp64=fp64, fp32=fp32)
 
+    def autotune(self, level, fp64=True, fp32=True, reps=5):
+        """Measure the launch schedules of the macro-cell kernels and the fused SOR cell phase on `level` and keep
+        the fastest (hymi_mesh_autotune; results are unchanged, bit for bit).  Returns the measured ms per
+        candidate: schedules first, then (colour passes, fused) per selected precision."""
+        report = np.zeros(64)
+        count = ctypes.c_int()
+        _lib.call("hymi_mesh_autotune", self.h, level, (1 if fp64 else 0) | (2 if fp32 else 0), reps, _dp(report),
+                  len(report), ctypes.byref(count))
+        return report[: count.value].tolist()
+
     def set_cell_sor(self, level, tile, fp64=True, fp32=False):
         """SOR cell phase on `level`: tile 0 = four colour passes, 24 = all four colours in one launch over
         24 x 24 tiles marched along z, for functions of the selected precisions with one right-hand side (same

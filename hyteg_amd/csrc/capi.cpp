@@ -8,6 +8,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -386,6 +387,29 @@ struct hymi_gmg_s
    }
 };
 
+// average duration in ms of `reps` calls of fn on stream s, bracketed by hipEvents recorded on s (one untimed call
+// first, which also builds any lazily created table)
+namespace {
+template < typename Fn >
+double timeLaunches( hipStream_t s, int reps, Fn&& fn )
+{
+   fn();
+   hipEvent_t e0, e1;
+   HYMI_HIP( hipEventCreate( &e0 ) );
+   HYMI_HIP( hipEventCreate( &e1 ) );
+   HYMI_HIP( hipEventRecord( e0, s ) );
+   for ( int k = 0; k < reps; k++ )
+      fn();
+   HYMI_HIP( hipEventRecord( e1, s ) );
+   HYMI_HIP( hipEventSynchronize( e1 ) );
+   float t = 0.f;
+   HYMI_HIP( hipEventElapsedTime( &t, e0, e1 ) );
+   (void) hipEventDestroy( e0 );
+   (void) hipEventDestroy( e1 );
+   return (double) t / reps;
+}
+} // namespace
+
 // ------------------------------------------------------------------------------------------------------
 extern "C" {
 
@@ -703,6 +727,8 @@ int hymi_function_create( hymi_mesh mesh, int minL, int maxL, int precision, int
       h->f.data  = std::move( F->data );
       h->f.count = std::move( F->count );
       F->data.clear();
+      for ( int L = h->f.minL; L <= h->f.maxL; L++ ) // so that hymi_dot_noalloc never builds a table
+         prepareDotTables( h->f, L );
       *out = h.release();
    } );
 }
@@ -1432,13 +1458,111 @@ int hymi_solve_refined( hymi_gmg g, hymi_function x, hymi_function b, int level,
    } );
 }
 
+int hymi_mesh_autotune( hymi_mesh mesh, int level, int precisions, int reps, double* report, int report_len,
+                        int* report_count )
+{
+   return guarded( [&] {
+      require( mesh && level >= 2 && level <= 12 && reps > 0 && precisions >= 1 && precisions <= 3, "bad argument" );
+      require( report_len >= 0 && ( report || report_len == 0 ), "bad report buffer" );
+      Mesh& M = mesh->m;
+      require( !M.ctx->hostOnly, "host-only context: tuning needs a device" );
+      HYMI_HIP( hipSetDevice( M.ctx->device ) );
+      hipStream_t   s = M.ctx->stream;
+      hymi_operator oh = nullptr;
+      if ( makeOperator( mesh, level, level, 0, &oh ) != HYMI_OK ) throw std::runtime_error( g_lastError );
+      std::unique_ptr< hymi_operator_s > op( oh );
+      Operator&                          A    = op->op;
+      const int                          flag = kInner | kNeumann;
+      std::vector< double >              rep;
+      auto scratchFunctions = [&]( int prec ) {
+         std::vector< std::unique_ptr< Function > > f;
+         for ( int k = 0; k < 3; k++ )
+            f.push_back( makeFunction( M, level, level, prec, 1 ) );
+         // finite values throughout: x = 1/2, b = 1, y = 0
+         launchLincomb( *f[0], level, kAll, 0, nullptr, nullptr, false, 0.5 );
+         launchLincomb( *f[1], level, kAll, 0, nullptr, nullptr, false, 1.0 );
+         launchLincomb( *f[2], level, kAll, 0, nullptr, nullptr, false, 0.0 );
+         return f;
+      };
+
+      // 1. launch schedule of the macro-cell kernels, measured on the apply, the residual and the four SOR colour
+      //    passes of the first selected precision (the schedule is shared by both precisions)
+      const int firstPrec = ( precisions & 1 ) ? kF64 : kF32;
+      auto      f         = scratchFunctions( firstPrec );
+      std::vector< MarchSchedule > cand;
+      if ( level >= 5 )
+      {
+         for ( int rows : { 4, 8, 16, 32 } )
+            for ( int waves : { 2, 4, 8 } )
+               for ( int block : { 1, 2 } )
+                  cand.push_back( MarchSchedule{ rows, waves, block, 3 } );
+      }
+      else
+         for ( int rows : { 1, 2, 4 } )
+            cand.push_back( MarchSchedule{ rows, 4, 1, 1 } );
+      double        best = -1.0;
+      MarchSchedule bestS{};
+      for ( const MarchSchedule& S : cand )
+      {
+         try
+         {
+            M.setSchedule( level, S );
+         } catch ( const std::invalid_argument& )
+         {
+            rep.push_back( -1.0 );
+            continue;
+         }
+         const double t = timeLaunches( s, reps, [&]() {
+            launchCellApplyOnly( A, *f[0], *f[2], level, flag );
+            launchApply( A, *f[0], *f[2], f[1].get(), level, flag, kResidual );
+            for ( int c = 0; c < 4; c++ )
+               launchCellSorOnly( A, *f[0], *f[1], 1.0, level, flag, c );
+         } );
+         rep.push_back( t );
+         if ( best < 0.0 || t < best )
+         {
+            best  = t;
+            bestS = S;
+         }
+      }
+      M.setSchedule( level, bestS );
+      f.clear();
+
+      // 2. the SOR cell phase per selected precision: four colour passes against the fused launch, measured as
+      //    the cell phases of two chained sweeps (one smoothing sequence of a V(2, 2) cycle)
+      int tileBits = M.plan( level ).sorTile ? M.plan( level ).sorPrecisions : 0;
+      if ( level >= 3 )
+         for ( int prec : { kF64, kF32 } )
+         {
+            const int bit = prec == kF64 ? 1 : 2;
+            if ( !( precisions & bit ) ) continue;
+            auto   g       = scratchFunctions( prec );
+            auto   timeSeq = [&]() {
+               return timeLaunches( s, reps, [&]() { launchCellSorSequence( A, *g[0], *g[1], 1.0, level, flag, 2 ); } );
+            };
+            M.setCellSor( level, 0, 0 );
+            const double tPass = timeSeq();
+            M.setCellSor( level, kSorTileSide, bit );
+            const double tFused = timeSeq();
+            rep.push_back( tPass );
+            rep.push_back( tFused );
+            tileBits = tFused < tPass ? ( tileBits | bit ) : ( tileBits & ~bit );
+         }
+      M.setCellSor( level, tileBits ? kSorTileSide : 0, tileBits );
+      HYMI_HIP( hipStreamSynchronize( s ) );
+      if ( report_count ) *report_count = (int) rep.size();
+      for ( int k = 0; k < (int) rep.size() && k < report_len; k++ )
+         report[k] = rep[k];
+   } );
+}
+
 int hymi_time_kernel( hymi_operator op, hymi_gmg gmg, hymi_function x, hymi_function b, hymi_function y, int level,
                       int which, int reps, double* ms )
 {
    return guarded( [&] {
       require( op && x && ms && reps > 0, "bad argument" );
       checkOpFunc( op, x, level );
-      if ( which == 1 || which == 3 || which == 4 || which == 6 || which == 7 ) checkOpFunc( op, b, level );
+      if ( which == 1 || which == 3 || which == 4 || which == 6 || which == 7 || which == 8 ) checkOpFunc( op, b, level );
       if ( which == 0 || which == 2 || which == 5 ) checkOpFunc( op, y, level );
       if ( which == 4 ) checkGmg( gmg, x, b, level );
       Operator&   A = op->op;
@@ -1455,6 +1579,7 @@ int hymi_time_kernel( hymi_operator op, hymi_gmg gmg, hymi_function x, hymi_func
          case 4: gmg->cycle( x->f, b->f, level ); break;
          case 6: launchCellSorPhase( A, x->f, b->f, 1.0, level, flag, false ); break;
          case 7: launchCellSorPhase( A, x->f, b->f, 1.0, level, flag, true ); break;
+         case 8: launchCellSorSequence( A, x->f, b->f, 1.0, level, flag, 2 ); break;
          case 5:
          {
             // plain copy y = x of the whole level (closures of all primitives): a bandwidth reference
@@ -1466,20 +1591,7 @@ int hymi_time_kernel( hymi_operator op, hymi_gmg gmg, hymi_function x, hymi_func
          default: throw std::invalid_argument( "bad kernel selector" );
          }
       };
-      once(); // warm-up (also builds any lazily created table)
-      hipEvent_t e0, e1;
-      HYMI_HIP( hipEventCreate( &e0 ) );
-      HYMI_HIP( hipEventCreate( &e1 ) );
-      HYMI_HIP( hipEventRecord( e0, s ) );
-      for ( int k = 0; k < reps; k++ )
-         once();
-      HYMI_HIP( hipEventRecord( e1, s ) );
-      HYMI_HIP( hipEventSynchronize( e1 ) );
-      float t = 0.f;
-      HYMI_HIP( hipEventElapsedTime( &t, e0, e1 ) );
-      (void) hipEventDestroy( e0 );
-      (void) hipEventDestroy( e1 );
-      *ms = (double) t / reps;
+      *ms = timeLaunches( s, reps, once );
    } );
 }
 

@@ -2927,8 +2927,11 @@ void prolongT( Function& f, int srcLevel, int flag, bool refreshCoarse )
                           gf, sg, nseg, Pf.dCellType.p, flag, uc, uf );
 }
 
+// Dot product of a and b over the owned DoFs of flag primitives: block partials into `partials` (room for
+// `cap` doubles), their fixed-order sum into out[0 .. NR).  With mayBuild = false the work-unit tables must
+// exist already (prepareDotTables): nothing is allocated.
 template < typename T, int NR >
-void dotT( Function& a, Function& b, int level, int flag, int slot )
+void dotInto( Function& a, Function& b, int level, int flag, double* partials, int64_t cap, double* out, bool mayBuild )
 {
    Mesh&       M   = *a.mesh;
    Context&    ctx = *M.ctx;
@@ -2937,33 +2940,51 @@ void dotT( Function& a, Function& b, int level, int flag, int slot )
    hipStream_t s   = ctx.stream;
    const int64_t nbLow = 64;
    int64_t       nbC   = 0;
+   auto          noTables = []() {
+      throw std::runtime_error( "dot: work-unit tables of the level are not prepared (a function of the level "
+                                "allocates them)" );
+   };
    if ( NR == 1 )
    {
       // flat windows of the transfer units, a fixed grid (few partials for the final sum)
       int64_t          nu = 0;
-      const MarchUnit* mu = level >= 2 ? M.transferUnits( P, &nu ) : nullptr;
-      nbC                 = nu > 0 ? std::min< int64_t >( kDotBlocks, ( nu + 3 ) / 4 ) : 0;
+      const MarchUnit* mu = nullptr;
+      if ( level >= 2 )
+      {
+         if ( !mayBuild && P.transferCount < 0 ) noTables();
+         mu = M.transferUnits( P, &nu );
+      }
+      nbC = nu > 0 ? std::min< int64_t >( kDotBlocks, ( nu + 3 ) / 4 ) : 0;
+      if ( ( nbC + nbLow ) * NR > cap ) throw std::logic_error( "dot: partial-sum buffer too small" );
       if ( nbC > 0 )
          hipLaunchKernelGGL( ( kDotFlat< T > ), dim3( (unsigned) nbC ), dim3( 256 ), 0, s, g, mu, nu, P.dCellType.p,
-                             flag, (const T*) a.ptr( level ), (const T*) b.ptr( level ), ctx.partials.p );
+                             flag, (const T*) a.ptr( level ), (const T*) b.ptr( level ), partials );
    }
-   else
+   else if ( level >= 2 )
    {
+      if ( !mayBuild && P.segs.find( segLen< NR >() ) == P.segs.end() ) noTables();
       int64_t        nseg   = 0;
       const CellSeg* sg     = M.segments( P, segLen< NR >(), &nseg );
       const int64_t  nbCell = std::min< int64_t >( kDotBlocks, ( nseg + 3 ) / 4 );
-      if ( nbCell > 0 && level >= 2 )
-         hipLaunchKernelGGL( ( kDotCell< T, NR > ), dim3( (unsigned) nbCell ), dim3( 256 ), 0, s, g, sg, nseg,
-                             P.dCellType.p, flag, (const T*) a.ptr( level ), (const T*) b.ptr( level ),
-                             ctx.partials.p );
-      nbC = ( level >= 2 ) ? nbCell : 0;
+      nbC                   = nbCell > 0 ? nbCell : 0;
+      if ( ( nbC + nbLow ) * NR > cap ) throw std::logic_error( "dot: partial-sum buffer too small" );
+      if ( nbC > 0 )
+         hipLaunchKernelGGL( ( kDotCell< T, NR > ), dim3( (unsigned) nbC ), dim3( 256 ), 0, s, g, sg, nseg,
+                             P.dCellType.p, flag, (const T*) a.ptr( level ), (const T*) b.ptr( level ), partials );
    }
-   if ( ( nbC + nbLow ) * NR > (int64_t) ctx.partials.n ) throw std::logic_error( "dot: partial-sum buffer too small" );
+   if ( ( nbC + nbLow ) * NR > cap ) throw std::logic_error( "dot: partial-sum buffer too small" );
    hipLaunchKernelGGL( ( kDotLow< T, NR > ), dim3( (unsigned) nbLow ), dim3( 256 ), 0, s, g, P.dFaceGeom.p, M.T.nF(),
                        P.dEdgeGeom.p, M.T.nE(), P.dVertexGeom.p, M.T.nV(), flag, (const T*) a.ptr( level ),
-                       (const T*) b.ptr( level ), ctx.partials.p + nbC * NR );
-   hipLaunchKernelGGL( ( kFinishDot< NR > ), dim3( 1 ), dim3( 256 ), 0, s, ctx.partials.p, nbC + nbLow,
-                       ctx.scalars.p + (int64_t) slot * NR );
+                       (const T*) b.ptr( level ), partials + nbC * NR );
+   hipLaunchKernelGGL( ( kFinishDot< NR > ), dim3( 1 ), dim3( 256 ), 0, s, partials, nbC + nbLow, out );
+}
+
+template < typename T, int NR >
+void dotT( Function& a, Function& b, int level, int flag, int slot )
+{
+   Context& ctx = *a.mesh->ctx;
+   dotInto< T, NR >( a, b, level, flag, ctx.partials.p, (int64_t) ctx.partials.n, ctx.scalars.p + (int64_t) slot * NR,
+                     true );
 }
 
 template < typename T, typename S, int NR >
@@ -3069,6 +3090,29 @@ struct DotF
    static void run( Function& a, Function& b, int L, int flag, int slot ) { dotT< T, NR >( a, b, L, flag, slot ); }
 };
 template < typename T, int NR >
+struct DotIntoF
+{
+   static void run( Function& a, Function& b, int L, int flag, double* partials, int64_t cap, double* out )
+   {
+      dotInto< T, NR >( a, b, L, flag, partials, cap, out, false );
+   }
+};
+template < typename T, int NR >
+struct PrepareDotF
+{
+   static void run( Function& f, int L )
+   {
+      Mesh&      M = *f.mesh;
+      LevelPlan& P = M.plan( L );
+      int64_t    n = 0;
+      if ( L < 2 ) return;
+      if ( NR == 1 )
+         (void) M.transferUnits( P, &n );
+      else
+         (void) M.segments( P, segLen< NR >(), &n );
+   }
+};
+template < typename T, int NR >
 struct LincombF
 {
    static void run( Function& d, int L, int flag, int n, const double* c, Function* const* s, bool acc, double k )
@@ -3134,6 +3178,13 @@ void launchDot( Function& a, Function& b, int level, int flag, int slot )
 {
    dispatch< DotF >( a.prec, a.nrhs, a, b, level, flag, slot );
 }
+
+void launchDotInto( Function& a, Function& b, int level, int flag, double* partials, int64_t cap, double* out )
+{
+   dispatch< DotIntoF >( a.prec, a.nrhs, a, b, level, flag, partials, cap, out );
+}
+
+void prepareDotTables( Function& f, int level ) { dispatch< PrepareDotF >( f.prec, f.nrhs, f, level ); }
 
 void launchAxpyScalar( Function& y, Function& x, int level, int flag, int slot, double sign )
 {
@@ -3232,6 +3283,19 @@ struct CellSorPhaseF
    }
 };
 template < typename T, int NR >
+struct CellSorSequenceF
+{
+   static void run( Operator& op, Function& x, const Function& b, double omega, int level, int flag, int count )
+   {
+      Mesh&      M  = *op.mesh;
+      LevelPlan& P  = M.plan( level );
+      auto&      Wt = op.weights( level );
+      for ( int k = 0; k < count; k++ ) // sweep k of a smoothing sequence is chained to sweep k - 1
+         cellSorPhase< T, NR >( M, P, wsel< T >( Wt.cell64, Wt.cell32 ), flag, (T*) x.ptr( level ),
+                                (const T*) b.ptr( level ), (T) omega, false, k );
+   }
+};
+template < typename T, int NR >
 struct CellSorOnlyF
 {
    static void run( Operator& op, Function& x, const Function& b, double omega, int level, int flag, int colour )
@@ -3253,6 +3317,11 @@ void launchCellApplyOnly( Operator& op, Function& src, Function& dst, int level,
 void launchCellSorPhase( Operator& op, Function& x, const Function& b, double omega, int level, int flag, bool tileOnly )
 {
    dispatch< CellSorPhaseF >( x.prec, x.nrhs, op, x, b, omega, level, flag, tileOnly );
+}
+
+void launchCellSorSequence( Operator& op, Function& x, const Function& b, double omega, int level, int flag, int count )
+{
+   dispatch< CellSorSequenceF >( x.prec, x.nrhs, op, x, b, omega, level, flag, count );
 }
 
 void launchCellSorOnly( Operator& op, Function& x, const Function& b, double omega, int level, int flag, int colour )

@@ -37,6 +37,11 @@ void launchLincomb( Function& dst, int level, int flag, int n, const double* c, 
 // per-rhs dot product over owned DoFs of primitives matching flag; result in ctx scalars[slot*nrhs + r]
 void launchDot( Function& a, Function& b, int level, int flag, int slot );
 // y += sign * scalars[slot*nrhs + r] * x   (closures of primitives matching flag)
+// the same dot product with block partials in `partials` (cap doubles) and the result in out[0 .. nrhs); the
+// level's work-unit tables must exist (prepareDotTables): nothing is allocated
+void launchDotInto( Function& a, Function& b, int level, int flag, double* partials, int64_t cap, double* out );
+// builds the work-unit tables the dot product of f's precision and nrhs reads on `level`
+void prepareDotTables( Function& f, int level );
 void launchAxpyScalar( Function& y, Function& x, int level, int flag, int slot, double sign );
 // p = r + scalars[slot*nrhs + r] * p
 void launchXpbyScalar( Function& p, Function& r, int level, int flag, int slot );
@@ -61,5 +66,8 @@ void launchCellApplyOnly( Operator& op, Function& src, Function& dst, int level,
 void launchCellSorOnly( Operator& op, Function& x, const Function& b, double omega, int level, int flag, int colour );
 // the whole SOR cell phase of a level as configured (fused or colour passes), or only the fused tile kernel
 void launchCellSorPhase( Operator& op, Function& x, const Function& b, double omega, int level, int flag, bool tileOnly );
+// the SOR cell phases of `count` sweeps of one smoothing sequence, as the smoother chains them (no
+// lower-dimensional phases in between)
+void launchCellSorSequence( Operator& op, Function& x, const Function& b, double omega, int level, int flag, int count );
 
 } // namespace hymi

@@ -89,7 +89,8 @@ constexpr int kCellSmallMaxLevel = 5;
 // fused macro-cell SOR sweep (one launch for the four colours): square xy tiles of 24 points per side, a halo of kSorTileHalo rings loaded around each tile and recomputed, kSorTileRing slab slots in LDS
 constexpr int kSorTileHalo = 4;
 constexpr int kSorTileRing = 10;
-inline bool sorTileSizeOk( int t ) { return t == 24; }
+constexpr int kSorTileSide = 24;
+inline bool   sorTileSizeOk( int t ) { return t == kSorTileSide; }
 
 struct EdgeGeom
 {

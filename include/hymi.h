@@ -202,7 +202,8 @@ int hymi_solve_refined( hymi_gmg g, hymi_function x, hymi_function b, int level,
  * hipEvents recorded on that stream.  which: 0 macro-cell apply kernel (x -> y), 1 macro-cell SOR colour
  * pass (x, b; colour 0), 2 full apply (x -> y), 3 full SOR sweep (x, b), 4 one V-cycle of gmg (x, b),
  * 5 plain copy y = x of the whole level (a bandwidth reference), 6 the SOR cell phase as configured for the level
- * (hymi_mesh_set_cell_sor; x, b), 7 the fused cell phase's tile kernel alone (x, b). */
+ * (hymi_mesh_set_cell_sor; x, b), 7 the fused cell phase's tile kernel alone (x, b), 8 the cell phases of two
+ * chained sweeps, as one smoothing sequence of a V(2, 2) cycle runs them (x, b). */
 int hymi_time_kernel( hymi_operator op, hymi_gmg gmg, hymi_function x, hymi_function b, hymi_function y, int level,
                       int which, int reps, double* ms_per_launch );
 
@@ -228,6 +229,23 @@ int hymi_mesh_get_schedule( hymi_mesh mesh, int level, int out[4] );
  * interior point by point (src/hyteg/p1functionspace/P1Operator.hpp:341-420) */
 int hymi_mesh_set_cell_sor( hymi_mesh mesh, int level, int tile, int precisions );
 int hymi_mesh_get_cell_sor( hymi_mesh mesh, int level, int out[2] );
+/* Autotuner.  No reference counterpart: the reference fixes its kernels when it is built
+ * (src/hyteg/HytegDefinitions.in.hpp:24-38).  On `level` (>= 2) of the mesh it measures, with hipEvents over
+ * `reps` launches on scratch functions of that level:
+ *   1. every launch schedule of the macro-cell kernels in its candidate list -- levels >= 5: the column march
+ *      (kind 3) with 4, 8, 16 or 32 slabs per unit, 2, 4 or 8 waves and prefetch 1 or 2 (24 candidates, in
+ *      that nesting order); levels < 5: flat windows (kind 1) with 1, 2 or 4 windows per unit -- each timed
+ *      on the macro-cell apply, the residual and the four SOR colour passes of the first precision selected
+ *      in `precisions` (bit 0 FP64, bit 1 FP32; the schedule is shared by both), and keeps the fastest;
+ *   2. for each selected precision (FP64 first), on levels >= 3: the SOR cell phase of two chained sweeps as
+ *      four colour passes and as the fused launch, and keeps the faster.
+ * The choice is stored in the mesh as hymi_mesh_set_schedule and hymi_mesh_set_cell_sor would store it (the
+ * results do not change: every schedule gives the same values bit for bit); captured V-cycle graphs are
+ * captured again.  report (report_len doubles, may be NULL with report_len 0) receives the ms of every
+ * candidate in the order above (-1 for a schedule the level cannot run), then per selected precision the
+ * pair (colour passes, fused); report_count = the number of entries measured. */
+int hymi_mesh_autotune( hymi_mesh mesh, int level, int precisions, int reps, double* report, int report_len,
+                        int* report_count );
 
 #ifdef __cplusplus
 }
