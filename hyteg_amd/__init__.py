@@ -228,6 +228,19 @@ class P1Function:
             raise ValueError("wrong number of values")
         _lib.call("hymi_function_upload_owned", self.h, level, _dp(v))
 
+    def download_column(self, level, column):
+        """Column `column` of an interleaved function (owned order)."""
+        out = np.zeros(self.storage.owned_count(level))
+        _lib.call("hymi_function_download_owned_column", self.h, level, column, _dp(out))
+        return out
+
+    def select_columns(self, level, keep):
+        """Zero every column not in `keep` (an iterable of column indices) on the whole level."""
+        mask = 0
+        for r in keep:
+            mask |= 1 << int(r)
+        _lib.call("hymi_function_select_columns", self.h, level, mask)
+
     def download_primitive(self, level, cls, idx):
         n = ctypes.c_int64()
         _lib.call("hymi_function_primitive_size", self.h, level, cls, idx, ctypes.byref(n))
@@ -503,6 +516,19 @@ class MixedPrecisionRefinementSolver:
             self._templates[nrhs] = P1Function("refinement-template", self.storage, self.inner.minLevel,
                                                self.inner.minLevel, precision=self.innerPrecision, nrhs=nrhs)
         return self.inner._handle(A, self._templates[nrhs])
+
+    def step(self, A, x, b, level, cycles=None, columns=None):
+        """One refinement step (hymi_refine_step): with cycles = 0 only ||b - A x|| per column; else the
+        correction from the residual the previous step left, added to the given columns (default all), and
+        the new residual norms.  Returns the norms (one per column)."""
+        h = self._innerHandle(A, x.nrhs)
+        cycles = self.cyclesPerOuter if cycles is None else cycles
+        mask = 0
+        for r in (range(x.nrhs) if columns is None else columns):
+            mask |= 1 << int(r)
+        out = np.zeros(x.nrhs)
+        _lib.call("hymi_refine_step", h, x.h, b.h, level, int(cycles), mask, _dp(out))
+        return out
 
     def solve(self, A, x, b, level):
         h = self._innerHandle(A, x.nrhs)

@@ -103,6 +103,9 @@ SIGNATURES = {
     "hymi_mesh_set_schedule": [_vp, _i, _i, _i, _i, _i],
     "hymi_interpolate_modes": [_vp, _i, _i, _i, _P(_d)],
     "hymi_solve_refined": [_vp, _vp, _vp, _i, _d, _i, _i, _P(_i), _P(_d), _P(_d)],
+    "hymi_refine_step": [_vp, _vp, _vp, _i, _i, ctypes.c_uint, _P(_d)],
+    "hymi_function_select_columns": [_vp, _i, ctypes.c_uint],
+    "hymi_function_download_owned_column": [_vp, _i, _i, _P(_d)],
     "hymi_mesh_get_schedule": [_vp, _i, _P(_i)],
     "hymi_mesh_set_cell_sor": [_vp, _i, _i, _i],
     "hymi_mesh_get_cell_sor": [_vp, _i, _P(_i)],

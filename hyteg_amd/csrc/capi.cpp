@@ -295,6 +295,59 @@ struct hymi_gmg_s
    // solver's own precision (allocated on first use, kept for later solves)
    std::unique_ptr< Function > refR, refRc, refEc;
    int                         refPrec = -1;
+   bool                        refSplit = false;
+   // (x, b, level) whose residual refR / refRc hold after refineResidual; refineCorrect needs it
+   std::tuple< uint64_t, uint64_t, int > refKey{ 0, 0, -1 };
+
+   void refineAlloc( const Function& x )
+   {
+      if ( refR && refPrec == x.prec ) return;
+      Mesh& M = *A->mesh;
+      refR    = makeFunction( M, minL, maxL, x.prec, nrhs );
+      refRc   = makeFunction( M, minL, maxL, prec, nrhs );
+      refEc   = makeFunction( M, minL, maxL, prec, nrhs );
+      refPrec = x.prec;
+      refKey  = { 0, 0, -1 };
+   }
+
+   // r = b - A x in the outer precision and ||r|| per right-hand side into out (synchronizes).  With the column
+   // schedule on the level, the cell residual goes rounded straight into the FP32 right-hand side and its norm
+   // is summed on the way (no FP64 cell residual is written or read again).
+   void refineResidual( Function& x, Function& b, int level, std::vector< double >& out, bool refresh )
+   {
+      Context&  ctx  = *A->mesh->ctx;
+      const int flag = kInner | kNeumann;
+      refSplit       = launchResidualSplit( *A, x, b, *refR, *refRc, level, flag, sUser, refresh );
+      if ( !refSplit )
+      {
+         launchApply( *A, x, *refR, &b, level, flag, kResidual, refresh );
+         launchDot( *refR, *refR, level, flag, sUser );
+      }
+      out.resize( nrhs );
+      HYMI_HIP( hipStreamSynchronize( ctx.stream ) );
+      HYMI_LEGACY_LOCK;
+      HYMI_HIP( hipMemcpyAsync( out.data(), ctx.scalars.p + sUser * nrhs, sizeof( double ) * nrhs, hipMemcpyDeviceToHost,
+                                ctx.stream ) );
+      HYMI_HIP( hipStreamSynchronize( ctx.stream ) );
+      for ( double& v : out )
+         v = std::sqrt( std::max( v, 0.0 ) );
+      refKey = { x.uid, b.uid, level };
+   }
+
+   // correction from the residual refineResidual left: e = 0, `cycles` V-cycles on A e = r in the solver's
+   // precision (replayed from a captured graph), x += e (on the columns in mask; all if allColumns).  e is zero
+   // on every Dirichlet value and copy (it starts at zero and no flag kernel writes them), so adding it over
+   // all primitives changes no boundary value.
+   void refineCorrect( Function& x, int level, int cycles, bool allColumns, unsigned mask )
+   {
+      if ( !refSplit ) launchConvert( *refRc, level, *refR, level, kAll );
+      launchLincomb( *refEc, level, kAll, 0, nullptr, nullptr, false, 0.0 );
+      replay( *refEc, *refRc, level, cycles );
+      if ( allColumns )
+         launchAddFrom( x, *refEc, level, kAll );
+      else
+         launchAddFromMasked( x, *refEc, level, kAll, mask );
+   }
 
    void cycle( Function& x, Function& b, int L )
    {
@@ -407,6 +460,18 @@ double timeLaunches( hipStream_t s, int reps, Fn&& fn )
    (void) hipEventDestroy( e0 );
    (void) hipEventDestroy( e1 );
    return (double) t / reps;
+}
+} // namespace
+
+namespace {
+void checkRefinement( hymi_gmg g, hymi_function x, hymi_function b, int level )
+{
+   require( level >= g->minL && level <= g->maxL, "level outside the solver's range" );
+   require( x->f.mesh == g->A->mesh && b->f.mesh == g->A->mesh, "functions live on another mesh" );
+   require( x->f.prec == b->f.prec, "x and b must share a precision" );
+   require( x->f.nrhs == g->nrhs && b->f.nrhs == g->nrhs, "nrhs differs from the solver's" );
+   requireLevel( x->f, level );
+   requireLevel( b->f, level );
 }
 } // namespace
 
@@ -1387,74 +1452,93 @@ int hymi_solve_refined( hymi_gmg g, hymi_function x, hymi_function b, int level,
 {
    return guarded( [&] {
       require( g && x && b && outerDone && relRes, "null argument" );
-      require( level >= g->minL && level <= g->maxL, "level outside the solver's range" );
-      require( x->f.mesh == g->A->mesh && b->f.mesh == g->A->mesh, "functions live on another mesh" );
-      require( x->f.prec == b->f.prec, "x and b must share a precision" );
-      require( x->f.nrhs == g->nrhs && b->f.nrhs == g->nrhs, "nrhs differs from the solver's" );
-      requireLevel( x->f, level );
-      requireLevel( b->f, level );
+      checkRefinement( g, x, b, level );
       require( tol > 0.0 && maxOuter >= 0 && cyclesPerOuter >= 1, "bad tolerance or iteration counts" );
-      Mesh&     M    = *g->A->mesh;
-      Context&  ctx  = *M.ctx;
-      const int nr   = g->nrhs;
-      const int flag = kInner | kNeumann;
-      if ( !g->refR || g->refPrec != x->f.prec )
-      {
-         g->refR    = makeFunction( M, g->minL, g->maxL, x->f.prec, nr );
-         g->refRc   = makeFunction( M, g->minL, g->maxL, g->prec, nr );
-         g->refEc   = makeFunction( M, g->minL, g->maxL, g->prec, nr );
-         g->refPrec = x->f.prec;
-      }
+      const int nr = g->nrhs;
+      g->refineAlloc( x->f );
       std::vector< double > r0( nr ), rn( nr );
-      // r = b - A x in the outer precision, and ||r|| per right-hand side.  x's closure copies are refreshed
-      // before the first residual only: the update x += e below runs over every primitive's whole closure
-      // array, and e's copies are current (a V-cycle ends with a smoothing sweep, after which every copy
-      // is), so the update keeps x's copies equal to their owners.
-      // With the column schedule on the level, the cell residual goes rounded straight into the FP32
-      // right-hand side and its norm is summed on the way (no FP64 cell residual is written or read again).
-      bool split          = false;
-      auto residualNorms = [&]( std::vector< double >& out, bool refresh ) {
-         split = launchResidualSplit( *g->A, x->f, b->f, *g->refR, *g->refRc, level, flag, sUser, refresh );
-         if ( !split )
-         {
-            launchApply( *g->A, x->f, *g->refR, &b->f, level, flag, kResidual, refresh );
-            launchDot( *g->refR, *g->refR, level, flag, sUser );
-         }
-         HYMI_HIP( hipStreamSynchronize( ctx.stream ) );
-         HYMI_LEGACY_LOCK;
-         HYMI_HIP( hipMemcpyAsync( out.data(), ctx.scalars.p + sUser * nr, sizeof( double ) * nr, hipMemcpyDeviceToHost,
-                                   ctx.stream ) );
-         HYMI_HIP( hipStreamSynchronize( ctx.stream ) );
-         for ( double& v : out )
-            v = std::sqrt( std::max( v, 0.0 ) );
-      };
-      auto worst = [&]() {
+      auto                  worst = [&]() {
          double m = 0.0;
          for ( int r = 0; r < nr; r++ )
             m = std::max( m, r0[r] > 0.0 ? rn[r] / r0[r] : 0.0 );
          return m;
       };
-      residualNorms( r0, true );
+      // x's closure copies are refreshed before the first residual only: the update x += e runs over every
+      // primitive's whole closure array and e's copies are current (a V-cycle ends with a smoothing sweep, after
+      // which every copy is), so the update keeps x's copies equal to their owners
+      g->refineResidual( x->f, b->f, level, r0, true );
       rn         = r0;
       double rel = worst();
       if ( history ) history[0] = rel;
       int k = 0;
       for ( ; k < maxOuter && rel > tol; k++ )
       {
-         // correction equation A e = r in the solver's precision, e from zero
-         if ( !split ) launchConvert( *g->refRc, level, *g->refR, level, kAll );
-         launchLincomb( *g->refEc, level, kAll, 0, nullptr, nullptr, false, 0.0 );
-         g->replay( *g->refEc, *g->refRc, level, cyclesPerOuter );
-         // e is zero on every Dirichlet value and copy (it starts at zero and no flag kernel writes them),
-         // so adding it over all primitives changes no boundary value
-         launchAddFrom( x->f, *g->refEc, level, kAll );
-         residualNorms( rn, g->nu2 == 0 || level == g->minL ); // without a final sweep e's copies may be stale
+         g->refineCorrect( x->f, level, cyclesPerOuter, true, 0u );
+         g->refineResidual( x->f, b->f, level, rn, g->nu2 == 0 || level == g->minL ); // no final sweep: stale copies
          rel = worst();
          if ( history ) history[k + 1] = rel;
       }
       *outerDone = k;
       for ( int r = 0; r < nr; r++ )
          relRes[r] = r0[r] > 0.0 ? rn[r] / r0[r] : 0.0;
+   } );
+}
+
+int hymi_refine_step( hymi_gmg g, hymi_function x, hymi_function b, int level, int cycles, unsigned int mask,
+                      double* res_norms )
+{
+   return guarded( [&] {
+      require( g && x && b && res_norms, "null argument" );
+      checkRefinement( g, x, b, level );
+      require( cycles >= 0, "cycles must be >= 0" );
+      g->refineAlloc( x->f );
+      std::vector< double > rn;
+      if ( cycles > 0 && mask != 0u )
+      {
+         if ( g->refKey != std::make_tuple( x->f.uid, b->f.uid, level ) )
+            throw std::invalid_argument( "refine step: no residual of these functions on this level; call with "
+                                         "cycles = 0 first" );
+         g->refineCorrect( x->f, level, cycles, false, mask );
+         g->refineResidual( x->f, b->f, level, rn, g->nu2 == 0 || level == g->minL );
+      }
+      else
+         g->refineResidual( x->f, b->f, level, rn, true );
+      for ( int r = 0; r < g->nrhs; r++ )
+         res_norms[r] = rn[r];
+   } );
+}
+
+int hymi_function_select_columns( hymi_function f, int level, unsigned int keep_mask )
+{
+   return guarded( [&] {
+      require( f, "null argument" );
+      requireLevel( f->f, level );
+      HYMI_HIP( hipSetDevice( f->f.mesh->ctx->device ) );
+      launchColumnSelect( f->f, level, keep_mask );
+   } );
+}
+
+int hymi_function_download_owned_column( hymi_function fh, int level, int column, double* host )
+{
+   return guarded( [&] {
+      require( fh && host, "null argument" );
+      Function& f = fh->f;
+      requireLevel( f, level );
+      require( column >= 0 && column < f.nrhs, "column out of range" );
+      Mesh&          M = *f.mesh;
+      const int64_t  n = M.plan( level ).total;
+      DevBuf< double > tmp;
+      tmp.alloc( (size_t) n );
+      launchColumnGather( f, level, column, tmp.p );
+      std::vector< double > buf( (size_t) n );
+      HYMI_HIP( hipStreamSynchronize( M.ctx->stream ) );
+      {
+         HYMI_LEGACY_LOCK;
+         HYMI_HIP( hipMemcpy( buf.data(), tmp.p, (size_t) n * 8, hipMemcpyDeviceToHost ) );
+      }
+      auto idx = ownedIndex( M, level );
+      for ( size_t k = 0; k < idx.size(); k++ )
+         host[k] = buf[(size_t) idx[k]];
    } );
 }
 

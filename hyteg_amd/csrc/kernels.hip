@@ -2244,6 +2244,44 @@ __global__ void kLincomb( const Range* __restrict__ ranges, T* __restrict__ dst,
    }
 }
 
+// Column operations on interleaved functions ([dof][column], NR columns; a Range offset is a dof index, so the
+// column of element o * NR + k is k mod NR).  kColumnSelect zeroes the columns not in `keep`; kColumnAddFrom
+// adds src to dst on the columns in `mask` with the arithmetic of kLincomb's accumulate form
+// (v = constant + c0 src, then v + dst); kColumnGather copies column `col` of n dofs to a contiguous array.
+template < typename T, int NR >
+__global__ void kColumnSelect( const Range* __restrict__ ranges, T* __restrict__ dst, unsigned keep )
+{
+   const Range   R   = ranges[blockIdx.x];
+   const int64_t len = (int64_t) R.len * NR;
+   const int64_t o   = R.off * NR;
+   for ( int64_t k = threadIdx.x; k < len; k += blockDim.x )
+      if ( !( ( keep >> ( k % NR ) ) & 1u ) ) dst[o + k] = T( 0 );
+}
+
+template < typename T, typename S, int NR >
+__global__ void kColumnAddFrom( const Range* __restrict__ ranges, T* __restrict__ dst, const S* __restrict__ src,
+                                unsigned mask, double c0, double constant )
+{
+   const Range   R   = ranges[blockIdx.x];
+   const int64_t len = (int64_t) R.len * NR;
+   const int64_t o   = R.off * NR;
+   for ( int64_t k = threadIdx.x; k < len; k += blockDim.x )
+      if ( ( mask >> ( k % NR ) ) & 1u )
+      {
+         double v = constant;
+         v += c0 * (double) src[o + k];
+         v += (double) dst[o + k];
+         dst[o + k] = (T) v;
+      }
+}
+
+template < typename T, int NR >
+__global__ void kColumnGather( const T* __restrict__ src, int64_t n, int col, double* __restrict__ out )
+{
+   for ( int64_t k = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t) gridDim.x * blockDim.x )
+      out[k] = (double) src[k * NR + col];
+}
+
 template < typename T, int NR >
 __global__ void kAxpyScalar( const Range* __restrict__ ranges, T* __restrict__ y, const T* __restrict__ x,
                              const double* __restrict__ sc, double sign )
@@ -3373,6 +3411,74 @@ bool launchResidualSplit( Operator& op, Function& x, const Function& b, Function
                        P.dVertexGeom.p, nV, flag, (const double*) y, (const double*) y, part + n );
    hipLaunchKernelGGL( ( kFinishDot< 1 > ), dim3( 1 ), dim3( 256 ), 0, s, part, n + 64, ctx.scalars.p + slot );
    return true;
+}
+
+template < typename T, int NR >
+struct ColumnSelectF
+{
+   static void run( Function& f, int level, unsigned keep )
+   {
+      Mesh&        M  = *f.mesh;
+      LevelPlan&   P  = M.plan( level );
+      int64_t      nr = 0;
+      const Range* R  = M.closureRanges( P, kAll, &nr );
+      if ( nr > 0 )
+         hipLaunchKernelGGL( ( kColumnSelect< T, NR > ), dim3( (unsigned) nr ), dim3( 256 ), 0, M.ctx->stream, R,
+                             (T*) f.ptr( level ), keep );
+   }
+};
+template < typename T, int NR >
+struct ColumnGatherF
+{
+   static void run( Function& f, int level, int col, double* out )
+   {
+      Mesh&         M = *f.mesh;
+      const int64_t n = M.plan( level ).total;
+      const int64_t blocks = std::min< int64_t >( 4096, ( n + 255 ) / 256 );
+      hipLaunchKernelGGL( ( kColumnGather< T, NR > ), dim3( (unsigned) std::max< int64_t >( 1, blocks ) ), dim3( 256 ), 0,
+                          M.ctx->stream, (const T*) f.ptr( level ), n, col, out );
+   }
+};
+
+void launchColumnSelect( Function& f, int level, unsigned keep ) { dispatch< ColumnSelectF >( f.prec, f.nrhs, f, level, keep ); }
+
+void launchColumnGather( Function& f, int level, int col, double* out )
+{
+   dispatch< ColumnGatherF >( f.prec, f.nrhs, f, level, col, out );
+}
+
+void launchAddFromMasked( Function& x, Function& e, int level, int flag, unsigned mask )
+{
+   if ( x.nrhs != e.nrhs || x.mesh != e.mesh ) throw std::runtime_error( "add: functions must share mesh and nrhs" );
+   Mesh&        M  = *x.mesh;
+   LevelPlan&   P  = M.plan( level );
+   int64_t      nr = 0;
+   const Range* R  = M.closureRanges( P, flag, &nr );
+   if ( nr == 0 ) return;
+   hipStream_t s = M.ctx->stream;
+   auto        go = [&]( auto* d, auto* sp, auto nrTag ) {
+      using TD          = std::remove_pointer_t< decltype( d ) >;
+      using TS          = std::remove_const_t< std::remove_pointer_t< decltype( sp ) > >;
+      constexpr int NRv = decltype( nrTag )::value;
+      hipLaunchKernelGGL( ( kColumnAddFrom< TD, TS, NRv > ), dim3( (unsigned) nr ), dim3( 256 ), 0, s, R, d, sp, mask, 1.0,
+                          0.0 );
+   };
+   auto run = [&]( auto nrTag ) {
+      if ( x.prec == kF64 && e.prec == kF32 )
+         go( (double*) x.ptr( level ), (const float*) e.ptr( level ), nrTag );
+      else if ( x.prec == kF64 && e.prec == kF64 )
+         go( (double*) x.ptr( level ), (const double*) e.ptr( level ), nrTag );
+      else if ( x.prec == kF32 && e.prec == kF32 )
+         go( (float*) x.ptr( level ), (const float*) e.ptr( level ), nrTag );
+      else
+         go( (float*) x.ptr( level ), (const double*) e.ptr( level ), nrTag );
+   };
+   if ( x.nrhs == 1 )
+      run( std::integral_constant< int, 1 >{} );
+   else if ( x.nrhs == 32 )
+      run( std::integral_constant< int, 32 >{} );
+   else
+      throw std::runtime_error( "unsupported nrhs (1 or 32)" );
 }
 
 void launchAddFrom( Function& x, Function& e, int level, int flag )

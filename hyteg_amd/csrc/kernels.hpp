@@ -52,6 +52,12 @@ void launchScalarCopy( Context& ctx, int nrhs, int dst, int src );
 void launchConvert( Function& dst, int dstLevel, Function& src, int srcLevel, int flag );
 // x (FP64) += e (FP32 or FP64)
 void launchAddFrom( Function& x, Function& e, int level, int flag );
+// the same on the columns r with bit r of mask set (interleaved functions; bit 0 for one right-hand side)
+void launchAddFromMasked( Function& x, Function& e, int level, int flag, unsigned mask );
+// zeroes the columns r without bit r of keep, on every primitive's whole closure array
+void launchColumnSelect( Function& f, int level, unsigned keep );
+// copies column col of the whole level buffer (plan total values) into out (device, FP64)
+void launchColumnGather( Function& f, int level, int col, double* out );
 // Residual of iterative refinement: r = b - A x with FP64 x and b; vertex, edge and face values go to r64
 // and, rounded, to rc; cell values go rounded straight to rc; ||r||^2 (FP64, owned DoFs of flag primitives)
 // to scalars[slot].  Needs the column schedule on the level; returns false (nothing launched) otherwise.

@@ -196,6 +196,23 @@ int hymi_fmg_solve( hymi_gmg g, hymi_function x, hymi_function b, int level, int
 int hymi_solve_refined( hymi_gmg g, hymi_function x, hymi_function b, int level, double tol, int max_outer,
                         int cycles_per_outer, int* outer_done, double* rel_res, double* history );
 
+/* One refinement step for a serving loop that admits and retires right-hand sides one column at a time.
+ * No single reference counterpart: it is one pass of the loop of hymi_solve_refined (cycle-and-check,
+ * apps/MultigridStudies/MultigridStudies.cpp:616-690) with the decision left to the caller.
+ * cycles = 0: res_norms[r] = ||b - A x|| per column (FP64) for the current x; nothing changes.
+ * cycles > 0: from the residual the previous call on the same x, b and level left (x and b must not have
+ * changed since; otherwise HYMI_ERR_INVALID), e = 0, `cycles` V-cycles of g on A e = r in g's precision,
+ * x += e on the columns r with bit r of mask set (bit 0 for one right-hand side), then res_norms as above.
+ * Every column is computed independently of the others, so a column's values depend only on its own data
+ * and the sequence of calls, not on what the other columns hold. */
+int hymi_refine_step( hymi_gmg g, hymi_function x, hymi_function b, int level, int cycles, unsigned int mask,
+                      double* res_norms );
+/* f[:, r] = 0 for every column r without bit r of keep_mask, on every primitive of the level (closure copies
+ * included).  No reference counterpart (the reference has one right-hand side per function). */
+int hymi_function_select_columns( hymi_function f, int level, unsigned int keep_mask );
+/* column `column` of an interleaved function in the owned-DoF order of hymi_function_download_owned */
+int hymi_function_download_owned_column( hymi_function f, int level, int column, double* host );
+
 /* ---- measurement -----------------------------------------------------------------------------------
  * No reference counterpart (the reference times with its WcTimingTree, src/hyteg/operators/Operator.hpp:
  * 99-115).  Average duration in ms of `reps` back-to-back launches on the context stream, bracketed by

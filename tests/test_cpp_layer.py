@@ -28,7 +28,7 @@ import pytest  # noqa: E402
 @pytest.mark.gpu
 def test_cpp_layer_device_run(tmp_path):
     """The same program on the GPU: 5 graph-replayed V(3,3) cycles through hymi::GeometricMultigridSolver reduce
-    the residual below 1e-3 of the right-hand side, and the refined FP32 solve reaches 1e-10."""
+    the residual below 1e-2 of the right-hand side, and the refined FP32 solve reaches 1e-10."""
     import hyteg_amd
     lib_dir = os.path.dirname(hyteg_amd._lib.LIB_PATH)
     exe = str(tmp_path / "hymi_cpp_check")

@@ -8,8 +8,9 @@
 * Config 4 (32 interleaved right-hand sides): every column of the interleaved apply matches the oracle to
   1e-13, of an SOR sweep to 1e-12, and the batched FP32-in-FP64 refined solve matches the oracle's FP64
   multigrid solution column by column to 1e-6.
-* Config 5 (serving): a stream with n = 1..3 and levels 3..5; every result equals the same request served
-  alone bit for bit, and sampled results match the oracle's FP64 solution to 1e-6.
+* Config 5 (serving): a stream with n = 1..3 and levels 3..5, small problems batched in 32-column lanes and
+  large ones on solve lanes; every result equals the same request served alone bit for bit, and sampled
+  results match the oracle's FP64 solution to 1e-6.
 
 The refined solves run to a relative residual of 1e-10 here and the oracle's to 1e-13: the difference of the
 two solutions is then bounded by the condition number (about 4^L) times 1e-10.
@@ -203,19 +204,21 @@ def test_config4_columns_match_oracle(gpu_ctx, oracle_lib):
 
 def test_config5_serving_matches_alone_and_oracle(gpu_ctx, oracle_lib):
     """A served stream with n = 1, 2, 3 sub-cubes per side and levels 3..5 (tolerance 1e-10): each result in the
-    mix of three workers equals the same request served alone, bit for bit; a sample of results (the two largest
-    n = 3 ones, the largest other and the smallest) equals the oracle's FP64 solution of b = M I_h f to 1e-6."""
+    stream (batch lanes and solve lanes at work together) equals the same request served alone, bit for bit; a
+    sample of results (the two largest n = 3 ones, the largest other and the smallest) equals the oracle's FP64
+    solution of b = M I_h f to 1e-6."""
     import hyteg_amd as H
     import oracle as O
     from hyteg_amd.serving import PoissonServer, poisson_stream
     reqs = poisson_stream(10, rate=1000.0, seed=11, ns=(1, 2, 3), levels=(3, 4, 5), tol=1e-10)
     assert {r.n for r in reqs} >= {3} and max(r.level for r in reqs) == 5
-    say("[config5] alone")
-    alone = PoissonServer(slots=1, keep_solutions=True)
-    ra = alone.serve(reqs, realtime=False)
+    kw = dict(slots=2, keep_solutions=True, batch_max_unknowns=300_000)   # n = 3 on level 5: a solve lane
+    say("[config5] each request alone")
+    alone = PoissonServer(**kw)
+    ra = alone.serve(reqs, one_at_a_time=True)
     alone.close()
-    say("[config5] mix of 3 workers")
-    mix = PoissonServer(slots=3, keep_solutions=True)
+    say("[config5] the stream (batch lanes and solve lanes)")
+    mix = PoissonServer(**kw)
     rm = mix.serve(reqs, realtime=True)
     mix.close()
     assert not alone.errors and not mix.errors, (alone.errors, mix.errors)
