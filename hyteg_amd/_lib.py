@@ -77,6 +77,7 @@ SIGNATURES = {
     "hymi_assign": [_vp, _i, _i, _i, _P(_d), _P(_vp)],
     "hymi_add": [_vp, _i, _i, _i, _P(_d), _P(_vp)],
     "hymi_dot": [_vp, _vp, _i, _i, _P(_d)],
+    "hymi_dot_noalloc": [_vp, _vp, _i, _i, _vp],
     "hymi_convert": [_vp, _vp, _i, _i],
     "hymi_laplace_create": [_vp, _i, _i, _P(_vp)],
     "hymi_mass_create": [_vp, _i, _i, _P(_vp)],

@@ -48,6 +48,43 @@ namespace {
 
 thread_local std::string g_lastError;
 
+// Fixed device scratch of hymi_dot_noalloc: the block partials of one dot product (at most kDotBlocks cell
+// workgroups plus 64 for the lower-dimensional primitives, times 32 right-hand sides: 528 KiB).  Each GPU the
+// library runs on holds its own copy until the process ends.  Host code never passes the array's name to a
+// kernel: it uses the address that hipGetSymbolAddress returned for the device (scratchOf).
+constexpr int64_t kScratchDoubles = kPartialsCap;
+__device__ __align__( 256 ) double scratch[kScratchDoubles];
+
+std::mutex&                 scratchLock()
+{
+   static std::mutex m;
+   return m;
+}
+std::map< int, double* >& scratchPtrs()
+{
+   static std::map< int, double* > m;
+   return m;
+}
+
+// records the device address of `scratch` on `device`, which must be current; once per device
+void lookUpScratch( int device )
+{
+   std::lock_guard< std::mutex > lk( scratchLock() );
+   if ( scratchPtrs().count( device ) ) return;
+   double*          ptr = nullptr;
+   const hipError_t rc  = hipGetSymbolAddress( (void**) &ptr, HIP_SYMBOL( scratch ) );
+   if ( rc != hipSuccess || ptr == nullptr ) throw HipError{ rc != hipSuccess ? rc : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)" };
+   scratchPtrs()[device] = ptr;
+}
+
+double* scratchOf( int device )
+{
+   std::lock_guard< std::mutex > lk( scratchLock() );
+   auto                          it = scratchPtrs().find( device );
+   if ( it == scratchPtrs().end() ) throw std::logic_error( "no scratch address recorded for the device" );
+   return it->second;
+}
+
 int fail( int code, const std::string& msg )
 {
    g_lastError = msg;
@@ -504,6 +541,7 @@ int hymi_ctx_create( int device, hymi_ctx* out )
          HYMI_LEGACY_LOCK;
          HYMI_HIP( hipStreamCreateWithFlags( &c->c.stream, hipStreamNonBlocking ) );
       }
+      lookUpScratch( device );
       ensureScalars( c->c, 32 );
       // fixed for the context's lifetime: captured graphs hold these addresses
       c->c.partials.alloc( (size_t) kPartialsCap );
@@ -1071,6 +1109,20 @@ int hymi_dot( hymi_function a, hymi_function b, int level, int flag, double* out
       HYMI_HIP( hipMemcpyAsync( out, ctx.scalars.p + sUser * a->f.nrhs, sizeof( double ) * a->f.nrhs,
                                 hipMemcpyDeviceToHost, ctx.stream ) );
       HYMI_HIP( hipStreamSynchronize( ctx.stream ) );
+   } );
+}
+
+int hymi_dot_noalloc( hymi_function a, hymi_function b, int level, int flag, double* out_device )
+{
+   return guarded( [&] {
+      require( a && b && out_device, "null argument" );
+      require( a->f.mesh == b->f.mesh && a->f.prec == b->f.prec && a->f.nrhs == b->f.nrhs, "incompatible functions" );
+      requireLevel( a->f, level );
+      requireLevel( b->f, level );
+      Context& ctx = *a->f.mesh->ctx;
+      HYMI_HIP( hipSetDevice( ctx.device ) );
+      // partials beyond kScratchDoubles are refused inside (the grid is fixed: at most kDotBlocks + 64 blocks)
+      launchDotInto( a->f, b->f, level, flag, scratchOf( ctx.device ), kScratchDoubles, out_device );
    } );
 }
 

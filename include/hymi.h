@@ -120,6 +120,13 @@ int hymi_assign( hymi_function dst, int level, int flag, int n, const double* co
 int hymi_add( hymi_function dst, int level, int flag, int n, const double* coeffs, const hymi_function* srcs );
 /* dotGlobal (VertexDoFFunction.cpp:1410-1498); writes nrhs values */
 int hymi_dot( hymi_function a, hymi_function b, int level, int flag, double* out );
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code under hipGraph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): hymi_dot with the block partials in a fixed
+ * device array of the library (528 KiB per GPU, held until the process ends; enough for every level and up
+ * to 32 right-hand sides) and the nrhs results written to the caller's DEVICE memory out_device,
+ * asynchronously on the context stream (nothing is synchronized, allocated or freed).  All calls on one GPU
+ * share the array: calls that overlap on one GPU (from different contexts' streams) race. */
+int hymi_dot_noalloc( hymi_function a, hymi_function b, int level, int flag, double* out_device );
 /* copy between functions of different precision on one level (mixed-precision refinement) */
 int hymi_convert( hymi_function dst, hymi_function src, int level, int flag );
 
