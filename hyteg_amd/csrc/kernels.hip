@@ -827,15 +827,33 @@ struct ColourStarts
 {
    int s[5];
 };
-template < typename T >
+// With F2C = true (forward sweeps) the workgroup first copies its four faces' closures into the cell's closure
+// (the face-to-cell refresh of the sweep, as kF2C does it for this cell): the faces are final once the face
+// phase is done, and the cell's copies are read by no other workgroup.  Cells the flag leaves alone copy too.
+template < typename T, bool F2C_ >
 __global__ __launch_bounds__( 1024 ) void kCellSorSmall( LV g, const int* __restrict__ pts, ColourStarts cs,
                                                         const T* __restrict__ W8, const int* __restrict__ ctype, int flag,
-                                                        T* __restrict__ u, const T* __restrict__ b, T omega, int reverse )
+                                                        T* __restrict__ u, const T* __restrict__ b, T omega, int reverse,
+                                                        const F2C* __restrict__ f2c )
 {
-   const int cell = blockIdx.x;
-   if ( !( ctype[cell] & flag ) ) return;
+   const int     cell = blockIdx.x;
    const int     w = g.w, t0 = tet32( w );
    const int64_t base = g.offC + (int64_t) cell * g.cS;
+   if constexpr ( F2C_ )
+   {
+      const int nF = w * ( w + 1 ) >> 1;
+      for ( int e = (int) threadIdx.x; e < 4 * nF; e += blockDim.x )
+      {
+         const F2C m = f2c[4 * cell + e / nF];
+         const int p = e % nF, j = triRow32( w, p ), i = p - j * ( 2 * w + 1 - j ) / 2;
+         const int x = m.m.base[0] + i * m.m.a[0] + j * m.m.b[0];
+         const int y = m.m.base[1] + i * m.m.a[1] + j * m.m.b[1];
+         const int z = m.m.base[2] + i * m.m.a[2] + j * m.m.b[2];
+         u[base + crow32( w, t0, y, z ) + x] = u[g.offF + (int64_t) m.face * g.fS + p];
+      }
+      __syncthreads();
+   }
+   if ( !( ctype[cell] & flag ) ) return; // uniform over the workgroup
    T*            cu   = u + base;
    const T*      cb   = b + base;
    const T*      c    = W8 + (int64_t) cell * 15;
@@ -1635,20 +1653,57 @@ __global__ __launch_bounds__( 64 ) void kVertexSorPush( LV g, const VertexGeom* 
 // staged in LDS once, so the per-point sums issue independent value loads and the colour updates issue none
 constexpr int kEdgeTermBatch = 16;
 constexpr int kEdgeThreads   = 256; // threads per edge workgroup; also the slices x points of a short edge
-template < typename T >
+// The vertex phase of a forward sweep inside the edge workgroups (VERT = true): wave j < 2 computes the SOR
+// update of the edge's vertex j as kVertexSorPush does (same terms in the same order, per-lane partial sums,
+// then the wave sum), reading each term from a face's closure copy of the edge point (vsnap) and the vertex's
+// old value from this edge's own end copy, so that no other workgroup of the launch writes what it reads; the
+// copies equal their owners at the start of a sweep, so the result is the same bit for bit.  Every edge writes
+// its own end copies, and the vertex's first edge also its own value.  Edges the flag leaves alone still do
+// this part.
+template < typename T, bool VERT >
 __global__ __launch_bounds__( kEdgeThreads ) void kEdgeSorFused( LV g, const EdgeGeom* __restrict__ eg,
                                                         const EdgeTerm* __restrict__ terms, const T* __restrict__ EW,
                                                         int flag, T* __restrict__ u, const T* __restrict__ b, T omega,
-                                                        int reverse )
+                                                        int reverse, const VertexGeom* __restrict__ vg,
+                                                        const int64_t* __restrict__ vsnap,
+                                                        const int32_t* __restrict__ ends, const T* __restrict__ VW )
 {
    __shared__ int64_t sPrim[kEdgeTermsMax];                  // element offset of the neighbour primitive
    __shared__ int     sB[3][kEdgeTermsMax], sI[3][kEdgeTermsMax], sKind[kEdgeTermsMax];
    __shared__ T       sW[kEdgeTermsMax];
    __shared__ T       tsum[kEdgeMaxW], su[kEdgeMaxW], sb[kEdgeMaxW];
+   __shared__ T       sVert[2];
    const EdgeGeom& G = eg[blockIdx.x];
-   if ( !( G.type & flag ) ) return;
-   const T*      ew = EW + 3 * (int64_t) G.edge + G.termOff;
-   const int64_t eo = g.offE + (int64_t) G.edge * g.eS;
+   const int64_t   eo = g.offE + (int64_t) G.edge * g.eS;
+   if constexpr ( VERT )
+   {
+      const int wave = (int) ( threadIdx.x >> 6 ), lane = (int) ( threadIdx.x & 63 );
+      if ( wave < 2 )
+      {
+         const int        v   = ends[4 * G.edge + wave];
+         const VertexGeom V   = vg[v];
+         const int64_t    eIx = eo + ( wave == 0 ? 0 : g.w - 1 );
+         const T          old = u[eIx];
+         T                nv  = old;
+         if ( V.type & flag )
+         {
+            const T* vw   = VW + V.vertex + V.termOff;
+            T        part = T( 0 );
+            for ( int k = lane; k < V.nTerms; k += 64 )
+               part += vw[1 + k] * u[vsnap[V.termOff + k]];
+            const T acc = waveSum( part );
+            nv          = ( T( 1 ) - omega ) * old + omega / vw[0] * ( b[V.vertex] - acc );
+         }
+         if ( lane == 0 )
+         {
+            sVert[wave] = nv;
+            u[eIx]      = nv;
+            if ( ends[4 * G.edge + 2 + wave] ) u[V.vertex] = nv;
+         }
+      }
+   }
+   if ( !( G.type & flag ) ) return; // uniform over the workgroup
+   const T* ew = EW + 3 * (int64_t) G.edge + G.termOff;
    const int     nT = G.nTerms; // <= kEdgeTermsMax (checked when the plan is built)
    const int     w = g.w, t0 = tet32( w );
    // term k at edge point x: face element fidx(w, B0 + x I0, B1 + x I1) or cell element
@@ -1672,6 +1727,12 @@ __global__ __launch_bounds__( kEdgeThreads ) void kEdgeSorFused( LV g, const Edg
    }
    const T rl = omega / ew[1], eW = ew[0], eE = ew[2];
    __syncthreads();
+   if constexpr ( VERT )
+      if ( threadIdx.x == 0 ) // the new end values (the loads above may have read the end copies before they were written)
+      {
+         su[0]     = sVert[0];
+         su[w - 1] = sVert[1];
+      }
    // The neighbour terms read face and cell points, which the edge phase does not change: the term sums of all
    // points are formed first (each thread's loads in flight together) and kept in LDS, and the two colour
    // updates then read only LDS.  Colour 0 = even edge points, colour 1 = odd ones; a backward sweep takes them
@@ -2671,8 +2732,25 @@ void cellSorFused( Mesh& M, LevelPlan& P, const T* CW, int flag, T* u, const T* 
 // The SOR cell phase of one level: the fused launch where it is selected for the function's precision (one
 // right-hand side), the one-workgroup-per-cell launch on small levels, else the four colour passes; backward =
 // colours in reverse; chain = index of the sweep within its smoothing sequence (see cellSorFused).
+// Whether the SOR cell phase of the level runs as one workgroup per macro-cell (kCellSorSmall) for a function
+// of precision T with NR right-hand sides.
 template < typename T, int NR >
-void cellSorPhase( Mesh& M, LevelPlan& P, const T* CW, int flag, T* u, const T* b, T om, bool backward, int chain = 0 )
+bool cellSorIsSmall( const LevelPlan& P )
+{
+   if constexpr ( NR != 1 )
+      return false; // interleaved right-hand sides: the colour passes
+   else
+   {
+      if ( P.sorTile > 0 && ( P.sorPrecisions & ( std::is_same< T, double >::value ? 1 : 2 ) ) ) return false;
+      return P.L <= kCellSmallMaxLevel && P.colourStart[4] > 0;
+   }
+}
+
+// withF2C: the one-workgroup-per-cell phase also does the face-to-cell refresh that precedes a forward cell
+// phase (the caller then skips it); ignored by the other forms of the phase.
+template < typename T, int NR >
+void cellSorPhase( Mesh& M, LevelPlan& P, const T* CW, int flag, T* u, const T* b, T om, bool backward, int chain = 0,
+                   bool withF2C = false )
 {
    const int rev = backward ? 1 : 0;
    if constexpr ( NR == 1 )
@@ -2681,7 +2759,7 @@ void cellSorPhase( Mesh& M, LevelPlan& P, const T* CW, int flag, T* u, const T* 
          cellSorFused< T >( M, P, CW, flag, u, b, om, rev, true, chain );
          return;
       }
-   if ( P.L <= kCellSmallMaxLevel && P.colourStart[4] > 0 )
+   if ( cellSorIsSmall< T, NR >( P ) )
    {
       ColourStarts cs;
       for ( int k = 0; k < 5; k++ )
@@ -2689,8 +2767,12 @@ void cellSorPhase( Mesh& M, LevelPlan& P, const T* CW, int flag, T* u, const T* 
       // threads: about one point of a colour each, from 64 (level 2) up to 1024 (level 5: 1124 per colour)
       const int perColour = ( P.colourStart[4] + 3 ) / 4;
       const int threads   = std::min( 1024, std::max( 64, ( perColour + 63 ) / 64 * 64 ) );
-      hipLaunchKernelGGL( ( kCellSorSmall< T > ), dim3( M.T.nC() ), dim3( threads ), 0, M.ctx->stream, viewOf( P ),
-                          P.dColourPts.p, cs, CW, P.dCellType.p, flag, u, b, om, rev );
+      if ( withF2C )
+         hipLaunchKernelGGL( ( kCellSorSmall< T, true > ), dim3( M.T.nC() ), dim3( threads ), 0, M.ctx->stream, viewOf( P ),
+                             P.dColourPts.p, cs, CW, P.dCellType.p, flag, u, b, om, rev, P.dF2C.p );
+      else
+         hipLaunchKernelGGL( ( kCellSorSmall< T, false > ), dim3( M.T.nC() ), dim3( threads ), 0, M.ctx->stream,
+                             viewOf( P ), P.dColourPts.p, cs, CW, P.dCellType.p, flag, u, b, om, rev, nullptr );
    }
    else
       for ( int cc = 0; cc < 4; cc++ )
@@ -2805,8 +2887,16 @@ void sorT( Operator& op, Function& xf, const Function& bf, double omega, int lev
       };
       auto edgePhase = [&]() {
          if ( P.w > kEdgeMaxW ) throw std::invalid_argument( "the fused edge SOR supports levels up to 10" );
-         hipLaunchKernelGGL( ( kEdgeSorFused< T > ), dim3( nE ), dim3( kEdgeThreads ), 0, s, g, P.dEdgeGeom.p, P.dEdgeTerms.p,
-                             wsel< T >( Wt.edge64, Wt.edge32 ), flag, u, b, om, rev );
+         hipLaunchKernelGGL( ( kEdgeSorFused< T, false > ), dim3( nE ), dim3( kEdgeThreads ), 0, s, g, P.dEdgeGeom.p,
+                             P.dEdgeTerms.p, wsel< T >( Wt.edge64, Wt.edge32 ), flag, u, b, om, rev, nullptr, nullptr,
+                             nullptr, nullptr );
+      };
+      // forward: the vertex phase computed inside the edge workgroups (one launch for both phases)
+      auto vertexEdgePhase = [&]() {
+         if ( P.w > kEdgeMaxW ) throw std::invalid_argument( "the fused edge SOR supports levels up to 10" );
+         hipLaunchKernelGGL( ( kEdgeSorFused< T, true > ), dim3( nE ), dim3( kEdgeThreads ), 0, s, g, P.dEdgeGeom.p,
+                             P.dEdgeTerms.p, wsel< T >( Wt.edge64, Wt.edge32 ), flag, u, b, om, rev, P.dVertexGeom.p,
+                             P.dVertexSnap.p, P.dEdgeEnds.p, wsel< T >( Wt.vertex64, Wt.vertex32 ) );
       };
       auto facePhase = [&]() {
          if ( nF == 0 ) return;
@@ -2837,13 +2927,17 @@ void sorT( Operator& op, Function& xf, const Function& bf, double omega, int lev
       };
       if ( !backward )
       {
-         vertexPhase();
-         edgePhase();
+         vertexEdgePhase();
          if ( level >= 2 )
          {
             facePhase();
-            refreshT< T, NR >( xf, level, 4 );
-            cellPhase();
+            if ( cellSorIsSmall< T, NR >( P ) ) // the cell workgroups refresh their own copies of the faces
+               cellSorPhase< T, NR >( M, P, wsel< T >( Wt.cell64, Wt.cell32 ), flag, u, b, om, false, chain, true );
+            else
+            {
+               refreshT< T, NR >( xf, level, 4 );
+               cellPhase();
+            }
          }
          else
             refreshT< T, NR >( xf, level, 2 | 4 ); // level 1: no face or cell interior; keep face copies current

@@ -401,6 +401,33 @@ void buildPlan( Mesh& M, LevelPlan& P, int level )
    if ( dev ) P.dVertexTerms.upload( P.hVertexTerms, s );
    if ( dev ) P.dVertexEnds.upload( P.hVertexEnds, s );
 
+   // the same vertex terms read from the closure copy of the first face of each edge (E2F entry of that face
+   // and edge), and each edge's end vertices with the edge that writes the vertex's own value (its first edge)
+   std::vector< int64_t > vsnap;
+   for ( int v = 0; v < T.nV(); v++ )
+      for ( int e : T.vEdges[v] )
+      {
+         const bool first = T.ev[e][0] == v;
+         const int  p     = first ? 1 : w - 2;
+         const int  f     = T.eFaces.at( e ).at( 0 );
+         int        le    = 0;
+         while ( le < 3 && e2f[3 * f + le].edge != e )
+            le++;
+         if ( le == 3 ) throw std::runtime_error( "plan validation failed: edge not in its face" );
+         const E2F& m = e2f[3 * f + le];
+         vsnap.push_back( P.faceOff( f ) + faceIndex( w, m.base[0] + p * m.inc[0], m.base[1] + p * m.inc[1] ) );
+      }
+   std::vector< int32_t > ends;
+   for ( int e = 0; e < T.nE(); e++ )
+   {
+      const int v0 = T.ev[e][0], v1 = T.ev[e][1];
+      ends.insert( ends.end(), { v0, v1, T.vEdges[v0].at( 0 ) == e ? 1 : 0, T.vEdges[v1].at( 0 ) == e ? 1 : 0 } );
+   }
+   for ( int64_t idx : vsnap )
+      if ( idx < P.offF || idx >= P.offC ) throw std::runtime_error( "plan validation failed: vertex snapshot term" );
+   if ( dev ) P.dVertexSnap.upload( vsnap, s );
+   if ( dev ) P.dEdgeEnds.upload( ends, s );
+
    validatePlan( T, P );
 
    std::vector< int32_t > ct( T.cType.begin(), T.cType.end() );
