@@ -365,7 +365,7 @@ constexpr int kMarchSor = 3; // MODE value of the in-place multicolour SOR pass
 // rounded to FP32 (the correction equation's right-hand side) and summed as r^2 in FP64, one partial per wave
 constexpr int kResidualSplit = 4;
 // finest level whose faces are small enough for one workgroup per face in the fused SOR phase
-constexpr int kFaceFusedMaxLevel = 4;
+constexpr int kFaceFusedMaxLevel = 5;
 constexpr int kFaceFusedMaxNp    = ( ( 1 << kFaceFusedMaxLevel ) - 2 ) * ( ( 1 << kFaceFusedMaxLevel ) - 1 ) / 2; // interior points
 
 // MODE kReplace / kAdd / kResidual: y = A u, y += A u, y = b - A u.
