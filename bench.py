@@ -172,11 +172,11 @@ def headline(H, n, L, tol, seed, cpo, steps, warmup):
 
 
 def cell_phase_roofline(H, S, A, L, prec, Ncell, reps):
-    """The SOR cell phase on level L in precision prec as the level runs it (fused launch or colour passes, as
-    selected), measured as two chained sweeps the way a V(2,2) smoothing sequence runs them; for the fused
-    phase also the tile kernel alone (unchained) and with its band copy (the first sweep of a sequence).
-    `achieved` = algorithmic bytes of the sequence (2 sweeps x 3 s per cell DoF: read x, b, write x) / its
-    measured time."""
+    """The SOR cell phase on level L in precision prec as the level runs it, measured as one smoothing sequence
+    of a V(2,2) cycle (two sweeps; selector 8): with the fused phase selected, two ping-pong passes (u -> scratch,
+    scratch -> u), else the colour passes of two sweeps.  `achieved` = algorithmic bytes of the sequence (2 sweeps
+    x 3 s per cell DoF: read x, b, write x) / its measured time.  For the fused phase the in-place form (tile
+    kernel alone and with its band copy, which a single sweep uses) is reported beside it."""
     s = 8 if prec == H.F64 else 4
     tn = "double" if s == 8 else "float"
     x, b = H.P1Function("rx", S, L, L, precision=prec), H.P1Function("rb", S, L, L, precision=prec)
@@ -188,21 +188,20 @@ def cell_phase_roofline(H, S, A, L, prec, Ncell, reps):
     seq_bytes = 2 * 3 * s * Ncell
     if fused:
         kname = f"kCellSorTile<{tn}>"
-        what = f"{kname} + kCellSorBand<{tn}>: SOR cell phase, all four colours in one launch"
+        what = f"{kname}: SOR cell phase, all four colours in one launch, two ping-pong passes"
     else:
         kname = KIND_NAMES[S.get_schedule(L)["kind"]].split(" ")[0] + f"<{tn}>"
-        what = f"{kname}: SOR cell phase as four colour passes"
+        what = f"{kname}: SOR cell phase as four colour passes, two sweeps"
     tag = f"cell_sor_{'tile' if fused else 'passes'}_{'f64' if s == 8 else 'f32'}_L{L}"
     r = {"bound": "hbm", "achieved": seq_bytes / (t_seq * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
          "traffic": traffic_from_profile(tag, kname)}
     r["frac"] = r["achieved"] / HBM_PEAK_GBS
-    r.update({"kernel": f"{what}, level {L}, two chained sweeps (one smoothing sequence of V(2,2))",
+    r.update({"kernel": f"{what}, level {L} (one smoothing sequence of V(2,2))",
               "algorithmic_bytes_per_sequence": seq_bytes, "ms_per_sequence": t_seq})
     if fused:
         t_tile = time_kernel(H, A, None, x, b, None, L, 7, reps)
         t_phase = time_kernel(H, A, None, x, b, None, L, 6, reps)
-        r.update({"ms_tile_unchained": t_tile, "ms_band_copy": t_phase - t_tile, "ms_tile_chained": t_seq - t_phase,
-                  "tile_unchained_frac": seq_bytes / 2 / (t_tile * 1e-3) / 1e9 / HBM_PEAK_GBS})
+        r.update({"in_place_sweep_ms": t_phase, "in_place_tile_ms": t_tile, "in_place_band_copy_ms": t_phase - t_tile})
     del x, b
     return r
 
@@ -253,7 +252,7 @@ def config2_vcycle(H, n, L, steps, warmup):
             "ms_per_vcycle": ms, "GDoF_per_s_per_vcycle": N / (ms * 1e-3) / 1e9, "vcycle_ms_events": t_vc,
             "rel_residual_after_cycles": float(np.sqrt(t.dotGlobal(t, L, H.INNER)) / r0),
             "cycles": 1 + max(1, warmup) + steps, "tuned": tuned_choices(S, L),
-            "cell_phase_two_chained_sweeps_ms": t_seq,
+            "cell_phase_two_sweeps_ms": t_seq,
             "cell_phase_GBs": (2 * 3 * 8 * Ncell / (t_seq * 1e-3) / 1e9) if t_seq else None}
 
 
@@ -401,7 +400,7 @@ def cpu_baseline(n, L, cycles, apply_runs, seed):
 def kernels_only(H, n, L, reps):
     """Every measured kernel of the headline workload alone, launched eagerly (no graphs): for rocprofv3 kernel
     traces and counter passes.  One V(2,2) cycle in FP32 and in FP64 (x = 0, b = the headline's right-hand
-    side), then `reps` launches each of the fused cell phase (two chained sweeps, FP32 and FP64), the FP64
+    side), then `reps` launches each of the SOR cell phase of two sweeps (FP32 and FP64), the FP64
     macro-cell apply and a plain copy of the level (the reference for scaling FETCH_SIZE / WRITE_SIZE)."""
     S = storage(H, n)
     A = H.P1ConstantLaplaceOperator(S, 0, L)

@@ -159,8 +159,9 @@ __device__ __forceinline__ int triRow32( int W, int f )
 // Face point (i, j) is copied to the cell point base + i a + j b of the pair's face-to-cell map.  Reads are
 // contiguous; all loads of a thread are issued before its stores.
 constexpr int kF2CBatch = 4;
+// u2 (may be null): a second array of the same layout whose cell closures receive the same copies
 template < typename T, int NR >
-__global__ __launch_bounds__( 256 ) void kF2C( LV g, const F2C* __restrict__ tab, T* __restrict__ u )
+__global__ __launch_bounds__( 256 ) void kF2C( LV g, const F2C* __restrict__ tab, T* __restrict__ u, T* __restrict__ u2 )
 {
    const F2C m  = tab[blockIdx.y];
    const int w  = g.w;
@@ -184,6 +185,13 @@ __global__ __launch_bounds__( 256 ) void kF2C( LV g, const F2C* __restrict__ tab
 #pragma unroll
    for ( int k = 0; k < kF2CBatch; k++ )
       if ( e0 + 256 * k < nV ) cu[o[k]] = v[k];
+   if ( u2 )
+   {
+      T* cu2 = u2 + ( g.offC + (int64_t) m.m.cell * g.cS ) * NR;
+#pragma unroll
+      for ( int k = 0; k < kF2CBatch; k++ )
+         if ( e0 + 256 * k < nV ) cu2[o[k]] = v[k];
+   }
 }
 
 // Closure refresh of several steps (V->E, E->F, F->C) in one launch.  Run one after another, each step copies
@@ -921,10 +929,15 @@ __device__ __forceinline__ bool sorTileBand( int lx, int ly, int i, int j, int T
 
 // Lanes: a wave updates positions of one colour offset c = (3x + 2y) mod 4 (plus 2 backward), so at every step
 // its colour, slab and LDS slot are uniform and live in scalar registers: wave v takes class v mod 4, 8 rows of
-// 8 positions 4 apart.  LDS rows are padded to R + PAD values (PAD = 1 keeps those reads free of bank
-// conflicts).  Loads and stores between memory and LDS walk the region and the core row by row instead.
+// 8 positions 4 apart.  Each 16 lanes of a wave take 4 rows x 4 positions (lanes 16 .. 31 the next 4 positions
+// of the same rows, lanes 32 .. 63 the next 4 rows): with rows padded to R + PAD values (PAD = 1) the 16 lanes
+// an 8-byte LDS read serves per cycle hit 32 distinct banks.  Loads and stores between memory and LDS walk the
+// region and the core row by row instead.
 constexpr int kSorTile = 24; // tile side: 32 x 32 region with the halo, 4 classes x 256 positions = 16 waves x 64
-template < typename T, int DP, int PAD, bool CHAINED >
+// MODE 0: in place, band positions to `band` (copied into u by kCellSorBand); 1: chained (halo from snapIn, band
+// positions to u and to `band`); 2: ping-pong (reads u, writes every updated position to `band`, reads no halo
+// from elsewhere: u is not written, so every tile sees start-of-sweep values)
+template < typename T, int DP, int PAD, int MODE >
 __global__ __launch_bounds__( 1024 ) void kCellSorTile( LV g, const MarchUnit* __restrict__ units, int64_t nUnits,
                                                       const int* __restrict__ slabTab, const T* __restrict__ W8,
                                                       const int* __restrict__ ctype, int flag, T* __restrict__ u,
@@ -935,6 +948,7 @@ __global__ __launch_bounds__( 1024 ) void kCellSorTile( LV g, const MarchUnit* _
    constexpr int NL = R * R / NT, RING = kSorTileRing;
    static_assert( R == 32 && NL == 1, "the lane mapping assumes a 32 x 32 region and 1024 threads" );
    __shared__ T lds[RING * RS];
+   __shared__ int sTab[kEdgeMaxW + 1];
    const int64_t ui = xcdBlock( nUnits );
    if ( ui >= nUnits ) return;
    const MarchUnit un = units[ui];
@@ -943,9 +957,12 @@ __global__ __launch_bounds__( 1024 ) void kCellSorTile( LV g, const MarchUnit* _
    const int64_t base = g.offC + (int64_t) un.cell * g.cS;
    // buffer views of the cell's arrays: loads and stores of dropped lanes need no branch, so the waits for
    // loads issued steps earlier can be counted exactly
+   for ( int i = (int) threadIdx.x; i <= w; i += NT )
+      sTab[i] = slabTab[i];
+   __syncthreads();
    const CellBuf< T > cu( u + base, t0 ), cband( band + base, t0 ), cb( b + base, t0 );
    // chained sweep: the band values of the previous sweep (snapIn), read for the halo
-   constexpr bool     chained = CHAINED;
+   constexpr bool     chained = MODE == 1, pingpong = MODE == 2;
    const CellBuf< T > csnap( chained ? snapIn + base : b + base, chained ? t0 : 0 );
    const T*           c = W8 + (int64_t) un.cell * 15;
    const T cE = c[dE], cW = c[dW], cN = c[dN], cS = c[dS], cNW = c[dNW], cSE = c[dSE];
@@ -956,8 +973,9 @@ __global__ __launch_bounds__( 1024 ) void kCellSorTile( LV g, const MarchUnit* _
    const int wave = __builtin_amdgcn_readfirstlane( tid >> 6 ), cls = wave & 3, sh = reverse ? 2 : 0;
    const int x0r = X0 - H, y0r = Y0 - H;
    // index of (x, y, s) in the cell = slabStart(s) + y (w + 1 - s) + x - y (y + 1) / 2; slabStart from the
-   // level's table (entries 0 .. w, a scalar load for a uniform s)
-   auto slabStart = [&]( int s ) { return slabTab[min( max( s, 0 ), w )]; };
+   // level's table (entries 0 .. w), copied into LDS first: a uniform LDS read instead of a scalar load, whose
+   // wait would also drain every LDS access in flight
+   auto slabStart = [&]( int s ) { return sTab[min( max( s, 0 ), w )]; };
 
    // row-major region positions (memory <-> LDS): LDS place, highest slab containing the point (-1: none) and
    // the slab-independent parts of its cell index
@@ -980,7 +998,9 @@ __global__ __launch_bounds__( 1024 ) void kCellSorTile( LV g, const MarchUnit* _
 #pragma unroll
    for ( int j = 0; j < NL; j++ )
    {
-      const int m = ( wave >> 2 ) * 64 + j * 64 + lane, ly = m >> 3, y = y0r + ly;
+      const int lq = lane & 15, lg = lane >> 4; // lane -> (row, position) as described above the kernel
+      const int lperm = ( ( lq >> 2 ) + ( ( lg >> 1 ) << 2 ) ) * 8 + ( lq & 3 ) + ( ( lg & 1 ) << 2 );
+      const int m = ( wave >> 2 ) * 64 + j * 64 + lperm, ly = m >> 3, y = y0r + ly;
       const int lx = 4 * ( m & 7 ) + ( ( 3 * ( cls - sh - 3 * x0r - 2 * y ) ) & 3 ), x = x0r + lx;
       uq[j]        = ly * RP + lx;
       kmax[j]      = min( min( lx - 1, R - 2 - lx ), min( ly - 1, R - 2 - ly ) );
@@ -1061,8 +1081,8 @@ __global__ __launch_bounds__( 1024 ) void kCellSorTile( LV g, const MarchUnit* _
          const bool wb = sw >= 1 && sw <= smax && ox[jc] + oy[jc] + sw <= w - 2;
          const T    v  = lds[swSlot * RS + ( wb ? oq[jc] : 0 )];
          const int  o  = wb ? swBase + oy[jc] * ( w + 1 - sw ) + obo[jc] : 0;
-         cband.store( o, v, wb && inBand[jc] );
-         cu.store( o, v, wb && ( chained || !inBand[jc] ) );
+         cband.store( o, v, wb && ( pingpong || inBand[jc] ) );
+         cu.store( o, v, wb && !pingpong && ( chained || !inBand[jc] ) );
       }
       // the wave's update: sweep position k on slab s, slots of s and its neighbours (uniform)
       const int k = kTab[P & 3], s = t - 2 * k;
@@ -1425,7 +1445,7 @@ __global__ void kFaceSor( LV g, const FaceGeom* __restrict__ fg, const T* __rest
 constexpr int kFaceGhostBatch = 2;
 template < typename T, bool WITH_E2F >
 __global__ __launch_bounds__( 256 ) void kFaceGhost( LV g, const FaceGeom* __restrict__ fg, const E2F* __restrict__ e2f,
-                                                     T* __restrict__ u, T* __restrict__ gh )
+                                                     T* __restrict__ u, T* __restrict__ gh, const T* __restrict__ cellSrc )
 {
    const FaceGeom& G  = fg[blockIdx.y];
    if ( WITH_E2F && blockIdx.x == gridDim.x - 1 )
@@ -1456,7 +1476,7 @@ __global__ __launch_bounds__( 256 ) void kFaceGhost( LV g, const FaceGeom* __res
          const int          y = M.base[1] + col * M.a[1] + row * M.b[1] + M.o[1];
          const int          z = M.base[2] + col * M.a[2] + row * M.b[2] + M.o[2];
          in[k][sd] = sd < nS && p < nP && x >= 0 && y >= 0 && z >= 0 && x + y + z <= w - 1;
-         v[k][sd]  = in[k][sd] ? u[g.offC + (int64_t) M.cell * g.cS + cellRel( w, t0, x, y, z )] : T( 0 );
+         v[k][sd]  = in[k][sd] ? cellSrc[g.offC + (int64_t) M.cell * g.cS + cellRel( w, t0, x, y, z )] : T( 0 );
       }
    }
 #pragma unroll
@@ -1666,7 +1686,8 @@ __global__ __launch_bounds__( kEdgeThreads ) void kEdgeSorFused( LV g, const Edg
                                                         int flag, T* __restrict__ u, const T* __restrict__ b, T omega,
                                                         int reverse, const VertexGeom* __restrict__ vg,
                                                         const int64_t* __restrict__ vsnap,
-                                                        const int32_t* __restrict__ ends, const T* __restrict__ VW )
+                                                        const int32_t* __restrict__ ends, const T* __restrict__ VW,
+                                                        const T* __restrict__ cellSrc )
 {
    __shared__ int64_t sPrim[kEdgeTermsMax];                  // element offset of the neighbour primitive
    __shared__ int     sB[3][kEdgeTermsMax], sI[3][kEdgeTermsMax], sKind[kEdgeTermsMax];
@@ -1758,7 +1779,7 @@ __global__ __launch_bounds__( kEdgeThreads ) void kEdgeSorFused( LV g, const Edg
             const int k = min( k0 + q, nT - 1 );
             const int c0 = sB[0][k] + x * sI[0][k], c1 = sB[1][k] + x * sI[1][k];
             const int rel = sKind[k] == 0 ? faceRel( w, c0, c1 ) : cellRel( w, t0, c0, c1, sB[2][k] + x * sI[2][k] );
-            v[q] = u[sPrim[k] + rel];
+            v[q] = ( sKind[k] == 0 ? (const T*) u : cellSrc )[sPrim[k] + rel]; // macro-cell terms from cellSrc
          }
 #pragma unroll
          for ( int q = 0; q < kEdgeTermBatch; q++ )
@@ -2716,16 +2737,48 @@ void cellSorFused( Mesh& M, LevelPlan& P, const T* CW, int flag, T* u, const T* 
    {
    case kSorTile:
       if ( snapIn )
-         hipLaunchKernelGGL( ( kCellSorTile< T, 3, 1, true > ), grid, dim3( 1024 ), 0, s, g, un, n, P.sorSlabStart.p, CW,
+         hipLaunchKernelGGL( ( kCellSorTile< T, 3, 1, 1 > ), grid, dim3( 1024 ), 0, s, g, un, n, P.sorSlabStart.p, CW,
                              P.dCellType.p, flag, u, band, snapIn, b, omega, reverse );
       else
-         hipLaunchKernelGGL( ( kCellSorTile< T, 3, 1, false > ), grid, dim3( 1024 ), 0, s, g, un, n, P.sorSlabStart.p, CW,
+         hipLaunchKernelGGL( ( kCellSorTile< T, 3, 1, 0 > ), grid, dim3( 1024 ), 0, s, g, un, n, P.sorSlabStart.p, CW,
                              P.dCellType.p, flag, u, band, snapIn, b, omega, reverse );
       if ( withBand )
          hipLaunchKernelGGL( ( kCellSorBand< T, kSorTile > ), bandGrid, dim3( 256 ), 0, s, g, nC, P.dCellType.p, flag, u,
                              band );
       break;
    default: throw std::logic_error( "bad cell SOR tile" );
+   }
+}
+
+// The fused cell phase of a ping-pong sweep: reads the macro-cell interiors (and closures) of src, writes every
+// updated interior value to dst (src is not written, so no band and no copy are needed)
+template < typename T >
+void cellSorPingPong( Mesh& M, LevelPlan& P, const T* CW, int flag, const T* src, T* dst, const T* b, T omega )
+{
+   if ( !P.sorUnits ) throw std::logic_error( "fused cell SOR phase launched before its buffers were prepared" );
+   if ( P.sorUnitCount == 0 ) return;
+   const dim3 grid( (unsigned) xcdGrid( P.sorUnitCount ) );
+   hipLaunchKernelGGL( ( kCellSorTile< T, 3, 1, 2 > ), grid, dim3( 1024 ), 0, M.ctx->stream, viewOf( P ), P.sorUnits->p,
+                       P.sorUnitCount, P.sorSlabStart.p, CW, P.dCellType.p, flag, const_cast< T* >( src ), dst,
+                       (const T*) nullptr, b, omega, 0 );
+}
+
+// Whether a forward smoothing sequence on this level can run its sweeps in ping-pong pairs: the fused cell phase
+// is selected for T (one right-hand side), the face phase reads the cells through ghost layers (levels above the
+// one-workgroup-per-face phase), and every macro-cell is updated (a cell the flag leaves alone would leave the
+// ping-pong array stale)
+template < typename T, int NR >
+bool pingPongOk( Mesh& M, const LevelPlan& P, int flag )
+{
+   if constexpr ( NR != 1 )
+      return false;
+   else
+   {
+      if ( !( P.sorTile > 0 && ( P.sorPrecisions & ( std::is_same< T, double >::value ? 1 : 2 ) ) ) ) return false;
+      if ( P.L <= kFaceFusedMaxLevel ) return false;
+      for ( int c = 0; c < M.T.nC(); c++ )
+         if ( !( M.T.cType[c] & flag ) ) return false;
+      return true;
    }
 }
 
@@ -2810,7 +2863,7 @@ void refreshT( Function& f, int level, int mask )
       hipLaunchKernelGGL( ( kF2C< T, NR > ),
                           dim3( (unsigned) ( ( (int64_t) P.w * ( P.w + 1 ) / 2 * NR + 256 * kF2CBatch - 1 ) / ( 256 * kF2CBatch ) ),
                                 4 * M.T.nC() ),
-                          dim3( 256 ), 0, s, g, P.dF2C.p, u );
+                          dim3( 256 ), 0, s, g, P.dF2C.p, u, (T*) nullptr );
 }
 
 template < typename T, int NR >
@@ -2863,9 +2916,14 @@ void applyT( Operator& op, Function& src, Function& dst, const Function* rhs, in
 // in reverse; each class reads the higher classes' new values straight from their arrays and lower-class
 // values from closure copies, which are refreshed only after the sweep.  Either way every closure copy is
 // current when the sweep ends.
+// pp (forward sweeps on levels where pingPongOk holds): 0 in place; 1 = the macro-cell interiors are read from
+// u and their new values written to the level's ping-pong array B (sorBand); 2 = read from B and written to u.
+// Within the sweep every kernel that reads macro-cell interiors (edge terms, face ghost layers) reads the array
+// that holds them at that point, and the face-to-cell refresh writes the closures of both arrays, so B is at every
+// point what u would be in place.
 template < typename T, int NR >
 void sorT( Operator& op, Function& xf, const Function& bf, double omega, int level, int flag, bool refreshFirst,
-           bool backward = false, int chain = 0 )
+           bool backward = false, int chain = 0, int pp = 0 )
 {
    Mesh&       M  = *op.mesh;
    LevelPlan&  P  = M.plan( level );
@@ -2879,6 +2937,10 @@ void sorT( Operator& op, Function& xf, const Function& bf, double omega, int lev
    if ( refreshFirst ) refreshT< T, NR >( xf, level, 7 );
    if ( NR == 1 && level >= 1 )
    {
+      if ( pp && ( backward || !pingPongOk< T, NR >( M, P, flag ) ) )
+         throw std::logic_error( "ping-pong sweep on a level or sweep that cannot run one" );
+      T*        ppArr       = pp ? reinterpret_cast< T* >( P.sorBand.p ) : nullptr;
+      const T*  cellSrc     = pp == 2 ? ppArr : u; // where the macro-cell interiors are during this sweep
       const int rev         = backward ? 1 : 0;
       auto      vertexPhase = [&]() {
          hipLaunchKernelGGL( ( kVertexSorPush< T > ), dim3( nV ), dim3( 64 ), 0, s, g, P.dVertexGeom.p,
@@ -2889,14 +2951,14 @@ void sorT( Operator& op, Function& xf, const Function& bf, double omega, int lev
          if ( P.w > kEdgeMaxW ) throw std::invalid_argument( "the fused edge SOR supports levels up to 10" );
          hipLaunchKernelGGL( ( kEdgeSorFused< T, false > ), dim3( nE ), dim3( kEdgeThreads ), 0, s, g, P.dEdgeGeom.p,
                              P.dEdgeTerms.p, wsel< T >( Wt.edge64, Wt.edge32 ), flag, u, b, om, rev, nullptr, nullptr,
-                             nullptr, nullptr );
+                             nullptr, nullptr, cellSrc );
       };
       // forward: the vertex phase computed inside the edge workgroups (one launch for both phases)
       auto vertexEdgePhase = [&]() {
          if ( P.w > kEdgeMaxW ) throw std::invalid_argument( "the fused edge SOR supports levels up to 10" );
          hipLaunchKernelGGL( ( kEdgeSorFused< T, true > ), dim3( nE ), dim3( kEdgeThreads ), 0, s, g, P.dEdgeGeom.p,
                              P.dEdgeTerms.p, wsel< T >( Wt.edge64, Wt.edge32 ), flag, u, b, om, rev, P.dVertexGeom.p,
-                             P.dVertexSnap.p, P.dEdgeEnds.p, wsel< T >( Wt.vertex64, Wt.vertex32 ) );
+                             P.dVertexSnap.p, P.dEdgeEnds.p, wsel< T >( Wt.vertex64, Wt.vertex32 ), cellSrc );
       };
       auto facePhase = [&]() {
          if ( nF == 0 ) return;
@@ -2912,10 +2974,10 @@ void sorT( Operator& op, Function& xf, const Function& bf, double omega, int lev
             const unsigned chunks = ( P.w * ( P.w + 1 ) / 2 + 256 * kFaceGhostBatch - 1 ) / ( 256 * kFaceGhostBatch );
             if ( !backward )
                hipLaunchKernelGGL( ( kFaceGhost< T, true > ), dim3( chunks + 1, nF ), dim3( 256 ), 0, s, g, P.dFaceGeom.p,
-                                   P.dE2F.p, u, gh );
+                                   P.dE2F.p, u, gh, cellSrc );
             else
                hipLaunchKernelGGL( ( kFaceGhost< T, false > ), dim3( chunks, nF ), dim3( 256 ), 0, s, g, P.dFaceGeom.p,
-                                   P.dE2F.p, u, gh );
+                                   P.dE2F.p, u, gh, cellSrc );
             for ( int cc = 0; cc < 3; cc++ )
                hipLaunchKernelGGL( ( kFaceSorRows< T > ), dim3( P.w - 3, nF ), dim3( 64 ), 0, s, g, P.dFaceGeom.p,
                                    wsel< T >( Wt.face64, Wt.face32 ), flag, u, b, om, backward ? 2 - cc : cc,
@@ -2931,7 +2993,21 @@ void sorT( Operator& op, Function& xf, const Function& bf, double omega, int lev
          if ( level >= 2 )
          {
             facePhase();
-            if ( cellSorIsSmall< T, NR >( P ) ) // the cell workgroups refresh their own copies of the faces
+            if ( pp )
+            {
+               // face-to-cell refresh into the closures of u and of B, then the cell phase from one to the other
+               hipLaunchKernelGGL( ( kF2C< T, 1 > ),
+                                   dim3( (unsigned) ( ( (int64_t) P.w * ( P.w + 1 ) / 2 + 256 * kF2CBatch - 1 ) /
+                                                      ( 256 * kF2CBatch ) ),
+                                         4 * M.T.nC() ),
+                                   dim3( 256 ), 0, s, g, P.dF2C.p, u, ppArr );
+               const T* CW = wsel< T >( Wt.cell64, Wt.cell32 );
+               if ( pp == 1 )
+                  cellSorPingPong< T >( M, P, CW, flag, u, ppArr, b, om );
+               else
+                  cellSorPingPong< T >( M, P, CW, flag, ppArr, u, b, om );
+            }
+            else if ( cellSorIsSmall< T, NR >( P ) ) // the cell workgroups refresh their own copies of the faces
                cellSorPhase< T, NR >( M, P, wsel< T >( Wt.cell64, Wt.cell32 ), flag, u, b, om, false, chain, true );
             else
             {
@@ -2978,10 +3054,27 @@ void sorT( Operator& op, Function& xf, const Function& bf, double omega, int lev
    }
 }
 
+// One smoothing sequence of `count` forward sweeps.  Where pingPongOk holds, the sweeps run in pairs (u -> B,
+// B -> u), after one in-place sweep if count is odd; otherwise sweep k is chained to sweep k - 1.
 template < typename T, int NR >
 void sorSweepsT( Operator& op, Function& xf, const Function& bf, double omega, int level, int flag, int count,
                  bool refreshFirst )
 {
+   Mesh&      M = *op.mesh;
+   LevelPlan& P = M.plan( level );
+   if ( count >= 2 && pingPongOk< T, NR >( M, P, flag ) )
+   {
+      M.prepareCellSor( P );
+      int k = 0;
+      if ( count % 2 == 1 )
+         sorT< T, NR >( op, xf, bf, omega, level, flag, refreshFirst, false, 0 ), k = 1;
+      for ( ; k < count; k += 2 )
+      {
+         sorT< T, NR >( op, xf, bf, omega, level, flag, refreshFirst && k == 0, false, 0, 1 );
+         sorT< T, NR >( op, xf, bf, omega, level, flag, false, false, 0, 2 );
+      }
+      return;
+   }
    for ( int k = 0; k < count; k++ ) // one smoothing sequence: sweep k is chained to sweep k - 1
       sorT< T, NR >( op, xf, bf, omega, level, flag, refreshFirst && k == 0, false, k );
 }
@@ -3422,9 +3515,24 @@ struct CellSorSequenceF
       Mesh&      M  = *op.mesh;
       LevelPlan& P  = M.plan( level );
       auto&      Wt = op.weights( level );
+      const T*   CW = wsel< T >( Wt.cell64, Wt.cell32 );
+      T*         u  = (T*) x.ptr( level );
+      const T*   bb = (const T*) b.ptr( level );
+      if ( count >= 2 && pingPongOk< T, NR >( M, P, flag ) ) // as the smoother runs them: ping-pong pairs
+      {
+         M.prepareCellSor( P );
+         T*  B = reinterpret_cast< T* >( P.sorBand.p );
+         int k = 0;
+         if ( count % 2 == 1 ) cellSorPhase< T, NR >( M, P, CW, flag, u, bb, (T) omega, false, 0 ), k = 1;
+         for ( ; k < count; k += 2 )
+         {
+            cellSorPingPong< T >( M, P, CW, flag, u, B, bb, (T) omega );
+            cellSorPingPong< T >( M, P, CW, flag, B, u, bb, (T) omega );
+         }
+         return;
+      }
       for ( int k = 0; k < count; k++ ) // sweep k of a smoothing sequence is chained to sweep k - 1
-         cellSorPhase< T, NR >( M, P, wsel< T >( Wt.cell64, Wt.cell32 ), flag, (T*) x.ptr( level ),
-                                (const T*) b.ptr( level ), (T) omega, false, k );
+         cellSorPhase< T, NR >( M, P, CW, flag, u, bb, (T) omega, false, k );
    }
 };
 template < typename T, int NR >

@@ -227,7 +227,8 @@ int hymi_function_download_owned_column( hymi_function f, int level, int column,
  * pass (x, b; colour 0), 2 full apply (x -> y), 3 full SOR sweep (x, b), 4 one V-cycle of gmg (x, b),
  * 5 plain copy y = x of the whole level (a bandwidth reference), 6 the SOR cell phase as configured for the level
  * (hymi_mesh_set_cell_sor; x, b), 7 the fused cell phase's tile kernel alone (x, b), 8 the cell phases of two
- * chained sweeps, as one smoothing sequence of a V(2, 2) cycle runs them (x, b). */
+ * sweeps as one smoothing sequence of a V(2, 2) cycle runs them (with the fused phase: two ping-pong passes
+ * through the level's scratch array; x, b). */
 int hymi_time_kernel( hymi_operator op, hymi_gmg gmg, hymi_function x, hymi_function b, hymi_function y, int level,
                       int which, int reps, double* ms_per_launch );
 
