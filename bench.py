@@ -340,15 +340,18 @@ def cpu_baseline(n, L, cycles, apply_runs, seed):
     """The CPU oracle (the reference's algorithm, lexicographic SOR, OpenMP over macro-primitives) on a bounded
     sample of the headline workload: the stencil apply (best of apply_runs runs, each at least 1 s of
     back-to-back applies after a warm-up, as ApplyBenchmark.cpp:78-94 times it) and `cycles` FP64 V(2,2)
-    cycles from x = 0 on the same right-hand side, the mean of cycles 2.. (MultigridStudies.cpp:243-255 logs
-    the per-cycle residual ratio, recorded here too)."""
+    cycles from x = 0 on the same right-hand side, the mean of cycles 2.. .  Per cycle it records, as
+    MultigridStudies.cpp:228-255 computes them, the discrete L2 residual and the discrete L2 error against
+    u = sin(pi x) sin(pi y) sin(pi z) (sums over inner DoFs divided by the number of all DoFs), and the
+    residual ratio."""
     import oracle as O
     m = O.Mesh.cuboid(n)
     A = O.Operator(m, 0, L)
     M = O.Operator(m, 0, L, 1)
     x, b, t = O.Function(m, 0, L), O.Function(m, 0, L), O.Function(m, 0, L)
     X = m.owned_coords(L)
-    t.set(L, 3 * np.pi ** 2 * np.prod(np.sin(np.pi * X), axis=1))
+    u_exact = np.prod(np.sin(np.pi * X), axis=1)
+    t.set(L, 3 * np.pi ** 2 * u_exact)
     M.apply(t, b, L, flag=O.ALL)
     bv = b.get(L)
     bv += 1e-3 * np.random.default_rng(seed).uniform(-1.0, 1.0, bv.shape)
@@ -371,13 +374,17 @@ def cpu_baseline(n, L, cycles, apply_runs, seed):
     def resnorm():
         A.apply(x, t, L, flag=O.INNER)
         r = b.get(L) - t.get(L)
-        inner = owned_inner
-        return float(np.sqrt(np.sum(r[inner] ** 2)))
+        return float(np.sqrt(np.sum(r[owned_inner] ** 2)))
+
+    def l2_error():
+        e = u_exact - x.get(L)
+        return float(np.sqrt(np.sum(e[owned_inner] ** 2) / n_all))
 
     owned_inner = m.owned_types(L) == O.INNER
+    n_all = len(owned_inner)
     bkeep = b.get(L)
     g = O.GMG(m, 0, L, 2, 2, 1.0, 0)
-    res = [resnorm()]
+    res, err = [resnorm()], [l2_error()]
     times = []
     for _ in range(cycles):
         t0 = O.wtime()
@@ -385,6 +392,7 @@ def cpu_baseline(n, L, cycles, apply_runs, seed):
         times.append(O.wtime() - t0)
         b.set(L, bkeep)
         res.append(resnorm())
+        err.append(l2_error())
     N = interior_count(n, L)
     dt = float(np.mean(times[1:])) if cycles > 1 else times[0]
     Ncell = cell_interior_count(n, L)
@@ -393,6 +401,7 @@ def cpu_baseline(n, L, cycles, apply_runs, seed):
                       f"x = 0, value = unknowns / mean time of cycles 2..{cycles} ({dt:.2f} s per cycle); apply "
                       f"best of {apply_runs} runs of >= 1 s",
             "vcycle_seconds": times, "residual_ratio_per_cycle": [res[k + 1] / res[k] for k in range(cycles)],
+            "l2_residual_per_cycle": [r / np.sqrt(n_all) for r in res[1:]], "l2_error_per_cycle": err[1:],
             "apply_ms": best * 1e3, "apply_GDoF_per_s": N / best / 1e9,
             "apply_cell_GBs": 2 * 8 * Ncell / best / 1e9}
 
@@ -421,6 +430,11 @@ def kernels_only(H, n, L, reps):
     return out
 
 
+def stage(what):
+    """One progress line on stderr per finished part of the run (stdout carries only the JSON line)."""
+    print(f"[bench] {what} done", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -432,7 +446,7 @@ def main():
     ap.add_argument("--cycles-per-step", type=int, default=4, help="FP32 V-cycles per FP64 refinement step")
     ap.add_argument("--config2-level", type=int, default=7)
     ap.add_argument("--config2-steps", type=int, default=20)
-    ap.add_argument("--cpu-cycles", type=int, default=4)
+    ap.add_argument("--cpu-cycles", type=int, default=10)
     ap.add_argument("--cpu-apply-runs", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-config2", action="store_true")
@@ -456,6 +470,7 @@ def main():
         print(json.dumps(kernels_only(H, n, L, max(3, args.steps))))
         return
     hl = headline(H, n, L, args.tol, SEED, args.cycles_per_step, args.steps, args.warmup)
+    stage("headline solves")
     N = hl["N"]
     Ncell = cell_interior_count(n, L)
     S, A = hl["S"], hl["A"]
@@ -463,6 +478,7 @@ def main():
     roof32 = cell_phase_roofline(H, S, A, L, H.F32, Ncell, reps)
     roof64 = cell_phase_roofline(H, S, A, L, H.F64, Ncell, reps)
     roof_apply = apply_roofline(H, S, A, L, Ncell, reps)
+    stage("kernel rooflines")
     for k in ("x", "b", "t", "S", "A"):
         hl.pop(k)
     del S, A
@@ -503,10 +519,13 @@ def main():
     }
     if not args.no_config2:
         out["config2_vcycle"] = config2_vcycle(H, n, args.config2_level, args.config2_steps, 3)
+        stage("level-7 FP64 V(3,3) cycles")
     if not args.no_batched:
         out["batched_rhs"] = batched_rhs(H, n, 6, 32, 1e-8, SEED, args.cycles_per_step)
+        stage("32 right-hand sides")
     if not args.no_serving:
         out["serving"] = serving_run(args.serving_requests, 200.0, args.serving_slots, SEED)
+        stage("serving")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, L, args.cpu_cycles, args.cpu_apply_runs, SEED)
     print(json.dumps(out))

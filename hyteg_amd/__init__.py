@@ -155,18 +155,17 @@ class PrimitiveStorage:
         -1 / 0 = level default."""
         _lib.call("hymi_mesh_set_schedule", self.h, level, kind, rows, waves, block)
 
-    def use_column_schedules(self, maxLevel, minLevel=5, fusedSorFrom=7, fusedSorFp32From=8):
+    def use_column_schedules(self, maxLevel, minLevel=5, fusedSorFrom=7, fusedSorFp32From=7):
         """Select the column march (kind 3) with its measured best parameters on levels minLevel..maxLevel:
         16 slabs per unit and 4 waves per workgroup from level 7 up, 8 and 4 on level 6, 4 and 2 below;
         loads one slab ahead.  The SOR cell phase runs as one fused launch (set_cell_sor(level, 24)) for FP64
         from level fusedSorFrom up and for FP32 from level fusedSorFp32From up: where it beats four colour
-        passes.  FP32 gains less from it, since the fused march is bound by step latency rather than bytes."""
+        passes (the library defaults).  On other levels of the range it is switched off."""
         for L in range(minLevel, maxLevel + 1):
             rows, waves = (16, 4) if L >= 7 else (8, 4) if L == 6 else (4, 2)
             self.set_schedule(L, rows, waves, 1, 3)
             fp64, fp32 = L >= max(fusedSorFrom, 3), L >= max(fusedSorFp32From, 3)
-            if fp64 or fp32:
-                self.set_cell_sor(L, 24, fp64=fp64, fp32=fp32)
+            self.set_cell_sor(L, 24 if fp64 or fp32 else 0, fp64=fp64, fp32=fp32)
 
     def autotune(self, level, fp64=True, fp32=True, reps=5):
         """Measure the launch schedules of the macro-cell kernels and the fused SOR cell phase on `level` and keep

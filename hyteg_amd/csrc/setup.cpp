@@ -522,9 +522,9 @@ MarchSchedule defaultSchedule( int level )
    return S;
 }
 
-// Measured default of the fused SOR cell phase (tile 24): FP64 from level 7 up, FP32 from level 8 up, where it
-// beats the four colour passes; returns the precision bits (0: colour passes)
-int defaultCellSorPrecisions( int level ) { return ( level >= 7 ? 1 : 0 ) | ( level >= 8 ? 2 : 0 ); }
+// Measured default of the fused SOR cell phase (tile 24): both precisions from level 7 up, where (run as ping-pong
+// pairs) it beats the four colour passes; returns the precision bits (0: colour passes)
+int defaultCellSorPrecisions( int level ) { return level >= 7 ? 3 : 0; }
 
 void checkSchedule( const MarchSchedule& S, int level )
 {

@@ -249,7 +249,7 @@ int hymi_mesh_get_schedule( hymi_mesh mesh, int level, int out[4] );
 /* SOR cell phase of one level: tile = 0 runs the four colours as four passes; tile = 24 runs all four in
  * one launch over 24 x 24 xy tiles marched along z (levels >= 3, functions with one right-hand side whose
  * precision is in `precisions`: bit 0 FP64, bit 1 FP32; others keep the passes; same values bit for bit).
- * The default is tile 24 for FP64 from level 7 up and for FP32 from level 8 up, the four passes elsewhere.
+ * The default is tile 24 for both precisions from level 7 up, the four passes elsewhere.
  * get returns { tile, precisions }.  No reference counterpart: the reference's smooth_sor runs the macro-cell
  * interior point by point (src/hyteg/p1functionspace/P1Operator.hpp:341-420) */
 int hymi_mesh_set_cell_sor( hymi_mesh mesh, int level, int tile, int precisions );

@@ -161,16 +161,18 @@ def test_cell_sor_setting(H, hctx):
     S.set_cell_sor(6, 24)
     S.set_cell_sor(3, 24, fp32=True)
     S.set_cell_sor(8, 24, fp64=False, fp32=True)
-    # level 7 keeps the library default: the fused phase for FP64
-    assert [S.get_cell_sor_setting(L) for L in (6, 3, 8, 7)] == [(24, 1), (24, 3), (24, 2), (24, 1)]
+    # level 7 keeps the library default: the fused phase for both precisions
+    assert [S.get_cell_sor_setting(L) for L in (6, 3, 8, 7)] == [(24, 1), (24, 3), (24, 2), (24, 3)]
     S.set_cell_sor(6, 0)
     assert S.get_cell_sor(6) == 0
     for level, tile, kw in ((6, 8, {}), (6, 16, {}), (6, 32, {}), (2, 24, {}), (6, 24, {"fp64": False})):
         with pytest.raises(H.HymiError):
             S.set_cell_sor(level, tile, **kw)
     S2 = H.PrimitiveStorage(H.MeshInfo.meshCuboid((0, 0, 0), (1, 1, 1), 1, 1, 1), hctx)
-    assert [S2.get_cell_sor_setting(L) for L in (5, 6, 7, 8)] == [(0, 0), (0, 0), (24, 1), (24, 3)]
+    assert [S2.get_cell_sor_setting(L) for L in (5, 6, 7, 8)] == [(0, 0), (0, 0), (24, 3), (24, 3)]
     assert [S2.get_schedule(L)["kind"] for L in (3, 4, 5, 6, 7, 8)] == [1, 1, 3, 3, 3, 3]
     assert [(S2.get_schedule(L)["rows"], S2.get_schedule(L)["waves"]) for L in (5, 6, 7)] == [(4, 2), (8, 4), (16, 4)]
     S.use_column_schedules(8)
-    assert [S.get_cell_sor_setting(L) for L in (5, 6, 7, 8)] == [(0, 0), (0, 0), (24, 1), (24, 3)]
+    assert [S.get_cell_sor_setting(L) for L in (5, 6, 7, 8)] == [(0, 0), (0, 0), (24, 3), (24, 3)]
+    S.use_column_schedules(8, fusedSorFp32From=8)
+    assert [S.get_cell_sor_setting(L) for L in (6, 7, 8)] == [(0, 0), (24, 1), (24, 3)]
