@@ -101,6 +101,7 @@ struct LevelPlan
    DevBuf< V2E >        dV2E;
    DevBuf< E2F >        dE2F;
    DevBuf< F2C >        dF2C;
+   DevBuf< CellFaceRead > dCellFaceRead; // per cell (read by the ping-pong cell phase)
    DevBuf< FaceGeom >   dFaceGeom;
    DevBuf< EdgeGeom >   dEdgeGeom;
    DevBuf< EdgeTerm >   dEdgeTerms;

@@ -194,6 +194,37 @@ __global__ __launch_bounds__( 256 ) void kF2C( LV g, const F2C* __restrict__ tab
    }
 }
 
+// Face-to-cell copy of the points within distance 1 of a face's boundary only (rows 0 and 1, and the first two and
+// last two points of every other row: 6w - 9 points), from the faces of u into the cell closures of dst.  These are
+// the closure points that the edge terms and the face ghost layers of the next sweep read from a macro-cell (the
+// fused cell phase reads closure points from the faces themselves).  Block = (chunk of 256 points, cell face).
+template < typename T >
+__global__ __launch_bounds__( 256 ) void kF2CBand( LV g, const F2C* __restrict__ tab, const T* __restrict__ u,
+                                                   T* __restrict__ dst )
+{
+   const F2C m  = tab[blockIdx.y];
+   const int w  = g.w, nb = 6 * w - 9;
+   const int e  = (int) blockIdx.x * 256 + (int) threadIdx.x;
+   if ( e >= nb ) return;
+   int i, j;
+   if ( e < w )
+      j = 0, i = e;
+   else if ( e < 2 * w - 1 )
+      j = 1, i = e - w;
+   else
+   {
+      const int k = ( e - ( 2 * w - 1 ) ) & 3;
+      j           = 2 + ( ( e - ( 2 * w - 1 ) ) >> 2 );
+      i           = k < 2 ? k : w - j - 4 + k;
+   }
+   if ( i < 0 || i > w - 1 - j ) return; // short rows near the top corner
+   const int x = m.m.base[0] + i * m.m.a[0] + j * m.m.b[0];
+   const int y = m.m.base[1] + i * m.m.a[1] + j * m.m.b[1];
+   const int z = m.m.base[2] + i * m.m.a[2] + j * m.m.b[2];
+   dst[g.offC + (int64_t) m.m.cell * g.cS + crow32( w, tet32( w ), y, z ) + x] =
+       u[g.offF + (int64_t) m.face * g.fS + faceRel( w, i, j )];
+}
+
 // Closure refresh of several steps (V->E, E->F, F->C) in one launch.  Run one after another, each step copies
 // what the previous one refreshed; here every copy reads the primitive that owns the value instead: an edge
 // end copy the vertex (when V->E is part of the refresh, else the edge's end copy as it stands), a face
@@ -935,13 +966,18 @@ __device__ __forceinline__ bool sorTileBand( int lx, int ly, int i, int j, int T
 // region and the core row by row instead.
 constexpr int kSorTile = 24; // tile side: 32 x 32 region with the halo, 4 classes x 256 positions = 16 waves x 64
 // MODE 0: in place, band positions to `band` (copied into u by kCellSorBand); 1: chained (halo from snapIn, band
-// positions to u and to `band`); 2: ping-pong (reads u, writes every updated position to `band`, reads no halo
-// from elsewhere: u is not written, so every tile sees start-of-sweep values)
+// positions to u and to `band`); 2: ping-pong (reads the interior from u, writes every updated position to
+// `band`, reads no halo from elsewhere: u is not written, so every tile sees start-of-sweep values).  In mode 2
+// the closure points (the cell's four boundary planes) are read from the face arrays of fsrc through the
+// inverse face maps cfr, so the cell closures of u need not be current.
+// FP32: at most 64 VGPRs, so two workgroups (32 waves) fit on a CU (the LDS ring takes 46 KB each); FP64 runs
+// one workgroup per CU (88 KB of LDS)
 template < typename T, int DP, int PAD, int MODE >
-__global__ __launch_bounds__( 1024 ) void kCellSorTile( LV g, const MarchUnit* __restrict__ units, int64_t nUnits,
+__global__ __launch_bounds__( 1024, sizeof( T ) == 4 ? 8 : 4 ) void kCellSorTile( LV g, const MarchUnit* __restrict__ units, int64_t nUnits,
                                                       const int* __restrict__ slabTab, const T* __restrict__ W8,
                                                       const int* __restrict__ ctype, int flag, T* __restrict__ u,
                                                       T* __restrict__ band, const T* __restrict__ snapIn,
+                                                      const T* __restrict__ fsrc, const CellFaceRead* __restrict__ cfr,
                                                       const T* __restrict__ b, T omega, int reverse )
 {
    constexpr int TS = kSorTile, H = kSorTileHalo, R = TS + 2 * H, RP = R + PAD, RS = R * RP, NT = 1024;
@@ -964,6 +1000,8 @@ __global__ __launch_bounds__( 1024 ) void kCellSorTile( LV g, const MarchUnit* _
    // chained sweep: the band values of the previous sweep (snapIn), read for the halo
    constexpr bool     chained = MODE == 1, pingpong = MODE == 2;
    const CellBuf< T > csnap( chained ? snapIn + base : b + base, chained ? t0 : 0 );
+   // ping-pong: the face arrays (all faces of the level), read for the closure points
+   const CellBuf< T > cf( pingpong ? fsrc + g.offF : b + base, pingpong ? g.offC - g.offF : 0 );
    const T*           c = W8 + (int64_t) un.cell * 15;
    const T cE = c[dE], cW = c[dW], cN = c[dN], cS = c[dS], cNW = c[dNW], cSE = c[dSE];
    const T cTC = c[dTC], cTW = c[dTW], cTS = c[dTS], cTSE = c[dTSE], cBC = c[dBC], cBN = c[dBN], cBE = c[dBE],
@@ -981,6 +1019,17 @@ __global__ __launch_bounds__( 1024 ) void kCellSorTile( LV g, const MarchUnit* _
    // the slab-independent parts of its cell index
    // and, in a chained sweep, the highest slab in which a halo point is interior (read from snapIn there)
    int lq[NL], lsx[NL], ly_[NL], lbo[NL], lsnap[NL];
+   // ping-pong: face-array offset of the position's point on the plane x + y + z = w - 1 (fdiag); for a position
+   // on x = 0 or y = 0 (fcol), the point of slab s is (i, j) = (fcA + fcS s, fcB + fcT s) of its face, whose
+   // offset is folded into fcA (the slopes fcS, fcT in {-1, 0, 1} packed into fcST as (fcS + 1) | (fcT + 1) << 2)
+   int  fdiag[NL], fcA[NL], fcB[NL], fcST[NL];
+   bool fcol[NL];
+   const CellFaceRead* fr = pingpong ? cfr + un.cell : nullptr;
+   const int           fS = (int) g.fS;
+   auto faceAt = [&]( int pl, int pu, int pv ) {
+      const int* c = fr->c[pl];
+      return fr->face[pl] * fS + faceRel( w, c[0] + c[1] * pu + c[2] * pv, c[3] + c[4] * pu + c[5] * pv );
+   };
 #pragma unroll
    for ( int j = 0; j < NL; j++ )
    {
@@ -991,6 +1040,16 @@ __global__ __launch_bounds__( 1024 ) void kCellSorTile( LV g, const MarchUnit* _
       ly_[j]          = y;
       lbo[j]          = x - ( ( y * ( y + 1 ) ) >> 1 );
       lsnap[j]        = ( chained && !core && x >= 1 && y >= 1 ) ? w - 2 - x - y : -1;
+      if constexpr ( pingpong )
+      {
+         fdiag[j]      = faceAt( 3, x, y );
+         fcol[j]       = x == 0 || y == 0;
+         const int  pl = x == 0 ? 2 : 1, pu = x == 0 ? y : x;
+         const int* c  = fr->c[pl];
+         fcA[j]        = fr->face[pl] * fS + c[0] + c[1] * pu;
+         fcB[j]        = c[3] + c[4] * pu;
+         fcST[j]       = ( c[2] + 1 ) | ( ( c[5] + 1 ) << 2 );
+      }
    }
    // update positions of colour class cls: LDS place, largest sweep position k whose ring holds the point,
    // highest slab in which it is interior (-1: none), slab-independent index parts
@@ -1025,14 +1084,31 @@ __global__ __launch_bounds__( 1024 ) void kCellSorTile( LV g, const MarchUnit* _
       {
          const bool ok = s <= lsx[j], fromSnap = chained && s >= 1 && s <= lsnap[j];
          const int  o  = ok ? ss + ly_[j] * sy + lbo[j] : 0;
-         const T    vu = cu.load( o, ok && !fromSnap );
-         if constexpr ( chained )
+         if constexpr ( pingpong )
          {
-            const T vs = csnap.load( o, fromSnap ); // both loads issued: no branch around either
-            v[j]       = fromSnap ? vs : vu;
+            // a closure point comes from its face (slab s >= 1 here: slab 0 is loaded separately).  One load per
+            // lane, from either array, into one register: the lanes of the two branches are disjoint, and each
+            // branch issues exactly one load, so the count of loads in flight does not depend on the branch
+            const bool onFace = ok && ( fcol[j] || s == lsx[j] );
+            if ( onFace )
+            {
+               const int fj = fcB[j] + ( ( fcST[j] >> 2 ) - 1 ) * s;
+               v[j] = cf.load( fcol[j] ? fcA[j] + ( ( fcST[j] & 3 ) - 1 ) * s + faceRel( w, 0, fj ) : fdiag[j], true );
+            }
+            else
+               v[j] = cu.load( o, ok );
          }
          else
-            v[j] = vu;
+         {
+            const T vu = cu.load( o, ok && !fromSnap );
+            if constexpr ( chained )
+            {
+               const T vs = csnap.load( o, fromSnap ); // both loads issued: no branch around either
+               v[j]       = fromSnap ? vs : vu;
+            }
+            else
+               v[j] = vu;
+         }
       }
    };
    auto storeSlab = [&]( int slot, const T( &v )[NL] ) {
@@ -1082,7 +1158,7 @@ __global__ __launch_bounds__( 1024 ) void kCellSorTile( LV g, const MarchUnit* _
          const T    v  = lds[swSlot * RS + ( wb ? oq[jc] : 0 )];
          const int  o  = wb ? swBase + oy[jc] * ( w + 1 - sw ) + obo[jc] : 0;
          cband.store( o, v, wb && ( pingpong || inBand[jc] ) );
-         cu.store( o, v, wb && !pingpong && ( chained || !inBand[jc] ) );
+         if constexpr ( !pingpong ) cu.store( o, v, wb && ( chained || !inBand[jc] ) );
       }
       // the wave's update: sweep position k on slab s, slots of s and its neighbours (uniform)
       const int k = kTab[P & 3], s = t - 2 * k;
@@ -1112,7 +1188,21 @@ __global__ __launch_bounds__( 1024 ) void kCellSorTile( LV g, const MarchUnit* _
       __syncthreads();
    };
 
-   for ( int s = 0; s < 3; s++ )
+   // slab 0: in ping-pong mode every point of it lies on the plane z = 0 and is read from that face
+   if constexpr ( pingpong )
+   {
+#pragma unroll
+      for ( int j = 0; j < NL; j++ )
+      {
+         const int  q = tid + j * NT, x = x0r + q % R, y = y0r + q / R;
+         const bool ok = lsx[j] >= 0;
+         xbuf[0][j]    = cf.load( ok ? faceAt( 0, x, y ) : 0, ok );
+      }
+   }
+   else
+      loadSlab( 0, xbuf[0] );
+   storeSlab( 0, xbuf[0] );
+   for ( int s = 1; s < 3; s++ )
    {
       loadSlab( s, xbuf[0] );
       storeSlab( s, xbuf[0] );
@@ -2738,10 +2828,12 @@ void cellSorFused( Mesh& M, LevelPlan& P, const T* CW, int flag, T* u, const T* 
    case kSorTile:
       if ( snapIn )
          hipLaunchKernelGGL( ( kCellSorTile< T, 3, 1, 1 > ), grid, dim3( 1024 ), 0, s, g, un, n, P.sorSlabStart.p, CW,
-                             P.dCellType.p, flag, u, band, snapIn, b, omega, reverse );
+                             P.dCellType.p, flag, u, band, snapIn, (const T*) nullptr,
+                             (const CellFaceRead*) nullptr, b, omega, reverse );
       else
          hipLaunchKernelGGL( ( kCellSorTile< T, 3, 1, 0 > ), grid, dim3( 1024 ), 0, s, g, un, n, P.sorSlabStart.p, CW,
-                             P.dCellType.p, flag, u, band, snapIn, b, omega, reverse );
+                             P.dCellType.p, flag, u, band, snapIn, (const T*) nullptr,
+                             (const CellFaceRead*) nullptr, b, omega, reverse );
       if ( withBand )
          hipLaunchKernelGGL( ( kCellSorBand< T, kSorTile > ), bandGrid, dim3( 256 ), 0, s, g, nC, P.dCellType.p, flag, u,
                              band );
@@ -2750,17 +2842,20 @@ void cellSorFused( Mesh& M, LevelPlan& P, const T* CW, int flag, T* u, const T* 
    }
 }
 
-// The fused cell phase of a ping-pong sweep: reads the macro-cell interiors (and closures) of src, writes every
-// updated interior value to dst (src is not written, so no band and no copy are needed)
+// The fused cell phase of a ping-pong sweep: reads the macro-cell interiors of src and the closure points from
+// the face arrays of faces (the function's own array: src in the first sweep of a pair, dst in the second),
+// writes every updated interior value to dst (src is not written, so no band and no copy are needed; no
+// closure of either array is read or written)
 template < typename T >
-void cellSorPingPong( Mesh& M, LevelPlan& P, const T* CW, int flag, const T* src, T* dst, const T* b, T omega )
+void cellSorPingPong( Mesh& M, LevelPlan& P, const T* CW, int flag, const T* src, T* dst, const T* faces, const T* b,
+                      T omega )
 {
    if ( !P.sorUnits ) throw std::logic_error( "fused cell SOR phase launched before its buffers were prepared" );
    if ( P.sorUnitCount == 0 ) return;
    const dim3 grid( (unsigned) xcdGrid( P.sorUnitCount ) );
    hipLaunchKernelGGL( ( kCellSorTile< T, 3, 1, 2 > ), grid, dim3( 1024 ), 0, M.ctx->stream, viewOf( P ), P.sorUnits->p,
                        P.sorUnitCount, P.sorSlabStart.p, CW, P.dCellType.p, flag, const_cast< T* >( src ), dst,
-                       (const T*) nullptr, b, omega, 0 );
+                       (const T*) nullptr, faces, P.dCellFaceRead.p, b, omega, 0 );
 }
 
 // Whether a forward smoothing sequence on this level can run its sweeps in ping-pong pairs: the fused cell phase
@@ -2919,8 +3014,10 @@ void applyT( Operator& op, Function& src, Function& dst, const Function* rhs, in
 // pp (forward sweeps on levels where pingPongOk holds): 0 in place; 1 = the macro-cell interiors are read from
 // u and their new values written to the level's ping-pong array B (sorBand); 2 = read from B and written to u.
 // Within the sweep every kernel that reads macro-cell interiors (edge terms, face ghost layers) reads the array
-// that holds them at that point, and the face-to-cell refresh writes the closures of both arrays, so B is at every
-// point what u would be in place.
+// that holds them at that point, and the cell phase reads the closure points from the face arrays, so the
+// interiors of B are at every point what those of u would be in place.  Of the macro-cell closures, the second
+// sweep's edge terms and face ghost layers read B's points within distance 1 of the faces' edges: the first sweep
+// copies those from the faces (kF2CBand); the full face-to-cell refresh runs once, into u, after the second.
 template < typename T, int NR >
 void sorT( Operator& op, Function& xf, const Function& bf, double omega, int level, int flag, bool refreshFirst,
            bool backward = false, int chain = 0, int pp = 0 )
@@ -2995,17 +3092,25 @@ void sorT( Operator& op, Function& xf, const Function& bf, double omega, int lev
             facePhase();
             if ( pp )
             {
-               // face-to-cell refresh into the closures of u and of B, then the cell phase from one to the other
-               hipLaunchKernelGGL( ( kF2C< T, 1 > ),
-                                   dim3( (unsigned) ( ( (int64_t) P.w * ( P.w + 1 ) / 2 + 256 * kF2CBatch - 1 ) /
-                                                      ( 256 * kF2CBatch ) ),
-                                         4 * M.T.nC() ),
-                                   dim3( 256 ), 0, s, g, P.dF2C.p, u, ppArr );
+               // the cell phase from one array to the other, closure points read from the faces; after the
+               // pair, the face-to-cell refresh of u
                const T* CW = wsel< T >( Wt.cell64, Wt.cell32 );
                if ( pp == 1 )
-                  cellSorPingPong< T >( M, P, CW, flag, u, ppArr, b, om );
+               {
+                  // the next sweep's edge terms and face ghost layers read B's closures near the faces' edges
+                  hipLaunchKernelGGL( ( kF2CBand< T > ), dim3( (unsigned) ( ( 6 * P.w - 9 + 255 ) / 256 ), 4 * M.T.nC() ),
+                                      dim3( 256 ), 0, s, g, P.dF2C.p, (const T*) u, ppArr );
+                  cellSorPingPong< T >( M, P, CW, flag, u, ppArr, u, b, om );
+               }
                else
-                  cellSorPingPong< T >( M, P, CW, flag, ppArr, u, b, om );
+               {
+                  cellSorPingPong< T >( M, P, CW, flag, ppArr, u, u, b, om );
+                  hipLaunchKernelGGL( ( kF2C< T, 1 > ),
+                                      dim3( (unsigned) ( ( (int64_t) P.w * ( P.w + 1 ) / 2 + 256 * kF2CBatch - 1 ) /
+                                                         ( 256 * kF2CBatch ) ),
+                                            4 * M.T.nC() ),
+                                      dim3( 256 ), 0, s, g, P.dF2C.p, u, (T*) nullptr );
+               }
             }
             else if ( cellSorIsSmall< T, NR >( P ) ) // the cell workgroups refresh their own copies of the faces
                cellSorPhase< T, NR >( M, P, wsel< T >( Wt.cell64, Wt.cell32 ), flag, u, b, om, false, chain, true );
@@ -3526,8 +3631,8 @@ struct CellSorSequenceF
          if ( count % 2 == 1 ) cellSorPhase< T, NR >( M, P, CW, flag, u, bb, (T) omega, false, 0 ), k = 1;
          for ( ; k < count; k += 2 )
          {
-            cellSorPingPong< T >( M, P, CW, flag, u, B, bb, (T) omega );
-            cellSorPingPong< T >( M, P, CW, flag, B, u, bb, (T) omega );
+            cellSorPingPong< T >( M, P, CW, flag, u, B, u, bb, (T) omega );
+            cellSorPingPong< T >( M, P, CW, flag, B, u, u, bb, (T) omega );
          }
          return;
       }

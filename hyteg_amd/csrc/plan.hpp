@@ -122,6 +122,16 @@ struct F2C
    FaceCellMap m;
 };
 
+// Where the closure points of one macro-cell are in the face arrays, by boundary plane q of the cell: 0: z = 0,
+// 1: y = 0, 2: x = 0, 3: x + y + z = w - 1.  A point of plane q with free coordinates (u, v) = (x, y), (x, z),
+// (y, z), (x, y) is point (i, j) = (c[q][0] + c[q][1] u + c[q][2] v, c[q][3] + c[q][4] u + c[q][5] v) of face
+// face[q] (the inverse of that face's F2C map).
+struct CellFaceRead
+{
+   int32_t face[4];
+   int32_t c[4][6];
+};
+
 // Operator weights aligned with the geometry tables.
 //   face: 15 per face: in-plane C, W, E, N, S, NW, SE (summed over sides), then side0 off[4], side1 off[4]
 //   edge: 3 (W, C, E) per edge followed by one weight per EdgeTerm

@@ -173,6 +173,56 @@ FaceCellMap numericMap( const Topology& T, int c, int lf, int w )
 bool inCell( int w, int x, int y, int z ) { return x >= 0 && y >= 0 && z >= 0 && x + y + z <= w - 1; }
 bool inFace( int w, int i, int j ) { return i >= 0 && j >= 0 && i + j <= w - 1; }
 
+// The inverse of a cell's four F2C maps (cell point = base + i a + j b), one per boundary plane of the cell.  The
+// plane of a map is found from the images of the face's corners; on it, (i, j) follows from the two free
+// coordinates (u, v) by the inverse of the unimodular 2 x 2 part of the map.  Every corner and the face's
+// midpoints are mapped back and checked, so a wrong table is an exception here.
+CellFaceRead cellFaceRead( const F2C* maps, int w )
+{
+   CellFaceRead R{};
+   bool         seen[4] = { false, false, false, false };
+   const int    free[4][2] = { { 0, 1 }, { 0, 2 }, { 1, 2 }, { 0, 1 } }; // (u, v) coordinate indices per plane
+   for ( int lf = 0; lf < 4; lf++ )
+   {
+      const FaceCellMap& m = maps[lf].m;
+      auto point = [&]( int i, int j, int d ) { return m.base[d] + i * m.a[d] + j * m.b[d]; };
+      const int corners[3][2] = { { 0, 0 }, { w - 1, 0 }, { 0, w - 1 } };
+      int       q = -1;
+      for ( int cand = 0; cand < 4 && q < 0; cand++ )
+      {
+         bool on = true;
+         for ( const auto& k : corners )
+         {
+            const int x = point( k[0], k[1], 0 ), y = point( k[0], k[1], 1 ), z = point( k[0], k[1], 2 );
+            on = on && ( cand == 0 ? z == 0 : cand == 1 ? y == 0 : cand == 2 ? x == 0 : x + y + z == w - 1 );
+         }
+         if ( on ) q = cand;
+      }
+      if ( q < 0 || seen[q] ) throw std::runtime_error( "plan validation failed: cell face planes" );
+      seen[q]       = true;
+      const int du = free[q][0], dv = free[q][1];
+      const int det = m.a[du] * m.b[dv] - m.b[du] * m.a[dv];
+      if ( det != 1 && det != -1 ) throw std::runtime_error( "plan validation failed: face map not unimodular" );
+      int* c = R.c[q];
+      c[1]   = m.b[dv] * det; // 1 / det = det for det = +-1
+      c[2]   = -m.b[du] * det;
+      c[4]   = -m.a[dv] * det;
+      c[5]   = m.a[du] * det;
+      c[0]   = -( c[1] * m.base[du] + c[2] * m.base[dv] );
+      c[3]   = -( c[4] * m.base[du] + c[5] * m.base[dv] );
+      R.face[q] = maps[lf].face;
+      const int probe[6][2] = { { 0, 0 }, { w - 1, 0 }, { 0, w - 1 }, { ( w - 1 ) / 2, 0 }, { 0, ( w - 1 ) / 2 },
+                                { ( w - 1 ) / 3, ( w - 1 ) / 3 } };
+      for ( const auto& k : probe )
+      {
+         const int u = point( k[0], k[1], du ), v = point( k[0], k[1], dv );
+         if ( c[0] + c[1] * u + c[2] * v != k[0] || c[3] + c[4] * u + c[5] * v != k[1] )
+            throw std::runtime_error( "plan validation failed: inverse face map" );
+      }
+   }
+   return R;
+}
+
 // Every affine index map used by a kernel is checked at its extreme arguments on the host (the maps are
 // affine, so the extremes bound all arguments); a wrong map becomes an exception here, never an
 // out-of-bounds access on the device.
@@ -288,6 +338,10 @@ void buildPlan( Mesh& M, LevelPlan& P, int level )
          f2c.push_back( x );
       }
    if ( dev ) P.dF2C.upload( f2c, s );
+   std::vector< CellFaceRead > cfr( T.nC() );
+   for ( int c = 0; c < T.nC(); c++ )
+      cfr[c] = cellFaceRead( &f2c[4 * c], w );
+   if ( dev ) P.dCellFaceRead.upload( cfr, s );
 
    // faces: neighbour-cell maps and the four off-plane neighbours per side (in direction order)
    P.hFaceGeom.clear();
