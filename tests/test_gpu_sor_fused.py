@@ -1,7 +1,8 @@
 """GPU tests of the fused SOR cell phase (all four colours of the macro-cell interiors in one launch, tiles
 marched along z): it must give the four colour passes' values bit for bit -- forward and backward sweeps,
-levels 3 to 7 (FP64; FP32 functions keep the colour passes) -- and the oracle's to 1e-12 relative; V-cycles with it must equal V-cycles
-without it, also through captured graphs that were recorded before the switch."""
+levels 3 to 7, FP64 and (where selected for it) FP32 -- and the oracle's to 1e-12 relative; V-cycles with it
+must equal V-cycles without it, also through captured graphs that were recorded before the switch, and the
+ping-pong pairs of their smoothing sequences must leave every macro-cell closure as the colour passes do."""
 import numpy as np
 import pytest
 
@@ -137,3 +138,32 @@ def test_chained_sweeps_symmetric_and_w_cycles(gpu_ctx, oracle_lib):
             gmg.solve_graph(A, u, f, L, count=2)
             out.append(u.download(L))
         assert np.array_equal(out[0], out[1]), (cyc, nu)
+
+
+@pytest.mark.parametrize("prec_name", ["F64", "F32"])
+def test_pingpong_pairs_keep_closures(prec_name, gpu_ctx, oracle_lib):
+    """V(2,2) and V(3,3) cycles on the 48-tet cube at level 6 run their smoothing sequences as ping-pong pairs when
+    the fused cell phase is selected (closure points read from the faces, one face-to-cell copy per pair).  The
+    owned values and every macro-cell array with its closure must equal those of the colour passes bit for bit."""
+    import hyteg_amd as H
+    prec = getattr(H, prec_name)
+    L = 6
+    om, S = mesh_pair(("cube", 2), gpu_ctx)
+    A = H.P1ConstantLaplaceOperator(S, 0, L)
+    for nu in (2, 3):
+        say(f"[sor fused] ping-pong pairs cube n=2 level={L} {prec_name} V({nu},{nu})")
+        rng = np.random.default_rng(60 + nu)
+        x0, bv = rng.standard_normal((2, om.owned_count(L)))
+        out = []
+        for tile in (0, 24):
+            S.set_cell_sor(L, tile, fp64=True, fp32=True)
+            x, b = H.P1Function("x", S, 0, L, precision=prec), H.P1Function("b", S, 0, L, precision=prec)
+            x.upload(L, x0)
+            b.upload(L, bv)
+            g = H.GeometricMultigridSolver(S, H.SORSmoother(1.0), H.CGSolver(S, 0, 0), H.P1toP1LinearRestriction(),
+                                           H.P1toP1LinearProlongation(), 0, L, nu, nu)
+            g.solve(A, x, b, L)
+            out.append((x.download(L), [x.download_primitive(L, H.CELL, c) for c in range(48)]))
+        assert np.array_equal(out[0][0], out[1][0])
+        for c in range(48):
+            assert np.array_equal(out[0][1][c], out[1][1][c]), c
