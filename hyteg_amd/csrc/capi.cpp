@@ -450,8 +450,9 @@ struct hymi_gmg_s
       const int  extra  = smoothInc * ( maxL_invoked - L );
       smooth( x, b, L, flag, nu1 + extra, finest );
       launchApply( *A, x, *tmp, &b, L, flag, kResidual, nu1 + extra == 0 && finest );
-      // restriction written straight into b on the coarse level (== restrict + b.assign on flag DoFs)
-      launchRestrict( *tmp, b, L, flag );
+      // restriction written straight into b on the coarse level (== restrict + b.assign on flag DoFs); tmp is
+      // scratch, so its cell closures are refreshed only where the restriction reads them
+      launchRestrict( *tmp, b, L, flag, true );
       launchLincomb( x, L - 1, kAll, 0, nullptr, nullptr, false, 0.0 );
       recurse( x, b, L - 1 );
       // W-cycle (GeometricMultigridSolver.hpp:201-204): the coarse correction is visited again, from its

@@ -2936,6 +2936,18 @@ void refreshT( Function& f, int level, int mask )
    hipStream_t s = M.ctx->stream;
    T*          u = (T*) f.ptr( level );
    if ( M.T.nC() == 0 ) mask &= 3;
+   // bit 8: of the face-to-cell step only the points within distance 1 of the faces' edges (kF2CBand), for
+   // readers that take no other cell closure point (one right-hand side; else the whole copy)
+   if ( mask & 8 )
+   {
+      refreshT< T, NR >( f, level, mask & 3 );
+      if ( NR == 1 )
+         hipLaunchKernelGGL( ( kF2CBand< T > ), dim3( (unsigned) ( ( 6 * P.w - 9 + 255 ) / 256 ), 4 * M.T.nC() ),
+                             dim3( 256 ), 0, s, g, P.dF2C.p, (const T*) u, u );
+      else
+         refreshT< T, NR >( f, level, 4 );
+      return;
+   }
    if ( mask == 3 || mask == 5 || mask == 6 || mask == 7 ) // several steps: one launch reading the owners
    {
       const int     nA = ( mask & 1 ) ? (int) blocksFor( 2 * (int64_t) M.T.nE() * NR, 256 ) : 0;
@@ -3031,7 +3043,9 @@ void sorT( Operator& op, Function& xf, const Function& bf, double omega, int lev
    const T*    b  = (const T*) bf.ptr( level );
    const T     om = (T) omega;
    const int   nV = M.T.nV(), nE = M.T.nE(), nF = M.T.nF();
-   if ( refreshFirst ) refreshT< T, NR >( xf, level, 7 );
+   // a ping-pong pair reads no macro-cell closure point beyond distance 1 of the faces' edges before its final
+   // face-to-cell copy
+   if ( refreshFirst ) refreshT< T, NR >( xf, level, pp == 1 ? 1 | 2 | 8 : 7 );
    if ( NR == 1 && level >= 1 )
    {
       if ( pp && ( backward || !pingPongOk< T, NR >( M, P, flag ) ) )
@@ -3184,8 +3198,11 @@ void sorSweepsT( Operator& op, Function& xf, const Function& bf, double omega, i
       sorT< T, NR >( op, xf, bf, omega, level, flag, refreshFirst && k == 0, false, k );
 }
 
+// bandOnly: the source's cell closures are refreshed only within distance 1 of the faces' edges, which is all
+// the restriction reads of them (the cell-interior part gathers interior fine points only); for a scratch source
+// whose other closure points nobody reads afterwards
 template < typename T, int NR >
-void restrictT( Function& f, Function& dst, int srcLevel, int flag )
+void restrictT( Function& f, Function& dst, int srcLevel, int flag, bool bandOnly = false )
 {
    Mesh&       M  = *f.mesh;
    LevelPlan&  Pf = M.plan( srcLevel );
@@ -3194,7 +3211,7 @@ void restrictT( Function& f, Function& dst, int srcLevel, int flag )
    hipStream_t s  = M.ctx->stream;
    const T*    uf = (const T*) f.ptr( srcLevel );
    T*          uc = (T*) dst.ptr( srcLevel - 1 );
-   refreshT< T, NR >( f, srcLevel, 7 );
+   refreshT< T, NR >( f, srcLevel, bandOnly && NR == 1 ? 1 | 2 | 8 : 7 );
    const int nV = M.T.nV(), nE = M.T.nE(), nF = M.T.nF();
    hipLaunchKernelGGL( ( kRestrictVertex< T, NR > ), dim3( blocksFor( (int64_t) nV * NR, 256 ) ), dim3( 256 ), 0, s,
                        Pf.dVertexGeom.p, Pf.dVertexTerms.p, nV, flag, uf, uc );
@@ -3407,7 +3424,10 @@ struct SorSweepsF
 template < typename T, int NR >
 struct RestrictF
 {
-   static void run( Function& f, Function& d, int L, int flag ) { restrictT< T, NR >( f, d, L, flag ); }
+   static void run( Function& f, Function& d, int L, int flag, bool bandOnly )
+   {
+      restrictT< T, NR >( f, d, L, flag, bandOnly );
+   }
 };
 template < typename T, int NR >
 struct ProlongF
@@ -3486,11 +3506,11 @@ void launchSorSweeps( Operator& op, Function& x, const Function& b, double omega
    dispatch< SorSweepsF >( x.prec, x.nrhs, op, x, b, omega, level, flag, count, refreshFirst );
 }
 
-void launchRestrict( Function& f, Function& dst, int srcLevel, int flag )
+void launchRestrict( Function& f, Function& dst, int srcLevel, int flag, bool bandOnly )
 {
    if ( dst.prec != f.prec || dst.nrhs != f.nrhs || dst.mesh != f.mesh )
       throw std::runtime_error( "restrict: source and destination must share mesh, precision and nrhs" );
-   dispatch< RestrictF >( f.prec, f.nrhs, f, dst, srcLevel, flag );
+   dispatch< RestrictF >( f.prec, f.nrhs, f, dst, srcLevel, flag, bandOnly );
 }
 
 void launchProlongateAdd( Function& f, int srcLevel, int flag, bool refreshCoarse )

@@ -29,7 +29,8 @@ void launchSor( Operator& op, Function& x, const Function& b, double omega, int 
 void launchSorSweeps( Operator& op, Function& x, const Function& b, double omega, int level, int flag, int count,
                       bool refreshFirst = true );
 // coarse level of dst <- restriction of the fine level of f (dst may be f)
-void launchRestrict( Function& f, Function& dst, int srcLevel, int flag );
+// bandOnly: refresh the source's cell closures only where the restriction reads them (a scratch source)
+void launchRestrict( Function& f, Function& dst, int srcLevel, int flag, bool bandOnly = false );
 void launchProlongateAdd( Function& f, int srcLevel, int flag, bool refreshCoarse = true );
 // dst = sum_k c[k] * src[k] (+ dst if accumulate) + constant, on the closures of primitives matching flag
 void launchLincomb( Function& dst, int level, int flag, int n, const double* c, Function* const* src, bool accumulate,
