@@ -1384,7 +1384,7 @@ int hymi_gmg_create( hymi_operator op, hymi_function xt, int minL, int maxL, int
          LevelPlan& P = M.plan( L );
          int64_t    n = 0;
          M.segments( P, segLen, &n );
-         if ( g->nrhs > 1 ) M.segments( P, 4 * segLen, &n ); // colour passes with several right-hand sides
+         if ( g->nrhs > 1 ) M.segments( P, kRowSeg, &n ); // cell apply, SOR and prolongation, several right-hand sides
          M.marchUnits( P, &n );
          M.transferUnits( P, &n );
          if ( g->nrhs == 1 ) M.faceGhost( P );

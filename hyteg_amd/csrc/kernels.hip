@@ -308,87 +308,94 @@ __global__ __launch_bounds__( 256 ) void kRefresh( LV g, const V2E* __restrict__
 // --------------------------------------------------------------------------------------------------------
 // macro-cell interior: apply / residual, multicolour SOR
 
+// Several right-hand sides (interleaved [point][right-hand side]): a wave owns a row segment of kRowSeg points
+// (cell, slab z, row y, x0 ..) and walks it 64 / NR points at a time, lane = (point, right-hand side).  The seven
+// row offsets of the stencil are formed once per segment, in 32-bit arithmetic (a cell holds fewer than 2^31
+// values of all right-hand sides together, checked by the launchers).
 template < typename T, int NR, int MODE >
 __global__ __launch_bounds__( 256 ) void kCellApply( LV g, const CellSeg* __restrict__ segs, int64_t nseg,
                                                      const T* __restrict__ W, const int* __restrict__ ctype, int flag,
                                                      const T* __restrict__ u, T* __restrict__ y,
                                                      const T* __restrict__ b )
 {
-   const int64_t s = xcdBlock( ( nseg + 3 ) / 4 ) * 4 + ( threadIdx.x >> 6 );
+   const int64_t s = xcdBlock( ( nseg + 3 ) / 4 ) * 4 + __builtin_amdgcn_readfirstlane( threadIdx.x >> 6 );
    if ( s >= nseg ) return;
    const CellSeg sg = segs[s];
    if ( !( ctype[sg.cell] & flag ) ) return;
-   const int lane = threadIdx.x & 63;
-   const int x    = sg.x0 + lane / NR;
-   const int r    = lane % NR;
-   const int w    = g.w;
-   if ( x > w - 2 - sg.y - sg.z ) return;
-   const int64_t base = g.offC + (int64_t) sg.cell * g.cS;
-   const int64_t t0   = tet( w );
-   const int64_t r0   = base + crow( w, t0, sg.y, sg.z ) + x;
-   const int64_t rN   = base + crow( w, t0, sg.y + 1, sg.z ) + x;
-   const int64_t rS   = base + crow( w, t0, sg.y - 1, sg.z ) + x;
-   const int64_t rT   = base + crow( w, t0, sg.y, sg.z + 1 ) + x;
-   const int64_t rTS  = base + crow( w, t0, sg.y - 1, sg.z + 1 ) + x;
-   const int64_t rB   = base + crow( w, t0, sg.y, sg.z - 1 ) + x;
-   const int64_t rBN  = base + crow( w, t0, sg.y + 1, sg.z - 1 ) + x;
+   const int     lane = threadIdx.x & 63, r = lane % NR;
+   const int     w = g.w, t0 = tet32( w );
+   const int64_t base = ( g.offC + (int64_t) sg.cell * g.cS ) * NR;
+   const T*      cu   = u + base;
+   const int     r0 = crow32( w, t0, sg.y, sg.z ), rN = crow32( w, t0, sg.y + 1, sg.z ), rS = crow32( w, t0, sg.y - 1, sg.z );
+   const int     rT = crow32( w, t0, sg.y, sg.z + 1 ), rTS = crow32( w, t0, sg.y - 1, sg.z + 1 );
+   const int     rB = crow32( w, t0, sg.y, sg.z - 1 ), rBN = crow32( w, t0, sg.y + 1, sg.z - 1 );
+   const int     xEnd = min( sg.x0 + kRowSeg, w - 1 - sg.y - sg.z ); // interior x <= w - 2 - y - z
    const T*      c    = W + (int64_t) sg.cell * 15;
-#define U( i ) u[( i ) *NR + r]
-   T acc = c[dC] * U( r0 );
-   acc += c[dW] * U( r0 - 1 ) + c[dE] * U( r0 + 1 );
-   acc += c[dN] * U( rN ) + c[dNW] * U( rN - 1 );
-   acc += c[dS] * U( rS ) + c[dSE] * U( rS + 1 );
-   acc += c[dTC] * U( rT ) + c[dTW] * U( rT - 1 );
-   acc += c[dTS] * U( rTS ) + c[dTSE] * U( rTS + 1 );
-   acc += c[dBC] * U( rB ) + c[dBE] * U( rB + 1 );
-   acc += c[dBN] * U( rBN ) + c[dBNW] * U( rBN - 1 );
+   const T cC = c[dC], cW = c[dW], cE = c[dE], cN = c[dN], cS = c[dS], cNW = c[dNW], cSE = c[dSE];
+   const T cTC = c[dTC], cTW = c[dTW], cTS = c[dTS], cTSE = c[dTSE], cBC = c[dBC], cBN = c[dBN], cBE = c[dBE],
+           cBNW = c[dBNW];
+   for ( int x = sg.x0 + lane / NR; x < xEnd; x += 64 / NR )
+   {
+#define U( i ) cu[( ( i ) + x ) * NR + r]
+      T acc = cC * U( r0 );
+      acc += cW * U( r0 - 1 ) + cE * U( r0 + 1 );
+      acc += cN * U( rN ) + cNW * U( rN - 1 );
+      acc += cS * U( rS ) + cSE * U( rS + 1 );
+      acc += cTC * U( rT ) + cTW * U( rT - 1 );
+      acc += cTS * U( rTS ) + cTSE * U( rTS + 1 );
+      acc += cBC * U( rB ) + cBE * U( rB + 1 );
+      acc += cBN * U( rBN ) + cBNW * U( rBN - 1 );
 #undef U
-   const int64_t o = r0 * NR + r;
-   if ( MODE == kReplace )
-      y[o] = acc;
-   else if ( MODE == kAdd )
-      y[o] += acc;
-   else
-      y[o] = b[o] - acc;
+      const int64_t o = base + ( r0 + x ) * NR + r;
+      if ( MODE == kReplace )
+         y[o] = acc;
+      else if ( MODE == kAdd )
+         y[o] += acc;
+      else
+         y[o] = b[o] - acc;
+   }
 }
 
+// One colour of the SOR pass for several right-hand sides: the segment's points of the colour (every fourth x),
+// 64 / NR at a time, so no lane is launched for a point of another colour.
 template < typename T, int NR >
 __global__ __launch_bounds__( 256 ) void kCellSor( LV g, const CellSeg* __restrict__ segs, int64_t nseg,
                                                    const T* __restrict__ W, const int* __restrict__ ctype, int flag,
                                                    T* __restrict__ u, const T* __restrict__ b, T omega, int colour )
 {
-   // segments hold 4 x (64 / NR) points, among them 64 / NR of the colour (every fourth x): lane -> (the
-   // segment's j-th point of the colour, right-hand side r), so no lane is launched for a point of another colour
-   const int64_t s = xcdBlock( ( nseg + 3 ) / 4 ) * 4 + ( threadIdx.x >> 6 );
+   const int64_t s = xcdBlock( ( nseg + 3 ) / 4 ) * 4 + __builtin_amdgcn_readfirstlane( threadIdx.x >> 6 );
    if ( s >= nseg ) return;
    const CellSeg sg = segs[s];
    if ( !( ctype[sg.cell] & flag ) ) return;
-   const int lane = threadIdx.x & 63;
-   const int x    = sg.x0 + ( ( colour - sg.x0 - 2 * sg.y - 3 * sg.z ) & 3 ) + 4 * ( lane / NR );
-   const int r    = lane % NR;
-   const int w    = g.w;
-   if ( x > w - 2 - sg.y - sg.z || x >= sg.x0 + 4 * ( 64 / NR ) ) return;
-   const int64_t base = g.offC + (int64_t) sg.cell * g.cS;
-   const int64_t t0   = tet( w );
-   const int64_t r0   = base + crow( w, t0, sg.y, sg.z ) + x;
-   const int64_t rN   = base + crow( w, t0, sg.y + 1, sg.z ) + x;
-   const int64_t rS   = base + crow( w, t0, sg.y - 1, sg.z ) + x;
-   const int64_t rT   = base + crow( w, t0, sg.y, sg.z + 1 ) + x;
-   const int64_t rTS  = base + crow( w, t0, sg.y - 1, sg.z + 1 ) + x;
-   const int64_t rB   = base + crow( w, t0, sg.y, sg.z - 1 ) + x;
-   const int64_t rBN  = base + crow( w, t0, sg.y + 1, sg.z - 1 ) + x;
+   const int     lane = threadIdx.x & 63, r = lane % NR;
+   const int     w = g.w, t0 = tet32( w );
+   const int64_t base = ( g.offC + (int64_t) sg.cell * g.cS ) * NR;
+   T*            cu   = u + base;
+   const T*      cb   = b + base;
+   const int     r0 = crow32( w, t0, sg.y, sg.z ), rN = crow32( w, t0, sg.y + 1, sg.z ), rS = crow32( w, t0, sg.y - 1, sg.z );
+   const int     rT = crow32( w, t0, sg.y, sg.z + 1 ), rTS = crow32( w, t0, sg.y - 1, sg.z + 1 );
+   const int     rB = crow32( w, t0, sg.y, sg.z - 1 ), rBN = crow32( w, t0, sg.y + 1, sg.z - 1 );
+   const int     xEnd = min( sg.x0 + kRowSeg, w - 1 - sg.y - sg.z );
    const T*      c    = W + (int64_t) sg.cell * 15;
-#define U( i ) u[( i ) *NR + r]
-   T acc = c[dW] * U( r0 - 1 ) + c[dE] * U( r0 + 1 );
-   acc += c[dN] * U( rN ) + c[dNW] * U( rN - 1 );
-   acc += c[dS] * U( rS ) + c[dSE] * U( rS + 1 );
-   acc += c[dTC] * U( rT ) + c[dTW] * U( rT - 1 );
-   acc += c[dTS] * U( rTS ) + c[dTSE] * U( rTS + 1 );
-   acc += c[dBC] * U( rB ) + c[dBE] * U( rB + 1 );
-   acc += c[dBN] * U( rBN ) + c[dBNW] * U( rBN - 1 );
+   const T cW = c[dW], cE = c[dE], cN = c[dN], cS = c[dS], cNW = c[dNW], cSE = c[dSE];
+   const T cTC = c[dTC], cTW = c[dTW], cTS = c[dTS], cTSE = c[dTSE], cBC = c[dBC], cBN = c[dBN], cBE = c[dBE],
+           cBNW = c[dBNW];
+   const T   relax = omega / c[dC];
+   const int xc    = sg.x0 + ( ( colour - sg.x0 - 2 * sg.y - 3 * sg.z ) & 3 ); // first point of the colour
+   for ( int x = xc + 4 * ( lane / NR ); x < xEnd; x += 4 * ( 64 / NR ) )
+   {
+#define U( i ) cu[( ( i ) + x ) * NR + r]
+      T acc = cW * U( r0 - 1 ) + cE * U( r0 + 1 );
+      acc += cN * U( rN ) + cNW * U( rN - 1 );
+      acc += cS * U( rS ) + cSE * U( rS + 1 );
+      acc += cTC * U( rT ) + cTW * U( rT - 1 );
+      acc += cTS * U( rTS ) + cTSE * U( rTS + 1 );
+      acc += cBC * U( rB ) + cBE * U( rB + 1 );
+      acc += cBN * U( rBN ) + cBNW * U( rBN - 1 );
 #undef U
-   const int64_t o = r0 * NR + r;
-   u[o]            = ( T( 1 ) - omega ) * u[o] + omega / c[dC] * ( b[o] - acc );
+      const int o = ( r0 + x ) * NR + r;
+      cu[o]       = ( T( 1 ) - omega ) * cu[o] + relax * ( cb[o] - acc );
+   }
 }
 
 // --------------------------------------------------------------------------------------------------------
@@ -2233,6 +2240,8 @@ __device__ __forceinline__ void prolongDir( int px, int py, int pz, int& dx, int
    }
 }
 
+// Several right-hand sides: a wave owns a row segment of kRowSeg fine points and walks it 64 / NR points at a
+// time; the coarse rows of both x-parity classes are formed once per segment (32-bit, see kCellApply).
 template < typename T, int NR >
 __global__ __launch_bounds__( 256 ) void kProlongCell( LV gc, LV gf, const CellSeg* __restrict__ segs, int64_t nseg,
                                                        const int* __restrict__ ctype, int flag,
@@ -2243,24 +2252,27 @@ __global__ __launch_bounds__( 256 ) void kProlongCell( LV gc, LV gf, const CellS
    const CellSeg sg = segs[s];
    if ( !( ctype[sg.cell] & flag ) ) return;
    const int lane = threadIdx.x & 63;
-   const int x    = sg.x0 + lane / NR;
    const int r    = lane % NR;
    const int y = sg.y, z = sg.z, py = y & 1, pz = z & 1;
    // per x-parity class: direction and the coarse rows of (p - d)/2 and (p + d)/2 (offsets within the cell)
    int       d0x, d0y, d0z, d1x, d1y, d1z;
    prolongDir( 0, py, pz, d0x, d0y, d0z );
    prolongDir( 1, py, pz, d1x, d1y, d1z );
-   const int wc = gc.w, tc = (int) tet( wc );
-   const int m0 = (int) crow( wc, tc, ( y - d0y ) / 2, ( z - d0z ) / 2 ), p0 = (int) crow( wc, tc, ( y + d0y ) / 2, ( z + d0z ) / 2 );
-   const int m1 = (int) crow( wc, tc, ( y - d1y ) / 2, ( z - d1z ) / 2 ), p1 = (int) crow( wc, tc, ( y + d1y ) / 2, ( z + d1z ) / 2 );
-   if ( x > gf.w - 2 - y - z ) return;
-   const int px = x & 1;
-   const int dx = px ? d1x : d0x;
-   const int im = ( px ? m1 : m0 ) + ( x - dx ) / 2, ip = ( px ? p1 : p0 ) + ( x + dx ) / 2;
-   const T*  cu = uc + ( gc.offC + (int64_t) sg.cell * gc.cS ) * NR;
-   const T   v  = T( 0.5 ) * ( cu[(int64_t) im * NR + r] + cu[(int64_t) ip * NR + r] );
-   T*        fu = uf + ( gf.offC + (int64_t) sg.cell * gf.cS ) * NR;
-   fu[( (int64_t) crow( gf.w, (int) tet( gf.w ), y, z ) + x ) * NR + r] += v;
+   const int wc = gc.w, tc = tet32( wc );
+   const int m0 = crow32( wc, tc, ( y - d0y ) / 2, ( z - d0z ) / 2 ), p0 = crow32( wc, tc, ( y + d0y ) / 2, ( z + d0z ) / 2 );
+   const int m1 = crow32( wc, tc, ( y - d1y ) / 2, ( z - d1z ) / 2 ), p1 = crow32( wc, tc, ( y + d1y ) / 2, ( z + d1z ) / 2 );
+   const int rf   = crow32( gf.w, tet32( gf.w ), y, z );
+   const int xEnd = min( sg.x0 + kRowSeg, gf.w - 1 - y - z );
+   const T*  cu   = uc + ( gc.offC + (int64_t) sg.cell * gc.cS ) * NR;
+   T*        fu   = uf + ( gf.offC + (int64_t) sg.cell * gf.cS ) * NR;
+   for ( int x = sg.x0 + lane / NR; x < xEnd; x += 64 / NR )
+   {
+      const int px = x & 1;
+      const int dx = px ? d1x : d0x;
+      const int im = ( px ? m1 : m0 ) + ( x - dx ) / 2, ip = ( px ? p1 : p0 ) + ( x + dx ) / 2;
+      const T   v  = T( 0.5 ) * ( cu[im * NR + r] + cu[ip * NR + r] );
+      fu[( rf + x ) * NR + r] += v;
+   }
 }
 
 // Flat form for one right-hand side: a wave takes kTransferWindows windows of 64 consecutive elements of one
@@ -2658,6 +2670,14 @@ int segLen()
    return 64 / NR;
 }
 
+// the row-segment kernels for several right-hand sides index a cell's values in 32 bits
+template < int NR >
+void rowSegFits( const LevelPlan& P )
+{
+   if ( (int64_t) P.cStride * NR >= ( int64_t( 1 ) << 31 ) )
+      throw std::invalid_argument( "level too large for several right-hand sides (a macro-cell holds 2^31 values or more)" );
+}
+
 template < typename T >
 const T* wsel( const DevBuf< double >& d64, const DevBuf< float >& d32 );
 template <>
@@ -2771,7 +2791,8 @@ void cellApplyLaunch( Mesh& M, LevelPlan& P, const T* CW, int flag, int mode, co
       return;
    }
    int64_t        nseg = 0;
-   const CellSeg* sg   = M.segments( P, 64 / NR, &nseg );
+   rowSegFits< NR >( P );
+   const CellSeg* sg   = M.segments( P, kRowSeg, &nseg );
    if ( nseg == 0 ) return;
    const dim3 grid( (unsigned) xcdGrid( ( nseg + 3 ) / 4 ) ), block( 256 );
    if ( mode == kReplace )
@@ -2796,7 +2817,8 @@ void cellSorLaunch( Mesh& M, LevelPlan& P, const T* CW, int flag, T* u, const T*
       return;
    }
    int64_t        nseg = 0;
-   const CellSeg* sg   = M.segments( P, 4 * ( 64 / NR ), &nseg ); // built with the solver's tables (no capture)
+   rowSegFits< NR >( P );
+   const CellSeg* sg   = M.segments( P, kRowSeg, &nseg ); // built with the solver's tables (no capture)
    if ( nseg == 0 ) return;
    hipLaunchKernelGGL( ( kCellSor< T, NR > ), dim3( (unsigned) xcdGrid( ( nseg + 3 ) / 4 ) ), dim3( 256 ), 0, s, g, sg,
                        nseg, CW, P.dCellType.p, flag, u, b, omega, colour );
@@ -3268,7 +3290,8 @@ void prolongT( Function& f, int srcLevel, int flag, bool refreshCoarse )
       return;
    }
    int64_t        nseg = 0;
-   const CellSeg* sg   = M.segments( Pf, segLen< NR >(), &nseg );
+   rowSegFits< NR >( Pf );
+   const CellSeg* sg   = M.segments( Pf, kRowSeg, &nseg );
    if ( nseg > 0 )
       hipLaunchKernelGGL( ( kProlongCell< T, NR > ), dim3( (unsigned) xcdGrid( ( nseg + 3 ) / 4 ) ), dim3( 256 ), 0, s, gc,
                           gf, sg, nseg, Pf.dCellType.p, flag, uc, uf );

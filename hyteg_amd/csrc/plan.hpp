@@ -27,6 +27,9 @@ struct MarchUnit
 };
 constexpr int kMarchOutput = 62; // output lanes per wave (lanes 1..62)
 constexpr int kTransferWindows = 4; // 64-point windows per unit of the flat grid-transfer kernels
+// points per row segment of the macro-cell apply, SOR and prolongation kernels for several right-hand sides (a
+// wave walks its segment 64 / nrhs points at a time)
+constexpr int kRowSeg = 32;
 // LDS slab tiles: elements of one band (rows ya-1 .. ya+R of a slab of width <= w) and the limits the kernel
 // relies on (8 staged elements per thread and stage, 64 KB of LDS for the four slots in FP64)
 inline int tileCap( int w, int R ) { return ( R + 2 ) * w; }
