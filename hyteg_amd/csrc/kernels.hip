@@ -2045,37 +2045,37 @@ __global__ void kModesCell( LV g, const CellSeg* __restrict__ segs, int64_t nseg
 // --------------------------------------------------------------------------------------------------------
 // restriction (coarse owned point <- fine neighbourhood of 2p, weights 1 and 1/2)
 
+// Several right-hand sides: a wave owns a row segment of kRowSeg coarse points and walks it 64 / NR points at a
+// time; the fine rows around (2X, 2Y, 2Z) are formed once per segment (32-bit, see kCellApply).
 template < typename T, int NR >
 __global__ __launch_bounds__( 256 ) void kRestrictCell( LV gc, LV gf, const CellSeg* __restrict__ segs, int64_t nseg,
                                                         const int* __restrict__ ctype, int flag,
                                                         const T* __restrict__ uf, T* __restrict__ uc )
 {
-   const int64_t s = xcdBlock( ( nseg + 3 ) / 4 ) * 4 + ( threadIdx.x >> 6 );
+   const int64_t s = xcdBlock( ( nseg + 3 ) / 4 ) * 4 + __builtin_amdgcn_readfirstlane( threadIdx.x >> 6 );
    if ( s >= nseg ) return;
    const CellSeg sg = segs[s];
    if ( !( ctype[sg.cell] & flag ) ) return;
    const int lane = threadIdx.x & 63;
-   const int X    = sg.x0 + lane / NR;
    const int r    = lane % NR;
-   if ( X > gc.w - 2 - sg.y - sg.z ) return;
-   const int     w    = gf.w;
-   const int     x    = 2 * X, y = 2 * sg.y, z = 2 * sg.z;
-   const int64_t base = gf.offC + (int64_t) sg.cell * gf.cS;
-   const int64_t t0   = tet( w );
-   const int64_t r0   = base + crow( w, t0, y, z ) + x;
-   const int64_t rN   = base + crow( w, t0, y + 1, z ) + x;
-   const int64_t rS   = base + crow( w, t0, y - 1, z ) + x;
-   const int64_t rT   = base + crow( w, t0, y, z + 1 ) + x;
-   const int64_t rTS  = base + crow( w, t0, y - 1, z + 1 ) + x;
-   const int64_t rB   = base + crow( w, t0, y, z - 1 ) + x;
-   const int64_t rBN  = base + crow( w, t0, y + 1, z - 1 ) + x;
-#define U( i ) uf[( i ) *NR + r]
-   T acc = U( r0 - 1 ) + U( r0 + 1 ) + U( rN ) + U( rN - 1 ) + U( rS ) + U( rS + 1 ) + U( rT ) + U( rT - 1 ) +
-           U( rTS ) + U( rTS + 1 ) + U( rB ) + U( rB + 1 ) + U( rBN ) + U( rBN - 1 );
-   acc = U( r0 ) + T( 0.5 ) * acc;
+   const int w = gf.w, tf = tet32( w ), y = 2 * sg.y, z = 2 * sg.z;
+   const int r0 = crow32( w, tf, y, z ), rN = crow32( w, tf, y + 1, z ), rS = crow32( w, tf, y - 1, z );
+   const int rT = crow32( w, tf, y, z + 1 ), rTS = crow32( w, tf, y - 1, z + 1 );
+   const int rB = crow32( w, tf, y, z - 1 ), rBN = crow32( w, tf, y + 1, z - 1 );
+   const int rc   = crow32( gc.w, tet32( gc.w ), sg.y, sg.z );
+   const int XEnd = min( sg.x0 + kRowSeg, gc.w - 1 - sg.y - sg.z );
+   const T*  fu   = uf + ( gf.offC + (int64_t) sg.cell * gf.cS ) * NR;
+   T*        cu   = uc + ( gc.offC + (int64_t) sg.cell * gc.cS ) * NR;
+   for ( int X = sg.x0 + lane / NR; X < XEnd; X += 64 / NR )
+   {
+      const int x = 2 * X;
+#define U( i ) fu[( ( i ) + x ) * NR + r]
+      T acc = U( r0 - 1 ) + U( r0 + 1 ) + U( rN ) + U( rN - 1 ) + U( rS ) + U( rS + 1 ) + U( rT ) + U( rT - 1 ) +
+              U( rTS ) + U( rTS + 1 ) + U( rB ) + U( rB + 1 ) + U( rBN ) + U( rBN - 1 );
+      acc = U( r0 ) + T( 0.5 ) * acc;
 #undef U
-   const int64_t cbase = gc.offC + (int64_t) sg.cell * gc.cS;
-   uc[( cbase + cidx( gc.w, X, sg.y, sg.z ) ) * NR + r] = acc;
+      cu[( rc + X ) * NR + r] = acc;
+   }
 }
 
 // Flat form for one right-hand side: a wave takes kTransferWindows windows of 64 consecutive elements of one
@@ -3254,7 +3254,8 @@ void restrictT( Function& f, Function& dst, int srcLevel, int flag, bool bandOnl
       return;
    }
    int64_t        nseg = 0;
-   const CellSeg* sg   = M.segments( Pc, segLen< NR >(), &nseg );
+   rowSegFits< NR >( Pf );
+   const CellSeg* sg   = M.segments( Pc, kRowSeg, &nseg );
    if ( nseg > 0 )
       hipLaunchKernelGGL( ( kRestrictCell< T, NR > ), dim3( (unsigned) xcdGrid( ( nseg + 3 ) / 4 ) ), dim3( 256 ), 0, s, gc,
                           gf, sg, nseg, Pc.dCellType.p, flag, uf, uc );
