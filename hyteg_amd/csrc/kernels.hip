@@ -3173,7 +3173,7 @@ void sorT( Operator& op, Function& xf, const Function& bf, double omega, int lev
       return;
    }
    if ( backward && NR != 1 ) throw std::invalid_argument( "backward SOR needs one right-hand side" );
-   if ( !refreshFirst ) refreshT< T, NR >( xf, level, 7 );
+   // (closure copies are current: refreshed above when asked for, and every sweep ends with all of them current)
    hipLaunchKernelGGL( ( kVertexSor< T, NR > ), dim3( blocksFor( (int64_t) nV * NR, 256 ) ), dim3( 256 ), 0, s, g,
                        P.dVertexGeom.p, P.dVertexTerms.p, nV, wsel< T >( Wt.vertex64, Wt.vertex32 ), flag, u, b, om );
    refreshT< T, NR >( xf, level, 1 );
