@@ -1541,8 +1541,11 @@ __global__ void kFaceSor( LV g, const FaceGeom* __restrict__ fg, const T* __rest
 // kE2F; the two parts touch disjoint data: cell points are read, face boundary points written).
 constexpr int kFaceGhostBatch = 2;
 template < typename T, bool WITH_E2F >
+// With snap (not null), the face's own points of rows 0 .. w-2 are also copied to snap (face layout): the
+// start-of-phase values the one-launch colour phase stages its neighbours' rows from.
 __global__ __launch_bounds__( 256 ) void kFaceGhost( LV g, const FaceGeom* __restrict__ fg, const E2F* __restrict__ e2f,
-                                                     T* __restrict__ u, T* __restrict__ gh, const T* __restrict__ cellSrc )
+                                                     T* __restrict__ u, T* __restrict__ gh, const T* __restrict__ cellSrc,
+                                                     T* __restrict__ snap )
 {
    const FaceGeom& G  = fg[blockIdx.y];
    if ( WITH_E2F && blockIdx.x == gridDim.x - 1 )
@@ -1558,12 +1561,14 @@ __global__ __launch_bounds__( 256 ) void kFaceGhost( LV g, const FaceGeom* __res
    }
    const int       w  = g.w, t0 = tet32( w ), nP = ( w * ( w + 1 ) >> 1 ) - 1, nS = G.nSides; // rows 0 .. w-2
    const int       p0 = (int) blockIdx.x * 256 * kFaceGhostBatch + (int) threadIdx.x;
-   T               v[kFaceGhostBatch][2];
+   T               v[kFaceGhostBatch][2], sv[kFaceGhostBatch];
    bool            in[kFaceGhostBatch][2];
+   const int64_t   fo = g.offF + (int64_t) G.face * g.fS;
 #pragma unroll
    for ( int k = 0; k < kFaceGhostBatch; k++ )
    {
       const int p = p0 + 256 * k, pc = p < nP ? p : 0;
+      sv[k]       = snap ? u[fo + pc] : T( 0 );
       const int row = triRow32( w, pc ), col = pc - row * ( 2 * w + 1 - row ) / 2;
 #pragma unroll
       for ( int sd = 0; sd < 2; sd++ )
@@ -1581,6 +1586,10 @@ __global__ __launch_bounds__( 256 ) void kFaceGhost( LV g, const FaceGeom* __res
 #pragma unroll
       for ( int sd = 0; sd < 2; sd++ )
          if ( in[k][sd] ) gh[( 2 * (int64_t) G.face + sd ) * g.fS + p0 + 256 * k] = v[k][sd];
+   if ( snap )
+#pragma unroll
+      for ( int k = 0; k < kFaceGhostBatch; k++ )
+         if ( p0 + 256 * k < nP ) snap[(int64_t) G.face * g.fS + p0 + 256 * k] = sv[k];
 }
 
 template < typename T >
@@ -1616,6 +1625,80 @@ __global__ __launch_bounds__( 64 ) void kFaceSorRows( LV g, const FaceGeom* __re
       const int o = fi( i, j );
       fu[o]       = ( T( 1 ) - omega ) * fu[o] + rl * ( fb[o] - acc );
    }
+}
+
+// The three colour passes of the face phase in one launch, for faces too large for one workgroup per face.
+// Block (row block, face) owns interior rows [j0, j0 + RB) of the face.  It stages rows j0-3 .. j0+RB+2 of the
+// face (current values) and, for the rows it updates, q = b - (off-plane terms from the ghost layers gh) in LDS,
+// then runs the colours in order with a barrier between them.  A point's in-plane neighbours lie in its own row
+// and the rows beside it and have the two other colours, so the k-th colour of the sweep (k = 0, 1, 2) is
+// updated on the owned rows widened by 2 - k rows on either side: the values a later colour reads one row
+// beyond were computed here from the same inputs with the same expression as by the block that owns them.
+// Only the owned rows are written back.  (The off-plane sum is formed before the in-plane one, so the values
+// differ from kFaceSorRows' in rounding only.)  LDS: (2 RB + 10) rows of w values (dynamic).
+template < typename T >
+__global__ __launch_bounds__( 1024 ) void kFaceSorBlocks( LV g, const FaceGeom* __restrict__ fg, const T* __restrict__ FW,
+                                                         int flag, T* __restrict__ u, const T* __restrict__ b, T omega,
+                                                         int reverse, const T* __restrict__ gh, const T* __restrict__ snap,
+                                                         int RB )
+{
+   extern __shared__ unsigned char smem[];
+   T*              V = reinterpret_cast< T* >( smem );
+   const FaceGeom& G = fg[blockIdx.y];
+   if ( !( G.type & flag ) ) return; // uniform over the workgroup
+   const int w = g.w, jLast = w - 3; // interior rows 1 .. w-3
+   const int j0 = 1 + (int) blockIdx.x * RB, j1 = min( j0 + RB, jLast + 1 );
+   if ( j0 > jLast ) return;
+   const int jA = max( 0, j0 - 3 ), jB = min( w - 1, j1 + 2 ); // staged face rows
+   const int jq0 = max( 1, j0 - 2 ), jq1 = min( jLast, j1 + 1 ); // rows with updates
+   T*        Q  = V + ( RB + 6 ) * w;
+   const T*  fw = FW + (int64_t) G.face * kFaceW;
+   const T   rl = omega / fw[0];
+   T*        fu = u + g.offF + (int64_t) G.face * g.fS;
+   const T*  fb = b + g.offF + (int64_t) G.face * g.fS;
+   const T*  fs = snap + (int64_t) G.face * g.fS;
+   auto      fi = [&]( int ii, int jj ) { return jj * ( w + 1 ) - ( jj * ( jj + 1 ) >> 1 ) + ii; };
+   // one wave per row, lanes along the row
+   const int wave = (int) ( threadIdx.x >> 6 ), lane = (int) ( threadIdx.x & 63 ), nw = (int) ( blockDim.x >> 6 );
+   // interior points from the snapshot (another block may already have written its rows), boundary points
+   // (the edges' copies, final before the phase and written by none of its blocks) from u
+   for ( int j = jA + wave; j <= jB; j += nw )
+      for ( int i = lane; i <= w - 1 - j; i += 64 )
+         V[( j - jA ) * w + i] = ( i == 0 || j == 0 || i == w - 1 - j ) ? fu[fi( i, j )] : fs[fi( i, j )];
+   for ( int j = jq0 + wave; j <= jq1; j += nw )
+      for ( int i = 1 + lane; i <= w - 2 - j; i += 64 )
+      {
+         T off = T( 0 );
+         for ( int sd = 0; sd < G.nSides; sd++ )
+         {
+            const T* go = gh + ( 2 * (int64_t) G.face + sd ) * g.fS;
+#pragma unroll
+            for ( int k = 0; k < 4; k++ )
+               off += fw[7 + 4 * sd + k] * go[fi( i + G.offCol[sd][k], j + G.offRow[sd][k] )];
+         }
+         Q[( j - jq0 ) * w + i] = fb[fi( i, j )] - off;
+      }
+   __syncthreads();
+   for ( int cc = 0; cc < 3; cc++ )
+   {
+      const int colour = reverse ? 2 - cc : cc, ext = 2 - cc;
+      const int ra = max( 1, j0 - ext ), rb = min( jLast, j1 - 1 + ext );
+      for ( int j = ra + wave; j <= rb; j += nw )
+      {
+         const int i0 = 1 + ( ( colour - 1 - 2 * j ) % 3 + 3 ) % 3;
+         for ( int i = i0 + 3 * lane; i <= w - 2 - j; i += 3 * 64 )
+         {
+            const T* R0  = V + ( j - jA ) * w + i;
+            const T  acc = fw[1] * R0[-1] + fw[2] * R0[1] + fw[3] * R0[w] + fw[4] * R0[-w] + fw[5] * R0[w - 1] +
+                          fw[6] * R0[1 - w];
+            V[( j - jA ) * w + i] = ( T( 1 ) - omega ) * R0[0] + rl * ( Q[( j - jq0 ) * w + i] - acc );
+         }
+      }
+      __syncthreads();
+   }
+   for ( int j = j0 + wave; j < j1; j += nw )
+      for ( int i = 1 + lane; i <= w - 2 - j; i += 64 )
+         fu[fi( i, j )] = V[( j - jA ) * w + i];
 }
 
 // --------------------------------------------------------------------------------------------------------
@@ -3103,18 +3186,30 @@ void sorT( Operator& op, Function& xf, const Function& bf, double omega, int lev
             // large faces: one workgroup per face would leave most of the chip idle.  Forward, the faces'
             // copies of the just-updated edges are refreshed first; backward, the edges are not updated yet.
             // (forward: the edge-to-face copy runs as one more block per face of the ghost launch)
-            T*             gh     = (T*) M.faceGhost( P );
+            T*             gh     = (T*) M.faceGhost( P ); // two ghost layers per face, then the snapshot
+            T*             snap   = gh + 2 * (int64_t) nF * P.fStride;
             const unsigned chunks = ( P.w * ( P.w + 1 ) / 2 + 256 * kFaceGhostBatch - 1 ) / ( 256 * kFaceGhostBatch );
+            // the three colours in one launch where a block of at least 4 rows fits in 64 KB of LDS
+            int rb = 8;
+            while ( rb >= 4 && (size_t) ( 2 * rb + 10 ) * P.w * sizeof( T ) > 64 * 1024 )
+               rb /= 2;
+            T* snapArg = rb >= 4 ? snap : nullptr;
             if ( !backward )
                hipLaunchKernelGGL( ( kFaceGhost< T, true > ), dim3( chunks + 1, nF ), dim3( 256 ), 0, s, g, P.dFaceGeom.p,
-                                   P.dE2F.p, u, gh, cellSrc );
+                                   P.dE2F.p, u, gh, cellSrc, snapArg );
             else
                hipLaunchKernelGGL( ( kFaceGhost< T, false > ), dim3( chunks, nF ), dim3( 256 ), 0, s, g, P.dFaceGeom.p,
-                                   P.dE2F.p, u, gh, cellSrc );
-            for ( int cc = 0; cc < 3; cc++ )
-               hipLaunchKernelGGL( ( kFaceSorRows< T > ), dim3( P.w - 3, nF ), dim3( 64 ), 0, s, g, P.dFaceGeom.p,
-                                   wsel< T >( Wt.face64, Wt.face32 ), flag, u, b, om, backward ? 2 - cc : cc,
-                                   (const T*) gh );
+                                   P.dE2F.p, u, gh, cellSrc, snapArg );
+            if ( rb >= 4 )
+               hipLaunchKernelGGL( ( kFaceSorBlocks< T > ), dim3( ( P.w - 3 + rb - 1 ) / rb, nF ), dim3( 1024 ),
+                                   ( 2 * rb + 10 ) * P.w * sizeof( T ), s, g, P.dFaceGeom.p,
+                                   wsel< T >( Wt.face64, Wt.face32 ), flag, u, b, om, backward ? 1 : 0, (const T*) gh,
+                                   (const T*) snap, rb );
+            else
+               for ( int cc = 0; cc < 3; cc++ )
+                  hipLaunchKernelGGL( ( kFaceSorRows< T > ), dim3( P.w - 3, nF ), dim3( 64 ), 0, s, g, P.dFaceGeom.p,
+                                      wsel< T >( Wt.face64, Wt.face32 ), flag, u, b, om, backward ? 2 - cc : cc,
+                                      (const T*) gh );
          }
       };
       auto cellPhase = [&]() {

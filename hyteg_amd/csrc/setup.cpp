@@ -730,9 +730,10 @@ const MarchUnit* Mesh::transferUnits( LevelPlan& P, int64_t* count )
    return P.transfer->p;
 }
 
+// the SOR face phase's scratch: two ghost layers per face and a snapshot of the faces (face layout each)
 void* Mesh::faceGhost( LevelPlan& P )
 {
-   const size_t n = 2 * (size_t) T.nF() * (size_t) P.fStride;
+   const size_t n = 3 * (size_t) T.nF() * (size_t) P.fStride;
    if ( P.faceGhost.n < n ) P.faceGhost.alloc( n );
    return P.faceGhost.p;
 }
