@@ -2183,23 +2183,13 @@ __global__ __launch_bounds__( 256 ) void kRestrictFlat( LV gc, LV gf, const Marc
    const CellBuf< T > fu( uf + gf.offC + (int64_t) U.cell * gf.cS, gf.cS );
    const CellBuf< T > cu( uc + gc.offC + (int64_t) U.cell * gc.cS, gc.cS );
    auto      rs = [&]( int WW, int yy ) { return yy * ( WW + 1 ) - ( yy * ( yy + 1 ) >> 1 ); };
-   int       e0 = U.x0, y0 = U.y0;
+   int       e0 = U.x0;
    for ( int j = 0; j < kTransferWindows; j++, e0 += 64 )
    {
       if ( e0 >= nEl ) break;
-      while ( rs( Wd, y0 + 1 ) <= e0 )
-         y0++;
-      int yL = y0;
-      while ( yL + 1 < Wd && rs( Wd, yL + 1 ) <= e0 + 63 )
-         yL++;
-      const int e  = e0 + lane;
-      int       Y  = y0, next = rs( Wd, y0 + 1 );
-      for ( int it = y0; it < yL; it++ )
-         if ( e >= next )
-         {
-            Y++;
-            next += Wd - Y;
-         }
+      // the lane's row from the closed form (no scalar search over the rows of the window)
+      const int  e  = e0 + lane;
+      const int  Y  = triRow32( Wd, min( e, nEl - 1 ) );
       const int  X  = e - rs( Wd, Y );
       const bool ok = e < nEl && X >= 1 && Y >= 1 && X + Y <= Wd - 2;
       const int  x = 2 * X, y = 2 * Y;
@@ -2384,7 +2374,7 @@ __global__ __launch_bounds__( 256 ) void kProlongFlat( LV gc, LV gf, const March
    const CellBuf< T > cu( uc + gc.offC + (int64_t) U.cell * gc.cS, gc.cS );
    const CellBuf< T > fu( uf + gf.offC + (int64_t) U.cell * gf.cS, gf.cS );
    auto      rs = [&]( int WW, int yy ) { return yy * ( WW + 1 ) - ( yy * ( yy + 1 ) >> 1 ); };
-   int       e0 = U.x0, y0 = U.y0;
+   int       e0 = U.x0;
    // all windows' loads are issued before the first sum (twelve loads in flight per lane); lanes without
    // an interior point read index 0 and store nothing
    T    fv[kTransferWindows], ca[kTransferWindows], cb[kTransferWindows];
@@ -2398,20 +2388,10 @@ __global__ __launch_bounds__( 256 ) void kProlongFlat( LV gc, LV gf, const March
       int im = 0, ip = 0;
       if ( e0 < nEl )
       {
-         while ( rs( Wd, y0 + 1 ) <= e0 )
-            y0++;
-         int yL = y0;
-         while ( yL + 1 < Wd && rs( Wd, yL + 1 ) <= e0 + 63 )
-            yL++;
+         // the lane's row from the closed form (no scalar search over the rows of the window)
          const int e  = e0 + lane;
-         int       yy = y0, next = rs( Wd, y0 + 1 );
-         for ( int it = y0; it < yL; it++ )
-            if ( e >= next )
-            {
-               yy++;
-               next += Wd - yy;
-            }
-         const int x = e - rs( Wd, yy );
+         const int yy = triRow32( Wd, min( e, nEl - 1 ) );
+         const int x  = e - rs( Wd, yy );
          ok[j]       = e < nEl && x >= 1 && yy >= 1 && x + yy <= Wd - 2;
          if ( ok[j] )
          {
