@@ -1704,22 +1704,39 @@ __global__ __launch_bounds__( 1024 ) void kFaceSorBlocks( LV g, const FaceGeom* 
 // --------------------------------------------------------------------------------------------------------
 // macro-edge interior
 
+// Neighbour terms of the edge and vertex kernels are loaded kTermBatch at a time (all loads of a batch in
+// flight together) and summed in term order, so the result is that of the plain loop.
+constexpr int kTermBatch = 8;
+
+template < typename T, int NR >
+__device__ __forceinline__ int64_t edgeTermIndex( const LV& g, const EdgeTerm& e, int x, int t0 )
+{
+   const int w = g.w;
+   if ( e.kind == 0 )
+      return g.offF + (int64_t) e.prim * g.fS + faceRel( w, e.base[0] + x * e.inc[0], e.base[1] + x * e.inc[1] );
+   return g.offC + (int64_t) e.prim * g.cS +
+          cellRel( w, t0, e.base[0] + x * e.inc[0], e.base[1] + x * e.inc[1], e.base[2] + x * e.inc[2] );
+}
+
 template < typename T, int NR >
 __device__ __forceinline__ T edgeTermsSum( const LV& g, const EdgeGeom& G, const EdgeTerm* __restrict__ terms,
                                            const T* __restrict__ tw, const T* __restrict__ u, int x, int r )
 {
    T         acc = T( 0 );
-   const int w = g.w, t0 = tet32( w );
-   for ( int k = 0; k < G.nTerms; k++ )
+   const int t0 = tet32( g.w ), n = G.nTerms;
+   for ( int k0 = 0; k0 < n; k0 += kTermBatch )
    {
-      const EdgeTerm& e = terms[G.termOff + k];
-      int64_t         idx;
-      if ( e.kind == 0 )
-         idx = g.offF + (int64_t) e.prim * g.fS + faceRel( w, e.base[0] + x * e.inc[0], e.base[1] + x * e.inc[1] );
-      else
-         idx = g.offC + (int64_t) e.prim * g.cS +
-               cellRel( w, t0, e.base[0] + x * e.inc[0], e.base[1] + x * e.inc[1], e.base[2] + x * e.inc[2] );
-      acc += tw[k] * u[idx * NR + r];
+      T v[kTermBatch], wt[kTermBatch];
+#pragma unroll
+      for ( int q = 0; q < kTermBatch; q++ )
+      {
+         const int k = min( k0 + q, n - 1 );
+         v[q]        = u[edgeTermIndex< T, NR >( g, terms[G.termOff + k], x, t0 ) * NR + r];
+         wt[q]       = tw[k];
+      }
+#pragma unroll
+      for ( int q = 0; q < kTermBatch; q++ )
+         if ( k0 + q < n ) acc += wt[q] * v[q];
    }
    return acc;
 }
@@ -1784,8 +1801,20 @@ __global__ void kVertexApply( LV g, const VertexGeom* __restrict__ vg, const int
    if ( !( G.type & flag ) ) return;
    const T* vw  = VW + G.vertex + G.termOff;
    T        acc = vw[0] * u[(int64_t) G.vertex * NR + r];
-   for ( int k = 0; k < G.nTerms; k++ )
-      acc += vw[1 + k] * u[terms[G.termOff + k] * NR + r];
+   for ( int k0 = 0; k0 < G.nTerms; k0 += kTermBatch )
+   {
+      T v[kTermBatch], wt[kTermBatch];
+#pragma unroll
+      for ( int q = 0; q < kTermBatch; q++ )
+      {
+         const int k = min( k0 + q, G.nTerms - 1 );
+         v[q]        = u[terms[G.termOff + k] * NR + r];
+         wt[q]       = vw[1 + k];
+      }
+#pragma unroll
+      for ( int q = 0; q < kTermBatch; q++ )
+         if ( k0 + q < G.nTerms ) acc += wt[q] * v[q];
+   }
    const int64_t o = (int64_t) G.vertex * NR + r;
    if ( MODE == kReplace )
       y[o] = acc;
@@ -1807,8 +1836,20 @@ __global__ void kVertexSor( LV g, const VertexGeom* __restrict__ vg, const int64
    if ( !( G.type & flag ) ) return;
    const T* vw  = VW + G.vertex + G.termOff;
    T        acc = T( 0 );
-   for ( int k = 0; k < G.nTerms; k++ )
-      acc += vw[1 + k] * u[terms[G.termOff + k] * NR + r];
+   for ( int k0 = 0; k0 < G.nTerms; k0 += kTermBatch )
+   {
+      T v[kTermBatch], wt[kTermBatch];
+#pragma unroll
+      for ( int q = 0; q < kTermBatch; q++ )
+      {
+         const int k = min( k0 + q, G.nTerms - 1 );
+         v[q]        = u[terms[G.termOff + k] * NR + r];
+         wt[q]       = vw[1 + k];
+      }
+#pragma unroll
+      for ( int q = 0; q < kTermBatch; q++ )
+         if ( k0 + q < G.nTerms ) acc += wt[q] * v[q];
+   }
    const int64_t o = (int64_t) G.vertex * NR + r;
    u[o]            = ( T( 1 ) - omega ) * u[o] + omega / vw[0] * ( b[o] - acc );
 }
@@ -2258,17 +2299,21 @@ __global__ void kRestrictEdge( LV gc, LV gf, const EdgeGeom* __restrict__ egFine
    const int64_t eo  = gf.offE + (int64_t) G.edge * gf.eS;
    T             acc = uf[( eo + x - 1 ) * NR + r] * T( 0.5 ) + uf[( eo + x + 1 ) * NR + r] * T( 0.5 );
    acc += uf[( eo + x ) * NR + r];
-   for ( int k = 0; k < G.nTerms; k++ )
+   const int t0 = tet32( gf.w ), n = G.nTerms;
+   for ( int k0 = 0; k0 < n; k0 += kTermBatch )
    {
-      const EdgeTerm& e = terms[G.termOff + k];
-      if ( e.wr == 0.0f ) continue;
-      int64_t idx;
-      if ( e.kind == 0 )
-         idx = gf.offF + (int64_t) e.prim * gf.fS + faceRel( gf.w, e.base[0] + x * e.inc[0], e.base[1] + x * e.inc[1] );
-      else
-         idx = gf.offC + (int64_t) e.prim * gf.cS +
-               cellRel( gf.w, tet32( gf.w ), e.base[0] + x * e.inc[0], e.base[1] + x * e.inc[1], e.base[2] + x * e.inc[2] );
-      acc += T( e.wr ) * uf[idx * NR + r];
+      T     v[kTermBatch];
+      float wr[kTermBatch];
+#pragma unroll
+      for ( int q = 0; q < kTermBatch; q++ )
+      {
+         const EdgeTerm& e = terms[G.termOff + min( k0 + q, n - 1 )];
+         wr[q]             = e.wr;
+         v[q]              = uf[edgeTermIndex< T, NR >( gf, e, x, t0 ) * NR + r];
+      }
+#pragma unroll
+      for ( int q = 0; q < kTermBatch; q++ )
+         if ( k0 + q < n && wr[q] != 0.0f ) acc += T( wr[q] ) * v[q];
    }
    uc[( gc.offE + (int64_t) G.edge * gc.eS + X ) * NR + r] = acc;
 }
@@ -2284,8 +2329,16 @@ __global__ void kRestrictVertex( const VertexGeom* __restrict__ vgFine, const in
    const VertexGeom G = vgFine[v];
    if ( !( G.type & flag ) ) return;
    T acc = T( 0 );
-   for ( int k = 0; k < G.nTerms; k++ )
-      acc += uf[termsFine[G.termOff + k] * NR + r];
+   for ( int k0 = 0; k0 < G.nTerms; k0 += kTermBatch )
+   {
+      T v[kTermBatch];
+#pragma unroll
+      for ( int q = 0; q < kTermBatch; q++ )
+         v[q] = uf[termsFine[G.termOff + min( k0 + q, G.nTerms - 1 )] * NR + r];
+#pragma unroll
+      for ( int q = 0; q < kTermBatch; q++ )
+         if ( k0 + q < G.nTerms ) acc += v[q];
+   }
    uc[(int64_t) G.vertex * NR + r] = uf[(int64_t) G.vertex * NR + r] + T( 0.5 ) * acc;
 }
 
