@@ -176,3 +176,35 @@ def test_cell_sor_setting(H, hctx):
     assert [S.get_cell_sor_setting(L) for L in (5, 6, 7, 8)] == [(0, 0), (0, 0), (24, 3), (24, 3)]
     S.use_column_schedules(8, fusedSorFp32From=8)
     assert [S.get_cell_sor_setting(L) for L in (6, 7, 8)] == [(0, 0), (24, 1), (24, 3)]
+
+
+def _tri_row_f32(W, f):
+    """The row lookup the device kernels use (kernels.hip triRow32): a float32 estimate from the smaller root of
+    the row-start quadratic R(y) = y (2W + 1 - y) / 2, then one correction step either way."""
+    f = np.asarray(f, dtype=np.int64)
+    q = np.float32(2 * W + 1)
+    y = (np.float32(0.5) * (q - np.sqrt(q * q - np.float32(8.0) * f.astype(np.float32)))).astype(np.int64)
+    y += ((y + 1) * (2 * W - y) // 2 <= f).astype(np.int64)
+    y -= (y * (2 * W + 1 - y) // 2 > f).astype(np.int64)
+    return y
+
+
+def test_row_lookup_closed_form_at_every_row_boundary():
+    """The flat transfer, face-ghost and band-copy kernels find the row of a flat triangle index in closed form.
+    Against a search over the row starts it must hold at both sides of every row boundary, for triangle widths
+    up to those of level 11 (a perturbed estimate is caught too: the check is not vacuous)."""
+    widths = list(range(2, 300)) + list(range(300, 2060, 7)) + [1025, 2049, 2050]
+    for W in widths:
+        n = W * (W + 1) // 2
+        ys = np.arange(W)
+        starts = ys * (2 * W + 1 - ys) // 2
+        f = np.concatenate([starts, starts + 1, starts - 1, starts + (W - ys) - 1])
+        f = f[(f >= 0) & (f < n)]
+        exact = np.searchsorted(starts, f, side="right") - 1
+        assert np.array_equal(_tri_row_f32(W, f), exact), W
+    # the check can fail: with the row length off by one (rows of W + 1 - y points) the lookup misses boundaries
+    W = 257
+    ys = np.arange(W)
+    starts = ys * (2 * W + 1 - ys) // 2
+    f = np.concatenate([starts, starts - 1])[1:]
+    assert not np.array_equal(_tri_row_f32(W + 1, f), np.searchsorted(starts, f, side="right") - 1)
