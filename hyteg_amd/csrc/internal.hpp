@@ -110,7 +110,8 @@ struct LevelPlan
    DevBuf< int64_t >    dVertexEnds;
    // the vertex terms again, each read from a face's closure copy of the same edge point (a copy no kernel of
    // the vertex and edge phases writes), and per edge {vertex 0, vertex 1, 1 if the edge writes vertex 0's own
-   // value, the same for vertex 1}: the forward sweep's vertex phase runs inside the edge workgroups
+   // value, the same for vertex 1, the two vertices' term offsets, term counts and types}: the forward sweep's
+   // vertex phase runs inside the edge workgroups
    DevBuf< int64_t >    dVertexSnap;
    DevBuf< int32_t >    dEdgeEnds;
    DevBuf< int32_t >    dCellType;

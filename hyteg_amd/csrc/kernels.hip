@@ -1905,8 +1905,7 @@ template < typename T, bool VERT >
 __global__ __launch_bounds__( kEdgeThreads ) void kEdgeSorFused( LV g, const EdgeGeom* __restrict__ eg,
                                                         const EdgeTerm* __restrict__ terms, const T* __restrict__ EW,
                                                         int flag, T* __restrict__ u, const T* __restrict__ b, T omega,
-                                                        int reverse, const VertexGeom* __restrict__ vg,
-                                                        const int64_t* __restrict__ vsnap,
+                                                        int reverse, const int64_t* __restrict__ vsnap,
                                                         const int32_t* __restrict__ ends, const T* __restrict__ VW,
                                                         const T* __restrict__ cellSrc )
 {
@@ -1922,25 +1921,27 @@ __global__ __launch_bounds__( kEdgeThreads ) void kEdgeSorFused( LV g, const Edg
       const int wave = (int) ( threadIdx.x >> 6 ), lane = (int) ( threadIdx.x & 63 );
       if ( wave < 2 )
       {
-         const int        v   = ends[4 * G.edge + wave];
-         const VertexGeom V   = vg[v];
-         const int64_t    eIx = eo + ( wave == 0 ? 0 : g.w - 1 );
-         const T          old = u[eIx];
-         T                nv  = old;
-         if ( V.type & flag )
+         // the edge's end vertex data straight from the per-edge table (edge id = geometry index, checked on the
+         // host), so the term loads follow after one round trip
+         const int32_t* en     = ends + 10 * (int64_t) blockIdx.x;
+         const int      v      = en[wave], vTermOff = en[4 + wave], vN = en[6 + wave], vType = en[8 + wave];
+         const int64_t  eIx    = g.offE + (int64_t) blockIdx.x * g.eS + ( wave == 0 ? 0 : g.w - 1 );
+         const T        old    = u[eIx];
+         T              nv     = old;
+         if ( vType & flag )
          {
-            const T* vw   = VW + V.vertex + V.termOff;
+            const T* vw   = VW + v + vTermOff;
             T        part = T( 0 );
-            for ( int k = lane; k < V.nTerms; k += 64 )
-               part += vw[1 + k] * u[vsnap[V.termOff + k]];
+            for ( int k = lane; k < vN; k += 64 )
+               part += vw[1 + k] * u[vsnap[vTermOff + k]];
             const T acc = waveSum( part );
-            nv          = ( T( 1 ) - omega ) * old + omega / vw[0] * ( b[V.vertex] - acc );
+            nv          = ( T( 1 ) - omega ) * old + omega / vw[0] * ( b[v] - acc );
          }
          if ( lane == 0 )
          {
             sVert[wave] = nv;
             u[eIx]      = nv;
-            if ( ends[4 * G.edge + 2 + wave] ) u[V.vertex] = nv;
+            if ( en[2 + wave] ) u[v] = nv;
          }
       }
    }
@@ -2671,24 +2672,13 @@ __global__ __launch_bounds__( 256 ) void kDotFlat( LV g, const MarchUnit* __rest
       const int      z = U.z, Wd = w - z, nEl = Wd * ( Wd + 1 ) / 2, S0 = t0 - tet32( Wd );
       const T*       ca = a + g.offC + (int64_t) U.cell * g.cS + S0;
       const T*       cb = b + g.offC + (int64_t) U.cell * g.cS + S0;
-      int            e0 = U.x0, y0 = U.y0;
+      int            e0 = U.x0;
       for ( int j = 0; j < kTransferWindows; j++, e0 += 64 )
       {
          if ( e0 >= nEl ) break;
-         while ( rs( Wd, y0 + 1 ) <= e0 )
-            y0++;
-         int yL = y0;
-         while ( yL + 1 < Wd && rs( Wd, yL + 1 ) <= e0 + 63 )
-            yL++;
          const int e  = e0 + lane;
-         int       yy = y0, next = rs( Wd, y0 + 1 );
-         for ( int it = y0; it < yL; it++ )
-            if ( e >= next )
-            {
-               yy++;
-               next += Wd - yy;
-            }
-         const int x = e - rs( Wd, yy );
+         const int yy = triRow32( Wd, min( e, nEl - 1 ) ); // the lane's row in closed form
+         const int x  = e - rs( Wd, yy );
          if ( e < nEl && x >= 1 && yy >= 1 && x + yy <= Wd - 2 ) v += (double) ca[e] * (double) cb[e];
       }
    }
@@ -3200,14 +3190,14 @@ void sorT( Operator& op, Function& xf, const Function& bf, double omega, int lev
          if ( P.w > kEdgeMaxW ) throw std::invalid_argument( "the fused edge SOR supports levels up to 10" );
          hipLaunchKernelGGL( ( kEdgeSorFused< T, false > ), dim3( nE ), dim3( kEdgeThreads ), 0, s, g, P.dEdgeGeom.p,
                              P.dEdgeTerms.p, wsel< T >( Wt.edge64, Wt.edge32 ), flag, u, b, om, rev, nullptr, nullptr,
-                             nullptr, nullptr, cellSrc );
+                             nullptr, cellSrc );
       };
       // forward: the vertex phase computed inside the edge workgroups (one launch for both phases)
       auto vertexEdgePhase = [&]() {
          if ( P.w > kEdgeMaxW ) throw std::invalid_argument( "the fused edge SOR supports levels up to 10" );
          hipLaunchKernelGGL( ( kEdgeSorFused< T, true > ), dim3( nE ), dim3( kEdgeThreads ), 0, s, g, P.dEdgeGeom.p,
-                             P.dEdgeTerms.p, wsel< T >( Wt.edge64, Wt.edge32 ), flag, u, b, om, rev, P.dVertexGeom.p,
-                             P.dVertexSnap.p, P.dEdgeEnds.p, wsel< T >( Wt.vertex64, Wt.vertex32 ), cellSrc );
+                             P.dEdgeTerms.p, wsel< T >( Wt.edge64, Wt.edge32 ), flag, u, b, om, rev, P.dVertexSnap.p,
+                             P.dEdgeEnds.p, wsel< T >( Wt.vertex64, Wt.vertex32 ), cellSrc );
       };
       auto facePhase = [&]() {
          if ( nF == 0 ) return;

@@ -471,11 +471,18 @@ void buildPlan( Mesh& M, LevelPlan& P, int level )
          const E2F& m = e2f[3 * f + le];
          vsnap.push_back( P.faceOff( f ) + faceIndex( w, m.base[0] + p * m.inc[0], m.base[1] + p * m.inc[1] ) );
       }
+   // per edge (in edge order, which is the order of the edge geometry): both end vertices, whether the edge
+   // writes each vertex's own value, and each vertex's term offset, term count and type (what the vertex phase
+   // inside the edge workgroups reads, without a second lookup through the vertex geometry)
    std::vector< int32_t > ends;
    for ( int e = 0; e < T.nE(); e++ )
    {
+      if ( P.hEdgeGeom[e].edge != e ) throw std::runtime_error( "plan validation failed: edge geometry order" );
       const int v0 = T.ev[e][0], v1 = T.ev[e][1];
-      ends.insert( ends.end(), { v0, v1, T.vEdges[v0].at( 0 ) == e ? 1 : 0, T.vEdges[v1].at( 0 ) == e ? 1 : 0 } );
+      const VertexGeom &g0 = P.hVertexGeom[v0], &g1 = P.hVertexGeom[v1];
+      if ( g0.vertex != v0 || g1.vertex != v1 ) throw std::runtime_error( "plan validation failed: vertex geometry order" );
+      ends.insert( ends.end(), { v0, v1, T.vEdges[v0].at( 0 ) == e ? 1 : 0, T.vEdges[v1].at( 0 ) == e ? 1 : 0, g0.termOff,
+                                 g1.termOff, g0.nTerms, g1.nTerms, g0.type, g1.type } );
    }
    for ( int64_t idx : vsnap )
       if ( idx < P.offF || idx >= P.offC ) throw std::runtime_error( "plan validation failed: vertex snapshot term" );
