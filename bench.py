@@ -206,6 +206,30 @@ def cell_phase_roofline(H, S, A, L, prec, Ncell, reps):
     return r
 
 
+def residual_restriction(H, S, A, L, Ncell, reps):
+    """The residual on level L restricted to level L-1 as a V-cycle runs them (selector 9), in one pass
+    (kResidualRestrict) and as two passes (the residual, then the restriction), FP32 and FP64.  `fused_GBs` =
+    algorithmic bytes of the one pass (read x and b at the fine cell DoFs, write the coarse cell DoFs: 2 s + s / 8 per
+    fine cell DoF) / its measured time."""
+    out = {}
+    default = S.get_residual_restrict(L)
+    for prec, name, s in ((H.F32, "fp32", 4), (H.F64, "fp64", 8)):
+        x, b = H.P1Function("qx", S, L - 1, L, precision=prec), H.P1Function("qb", S, L - 1, L, precision=prec)
+        r = H.P1Function("qr", S, L, L, precision=prec)
+        x.interpolate_modes(sine_modes(), L)
+        b.interpolate(1e-3, L, H.ALL)
+        t = {}
+        for mode in (1, 0):
+            S.set_residual_restrict(L, mode)
+            t[mode] = time_kernel(H, A, None, x, b, r, L, 9, reps)
+        out[name] = {"fused_ms": t[1], "two_pass_ms": t[0],
+                     "fused_GBs": (2 * s + s / 8) * Ncell / (t[1] * 1e-3) / 1e9}
+        del x, b, r
+    S.set_residual_restrict(L, -1)
+    out["default_fused"] = bool(default)
+    return out
+
+
 def apply_roofline(H, S, A, L, Ncell, reps):
     """The macro-cell 15-point apply kernel (FP64) and a plain copy of the same level, both timed live."""
     x, y = H.P1Function("ax", S, L, L), H.P1Function("ay", S, L, L)
@@ -409,8 +433,9 @@ def cpu_baseline(n, L, cycles, apply_runs, seed):
 def kernels_only(H, n, L, reps):
     """Every measured kernel of the headline workload alone, launched eagerly (no graphs): for rocprofv3 kernel
     traces and counter passes.  One V(2,2) cycle in FP32 and in FP64 (x = 0, b = the headline's right-hand
-    side), then `reps` launches each of the SOR cell phase of two sweeps (FP32 and FP64), the FP64
-    macro-cell apply and a plain copy of the level (the reference for scaling FETCH_SIZE / WRITE_SIZE)."""
+    side), then `reps` launches each of the SOR cell phase of two sweeps (FP32 and FP64), the residual and
+    restriction in one pass and in two (FP32 and FP64), the FP64 macro-cell apply and a plain copy of the level (the
+    reference for scaling FETCH_SIZE / WRITE_SIZE)."""
     S = storage(H, n)
     A = H.P1ConstantLaplaceOperator(S, 0, L)
     b, t = noisy_sine_rhs(H, S, L, SEED)
@@ -424,6 +449,7 @@ def kernels_only(H, n, L, reps):
     Ncell = cell_interior_count(n, L)
     out["cell_phase_fp32"] = cell_phase_roofline(H, S, A, L, H.F32, Ncell, reps)["ms_per_sequence"]
     out["cell_phase_fp64"] = cell_phase_roofline(H, S, A, L, H.F64, Ncell, reps)["ms_per_sequence"]
+    out["residual_restriction"] = residual_restriction(H, S, A, L, Ncell, reps)
     ra = apply_roofline(H, S, A, L, Ncell, reps)
     out.update({"apply_ms": ra["ms_per_launch"], "copy_ms": ra["plain_copy_ms"],
                 "copy_bytes_each_way": 8 * S.owned_count(L)})
@@ -478,6 +504,7 @@ def main():
     roof32 = cell_phase_roofline(H, S, A, L, H.F32, Ncell, reps)
     roof64 = cell_phase_roofline(H, S, A, L, H.F64, Ncell, reps)
     roof_apply = apply_roofline(H, S, A, L, Ncell, reps)
+    res_restrict = residual_restriction(H, S, A, L, Ncell, reps)
     stage("kernel rooflines")
     for k in ("x", "b", "t", "S", "A"):
         hl.pop(k)
@@ -515,6 +542,7 @@ def main():
         "roofline": roof32,
         "roofline_fp64": roof64,
         "roofline_apply": roof_apply,
+        "residual_restriction": res_restrict,
         "mixed_precision": {k: v for k, v in hl.items() if k != "N"},
     }
     if not args.no_config2:

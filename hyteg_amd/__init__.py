@@ -183,6 +183,17 @@ class PrimitiveStorage:
         values bit for bit)."""
         _lib.call("hymi_mesh_set_cell_sor", self.h, level, int(tile), (1 if fp64 else 0) | (2 if fp32 else 0))
 
+    def set_residual_restrict(self, level, mode):
+        """Residual and restriction of a V-cycle from fine `level`: 1 = the macro-cell interiors in one pass, 0 = two
+        passes, -1 = default (one pass from level 8 up).  Results agree to rounding."""
+        _lib.call("hymi_mesh_set_residual_restrict", self.h, level, int(mode))
+
+    def get_residual_restrict(self, level):
+        """1 where `level` runs the fused residual and restriction."""
+        out = ctypes.c_int()
+        _lib.call("hymi_mesh_get_residual_restrict", self.h, level, ctypes.byref(out))
+        return out.value
+
     def get_cell_sor(self, level):
         """Tile of the fused cell phase on `level` (0: colour passes)."""
         return self.get_cell_sor_setting(level)[0]

@@ -110,6 +110,8 @@ SIGNATURES = {
     "hymi_mesh_get_schedule": [_vp, _i, _P(_i)],
     "hymi_mesh_set_cell_sor": [_vp, _i, _i, _i],
     "hymi_mesh_get_cell_sor": [_vp, _i, _P(_i)],
+    "hymi_mesh_set_residual_restrict": [_vp, _i, _i],
+    "hymi_mesh_get_residual_restrict": [_vp, _i, _P(_i)],
     "hymi_mesh_autotune": [_vp, _i, _i, _i, _P(_d), _i, _P(_i)],
 }
 

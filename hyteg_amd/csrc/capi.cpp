@@ -449,10 +449,9 @@ struct hymi_gmg_s
       // below the invoked one
       const int  extra  = smoothInc * ( maxL_invoked - L );
       smooth( x, b, L, flag, nu1 + extra, finest );
-      launchApply( *A, x, *tmp, &b, L, flag, kResidual, nu1 + extra == 0 && finest );
-      // restriction written straight into b on the coarse level (== restrict + b.assign on flag DoFs); tmp is
-      // scratch, so its cell closures are refreshed only where the restriction reads them
-      launchRestrict( *tmp, b, L, flag, true );
+      // residual into tmp, restricted straight into b on the coarse level (== restrict + b.assign on flag DoFs);
+      // tmp is scratch, so its cell closures are refreshed only where the restriction reads them
+      launchResidualRestrict( *A, x, *tmp, b, L, flag, nu1 + extra == 0 && finest );
       launchLincomb( x, L - 1, kAll, 0, nullptr, nullptr, false, 0.0 );
       recurse( x, b, L - 1 );
       // W-cycle (GeometricMultigridSolver.hpp:201-204): the coarse correction is visited again, from its
@@ -767,6 +766,22 @@ int hymi_mesh_set_cell_sor( hymi_mesh mesh, int level, int tile, int precisions 
    return guarded( [&] {
       require( mesh && level >= 0 && level <= 12, "bad argument" );
       mesh->m.setCellSor( level, tile, precisions );
+   } );
+}
+
+int hymi_mesh_set_residual_restrict( hymi_mesh mesh, int level, int mode )
+{
+   return guarded( [&] {
+      require( mesh && level >= 0 && level <= 12, "bad argument" );
+      mesh->m.setResidualRestrict( level, mode );
+   } );
+}
+
+int hymi_mesh_get_residual_restrict( hymi_mesh mesh, int level, int* out )
+{
+   return guarded( [&] {
+      require( mesh && out && level >= 0 && level <= 12, "bad argument" );
+      *out = mesh->m.residualRestrictOn( level ) ? 1 : 0;
    } );
 }
 
@@ -1392,6 +1407,11 @@ int hymi_gmg_create( hymi_operator op, hymi_function xt, int minL, int maxL, int
          M.closureRanges( P, kInner | kNeumann | kFreeslip, &n );
          M.closureRanges( P, kAll, &n );
          if ( g->nrhs == 1 ) M.prepareCellSor( P ); // the fused cell phase's buffers, if the level selects it
+         if ( g->nrhs == 1 && L > minL && L >= 3 )
+         {
+            int extra = 0;
+            M.residualRestrictUnits( M.plan( L - 1 ), &n, &extra );
+         }
       }
       // the context's scalar slots and partial-sum buffer were sized for 32 right-hand sides when it was
       // created and are never reallocated (graphs of other solvers on this context hold their addresses)
@@ -1700,8 +1720,14 @@ int hymi_time_kernel( hymi_operator op, hymi_gmg gmg, hymi_function x, hymi_func
       require( op && x && ms && reps > 0, "bad argument" );
       checkOpFunc( op, x, level );
       if ( which == 1 || which == 3 || which == 4 || which == 6 || which == 7 || which == 8 ) checkOpFunc( op, b, level );
-      if ( which == 0 || which == 2 || which == 5 ) checkOpFunc( op, y, level );
+      if ( which == 0 || which == 2 || which == 5 || which == 9 ) checkOpFunc( op, y, level );
       if ( which == 4 ) checkGmg( gmg, x, b, level );
+      if ( which == 9 )
+      {
+         checkOpFunc( op, b, level );
+         require( level >= 1, "residual and restriction need level >= 1" );
+         requireLevel( b->f, level - 1 );
+      }
       Operator&   A = op->op;
       hipStream_t s = A.mesh->ctx->stream;
       const int   flag = kInner | kNeumann;
@@ -1717,6 +1743,7 @@ int hymi_time_kernel( hymi_operator op, hymi_gmg gmg, hymi_function x, hymi_func
          case 6: launchCellSorPhase( A, x->f, b->f, 1.0, level, flag, false ); break;
          case 7: launchCellSorPhase( A, x->f, b->f, 1.0, level, flag, true ); break;
          case 8: launchCellSorSequence( A, x->f, b->f, 1.0, level, flag, 2 ); break;
+         case 9: launchResidualRestrict( A, x->f, y->f, b->f, level, flag, false ); break;
          case 5:
          {
             // plain copy y = x of the whole level (closures of all primitives): a bandwidth reference

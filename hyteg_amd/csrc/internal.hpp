@@ -133,6 +133,15 @@ struct LevelPlan
    // grid-transfer kernels; fields as for the flat macro-cell kernel (x0 = first element, y0 = its row)
    std::unique_ptr< DevBuf< MarchUnit > > transfer;
    int64_t                                transferCount = -1;
+   // work units of the fused residual-and-restriction kernel (this level as the coarse one): cell, run of coarse
+   // slabs Za | Zb << 16, x0, y0 of the tile (1 + multiples of kResBandX, kResBandY), longest runs first; then the fine
+   // points next to the faces that no tile computes, as (0, z, x, y)
+   std::unique_ptr< DevBuf< MarchUnit > > resUnits;
+   int64_t                                resUnitCount  = -1;
+   int                                    resExtraCount = 0;
+   // residual and restriction from this level as one pass over the macro-cell interiors: -1 the default (on from
+   // kResRestrictMinLevel), 0 off, 1 on
+   int resRestrict = -1;
    // face ghost layers of the SOR sweep: per face and side, the neighbour cell's first interior layer in
    // face coordinates (2 nF fStride elements, FP64-sized so that either precision fits)
    DevBuf< double > faceGhost;
@@ -169,6 +178,7 @@ struct Mesh
    const CellSeg* segments( LevelPlan& P, int len, int64_t* count );
    const MarchUnit* marchUnits( LevelPlan& P, int64_t* count );
    const MarchUnit* transferUnits( LevelPlan& P, int64_t* count );
+   const MarchUnit* residualRestrictUnits( LevelPlan& Pc, int64_t* count, int* extra );
    // face ghost buffer of level P (allocated on first use)
    void* faceGhost( LevelPlan& P );
 
@@ -183,6 +193,10 @@ struct Mesh
    void selectCellSor( int level, int tile, int precisions );
    void prepareCellSor( LevelPlan& P );
    bool cellSorReady( const LevelPlan& P ) const;
+   // fused residual and restriction from `level` (one right-hand side): -1 default, 0 off, 1 on (level >= 3);
+   // bumps scheduleEpoch like setSchedule
+   void setResidualRestrict( int level, int mode );
+   bool residualRestrictOn( int level );
    uint64_t scheduleEpoch = 0;
    // affine frames of the primitives, level-independent: micro-vertex (x, y, z) of a primitive lies at
    // o + (x a + y b + z c) / (w - 1); 12 doubles (o, a, b, c) per primitive, vertices, edges, faces, cells

@@ -2247,6 +2247,149 @@ __global__ __launch_bounds__( 256 ) void kRestrictFlat( LV gc, LV gf, const Marc
    }
 }
 
+// Residual and restriction of the macro-cell interiors in one pass (one right-hand side): the fine residual
+// r = b - A u (the terms and summation order of kCellColumn) is restricted with kRestrictFlat's weights and
+// summation order straight into the coarse array.  A workgroup of kResBandY * 2 + 1 waves takes kResBandX x
+// kResBandY coarse points (X, Y) of one cell and a run of coarse slabs Za .. Zb (units from
+// Mesh::residualRestrictUnits of the coarse level, longest runs first).  Wave k marches fine row y = 2 Y0 - 1 + k
+// up in z with its lanes on x = 2 X0 - 2 + lane, like kCellColumn: the values of u (at y, y + 1 and y - 1) and of b
+// are loaded D slabs ahead into a register ring, the x neighbours come from the neighbour lanes, and lanes 1 .. 62
+// hold residuals.  Each residual goes to an LDS ring of five slabs; after fine slabs 2Z and 2Z+1 one barrier, then
+// coarse slab Z is gathered from fine slabs 2Z-1 .. 2Z+1 (five slots: the slab a step writes was last read two
+// steps earlier, before the barrier in between).  Every fine point a coarse interior point reads is a fine interior
+// point, so no closure values of the residual are needed.  The residual reaches memory only at the interior points
+// next to the faces (x, y or z == 1, x + y + z == w - 2): those are the cell points the face, edge and vertex
+// restriction read; the other interior points of r are left as they were.  Neighbouring tiles share two fine
+// columns and one fine row, written by the tile below only; the first slab of a run (2 Za - 1) is written by the
+// run below.  The few such points no tile reaches (listed after the tiles, nExtra of them per cell) are computed
+// from global memory by the run at (1, 1, 1).
+template < typename T, int D >
+__global__ __launch_bounds__( 64 * ( 2 * kResBandY + 1 ), sizeof( T ) == 4 ? 8 : 4 ) void kResidualRestrict(
+    LV gf, LV gc, const MarchUnit* __restrict__ units, int64_t nUnits, int nExtra, const T* __restrict__ W8,
+    const int* __restrict__ ctype, int flag, const T* __restrict__ u, const T* __restrict__ b, T* __restrict__ r,
+    T* __restrict__ bc )
+{
+   constexpr int NW = 2 * kResBandY + 1, NT = 64 * NW, NS = 3 + D, RING = 5;
+   static_assert( 2 * kResBandX + 2 <= 64 && kResBandX * kResBandY <= NT, "tile does not fit the lanes" );
+   __shared__ T rs[RING * NT]; // residual of fine slab z at (z % RING) * NT + 64 k + lane
+   const int64_t ui = blockIdx.x;
+   if ( ui >= nUnits ) return;
+   const MarchUnit un = units[ui];
+   if ( !( ctype[un.cell] & flag ) ) return;
+   const int     w = gf.w, wc = gc.w, t0 = tet32( w ), tc = tet32( wc );
+   const int     X0 = un.x0, Y0 = un.y0, Za = un.z & 0xffff, Zb = un.z >> 16, tid = (int) threadIdx.x;
+   const int64_t base = gf.offC + (int64_t) un.cell * gf.cS;
+   const CellBuf< T > cu( u + base, t0 ), cb( b + base, t0 ), cr( r + base, t0 );
+   const CellBuf< T > co( bc + gc.offC + (int64_t) un.cell * gc.cS, tc );
+   const T*           c = W8 + (int64_t) un.cell * 15;
+   const T cC = c[dC], cW = c[dW], cE = c[dE], cN = c[dN], cS = c[dS], cNW = c[dNW], cSE = c[dSE];
+   const T cTC = c[dTC], cTW = c[dTW], cTS = c[dTS], cTSE = c[dTSE], cBC = c[dBC], cBN = c[dBN], cBE = c[dBE],
+           cBNW = c[dBNW];
+   // index of fine (x, y, z) in the cell = t0 - tet(w - z) + y (w + 1 - z) - y (y + 1) / 2 + x
+   auto idx = [&]( int x, int y, int z ) { return t0 - tet32( w - z ) + y * ( w + 1 - z ) - ( ( y * ( y + 1 ) ) >> 1 ) + x; };
+
+   if ( X0 == 1 && Y0 == 1 && Za == 1 )
+      for ( int k = tid; k < nExtra; k += NT )
+      {
+         const MarchUnit e = units[nUnits + k];
+         const int       x = e.x0, y = e.y0, z = e.z;
+         const int       o = idx( x, y, z ), oN = idx( x, y + 1, z ), oS = idx( x, y - 1, z );
+         const int       oT = idx( x, y, z + 1 ), oTS = idx( x, y - 1, z + 1 );
+         const int       oB = idx( x, y, z - 1 ), oBN = idx( x, y + 1, z - 1 );
+         T               acc = cC * cu[o];
+         acc += cW * cu[o - 1] + cE * cu[o + 1];
+         acc += cN * cu[oN] + cNW * cu[oN - 1];
+         acc += cS * cu[oS] + cSE * cu[oS + 1];
+         acc += cTC * cu[oT] + cTW * cu[oT - 1];
+         acc += cTS * cu[oTS] + cTSE * cu[oTS + 1];
+         acc += cBC * cu[oB] + cBE * cu[oB + 1];
+         acc += cBN * cu[oBN] + cBNW * cu[oBN - 1];
+         cr.store( o, cb[o] - acc, true );
+      }
+
+   const int  lane = tid & 63, k = __builtin_amdgcn_readfirstlane( tid >> 6 );
+   const int  x = 2 * X0 - 2 + lane, y = 2 * Y0 - 1 + k;
+   const bool cand = lane >= 1 && lane <= 62;
+   // written to r: next to a face and not in the columns / row shared with the tile below
+   const bool own  = cand && ( X0 == 1 || lane >= 3 ) && ( Y0 == 1 || k >= 1 );
+   const int  zEnd = 2 * Zb + 2; // last slab a residual of the run reads
+   // register ring of slabs (u at y, y + 1, y - 1 and b at y); slab s in slot (s - 2 Za + 2) mod NS
+   T v0[NS], vp[NS], vm[NS], vb[NS];
+   int sl = 2 * Za - 2; // next slab to load
+   auto load = [&]( int j ) {
+      const bool go = sl <= zEnd;
+      v0[j]         = cu.load( idx( x, y, sl ), go && x + y + sl <= w - 1 );
+      vp[j]         = cu.load( idx( x, y + 1, sl ), go && x + y + 1 + sl <= w - 1 );
+      vm[j]         = cu.load( idx( x, y - 1, sl ), go && x + y - 1 + sl <= w - 1 );
+      vb[j]         = cb.load( idx( x, y, sl ), go && x + y + sl <= w - 2 );
+      sl++;
+   };
+   // residual at fine slab z from slots (B, Z, T) = slabs z - 1, z, z + 1; into rs and, where owned, r
+   auto residual = [&]( int z, int sB, int sZ, int sT, bool store ) {
+      const T Z0 = v0[sZ], Zp = vp[sZ], Zm = vm[sZ], T0 = v0[sT], Tm = vm[sT], B0 = v0[sB], Bp = vp[sB];
+      T       acc = cC * Z0;
+      acc += cW * laneDown( Z0 ) + cE * laneUp( Z0 );
+      acc += cN * Zp + cNW * laneDown( Zp );
+      acc += cS * Zm + cSE * laneUp( Zm );
+      acc += cTC * T0 + cTW * laneDown( T0 );
+      acc += cTS * Tm + cTSE * laneUp( Tm );
+      acc += cBC * B0 + cBE * laneUp( B0 );
+      acc += cBN * Bp + cBNW * laneDown( Bp );
+      const bool in = x + y + z <= w - 2;
+      const T    rv = in ? vb[sZ] - acc : T( 0 );
+      rs[( z % RING ) * NT + tid] = rv;
+      cr.store( idx( x, y, z ), rv, store && own && in && ( x == 1 || y == 1 || z == 1 || x + y + z == w - 2 ) );
+   };
+   // coarse point of this thread: (X, Y) = (X0 + I, Y0 + J), centre at lane 2I + 2 of row 2J + 1
+   const int  I = tid % kResBandX, J = tid / kResBandX, X = X0 + I, Y = Y0 + J;
+   const bool outT = tid < kResBandX * kResBandY;
+   const int  oce = ( 2 * J + 1 ) * 64 + 2 * I + 2;
+   auto       restrictSlab = [&]( int Z ) {
+      if ( !outT ) return;
+      const T* Zr = rs + ( ( 2 * Z ) % RING ) * NT + oce;
+      const T* Tr = rs + ( ( 2 * Z + 1 ) % RING ) * NT + oce;
+      const T* Br = rs + ( ( 2 * Z - 1 ) % RING ) * NT + oce;
+      T acc = Zr[-1] + Zr[1] + Zr[64] + Zr[63] + Zr[-64] + Zr[-63] + Tr[0] + Tr[-1] + Tr[-64] + Tr[-63] + Br[0] + Br[1] +
+              Br[64] + Br[63];
+      acc            = Zr[0] + T( 0.5 ) * acc;
+      const int Wd   = wc - Z;
+      co.store( tc - tet32( Wd ) + Y * ( Wd + 1 ) - ( ( Y * ( Y + 1 ) ) >> 1 ) + X, acc, X + Y + Z <= wc - 2 );
+   };
+
+   // prologue: slabs 2Za-2 .. 2Za-1+D, then the residual of slab 2Za-1 (sub-step 0; written to r by the run below
+   // when Za > 1)
+#pragma unroll
+   for ( int j = 0; j < D + 2; j++ )
+   {
+      load( j );
+      __builtin_amdgcn_sched_barrier( 0 );
+   }
+   load( D + 2 );
+   __builtin_amdgcn_sched_barrier( 0 );
+   residual( 2 * Za - 1, 0, 1, 2, Za == 1 );
+   // coarse steps in turns of NS (sub-steps 2i + 1 and 2i + 2 of step i; slots advance by one per sub-step)
+   for ( int Zt = Za;; Zt += NS )
+   {
+#pragma unroll
+      for ( int i = 0; i < NS; i++ )
+      {
+         const int Z = Zt + i;
+         if ( Z > Zb ) return;
+#pragma unroll
+         for ( int h = 1; h <= 2; h++ )
+         {
+            const int q = 2 * i + h; // sub-step within the turn, modulo NS
+            load( ( q + 2 + D ) % NS );
+            __builtin_amdgcn_sched_barrier( 0 );
+            residual( 2 * Z - 1 + h, q % NS, ( q + 1 ) % NS, ( q + 2 ) % NS, true );
+            __builtin_amdgcn_sched_barrier( 0 );
+         }
+         __syncthreads();
+         restrictSlab( Z );
+      }
+   }
+}
+
 template < typename T, int NR >
 __global__ void kRestrictFace( LV gc, LV gf, const FaceGeom* __restrict__ fgFine, int flag, const T* __restrict__ uf,
                                T* __restrict__ uc )
@@ -3101,9 +3244,10 @@ void refreshT( Function& f, int level, int mask )
                           dim3( 256 ), 0, s, g, P.dF2C.p, u, (T*) nullptr );
 }
 
+// cells = false leaves out the macro-cell interiors (the fused residual-and-restriction pass computes those)
 template < typename T, int NR >
 void applyT( Operator& op, Function& src, Function& dst, const Function* rhs, int level, int flag, int mode,
-             bool refreshSrc )
+             bool refreshSrc, bool cells = true )
 {
    Mesh&       M  = *op.mesh;
    LevelPlan&  P  = M.plan( level );
@@ -3140,7 +3284,7 @@ void applyT( Operator& op, Function& src, Function& dst, const Function* rhs, in
       if ( m > 0 && nF > 0 )
          LAUNCH_MODES( kFaceApply, dim3( blocksFor( m * ( m + 1 ) / 2 * NR, 256 ), nF ), dim3( 256 ), g, P.dFaceGeom.p,
                        FW, flag, u, y, b );
-      cellApplyLaunch< T, NR >( M, P, CW, flag, mode, u, y, b );
+      if ( cells ) cellApplyLaunch< T, NR >( M, P, CW, flag, mode, u, y, b );
    }
 #undef LAUNCH_MODES
 }
@@ -3341,8 +3485,9 @@ void sorSweepsT( Operator& op, Function& xf, const Function& bf, double omega, i
 // bandOnly: the source's cell closures are refreshed only within distance 1 of the faces' edges, which is all
 // the restriction reads of them (the cell-interior part gathers interior fine points only); for a scratch source
 // whose other closure points nobody reads afterwards
+// cells = false leaves out the macro-cell interiors of the coarse level
 template < typename T, int NR >
-void restrictT( Function& f, Function& dst, int srcLevel, int flag, bool bandOnly = false )
+void restrictT( Function& f, Function& dst, int srcLevel, int flag, bool bandOnly = false, bool cells = true )
 {
    Mesh&       M  = *f.mesh;
    LevelPlan&  Pf = M.plan( srcLevel );
@@ -3362,6 +3507,7 @@ void restrictT( Function& f, Function& dst, int srcLevel, int flag, bool bandOnl
    if ( m > 0 && nF > 0 )
       hipLaunchKernelGGL( ( kRestrictFace< T, NR > ), dim3( blocksFor( m * ( m + 1 ) / 2 * NR, 256 ), nF ), dim3( 256 ), 0,
                           s, gc, gf, Pf.dFaceGeom.p, flag, uf, uc );
+   if ( !cells ) return;
    if ( NR == 1 )
    {
       int64_t          nu = 0;
@@ -3377,6 +3523,35 @@ void restrictT( Function& f, Function& dst, int srcLevel, int flag, bool bandOnl
    if ( nseg > 0 )
       hipLaunchKernelGGL( ( kRestrictCell< T, NR > ), dim3( (unsigned) xcdGrid( ( nseg + 3 ) / 4 ) ), dim3( 256 ), 0, s, gc,
                           gf, sg, nseg, Pc.dCellType.p, flag, uf, uc );
+}
+
+// r = b - A x on level L, restricted into b on level L - 1 (flag DoFs); r is scratch.  One right-hand side on a
+// level whose plan selects it: the macro-cell interiors in one pass (kResidualRestrict), which writes r's cell
+// interiors only next to the faces, where the face, edge and vertex restriction read them; otherwise the residual
+// and the restriction as two passes.
+template < typename T, int NR >
+void residualRestrictT( Operator& op, Function& x, Function& r, Function& b, int L, int flag, bool refreshSrc )
+{
+   Mesh&      M  = *op.mesh;
+   LevelPlan& Pf = M.plan( L );
+   if ( NR > 1 || !M.residualRestrictOn( L ) )
+   {
+      applyT< T, NR >( op, x, r, &b, L, flag, kResidual, refreshSrc );
+      restrictT< T, NR >( r, b, L, flag, true );
+      return;
+   }
+   LevelPlan&       Pc = M.plan( L - 1 );
+   int64_t          n  = 0;
+   int              extra = 0;
+   const MarchUnit* mu = M.residualRestrictUnits( Pc, &n, &extra );
+   applyT< T, NR >( op, x, r, &b, L, flag, kResidual, refreshSrc, false );
+   auto& Wt = op.weights( L );
+   if ( n > 0 )
+      hipLaunchKernelGGL( ( kResidualRestrict< T, sizeof( T ) == 4 ? kResPrefetch : kResPrefetch64 > ), dim3( (unsigned) n ), dim3( 64 * ( 2 * kResBandY + 1 ) ), 0,
+                          M.ctx->stream, viewOf( Pf ), viewOf( Pc ), mu, n, extra, wsel< T >( Wt.cell64, Wt.cell32 ),
+                          Pf.dCellType.p, flag, (const T*) x.ptr( L ), (const T*) b.ptr( L ), (T*) r.ptr( L ),
+                          (T*) b.ptr( L - 1 ) );
+   restrictT< T, NR >( r, b, L, flag, true, false );
 }
 
 template < typename T, int NR >
@@ -3572,6 +3747,14 @@ struct RestrictF
    }
 };
 template < typename T, int NR >
+struct ResidualRestrictF
+{
+   static void run( Operator& op, Function& x, Function& r, Function& b, int L, int flag, bool refresh )
+   {
+      residualRestrictT< T, NR >( op, x, r, b, L, flag, refresh );
+   }
+};
+template < typename T, int NR >
 struct ProlongF
 {
    static void run( Function& f, int L, int flag, bool refresh ) { prolongT< T, NR >( f, L, flag, refresh ); }
@@ -3653,6 +3836,13 @@ void launchRestrict( Function& f, Function& dst, int srcLevel, int flag, bool ba
    if ( dst.prec != f.prec || dst.nrhs != f.nrhs || dst.mesh != f.mesh )
       throw std::runtime_error( "restrict: source and destination must share mesh, precision and nrhs" );
    dispatch< RestrictF >( f.prec, f.nrhs, f, dst, srcLevel, flag, bandOnly );
+}
+
+void launchResidualRestrict( Operator& op, Function& x, Function& r, Function& b, int level, int flag, bool refreshSrc )
+{
+   if ( r.prec != x.prec || b.prec != x.prec || r.nrhs != x.nrhs || b.nrhs != x.nrhs )
+      throw std::runtime_error( "residual and restriction: x, r and b must share precision and nrhs" );
+   dispatch< ResidualRestrictF >( x.prec, x.nrhs, op, x, r, b, level, flag, refreshSrc );
 }
 
 void launchProlongateAdd( Function& f, int srcLevel, int flag, bool refreshCoarse )

@@ -31,6 +31,10 @@ void launchSorSweeps( Operator& op, Function& x, const Function& b, double omega
 // coarse level of dst <- restriction of the fine level of f (dst may be f)
 // bandOnly: refresh the source's cell closures only where the restriction reads them (a scratch source)
 void launchRestrict( Function& f, Function& dst, int srcLevel, int flag, bool bandOnly = false );
+// r = b - A x on `level`, restricted into b on level - 1 (flag DoFs); r is scratch afterwards (of its macro-cell
+// interiors only the points next to the faces are written where the level fuses the two passes)
+void launchResidualRestrict( Operator& op, Function& x, Function& r, Function& b, int level, int flag,
+                             bool refreshSrc );
 void launchProlongateAdd( Function& f, int srcLevel, int flag, bool refreshCoarse = true );
 // dst = sum_k c[k] * src[k] (+ dst if accumulate) + constant, on the closures of primitives matching flag
 void launchLincomb( Function& dst, int level, int flag, int n, const double* c, Function* const* src, bool accumulate,

@@ -30,6 +30,15 @@ constexpr int kTransferWindows = 4; // 64-point windows per unit of the flat gri
 // points per row segment of the macro-cell apply, SOR and prolongation kernels for several right-hand sides (a
 // wave walks its segment 64 / nrhs points at a time)
 constexpr int kRowSeg = 32;
+// fused residual-and-restriction kernel: coarse points of a tile along x and y (a wave marches one fine row of
+// 2 kResBandX + 2 lanes; a workgroup holds the 2 kResBandY + 1 rows), coarse slabs per work unit (a tile's column
+// is split into runs of this length), slabs loaded ahead, and the finest level it runs on by default
+constexpr int kResBandX            = 30;
+constexpr int kResBandY            = 7;
+constexpr int kResChunk            = 32;
+constexpr int kResPrefetch         = 2; // FP32
+constexpr int kResPrefetch64       = 2; // FP64 (one workgroup per CU: two would leave 64 registers, too few)
+constexpr int kResRestrictMinLevel = 8;
 // LDS slab tiles: elements of one band (rows ya-1 .. ya+R of a slab of width <= w) and the limits the kernel
 // relies on (8 staged elements per thread and stage, 64 KB of LDS for the four slots in FP64)
 inline int tileCap( int w, int R ) { return ( R + 2 ) * w; }

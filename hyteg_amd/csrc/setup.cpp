@@ -708,6 +708,67 @@ const MarchUnit* Mesh::marchUnits( LevelPlan& P, int64_t* count )
    return P.march->p;
 }
 
+const MarchUnit* Mesh::residualRestrictUnits( LevelPlan& Pc, int64_t* count, int* extra )
+{
+   if ( Pc.resUnitCount < 0 )
+   {
+      std::vector< MarchUnit > v;
+      const int                wc = Pc.w, w = 2 * wc - 1;
+      // tiles (X0, Y0) = (1 + kResBandX a, 1 + kResBandY b) with a coarse interior point (X0 + Y0 <= wc - 3); a
+      // tile's coarse slabs 1 .. wc - 2 - X0 - Y0 in runs Za .. Zb of at most kResChunk slabs, z = Za | Zb << 16
+      for ( int c = 0; c < T.nC(); c++ )
+         for ( int Y0 = 1; Y0 <= wc - 4; Y0 += kResBandY )
+            for ( int X0 = 1; X0 + Y0 <= wc - 3; X0 += kResBandX )
+               for ( int Za = 1; Za <= wc - 2 - X0 - Y0; Za += kResChunk )
+                  v.push_back( { c, Za | std::min( Za + kResChunk - 1, wc - 2 - X0 - Y0 ) << 16, X0, Y0 } );
+      // longest runs first, so that they start in the first wave of workgroups
+      std::stable_sort( v.begin(), v.end(), []( const MarchUnit& a, const MarchUnit& b ) {
+         return ( a.z >> 16 ) - ( a.z & 0xffff ) > ( b.z >> 16 ) - ( b.z & 0xffff );
+      } );
+      // fine interior points next to a face (x, y or z == 1, x + y + z == w - 2) that no tile computes: a tile
+      // (X0, Y0) computes the fine points 2 X0 - 1 <= x <= 2 X0 + 2 kResBandX, 2 Y0 - 1 <= y <= 2 Y0 - 1 + 2 kResBandY
+      // of slabs z <= w - 2 - 2 X0 - 2 Y0.  They lie next to the cell's top vertex and where tile boundaries meet the
+      // face x + y + z = w - 2: a few per tile row, the same in every cell, appended to the table as (0, z, x, y)
+      auto covered = [&]( int x, int y, int z ) {
+         for ( int ka = ( x - 1 ) / ( 2 * kResBandX ) - 1; ka <= ( x - 1 ) / ( 2 * kResBandX ); ka++ )
+            for ( int kb = ( y - 1 ) / ( 2 * kResBandY ) - 1; kb <= ( y - 1 ) / ( 2 * kResBandY ); kb++ )
+            {
+               if ( ka < 0 || kb < 0 ) continue;
+               const int X0 = 1 + kResBandX * ka, Y0 = 1 + kResBandY * kb;
+               if ( X0 + Y0 <= wc - 3 && x >= 2 * X0 - 1 && x <= 2 * X0 + 2 * kResBandX && y >= 2 * Y0 - 1 &&
+                    y <= 2 * Y0 - 1 + 2 * kResBandY && z <= w - 2 - 2 * X0 - 2 * Y0 )
+                  return true;
+            }
+         return false;
+      };
+      std::vector< MarchUnit > extra;
+      if ( !v.empty() )
+         for ( int a = 1; a <= w - 4; a++ )
+            for ( int b = 1; a + b <= w - 3; b++ )
+            {
+               // the point (a, b) of the four layers x == 1, y == 1, z == 1, x + y + z == w - 2, each point once
+               const int P[4][3] = { { 1, a, b }, { a, 1, b }, { a, b, 1 }, { a, b, w - 2 - a - b } };
+               for ( int l = 0; l < 4; l++ )
+               {
+                  const int x = P[l][0], y = P[l][1], z = P[l][2];
+                  bool      earlier = false;
+                  for ( int m = 0; m < l; m++ )
+                     earlier = earlier || ( m == 0 && x == 1 ) || ( m == 1 && y == 1 ) || ( m == 2 && z == 1 );
+                  if ( !earlier && !covered( x, y, z ) ) extra.push_back( { 0, z, x, y } );
+               }
+            }
+      Pc.resUnitCount  = (int64_t) v.size();
+      Pc.resExtraCount = (int) extra.size();
+      v.insert( v.end(), extra.begin(), extra.end() );
+      Pc.resUnits = std::make_unique< DevBuf< MarchUnit > >();
+      if ( !v.empty() ) Pc.resUnits->upload( v, ctx->stream );
+      HYMI_HIP( hipStreamSynchronize( ctx->stream ) );
+   }
+   *count = Pc.resUnitCount;
+   *extra = Pc.resExtraCount;
+   return Pc.resUnits->p;
+}
+
 const MarchUnit* Mesh::transferUnits( LevelPlan& P, int64_t* count )
 {
    if ( P.transferCount < 0 )
@@ -759,6 +820,23 @@ void Mesh::setSchedule( int level, const MarchSchedule& S )
    scheduleEpoch++;
    int64_t n = 0;
    if ( !ctx->hostOnly ) marchUnits( P, &n );
+}
+
+void Mesh::setResidualRestrict( int level, int mode )
+{
+   if ( mode < -1 || mode > 1 ) throw std::invalid_argument( "residual-restriction mode must be -1, 0 or 1" );
+   if ( mode == 1 && level < 3 ) throw std::invalid_argument( "the fused residual and restriction needs level >= 3" );
+   LevelPlan& P = plan( level );
+   if ( P.resRestrict == mode ) return;
+   scheduleEpoch++;
+   if ( !ctx->hostOnly ) HYMI_HIP( hipStreamSynchronize( ctx->stream ) );
+   P.resRestrict = mode;
+}
+
+bool Mesh::residualRestrictOn( int level )
+{
+   const int m = plan( level ).resRestrict;
+   return m < 0 ? level >= kResRestrictMinLevel : m == 1;
 }
 
 void Mesh::setCellSor( int level, int tile, int precisions )

@@ -228,7 +228,8 @@ int hymi_function_download_owned_column( hymi_function f, int level, int column,
  * 5 plain copy y = x of the whole level (a bandwidth reference), 6 the SOR cell phase as configured for the level
  * (hymi_mesh_set_cell_sor; x, b), 7 the fused cell phase's tile kernel alone (x, b), 8 the cell phases of two
  * sweeps as one smoothing sequence of a V(2, 2) cycle runs them (with the fused phase: two ping-pong passes
- * through the level's scratch array; x, b). */
+ * through the level's scratch array; x, b), 9 the residual r = b - A x on `level` restricted into b on level - 1 as
+ * a V-cycle runs them (hymi_mesh_set_residual_restrict; y as r; b must hold level - 1). */
 int hymi_time_kernel( hymi_operator op, hymi_gmg gmg, hymi_function x, hymi_function b, hymi_function y, int level,
                       int which, int reps, double* ms_per_launch );
 
@@ -254,6 +255,15 @@ int hymi_mesh_get_schedule( hymi_mesh mesh, int level, int out[4] );
  * interior point by point (src/hyteg/p1functionspace/P1Operator.hpp:341-420) */
 int hymi_mesh_set_cell_sor( hymi_mesh mesh, int level, int tile, int precisions );
 int hymi_mesh_get_cell_sor( hymi_mesh mesh, int level, int out[2] );
+/* Residual and restriction of a V-cycle step from fine `level` to level - 1: mode 1 computes the macro-cell
+ * interiors' residual and restriction in one pass (one right-hand side, level >= 3), 0 runs them as two passes,
+ * -1 selects the default (one pass from level 8 up).  The cycle's results agree to rounding (same terms in the
+ * same order).  get returns 1 where the level runs the one pass.  Waits for the context stream; captured V-cycle
+ * graphs are captured again.  No reference counterpart: the reference computes the residual
+ * (src/hyteg/solvers/GeometricMultigridSolver.hpp:184-190) and restricts it (P1toP1LinearRestriction.cpp) as separate
+ * passes */
+int hymi_mesh_set_residual_restrict( hymi_mesh mesh, int level, int mode );
+int hymi_mesh_get_residual_restrict( hymi_mesh mesh, int level, int* out );
 /* Autotuner.  No reference counterpart: the reference fixes its kernels when it is built
  * (src/hyteg/HytegDefinitions.in.hpp:24-38).  On `level` (>= 2) of the mesh it measures, with hipEvents over
  * `reps` launches on scratch functions of that level:
