@@ -2263,8 +2263,17 @@ __global__ __launch_bounds__( 256 ) void kRestrictFlat( LV gc, LV gf, const Marc
 // columns and one fine row, written by the tile below only; the first slab of a run (2 Za - 1) is written by the
 // run below.  The few such points no tile reaches (listed after the tiles, nExtra of them per cell) are computed
 // from global memory by the run at (1, 1, 1).
+// a copy of v the compiler cannot see through: expressions of it are formed where they are used instead of being
+// hoisted out of unrolled loops into registers of their own
+__device__ __forceinline__ int opaqueInt( int v )
+{
+   int o;
+   asm volatile( "v_mov_b32 %0, %1" : "=v"( o ) : "v"( v ) );
+   return o;
+}
+
 template < typename T, int D >
-__global__ __launch_bounds__( 64 * ( 2 * kResBandY + 1 ), sizeof( T ) == 4 ? 8 : 4 ) void kResidualRestrict(
+__global__ __launch_bounds__( 64 * ( 2 * kResBandY + 1 ), 8 ) void kResidualRestrict(
     LV gf, LV gc, const MarchUnit* __restrict__ units, int64_t nUnits, int nExtra, const T* __restrict__ W8,
     const int* __restrict__ ctype, int flag, const T* __restrict__ u, const T* __restrict__ b, T* __restrict__ r,
     T* __restrict__ bc )
@@ -2313,15 +2322,24 @@ __global__ __launch_bounds__( 64 * ( 2 * kResBandY + 1 ), sizeof( T ) == 4 ? 8 :
    // written to r: next to a face and not in the columns / row shared with the tile below
    const bool own  = cand && ( X0 == 1 || lane >= 3 ) && ( Y0 == 1 || k >= 1 );
    const int  zEnd = 2 * Zb + 2; // last slab a residual of the run reads
-   // register ring of slabs (u at y, y + 1, y - 1 and b at y); slab s in slot (s - 2 Za + 2) mod NS
-   T v0[NS], vp[NS], vm[NS], vb[NS];
-   int sl = 2 * Za - 2; // next slab to load
+   // register ring of slabs (u at y, y + 1, y - 1 and b at y); slab s in slot (s - 2 Za + 2) mod NS.  Indices are
+   // advanced slab by slab on the scalar unit (a few operations per slab; recomputing tet(w - s) for every load
+   // made the scalar unit, which the CU's waves share, the bottleneck): rb = index of (0, y, sl), rows y + 1 and
+   // y - 1 lie w - sl - y further and w + 1 - sl - y nearer, and idx(0, y, s + 1) = idx(0, y, s) + T(w - s) - y with
+   // T(n) = n (n + 1) / 2.  rbs[slot] keeps each loaded slab's base for its residual's store.
+   T   v0[NS], vp[NS], vm[NS], vb[NS];
+   int rbs[NS];
+   int sl = 2 * Za - 2, rb = idx( 0, y, sl ); // next slab to load and its row base
    auto load = [&]( int j ) {
+      // bounds as x against scalar limits (per-lane sums would be hoisted out of the unrolled steps)
       const bool go = sl <= zEnd;
-      v0[j]         = cu.load( idx( x, y, sl ), go && x + y + sl <= w - 1 );
-      vp[j]         = cu.load( idx( x, y + 1, sl ), go && x + y + 1 + sl <= w - 1 );
-      vm[j]         = cu.load( idx( x, y - 1, sl ), go && x + y - 1 + sl <= w - 1 );
-      vb[j]         = cb.load( idx( x, y, sl ), go && x + y + sl <= w - 2 );
+      const int  lim = w - 1 - y - sl, up = w - sl - y, dn = w + 1 - sl - y, xo = opaqueInt( x );
+      rbs[j]         = rb;
+      v0[j]          = cu.load( rb + xo, go && xo <= lim );
+      vp[j]          = cu.load( rb + up + xo, go && xo <= lim - 1 );
+      vm[j]          = cu.load( rb - dn + xo, go && xo <= lim + 1 );
+      vb[j]          = cb.load( rb + xo, go && xo <= lim - 1 );
+      rb += ( ( ( w - sl ) * ( w - sl + 1 ) ) >> 1 ) - y;
       sl++;
    };
    // residual at fine slab z from slots (B, Z, T) = slabs z - 1, z, z + 1; into rs and, where owned, r
@@ -2335,25 +2353,30 @@ __global__ __launch_bounds__( 64 * ( 2 * kResBandY + 1 ), sizeof( T ) == 4 ? 8 :
       acc += cTS * Tm + cTSE * laneUp( Tm );
       acc += cBC * B0 + cBE * laneUp( B0 );
       acc += cBN * Bp + cBNW * laneDown( Bp );
-      const bool in = x + y + z <= w - 2;
-      const T    rv = in ? vb[sZ] - acc : T( 0 );
+      const int  lim = w - 2 - y - z, xo = opaqueInt( x ); // last interior x of the row
+      const bool in  = xo <= lim;
+      const T    rv  = in ? vb[sZ] - acc : T( 0 );
       rs[( z % RING ) * NT + tid] = rv;
-      cr.store( idx( x, y, z ), rv, store && own && in && ( x == 1 || y == 1 || z == 1 || x + y + z == w - 2 ) );
+      cr.store( rbs[sZ] + xo, rv, store && own && in && ( xo == 1 || y == 1 || z == 1 || xo == lim ) );
    };
-   // coarse point of this thread: (X, Y) = (X0 + I, Y0 + J), centre at lane 2I + 2 of row 2J + 1
+   // coarse point of this thread: (X, Y) = (X0 + I, Y0 + J), centre at lane 2I + 2 of row 2J + 1; oc = its index in
+   // coarse slab Z (advanced like rb)
    const int  I = tid % kResBandX, J = tid / kResBandX, X = X0 + I, Y = Y0 + J;
    const bool outT = tid < kResBandX * kResBandY;
    const int  oce = ( 2 * J + 1 ) * 64 + 2 * I + 2;
+   int        oc  = tc - tet32( wc - Za ) + Y * ( wc - Za + 1 ) - ( ( Y * ( Y + 1 ) ) >> 1 ) + X;
    auto       restrictSlab = [&]( int Z ) {
-      if ( !outT ) return;
-      const T* Zr = rs + ( ( 2 * Z ) % RING ) * NT + oce;
-      const T* Tr = rs + ( ( 2 * Z + 1 ) % RING ) * NT + oce;
-      const T* Br = rs + ( ( 2 * Z - 1 ) % RING ) * NT + oce;
-      T acc = Zr[-1] + Zr[1] + Zr[64] + Zr[63] + Zr[-64] + Zr[-63] + Tr[0] + Tr[-1] + Tr[-64] + Tr[-63] + Br[0] + Br[1] +
-              Br[64] + Br[63];
-      acc            = Zr[0] + T( 0.5 ) * acc;
-      const int Wd   = wc - Z;
-      co.store( tc - tet32( Wd ) + Y * ( Wd + 1 ) - ( ( Y * ( Y + 1 ) ) >> 1 ) + X, acc, X + Y + Z <= wc - 2 );
+      if ( outT )
+      {
+         const T* Zr = rs + ( ( 2 * Z ) % RING ) * NT + oce;
+         const T* Tr = rs + ( ( 2 * Z + 1 ) % RING ) * NT + oce;
+         const T* Br = rs + ( ( 2 * Z - 1 ) % RING ) * NT + oce;
+         T acc = Zr[-1] + Zr[1] + Zr[64] + Zr[63] + Zr[-64] + Zr[-63] + Tr[0] + Tr[-1] + Tr[-64] + Tr[-63] + Br[0] +
+                 Br[1] + Br[64] + Br[63];
+         acc = Zr[0] + T( 0.5 ) * acc;
+         co.store( oc, acc, X + Y + Z <= wc - 2 );
+      }
+      oc += ( ( ( wc - Z ) * ( wc - Z + 1 ) ) >> 1 ) - Y;
    };
 
    // prologue: slabs 2Za-2 .. 2Za-1+D, then the residual of slab 2Za-1 (sub-step 0; written to r by the run below

@@ -36,8 +36,8 @@ constexpr int kRowSeg = 32;
 constexpr int kResBandX            = 30;
 constexpr int kResBandY            = 7;
 constexpr int kResChunk            = 32;
-constexpr int kResPrefetch         = 2; // FP32
-constexpr int kResPrefetch64       = 2; // FP64 (one workgroup per CU: two would leave 64 registers, too few)
+constexpr int kResPrefetch         = 3; // FP32
+constexpr int kResPrefetch64       = 1; // FP64 (deeper would not fit 64 registers, two workgroups per CU)
 constexpr int kResRestrictMinLevel = 8;
 // LDS slab tiles: elements of one band (rows ya-1 .. ya+R of a slab of width <= w) and the limits the kernel
 // relies on (8 staged elements per thread and stage, 64 KB of LDS for the four slots in FP64)
