@@ -35,6 +35,49 @@ struct LV
 
 LV viewOf( const LevelPlan& P ) { return LV{ P.w, P.offE, P.offF, P.offC, P.eStride, P.fStride, P.cStride }; }
 
+// block index (x, y), block size and thread index of a thread-indexed kernel body: its own launch's, or a logical
+// block inside a merged launch (kLowApply, kLowRestrict, kLowProlong)
+struct Blk
+{
+   int x, y, dim, tid;
+};
+
+// One launch for the vertex, edge and face parts of a step, which write disjoint points and read none of each
+// other's outputs: its workgroups (256 threads) are first the vertex blocks (256 threads each), then the edge
+// blocks (64 threads: four per workgroup, one per wave), then the face blocks (256 threads), each given the (x, y)
+// block index it has in its own kernel's grid (nbV blocks; nxE x nyE; nxF x nyF).  On the coarse levels these
+// parts are a few microseconds each, and one launch replaces three.
+struct LowGrid
+{
+   int nbV, nxE, nyE, nxF, nyF;
+};
+inline unsigned lowBlocks( const LowGrid& L )
+{
+   return (unsigned) std::max( 1, L.nbV + ( L.nxE * L.nyE + 3 ) / 4 + L.nxF * L.nyF );
+}
+// the part of this workgroup (wave) and its logical block: 0 vertex, 1 edge, 2 face, -1 none
+__device__ __forceinline__ int lowPart( const LowGrid& L, Blk& B )
+{
+   int b = (int) blockIdx.x;
+   if ( b < L.nbV )
+   {
+      B = Blk{ b, 0, 256, (int) threadIdx.x };
+      return 0;
+   }
+   b -= L.nbV;
+   const int nE = L.nxE * L.nyE, nbE = ( nE + 3 ) / 4;
+   if ( b < nbE )
+   {
+      const int li = 4 * b + __builtin_amdgcn_readfirstlane( (int) ( threadIdx.x >> 6 ) );
+      B            = Blk{ li % L.nxE, li / L.nxE, 64, (int) ( threadIdx.x & 63 ) };
+      return li < nE ? 1 : -1;
+   }
+   b -= nbE;
+   if ( b >= L.nxF * L.nyF ) return -1;
+   B = Blk{ b % L.nxF, b / L.nxF, 256, (int) threadIdx.x };
+   return 2;
+}
+
 __device__ __forceinline__ int64_t tet( int64_t w ) { return w * ( w + 1 ) * ( w + 2 ) / 6; }
 __device__ __forceinline__ int64_t fidx( int64_t w, int64_t x, int64_t y ) { return y * ( w + 1 ) - y * ( y + 1 ) / 2 + x; }
 __device__ __forceinline__ int64_t cidx( int64_t w, int64_t x, int64_t y, int64_t z )
@@ -1480,14 +1523,14 @@ __device__ __forceinline__ T faceStencilSum( const LV& g, const FaceGeom& G, con
 }
 
 template < typename T, int NR, int MODE >
-__global__ void kFaceApply( LV g, const FaceGeom* __restrict__ fg, const T* __restrict__ FW, int flag,
+__device__ __forceinline__ void kFaceApply_b( const Blk& B, LV g, const FaceGeom* __restrict__ fg, const T* __restrict__ FW, int flag,
                             const T* __restrict__ u, T* __restrict__ y, const T* __restrict__ b )
 {
-   const FaceGeom& G = fg[blockIdx.y];
+   const FaceGeom& G = fg[B.y];
    if ( !( G.type & flag ) ) return;
    const int     m  = g.w - 3;
    const int64_t np = (int64_t) m * ( m + 1 ) / 2;
-   const int64_t t  = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+   const int64_t t  = (int64_t) B.x * B.dim + B.tid;
    const int     r  = (int) ( t % NR );
    const int64_t k  = t / NR;
    if ( k >= np ) return;
@@ -1503,6 +1546,13 @@ __global__ void kFaceApply( LV g, const FaceGeom* __restrict__ fg, const T* __re
       y[o] += acc;
    else
       y[o] = b[o] - acc;
+}
+
+template < typename T, int NR, int MODE >
+__global__ void kFaceApply( LV g, const FaceGeom* __restrict__ fg, const T* __restrict__ FW, int flag,
+                            const T* __restrict__ u, T* __restrict__ y, const T* __restrict__ b )
+{
+   kFaceApply_b< T, NR, MODE >( Blk{ (int) blockIdx.x, (int) blockIdx.y, (int) blockDim.x, (int) threadIdx.x }, g, fg, FW, flag, u, y, b );
 }
 
 template < typename T, int NR >
@@ -1742,13 +1792,13 @@ __device__ __forceinline__ T edgeTermsSum( const LV& g, const EdgeGeom& G, const
 }
 
 template < typename T, int NR, int MODE >
-__global__ void kEdgeApply( LV g, const EdgeGeom* __restrict__ eg, const EdgeTerm* __restrict__ terms,
+__device__ __forceinline__ void kEdgeApply_b( const Blk& B, LV g, const EdgeGeom* __restrict__ eg, const EdgeTerm* __restrict__ terms,
                             const T* __restrict__ EW, int flag, const T* __restrict__ u, T* __restrict__ y,
                             const T* __restrict__ b )
 {
-   const EdgeGeom& G = eg[blockIdx.y];
+   const EdgeGeom& G = eg[B.y];
    if ( !( G.type & flag ) ) return;
-   const int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+   const int64_t t = (int64_t) B.x * B.dim + B.tid;
    const int     r = (int) ( t % NR );
    const int     x = (int) ( t / NR ) + 1;
    if ( x > g.w - 2 ) return;
@@ -1763,6 +1813,14 @@ __global__ void kEdgeApply( LV g, const EdgeGeom* __restrict__ eg, const EdgeTer
       y[o] += acc;
    else
       y[o] = b[o] - acc;
+}
+
+template < typename T, int NR, int MODE >
+__global__ void kEdgeApply( LV g, const EdgeGeom* __restrict__ eg, const EdgeTerm* __restrict__ terms,
+                            const T* __restrict__ EW, int flag, const T* __restrict__ u, T* __restrict__ y,
+                            const T* __restrict__ b )
+{
+   kEdgeApply_b< T, NR, MODE >( Blk{ (int) blockIdx.x, (int) blockIdx.y, (int) blockDim.x, (int) threadIdx.x }, g, eg, terms, EW, flag, u, y, b );
 }
 
 template < typename T, int NR >
@@ -1789,11 +1847,11 @@ __global__ void kEdgeSor( LV g, const EdgeGeom* __restrict__ eg, const EdgeTerm*
 // macro-vertices
 
 template < typename T, int NR, int MODE >
-__global__ void kVertexApply( LV g, const VertexGeom* __restrict__ vg, const int64_t* __restrict__ terms, int nV,
+__device__ __forceinline__ void kVertexApply_b( const Blk& B, LV g, const VertexGeom* __restrict__ vg, const int64_t* __restrict__ terms, int nV,
                               const T* __restrict__ VW, int flag, const T* __restrict__ u, T* __restrict__ y,
                               const T* __restrict__ b )
 {
-   const int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+   const int64_t t = (int64_t) B.x * B.dim + B.tid;
    const int     r = (int) ( t % NR );
    const int64_t v = t / NR;
    if ( v >= nV ) return;
@@ -1822,6 +1880,33 @@ __global__ void kVertexApply( LV g, const VertexGeom* __restrict__ vg, const int
       y[o] += acc;
    else
       y[o] = b[o] - acc;
+}
+
+template < typename T, int NR, int MODE >
+__global__ void kVertexApply( LV g, const VertexGeom* __restrict__ vg, const int64_t* __restrict__ terms, int nV,
+                              const T* __restrict__ VW, int flag, const T* __restrict__ u, T* __restrict__ y,
+                              const T* __restrict__ b )
+{
+   kVertexApply_b< T, NR, MODE >( Blk{ (int) blockIdx.x, (int) blockIdx.y, (int) blockDim.x, (int) threadIdx.x }, g, vg, terms, nV, VW, flag, u, y, b );
+}
+
+// vertex, edge and face parts of an apply / residual in one launch (LowGrid)
+template < typename T, int NR, int MODE >
+__global__ __launch_bounds__( 256 ) void kLowApply( LowGrid L, LV g, const VertexGeom* __restrict__ vg,
+                                                    const int64_t* __restrict__ vterms, int nV, const T* __restrict__ VW,
+                                                    const EdgeGeom* __restrict__ eg, const EdgeTerm* __restrict__ eterms,
+                                                    const T* __restrict__ EW, const FaceGeom* __restrict__ fg,
+                                                    const T* __restrict__ FW, int flag, const T* __restrict__ u,
+                                                    T* __restrict__ y, const T* __restrict__ b )
+{
+   Blk       B;
+   const int part = lowPart( L, B );
+   if ( part == 0 )
+      kVertexApply_b< T, NR, MODE >( B, g, vg, vterms, nV, VW, flag, u, y, b );
+   else if ( part == 1 )
+      kEdgeApply_b< T, NR, MODE >( B, g, eg, eterms, EW, flag, u, y, b );
+   else if ( part == 2 )
+      kFaceApply_b< T, NR, MODE >( B, g, fg, FW, flag, u, y, b );
 }
 
 template < typename T, int NR >
@@ -2414,14 +2499,14 @@ __global__ __launch_bounds__( 64 * ( 2 * kResBandY + 1 ), 8 ) void kResidualRest
 }
 
 template < typename T, int NR >
-__global__ void kRestrictFace( LV gc, LV gf, const FaceGeom* __restrict__ fgFine, int flag, const T* __restrict__ uf,
+__device__ __forceinline__ void kRestrictFace_b( const Blk& B, LV gc, LV gf, const FaceGeom* __restrict__ fgFine, int flag, const T* __restrict__ uf,
                                T* __restrict__ uc )
 {
-   const FaceGeom& G = fgFine[blockIdx.y];
+   const FaceGeom& G = fgFine[B.y];
    if ( !( G.type & flag ) ) return;
    const int     m  = gc.w - 3;
    const int64_t np = (int64_t) m * ( m + 1 ) / 2;
-   const int64_t t  = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+   const int64_t t  = (int64_t) B.x * B.dim + B.tid;
    const int     r  = (int) ( t % NR );
    const int64_t k  = t / NR;
    if ( k >= np ) return;
@@ -2453,12 +2538,19 @@ __global__ void kRestrictFace( LV gc, LV gf, const FaceGeom* __restrict__ fgFine
 }
 
 template < typename T, int NR >
-__global__ void kRestrictEdge( LV gc, LV gf, const EdgeGeom* __restrict__ egFine, const EdgeTerm* __restrict__ terms,
+__global__ void kRestrictFace( LV gc, LV gf, const FaceGeom* __restrict__ fgFine, int flag, const T* __restrict__ uf,
+                               T* __restrict__ uc )
+{
+   kRestrictFace_b< T, NR >( Blk{ (int) blockIdx.x, (int) blockIdx.y, (int) blockDim.x, (int) threadIdx.x }, gc, gf, fgFine, flag, uf, uc );
+}
+
+template < typename T, int NR >
+__device__ __forceinline__ void kRestrictEdge_b( const Blk& B, LV gc, LV gf, const EdgeGeom* __restrict__ egFine, const EdgeTerm* __restrict__ terms,
                                int flag, const T* __restrict__ uf, T* __restrict__ uc )
 {
-   const EdgeGeom& G = egFine[blockIdx.y];
+   const EdgeGeom& G = egFine[B.y];
    if ( !( G.type & flag ) ) return;
-   const int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+   const int64_t t = (int64_t) B.x * B.dim + B.tid;
    const int     r = (int) ( t % NR );
    const int     X = (int) ( t / NR ) + 1;
    if ( X > gc.w - 2 ) return;
@@ -2486,10 +2578,17 @@ __global__ void kRestrictEdge( LV gc, LV gf, const EdgeGeom* __restrict__ egFine
 }
 
 template < typename T, int NR >
-__global__ void kRestrictVertex( const VertexGeom* __restrict__ vgFine, const int64_t* __restrict__ termsFine, int nV,
+__global__ void kRestrictEdge( LV gc, LV gf, const EdgeGeom* __restrict__ egFine, const EdgeTerm* __restrict__ terms,
+                               int flag, const T* __restrict__ uf, T* __restrict__ uc )
+{
+   kRestrictEdge_b< T, NR >( Blk{ (int) blockIdx.x, (int) blockIdx.y, (int) blockDim.x, (int) threadIdx.x }, gc, gf, egFine, terms, flag, uf, uc );
+}
+
+template < typename T, int NR >
+__device__ __forceinline__ void kRestrictVertex_b( const Blk& B, const VertexGeom* __restrict__ vgFine, const int64_t* __restrict__ termsFine, int nV,
                                  int flag, const T* __restrict__ uf, T* __restrict__ uc )
 {
-   const int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+   const int64_t t = (int64_t) B.x * B.dim + B.tid;
    const int     r = (int) ( t % NR );
    const int64_t v = t / NR;
    if ( v >= nV ) return;
@@ -2507,6 +2606,32 @@ __global__ void kRestrictVertex( const VertexGeom* __restrict__ vgFine, const in
          if ( k0 + q < G.nTerms ) acc += v[q];
    }
    uc[(int64_t) G.vertex * NR + r] = uf[(int64_t) G.vertex * NR + r] + T( 0.5 ) * acc;
+}
+
+template < typename T, int NR >
+__global__ void kRestrictVertex( const VertexGeom* __restrict__ vgFine, const int64_t* __restrict__ termsFine, int nV,
+                                 int flag, const T* __restrict__ uf, T* __restrict__ uc )
+{
+   kRestrictVertex_b< T, NR >( Blk{ (int) blockIdx.x, (int) blockIdx.y, (int) blockDim.x, (int) threadIdx.x }, vgFine, termsFine, nV, flag, uf, uc );
+}
+
+// vertex, edge and face parts of a restriction in one launch (LowGrid)
+template < typename T, int NR >
+__global__ __launch_bounds__( 256 ) void kLowRestrict( LowGrid L, LV gc, LV gf, const VertexGeom* __restrict__ vgFine,
+                                                       const int64_t* __restrict__ vtermsFine, int nV,
+                                                       const EdgeGeom* __restrict__ egFine,
+                                                       const EdgeTerm* __restrict__ etermsFine,
+                                                       const FaceGeom* __restrict__ fgFine, int flag,
+                                                       const T* __restrict__ uf, T* __restrict__ uc )
+{
+   Blk       B;
+   const int part = lowPart( L, B );
+   if ( part == 0 )
+      kRestrictVertex_b< T, NR >( B, vgFine, vtermsFine, nV, flag, uf, uc );
+   else if ( part == 1 )
+      kRestrictEdge_b< T, NR >( B, gc, gf, egFine, etermsFine, flag, uf, uc );
+   else if ( part == 2 )
+      kRestrictFace_b< T, NR >( B, gc, gf, fgFine, flag, uf, uc );
 }
 
 // --------------------------------------------------------------------------------------------------------
@@ -2632,14 +2757,14 @@ __global__ __launch_bounds__( 256 ) void kProlongFlat( LV gc, LV gf, const March
 }
 
 template < typename T, int NR >
-__global__ void kProlongFace( LV gc, LV gf, const FaceGeom* __restrict__ fg, int flag, const T* __restrict__ uc,
+__device__ __forceinline__ void kProlongFace_b( const Blk& B, LV gc, LV gf, const FaceGeom* __restrict__ fg, int flag, const T* __restrict__ uc,
                               T* __restrict__ uf )
 {
-   const FaceGeom& G = fg[blockIdx.y];
+   const FaceGeom& G = fg[B.y];
    if ( !( G.type & flag ) ) return;
    const int     m  = gf.w - 3;
    const int64_t np = (int64_t) m * ( m + 1 ) / 2;
-   const int64_t t  = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+   const int64_t t  = (int64_t) B.x * B.dim + B.tid;
    const int     r  = (int) ( t % NR );
    const int64_t k  = t / NR;
    if ( k >= np ) return;
@@ -2662,12 +2787,19 @@ __global__ void kProlongFace( LV gc, LV gf, const FaceGeom* __restrict__ fg, int
 }
 
 template < typename T, int NR >
-__global__ void kProlongEdge( LV gc, LV gf, const EdgeGeom* __restrict__ eg, int flag, const T* __restrict__ uc,
+__global__ void kProlongFace( LV gc, LV gf, const FaceGeom* __restrict__ fg, int flag, const T* __restrict__ uc,
                               T* __restrict__ uf )
 {
-   const EdgeGeom& G = eg[blockIdx.y];
+   kProlongFace_b< T, NR >( Blk{ (int) blockIdx.x, (int) blockIdx.y, (int) blockDim.x, (int) threadIdx.x }, gc, gf, fg, flag, uc, uf );
+}
+
+template < typename T, int NR >
+__device__ __forceinline__ void kProlongEdge_b( const Blk& B, LV gc, LV gf, const EdgeGeom* __restrict__ eg, int flag, const T* __restrict__ uc,
+                              T* __restrict__ uf )
+{
+   const EdgeGeom& G = eg[B.y];
    if ( !( G.type & flag ) ) return;
-   const int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+   const int64_t t = (int64_t) B.x * B.dim + B.tid;
    const int     r = (int) ( t % NR );
    const int     x = (int) ( t / NR ) + 1;
    if ( x > gf.w - 2 ) return;
@@ -2678,15 +2810,46 @@ __global__ void kProlongEdge( LV gc, LV gf, const EdgeGeom* __restrict__ eg, int
 }
 
 template < typename T, int NR >
-__global__ void kProlongVertex( const VertexGeom* __restrict__ vg, int nV, int flag, const T* __restrict__ uc,
+__global__ void kProlongEdge( LV gc, LV gf, const EdgeGeom* __restrict__ eg, int flag, const T* __restrict__ uc,
+                              T* __restrict__ uf )
+{
+   kProlongEdge_b< T, NR >( Blk{ (int) blockIdx.x, (int) blockIdx.y, (int) blockDim.x, (int) threadIdx.x }, gc, gf, eg, flag, uc, uf );
+}
+
+template < typename T, int NR >
+__device__ __forceinline__ void kProlongVertex_b( const Blk& B, const VertexGeom* __restrict__ vg, int nV, int flag, const T* __restrict__ uc,
                                 T* __restrict__ uf )
 {
-   const int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+   const int64_t t = (int64_t) B.x * B.dim + B.tid;
    const int     r = (int) ( t % NR );
    const int64_t v = t / NR;
    if ( v >= nV ) return;
    if ( !( vg[v].type & flag ) ) return;
    uf[v * NR + r] += uc[v * NR + r];
+}
+
+template < typename T, int NR >
+__global__ void kProlongVertex( const VertexGeom* __restrict__ vg, int nV, int flag, const T* __restrict__ uc,
+                                T* __restrict__ uf )
+{
+   kProlongVertex_b< T, NR >( Blk{ (int) blockIdx.x, (int) blockIdx.y, (int) blockDim.x, (int) threadIdx.x }, vg, nV, flag, uc, uf );
+}
+
+// vertex, edge and face parts of a prolongation in one launch (LowGrid)
+template < typename T, int NR >
+__global__ __launch_bounds__( 256 ) void kLowProlong( LowGrid L, LV gc, LV gf, const VertexGeom* __restrict__ vgFine,
+                                                      int nV, const EdgeGeom* __restrict__ egFine,
+                                                      const FaceGeom* __restrict__ fgFine, int flag,
+                                                      const T* __restrict__ uc, T* __restrict__ uf )
+{
+   Blk       B;
+   const int part = lowPart( L, B );
+   if ( part == 0 )
+      kProlongVertex_b< T, NR >( B, vgFine, nV, flag, uc, uf );
+   else if ( part == 1 )
+      kProlongEdge_b< T, NR >( B, gc, gf, egFine, flag, uc, uf );
+   else if ( part == 2 )
+      kProlongFace_b< T, NR >( B, gc, gf, fgFine, flag, uc, uf );
 }
 
 // --------------------------------------------------------------------------------------------------------
@@ -3296,19 +3459,22 @@ void applyT( Operator& op, Function& src, Function& dst, const Function* rhs, in
       else                                                                                                           \
          hipLaunchKernelGGL( ( KERNEL< T, NR, kResidual > ), GRID, BLOCK, 0, s, __VA_ARGS__ );                       \
    } while ( 0 )
-   LAUNCH_MODES( kVertexApply, dim3( blocksFor( (int64_t) nV * NR, 256 ) ), dim3( 256 ), g, P.dVertexGeom.p,
-                 P.dVertexTerms.p, nV, VW, flag, u, y, b );
+   // vertices, edge interiors and face interiors in one launch
+   LowGrid       lg{ (int) blocksFor( (int64_t) nV * NR, 256 ), 0, 0, 0, 0 };
+   const int64_t m = P.w - 3;
    if ( level >= 1 )
-      LAUNCH_MODES( kEdgeApply, dim3( blocksFor( (int64_t) ( P.w - 2 ) * NR, 64 ), nE ), dim3( 64 ), g, P.dEdgeGeom.p,
-                    P.dEdgeTerms.p, EW, flag, u, y, b );
-   if ( level >= 2 )
    {
-      const int64_t m = P.w - 3;
-      if ( m > 0 && nF > 0 )
-         LAUNCH_MODES( kFaceApply, dim3( blocksFor( m * ( m + 1 ) / 2 * NR, 256 ), nF ), dim3( 256 ), g, P.dFaceGeom.p,
-                       FW, flag, u, y, b );
-      if ( cells ) cellApplyLaunch< T, NR >( M, P, CW, flag, mode, u, y, b );
+      lg.nxE = (int) blocksFor( (int64_t) ( P.w - 2 ) * NR, 64 );
+      lg.nyE = nE;
    }
+   if ( level >= 2 && m > 0 && nF > 0 )
+   {
+      lg.nxF = (int) blocksFor( m * ( m + 1 ) / 2 * NR, 256 );
+      lg.nyF = nF;
+   }
+   LAUNCH_MODES( kLowApply, dim3( lowBlocks( lg ) ), dim3( 256 ), lg, g, P.dVertexGeom.p, P.dVertexTerms.p, nV, VW,
+                 P.dEdgeGeom.p, P.dEdgeTerms.p, EW, P.dFaceGeom.p, FW, flag, u, y, b );
+   if ( level >= 2 && cells ) cellApplyLaunch< T, NR >( M, P, CW, flag, mode, u, y, b );
 #undef LAUNCH_MODES
 }
 
@@ -3521,15 +3687,21 @@ void restrictT( Function& f, Function& dst, int srcLevel, int flag, bool bandOnl
    T*          uc = (T*) dst.ptr( srcLevel - 1 );
    refreshT< T, NR >( f, srcLevel, bandOnly && NR == 1 ? 1 | 2 | 8 : 7 );
    const int nV = M.T.nV(), nE = M.T.nE(), nF = M.T.nF();
-   hipLaunchKernelGGL( ( kRestrictVertex< T, NR > ), dim3( blocksFor( (int64_t) nV * NR, 256 ) ), dim3( 256 ), 0, s,
-                       Pf.dVertexGeom.p, Pf.dVertexTerms.p, nV, flag, uf, uc );
-   if ( Pc.w > 2 )
-      hipLaunchKernelGGL( ( kRestrictEdge< T, NR > ), dim3( blocksFor( (int64_t) ( Pc.w - 2 ) * NR, 64 ), nE ), dim3( 64 ),
-                          0, s, gc, gf, Pf.dEdgeGeom.p, Pf.dEdgeTerms.p, flag, uf, uc );
+   // coarse vertices, edge interiors and face interiors in one launch
+   LowGrid       lg{ (int) blocksFor( (int64_t) nV * NR, 256 ), 0, 0, 0, 0 };
    const int64_t m = Pc.w - 3;
+   if ( Pc.w > 2 )
+   {
+      lg.nxE = (int) blocksFor( (int64_t) ( Pc.w - 2 ) * NR, 64 );
+      lg.nyE = nE;
+   }
    if ( m > 0 && nF > 0 )
-      hipLaunchKernelGGL( ( kRestrictFace< T, NR > ), dim3( blocksFor( m * ( m + 1 ) / 2 * NR, 256 ), nF ), dim3( 256 ), 0,
-                          s, gc, gf, Pf.dFaceGeom.p, flag, uf, uc );
+   {
+      lg.nxF = (int) blocksFor( m * ( m + 1 ) / 2 * NR, 256 );
+      lg.nyF = nF;
+   }
+   hipLaunchKernelGGL( ( kLowRestrict< T, NR > ), dim3( lowBlocks( lg ) ), dim3( 256 ), 0, s, lg, gc, gf, Pf.dVertexGeom.p,
+                       Pf.dVertexTerms.p, nV, Pf.dEdgeGeom.p, Pf.dEdgeTerms.p, Pf.dFaceGeom.p, flag, uf, uc );
    if ( !cells ) return;
    if ( NR == 1 )
    {
@@ -3589,14 +3761,17 @@ void prolongT( Function& f, int srcLevel, int flag, bool refreshCoarse )
    T*          uf = (T*) f.ptr( srcLevel + 1 );
    if ( refreshCoarse ) refreshT< T, NR >( f, srcLevel, 7 );
    const int nV = M.T.nV(), nE = M.T.nE(), nF = M.T.nF();
-   hipLaunchKernelGGL( ( kProlongVertex< T, NR > ), dim3( blocksFor( (int64_t) nV * NR, 256 ) ), dim3( 256 ), 0, s,
-                       Pf.dVertexGeom.p, nV, flag, uc, uf );
-   hipLaunchKernelGGL( ( kProlongEdge< T, NR > ), dim3( blocksFor( (int64_t) ( Pf.w - 2 ) * NR, 64 ), nE ), dim3( 64 ), 0,
-                       s, gc, gf, Pf.dEdgeGeom.p, flag, uc, uf );
+   // fine vertices, edge interiors and face interiors in one launch
+   LowGrid       lg{ (int) blocksFor( (int64_t) nV * NR, 256 ), (int) blocksFor( (int64_t) ( Pf.w - 2 ) * NR, 64 ), nE,
+               0, 0 };
    const int64_t m = Pf.w - 3;
    if ( m > 0 && nF > 0 )
-      hipLaunchKernelGGL( ( kProlongFace< T, NR > ), dim3( blocksFor( m * ( m + 1 ) / 2 * NR, 256 ), nF ), dim3( 256 ), 0,
-                          s, gc, gf, Pf.dFaceGeom.p, flag, uc, uf );
+   {
+      lg.nxF = (int) blocksFor( m * ( m + 1 ) / 2 * NR, 256 );
+      lg.nyF = nF;
+   }
+   hipLaunchKernelGGL( ( kLowProlong< T, NR > ), dim3( lowBlocks( lg ) ), dim3( 256 ), 0, s, lg, gc, gf, Pf.dVertexGeom.p, nV,
+                       Pf.dEdgeGeom.p, Pf.dFaceGeom.p, flag, uc, uf );
    if ( NR == 1 )
    {
       int64_t          nu = 0;
@@ -4071,15 +4246,17 @@ bool launchResidualSplit( Operator& op, Function& x, const Function& b, Function
    float*        yc = (float*) rc.ptr( level );
    if ( refreshSrc ) refreshT< double, 1 >( x, level, 7 );
    const int nV = M.T.nV(), nE = M.T.nE(), nF = M.T.nF();
-   // vertices, edges and faces in FP64, as applyT does
-   hipLaunchKernelGGL( ( kVertexApply< double, 1, kResidual > ), dim3( blocksFor( nV, 256 ) ), dim3( 256 ), 0, s, g,
-                       P.dVertexGeom.p, P.dVertexTerms.p, nV, Wt.vertex64.p, flag, u, y, bb );
-   hipLaunchKernelGGL( ( kEdgeApply< double, 1, kResidual > ), dim3( blocksFor( P.w - 2, 64 ), nE ), dim3( 64 ), 0, s, g,
-                       P.dEdgeGeom.p, P.dEdgeTerms.p, Wt.edge64.p, flag, u, y, bb );
+   // vertices, edges and faces in FP64, as applyT does (one launch)
    const int64_t m = P.w - 3;
+   LowGrid       lg{ (int) blocksFor( nV, 256 ), (int) blocksFor( P.w - 2, 64 ), nE, 0, 0 };
    if ( m > 0 && nF > 0 )
-      hipLaunchKernelGGL( ( kFaceApply< double, 1, kResidual > ), dim3( blocksFor( m * ( m + 1 ) / 2, 256 ), nF ),
-                          dim3( 256 ), 0, s, g, P.dFaceGeom.p, Wt.face64.p, flag, u, y, bb );
+   {
+      lg.nxF = (int) blocksFor( m * ( m + 1 ) / 2, 256 );
+      lg.nyF = nF;
+   }
+   hipLaunchKernelGGL( ( kLowApply< double, 1, kResidual > ), dim3( lowBlocks( lg ) ), dim3( 256 ), 0, s, lg, g,
+                       P.dVertexGeom.p, P.dVertexTerms.p, nV, Wt.vertex64.p, P.dEdgeGeom.p, P.dEdgeTerms.p, Wt.edge64.p,
+                       P.dFaceGeom.p, Wt.face64.p, flag, u, y, bb );
    // cells: FP32 residual straight into rc, r^2 per wave into partials[0, n)
    marchLaunch< double, kResidualSplit >( M, P, Wt.cell64.p, flag, u, nullptr, bb, 0.0, 0, yc, part );
    // FP32 copy of the vertex, edge and face residuals (whole arrays)
