@@ -450,9 +450,9 @@ struct hymi_gmg_s
       const int  extra  = smoothInc * ( maxL_invoked - L );
       smooth( x, b, L, flag, nu1 + extra, finest );
       // residual into tmp, restricted straight into b on the coarse level (== restrict + b.assign on flag DoFs);
-      // tmp is scratch, so its cell closures are refreshed only where the restriction reads them
-      launchResidualRestrict( *A, x, *tmp, b, L, flag, nu1 + extra == 0 && finest );
-      launchLincomb( x, L - 1, kAll, 0, nullptr, nullptr, false, 0.0 );
+      // tmp is scratch, so its cell closures are refreshed only where the restriction reads them.  The coarse x
+      // is zeroed (all primitives) in the restriction's launch
+      launchResidualRestrict( *A, x, *tmp, b, L, flag, nu1 + extra == 0 && finest, &x );
       recurse( x, b, L - 1 );
       // W-cycle (GeometricMultigridSolver.hpp:201-204): the coarse correction is visited again, from its
       // result and with the same coarse right-hand side

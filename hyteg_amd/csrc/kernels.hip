@@ -42,20 +42,28 @@ struct Blk
    int x, y, dim, tid;
 };
 
-// One launch for the vertex, edge and face parts of a step, which write disjoint points and read none of each
-// other's outputs: its workgroups (256 threads) are first the vertex blocks (256 threads each), then the edge
-// blocks (64 threads: four per workgroup, one per wave), then the face blocks (256 threads), each given the (x, y)
-// block index it has in its own kernel's grid (nbV blocks; nxE x nyE; nxF x nyF).  On the coarse levels these
-// parts are a few microseconds each, and one launch replaces three.
+// One launch for the parts of a step that write disjoint points and read none of each other's outputs: the
+// vertex, edge and face parts, and (restriction, prolongation) the macro-cell part and the zeroing of another
+// function.  Its workgroups (256 threads) are the vertex blocks (256 threads each), the edge blocks (64 threads:
+// four per workgroup, one per wave), the face blocks (256 threads), the macro-cell blocks (256 threads; their
+// first block index is a multiple of 8, so that the XCD-aware unit order of the flat transfer kernels holds) and
+// the zeroing blocks (one per range), each given the (x, y) block index it has in its own kernel's grid (nbV
+// blocks; nxE x nyE; nxF x nyF; nbC; nbZ).  On the coarse levels these parts take a few microseconds each.
 struct LowGrid
 {
-   int nbV, nxE, nyE, nxF, nyF;
+   int nbV, nxE, nyE, nxF, nyF, nbC, nbZ;
 };
+__host__ __device__ inline int lowCellStart( const LowGrid& L )
+{
+   return ( L.nbV + ( L.nxE * L.nyE + 3 ) / 4 + L.nxF * L.nyF + 7 ) / 8 * 8;
+}
 inline unsigned lowBlocks( const LowGrid& L )
 {
-   return (unsigned) std::max( 1, L.nbV + ( L.nxE * L.nyE + 3 ) / 4 + L.nxF * L.nyF );
+   const int n = L.nbC + L.nbZ ? lowCellStart( L ) + L.nbC + L.nbZ : L.nbV + ( L.nxE * L.nyE + 3 ) / 4 + L.nxF * L.nyF;
+   return (unsigned) std::max( 1, n );
 }
-// the part of this workgroup (wave) and its logical block: 0 vertex, 1 edge, 2 face, -1 none
+// the part of this workgroup (wave) and its logical block: 0 vertex, 1 edge, 2 face, 3 macro-cell, 4 zeroing,
+// -1 none
 __device__ __forceinline__ int lowPart( const LowGrid& L, Blk& B )
 {
    int b = (int) blockIdx.x;
@@ -73,9 +81,29 @@ __device__ __forceinline__ int lowPart( const LowGrid& L, Blk& B )
       return li < nE ? 1 : -1;
    }
    b -= nbE;
-   if ( b >= L.nxF * L.nyF ) return -1;
-   B = Blk{ b % L.nxF, b / L.nxF, 256, (int) threadIdx.x };
-   return 2;
+   if ( b < L.nxF * L.nyF )
+   {
+      B = Blk{ b % L.nxF, b / L.nxF, 256, (int) threadIdx.x };
+      return 2;
+   }
+   b = (int) blockIdx.x - lowCellStart( L );
+   if ( b < 0 ) return -1;
+   if ( b < L.nbC )
+   {
+      B = Blk{ b, 0, 256, (int) threadIdx.x };
+      return 3;
+   }
+   b -= L.nbC;
+   if ( b >= L.nbZ ) return -1;
+   B = Blk{ b, 0, 256, (int) threadIdx.x };
+   return 4;
+}
+
+// XCD-aware order of logical block b of nLogical (as xcdBlock, for a logical block index)
+__device__ __forceinline__ int64_t xcdBlockOf( int64_t b, int64_t nLogical )
+{
+   const int64_t per = ( nLogical + 7 ) / 8;
+   return ( b % 8 ) * per + b / 8;
 }
 
 __device__ __forceinline__ int64_t tet( int64_t w ) { return w * ( w + 1 ) * ( w + 2 ) / 6; }
@@ -2293,15 +2321,15 @@ __global__ __launch_bounds__( 256 ) void kRestrictCell( LV gc, LV gf, const Cell
 // gathers the fifteen fine values around (2X, 2Y, 2Z) with 32-bit row offsets, summed in the order of
 // kRestrictCell.
 template < typename T >
-__global__ __launch_bounds__( 256 ) void kRestrictFlat( LV gc, LV gf, const MarchUnit* __restrict__ units,
+__device__ __forceinline__ void kRestrictFlat_b( const Blk& B, LV gc, LV gf, const MarchUnit* __restrict__ units,
                                                         int64_t nUnits, const int* __restrict__ ctype, int flag,
                                                         const T* __restrict__ uf, T* __restrict__ uc )
 {
-   const int64_t ui = xcdBlock( ( nUnits + 3 ) / 4 ) * 4 + __builtin_amdgcn_readfirstlane( threadIdx.x >> 6 );
+   const int64_t ui = xcdBlockOf( B.x, ( nUnits + 3 ) / 4 ) * 4 + __builtin_amdgcn_readfirstlane( B.tid >> 6 );
    if ( ui >= nUnits ) return;
    const MarchUnit U = units[ui];
    if ( !( ctype[U.cell] & flag ) ) return;
-   const int lane = threadIdx.x & 63;
+   const int lane = B.tid & 63;
    const int wc = gc.w, w = gf.w, Z = U.z, Wd = wc - Z;
    const int tc = tet32( wc ), tf = tet32( w );
    const int nEl = Wd * ( Wd + 1 ) / 2;
@@ -2330,6 +2358,14 @@ __global__ __launch_bounds__( 256 ) void kRestrictFlat( LV gc, LV gf, const Marc
       acc = fu[r0] + T( 0.5 ) * acc;
       cu.store( S0 + e, acc, ok );
    }
+}
+
+template < typename T >
+__global__ __launch_bounds__( 256 ) void kRestrictFlat( LV gc, LV gf, const MarchUnit* __restrict__ units,
+                                                        int64_t nUnits, const int* __restrict__ ctype, int flag,
+                                                        const T* __restrict__ uf, T* __restrict__ uc )
+{
+   kRestrictFlat_b< T >( Blk{ (int) blockIdx.x, 0, 256, (int) threadIdx.x }, gc, gf, units, nUnits, ctype, flag, uf, uc );
 }
 
 // Residual and restriction of the macro-cell interiors in one pass (one right-hand side): the fine residual
@@ -2615,14 +2651,28 @@ __global__ void kRestrictVertex( const VertexGeom* __restrict__ vgFine, const in
    kRestrictVertex_b< T, NR >( Blk{ (int) blockIdx.x, (int) blockIdx.y, (int) blockDim.x, (int) threadIdx.x }, vgFine, termsFine, nV, flag, uf, uc );
 }
 
-// vertex, edge and face parts of a restriction in one launch (LowGrid)
+// zeroes the ranges of dst (one range per logical block)
+template < typename T, int NR >
+__device__ __forceinline__ void kZeroRanges_b( const Blk& B, const Range* __restrict__ ranges, T* __restrict__ dst )
+{
+   const Range   R   = ranges[B.x];
+   const int64_t len = (int64_t) R.len * NR, o = R.off * NR;
+   for ( int64_t k = B.tid; k < len; k += B.dim )
+      dst[o + k] = T( 0 );
+}
+
+// a restriction in one launch (LowGrid): vertex, edge and face parts, the macro-cell part (one right-hand side:
+// kRestrictFlat's units of the coarse level) and the zeroing of the coarse level of another function (ranges zr)
 template < typename T, int NR >
 __global__ __launch_bounds__( 256 ) void kLowRestrict( LowGrid L, LV gc, LV gf, const VertexGeom* __restrict__ vgFine,
                                                        const int64_t* __restrict__ vtermsFine, int nV,
                                                        const EdgeGeom* __restrict__ egFine,
                                                        const EdgeTerm* __restrict__ etermsFine,
                                                        const FaceGeom* __restrict__ fgFine, int flag,
-                                                       const T* __restrict__ uf, T* __restrict__ uc )
+                                                       const T* __restrict__ uf, T* __restrict__ uc,
+                                                       const MarchUnit* __restrict__ cellUnits, int64_t nCellUnits,
+                                                       const int* __restrict__ ctypeC, const Range* __restrict__ zr,
+                                                       T* __restrict__ zdst )
 {
    Blk       B;
    const int part = lowPart( L, B );
@@ -2632,6 +2682,12 @@ __global__ __launch_bounds__( 256 ) void kLowRestrict( LowGrid L, LV gc, LV gf, 
       kRestrictEdge_b< T, NR >( B, gc, gf, egFine, etermsFine, flag, uf, uc );
    else if ( part == 2 )
       kRestrictFace_b< T, NR >( B, gc, gf, fgFine, flag, uf, uc );
+   else if ( part == 3 )
+   {
+      if constexpr ( NR == 1 ) kRestrictFlat_b< T >( B, gc, gf, cellUnits, nCellUnits, ctypeC, flag, uf, uc );
+   }
+   else if ( part == 4 )
+      kZeroRanges_b< T, NR >( B, zr, zdst );
 }
 
 // --------------------------------------------------------------------------------------------------------
@@ -2698,15 +2754,15 @@ __global__ __launch_bounds__( 256 ) void kProlongCell( LV gc, LV gf, const CellS
 // indices with 32-bit arithmetic.  The coarse slabs (z -/+ pz) / 2 are wave-uniform because the
 // z-component of the direction always equals the parity of z.
 template < typename T >
-__global__ __launch_bounds__( 256 ) void kProlongFlat( LV gc, LV gf, const MarchUnit* __restrict__ units,
+__device__ __forceinline__ void kProlongFlat_b( const Blk& B, LV gc, LV gf, const MarchUnit* __restrict__ units,
                                                        int64_t nUnits, const int* __restrict__ ctype, int flag,
                                                        const T* __restrict__ uc, T* __restrict__ uf )
 {
-   const int64_t ui = xcdBlock( ( nUnits + 3 ) / 4 ) * 4 + __builtin_amdgcn_readfirstlane( threadIdx.x >> 6 );
+   const int64_t ui = xcdBlockOf( B.x, ( nUnits + 3 ) / 4 ) * 4 + __builtin_amdgcn_readfirstlane( B.tid >> 6 );
    if ( ui >= nUnits ) return;
    const MarchUnit U = units[ui];
    if ( !( ctype[U.cell] & flag ) ) return;
-   const int lane = threadIdx.x & 63;
+   const int lane = B.tid & 63;
    const int w = gf.w, z = U.z, Wd = w - z, wc = gc.w;
    const int tf = tet32( w ), tc = tet32( wc );
    const int nEl = Wd * ( Wd + 1 ) / 2;
@@ -2754,6 +2810,14 @@ __global__ __launch_bounds__( 256 ) void kProlongFlat( LV gc, LV gf, const March
 #pragma unroll
    for ( int j = 0; j < kTransferWindows; j++ )
       fu.store( fo[j], fv[j] + T( 0.5 ) * ( ca[j] + cb[j] ), ok[j] );
+}
+
+template < typename T >
+__global__ __launch_bounds__( 256 ) void kProlongFlat( LV gc, LV gf, const MarchUnit* __restrict__ units,
+                                                       int64_t nUnits, const int* __restrict__ ctype, int flag,
+                                                       const T* __restrict__ uc, T* __restrict__ uf )
+{
+   kProlongFlat_b< T >( Blk{ (int) blockIdx.x, 0, 256, (int) threadIdx.x }, gc, gf, units, nUnits, ctype, flag, uc, uf );
 }
 
 template < typename T, int NR >
@@ -2835,12 +2899,15 @@ __global__ void kProlongVertex( const VertexGeom* __restrict__ vg, int nV, int f
    kProlongVertex_b< T, NR >( Blk{ (int) blockIdx.x, (int) blockIdx.y, (int) blockDim.x, (int) threadIdx.x }, vg, nV, flag, uc, uf );
 }
 
-// vertex, edge and face parts of a prolongation in one launch (LowGrid)
+// a prolongation in one launch (LowGrid): vertex, edge and face parts, and the macro-cell part (one right-hand
+// side: kProlongFlat's units of the fine level)
 template < typename T, int NR >
 __global__ __launch_bounds__( 256 ) void kLowProlong( LowGrid L, LV gc, LV gf, const VertexGeom* __restrict__ vgFine,
                                                       int nV, const EdgeGeom* __restrict__ egFine,
                                                       const FaceGeom* __restrict__ fgFine, int flag,
-                                                      const T* __restrict__ uc, T* __restrict__ uf )
+                                                      const T* __restrict__ uc, T* __restrict__ uf,
+                                                      const MarchUnit* __restrict__ cellUnits, int64_t nCellUnits,
+                                                      const int* __restrict__ ctypeF )
 {
    Blk       B;
    const int part = lowPart( L, B );
@@ -2850,6 +2917,10 @@ __global__ __launch_bounds__( 256 ) void kLowProlong( LowGrid L, LV gc, LV gf, c
       kProlongEdge_b< T, NR >( B, gc, gf, egFine, flag, uc, uf );
    else if ( part == 2 )
       kProlongFace_b< T, NR >( B, gc, gf, fgFine, flag, uc, uf );
+   else if ( part == 3 )
+   {
+      if constexpr ( NR == 1 ) kProlongFlat_b< T >( B, gc, gf, cellUnits, nCellUnits, ctypeF, flag, uc, uf );
+   }
 }
 
 // --------------------------------------------------------------------------------------------------------
@@ -3460,7 +3531,7 @@ void applyT( Operator& op, Function& src, Function& dst, const Function* rhs, in
          hipLaunchKernelGGL( ( KERNEL< T, NR, kResidual > ), GRID, BLOCK, 0, s, __VA_ARGS__ );                       \
    } while ( 0 )
    // vertices, edge interiors and face interiors in one launch
-   LowGrid       lg{ (int) blocksFor( (int64_t) nV * NR, 256 ), 0, 0, 0, 0 };
+   LowGrid       lg{ (int) blocksFor( (int64_t) nV * NR, 256 ), 0, 0, 0, 0, 0, 0 };
    const int64_t m = P.w - 3;
    if ( level >= 1 )
    {
@@ -3674,9 +3745,11 @@ void sorSweepsT( Operator& op, Function& xf, const Function& bf, double omega, i
 // bandOnly: the source's cell closures are refreshed only within distance 1 of the faces' edges, which is all
 // the restriction reads of them (the cell-interior part gathers interior fine points only); for a scratch source
 // whose other closure points nobody reads afterwards
-// cells = false leaves out the macro-cell interiors of the coarse level
+// cells = false leaves out the macro-cell interiors of the coarse level; zeroCoarse (may be null): a function whose
+// coarse level (all primitives) is set to zero in the same launch
 template < typename T, int NR >
-void restrictT( Function& f, Function& dst, int srcLevel, int flag, bool bandOnly = false, bool cells = true )
+void restrictT( Function& f, Function& dst, int srcLevel, int flag, bool bandOnly = false, bool cells = true,
+                Function* zeroCoarse = nullptr )
 {
    Mesh&       M  = *f.mesh;
    LevelPlan&  Pf = M.plan( srcLevel );
@@ -3687,8 +3760,9 @@ void restrictT( Function& f, Function& dst, int srcLevel, int flag, bool bandOnl
    T*          uc = (T*) dst.ptr( srcLevel - 1 );
    refreshT< T, NR >( f, srcLevel, bandOnly && NR == 1 ? 1 | 2 | 8 : 7 );
    const int nV = M.T.nV(), nE = M.T.nE(), nF = M.T.nF();
-   // coarse vertices, edge interiors and face interiors in one launch
-   LowGrid       lg{ (int) blocksFor( (int64_t) nV * NR, 256 ), 0, 0, 0, 0 };
+   // coarse vertices, edge interiors, face interiors and (one right-hand side) macro-cell interiors in one launch,
+   // with the zeroing of zeroCoarse
+   LowGrid       lg{ (int) blocksFor( (int64_t) nV * NR, 256 ), 0, 0, 0, 0, 0, 0 };
    const int64_t m = Pc.w - 3;
    if ( Pc.w > 2 )
    {
@@ -3700,18 +3774,20 @@ void restrictT( Function& f, Function& dst, int srcLevel, int flag, bool bandOnl
       lg.nxF = (int) blocksFor( m * ( m + 1 ) / 2 * NR, 256 );
       lg.nyF = nF;
    }
-   hipLaunchKernelGGL( ( kLowRestrict< T, NR > ), dim3( lowBlocks( lg ) ), dim3( 256 ), 0, s, lg, gc, gf, Pf.dVertexGeom.p,
-                       Pf.dVertexTerms.p, nV, Pf.dEdgeGeom.p, Pf.dEdgeTerms.p, Pf.dFaceGeom.p, flag, uf, uc );
-   if ( !cells ) return;
-   if ( NR == 1 )
+   int64_t          nu = 0, nz = 0;
+   const MarchUnit* mu = nullptr;
+   if ( cells && NR == 1 )
    {
-      int64_t          nu = 0;
-      const MarchUnit* mu = M.transferUnits( Pc, &nu );
-      if ( nu > 0 )
-         hipLaunchKernelGGL( ( kRestrictFlat< T > ), dim3( (unsigned) xcdGrid( ( nu + 3 ) / 4 ) ), dim3( 256 ), 0, s, gc, gf,
-                             mu, nu, Pc.dCellType.p, flag, uf, uc );
-      return;
+      mu     = M.transferUnits( Pc, &nu );
+      lg.nbC = nu > 0 ? (int) xcdGrid( ( nu + 3 ) / 4 ) : 0;
    }
+   const Range* zr = zeroCoarse ? M.closureRanges( Pc, kAll, &nz ) : nullptr;
+   lg.nbZ          = (int) nz;
+   T* zdst         = zeroCoarse ? (T*) zeroCoarse->ptr( srcLevel - 1 ) : nullptr;
+   hipLaunchKernelGGL( ( kLowRestrict< T, NR > ), dim3( lowBlocks( lg ) ), dim3( 256 ), 0, s, lg, gc, gf, Pf.dVertexGeom.p,
+                       Pf.dVertexTerms.p, nV, Pf.dEdgeGeom.p, Pf.dEdgeTerms.p, Pf.dFaceGeom.p, flag, uf, uc, mu, nu,
+                       Pc.dCellType.p, zr, zdst );
+   if ( !cells || NR == 1 ) return;
    int64_t        nseg = 0;
    rowSegFits< NR >( Pf );
    const CellSeg* sg   = M.segments( Pc, kRowSeg, &nseg );
@@ -3725,14 +3801,15 @@ void restrictT( Function& f, Function& dst, int srcLevel, int flag, bool bandOnl
 // interiors only next to the faces, where the face, edge and vertex restriction read them; otherwise the residual
 // and the restriction as two passes.
 template < typename T, int NR >
-void residualRestrictT( Operator& op, Function& x, Function& r, Function& b, int L, int flag, bool refreshSrc )
+void residualRestrictT( Operator& op, Function& x, Function& r, Function& b, int L, int flag, bool refreshSrc,
+                        Function* zeroCoarse )
 {
    Mesh&      M  = *op.mesh;
    LevelPlan& Pf = M.plan( L );
    if ( NR > 1 || !M.residualRestrictOn( L ) )
    {
       applyT< T, NR >( op, x, r, &b, L, flag, kResidual, refreshSrc );
-      restrictT< T, NR >( r, b, L, flag, true );
+      restrictT< T, NR >( r, b, L, flag, true, true, zeroCoarse );
       return;
    }
    LevelPlan&       Pc = M.plan( L - 1 );
@@ -3746,7 +3823,7 @@ void residualRestrictT( Operator& op, Function& x, Function& r, Function& b, int
                           M.ctx->stream, viewOf( Pf ), viewOf( Pc ), mu, n, extra, wsel< T >( Wt.cell64, Wt.cell32 ),
                           Pf.dCellType.p, flag, (const T*) x.ptr( L ), (const T*) b.ptr( L ), (T*) r.ptr( L ),
                           (T*) b.ptr( L - 1 ) );
-   restrictT< T, NR >( r, b, L, flag, true, false );
+   restrictT< T, NR >( r, b, L, flag, true, false, zeroCoarse );
 }
 
 template < typename T, int NR >
@@ -3761,26 +3838,25 @@ void prolongT( Function& f, int srcLevel, int flag, bool refreshCoarse )
    T*          uf = (T*) f.ptr( srcLevel + 1 );
    if ( refreshCoarse ) refreshT< T, NR >( f, srcLevel, 7 );
    const int nV = M.T.nV(), nE = M.T.nE(), nF = M.T.nF();
-   // fine vertices, edge interiors and face interiors in one launch
+   // fine vertices, edge interiors, face interiors and (one right-hand side) macro-cell interiors in one launch
    LowGrid       lg{ (int) blocksFor( (int64_t) nV * NR, 256 ), (int) blocksFor( (int64_t) ( Pf.w - 2 ) * NR, 64 ), nE,
-               0, 0 };
+               0, 0, 0, 0 };
    const int64_t m = Pf.w - 3;
    if ( m > 0 && nF > 0 )
    {
       lg.nxF = (int) blocksFor( m * ( m + 1 ) / 2 * NR, 256 );
       lg.nyF = nF;
    }
-   hipLaunchKernelGGL( ( kLowProlong< T, NR > ), dim3( lowBlocks( lg ) ), dim3( 256 ), 0, s, lg, gc, gf, Pf.dVertexGeom.p, nV,
-                       Pf.dEdgeGeom.p, Pf.dFaceGeom.p, flag, uc, uf );
+   int64_t          nu = 0;
+   const MarchUnit* tu = nullptr;
    if ( NR == 1 )
    {
-      int64_t          nu = 0;
-      const MarchUnit* tu = M.transferUnits( Pf, &nu );
-      if ( nu > 0 )
-         hipLaunchKernelGGL( ( kProlongFlat< T > ), dim3( (unsigned) xcdGrid( ( nu + 3 ) / 4 ) ), dim3( 256 ), 0, s, gc, gf,
-                             tu, nu, Pf.dCellType.p, flag, uc, uf );
-      return;
+      tu     = M.transferUnits( Pf, &nu );
+      lg.nbC = nu > 0 ? (int) xcdGrid( ( nu + 3 ) / 4 ) : 0;
    }
+   hipLaunchKernelGGL( ( kLowProlong< T, NR > ), dim3( lowBlocks( lg ) ), dim3( 256 ), 0, s, lg, gc, gf, Pf.dVertexGeom.p, nV,
+                       Pf.dEdgeGeom.p, Pf.dFaceGeom.p, flag, uc, uf, tu, nu, Pf.dCellType.p );
+   if ( NR == 1 ) return;
    int64_t        nseg = 0;
    rowSegFits< NR >( Pf );
    const CellSeg* sg   = M.segments( Pf, kRowSeg, &nseg );
@@ -3947,9 +4023,9 @@ struct RestrictF
 template < typename T, int NR >
 struct ResidualRestrictF
 {
-   static void run( Operator& op, Function& x, Function& r, Function& b, int L, int flag, bool refresh )
+   static void run( Operator& op, Function& x, Function& r, Function& b, int L, int flag, bool refresh, Function* z )
    {
-      residualRestrictT< T, NR >( op, x, r, b, L, flag, refresh );
+      residualRestrictT< T, NR >( op, x, r, b, L, flag, refresh, z );
    }
 };
 template < typename T, int NR >
@@ -4036,11 +4112,14 @@ void launchRestrict( Function& f, Function& dst, int srcLevel, int flag, bool ba
    dispatch< RestrictF >( f.prec, f.nrhs, f, dst, srcLevel, flag, bandOnly );
 }
 
-void launchResidualRestrict( Operator& op, Function& x, Function& r, Function& b, int level, int flag, bool refreshSrc )
+void launchResidualRestrict( Operator& op, Function& x, Function& r, Function& b, int level, int flag, bool refreshSrc,
+                             Function* zeroCoarse )
 {
    if ( r.prec != x.prec || b.prec != x.prec || r.nrhs != x.nrhs || b.nrhs != x.nrhs )
       throw std::runtime_error( "residual and restriction: x, r and b must share precision and nrhs" );
-   dispatch< ResidualRestrictF >( x.prec, x.nrhs, op, x, r, b, level, flag, refreshSrc );
+   if ( zeroCoarse && ( zeroCoarse->prec != x.prec || zeroCoarse->nrhs != x.nrhs || zeroCoarse->mesh != x.mesh ) )
+      throw std::runtime_error( "residual and restriction: the zeroed function must match x" );
+   dispatch< ResidualRestrictF >( x.prec, x.nrhs, op, x, r, b, level, flag, refreshSrc, zeroCoarse );
 }
 
 void launchProlongateAdd( Function& f, int srcLevel, int flag, bool refreshCoarse )
@@ -4248,7 +4327,7 @@ bool launchResidualSplit( Operator& op, Function& x, const Function& b, Function
    const int nV = M.T.nV(), nE = M.T.nE(), nF = M.T.nF();
    // vertices, edges and faces in FP64, as applyT does (one launch)
    const int64_t m = P.w - 3;
-   LowGrid       lg{ (int) blocksFor( nV, 256 ), (int) blocksFor( P.w - 2, 64 ), nE, 0, 0 };
+   LowGrid       lg{ (int) blocksFor( nV, 256 ), (int) blocksFor( P.w - 2, 64 ), nE, 0, 0, 0, 0 };
    if ( m > 0 && nF > 0 )
    {
       lg.nxF = (int) blocksFor( m * ( m + 1 ) / 2, 256 );

@@ -32,9 +32,10 @@ void launchSorSweeps( Operator& op, Function& x, const Function& b, double omega
 // bandOnly: refresh the source's cell closures only where the restriction reads them (a scratch source)
 void launchRestrict( Function& f, Function& dst, int srcLevel, int flag, bool bandOnly = false );
 // r = b - A x on `level`, restricted into b on level - 1 (flag DoFs); r is scratch afterwards (of its macro-cell
-// interiors only the points next to the faces are written where the level fuses the two passes)
+// interiors only the points next to the faces are written where the level fuses the two passes).  zeroCoarse (may
+// be null): a function whose level - 1 is set to zero on every primitive in the restriction's launch
 void launchResidualRestrict( Operator& op, Function& x, Function& r, Function& b, int level, int flag,
-                             bool refreshSrc );
+                             bool refreshSrc, Function* zeroCoarse = nullptr );
 void launchProlongateAdd( Function& f, int srcLevel, int flag, bool refreshCoarse = true );
 // dst = sum_k c[k] * src[k] (+ dst if accumulate) + constant, on the closures of primitives matching flag
 void launchLincomb( Function& dst, int level, int flag, int n, const double* c, Function* const* src, bool accumulate,
