@@ -1576,13 +1576,6 @@ __device__ __forceinline__ void kFaceApply_b( const Blk& B, LV g, const FaceGeom
       y[o] = b[o] - acc;
 }
 
-template < typename T, int NR, int MODE >
-__global__ void kFaceApply( LV g, const FaceGeom* __restrict__ fg, const T* __restrict__ FW, int flag,
-                            const T* __restrict__ u, T* __restrict__ y, const T* __restrict__ b )
-{
-   kFaceApply_b< T, NR, MODE >( Blk{ (int) blockIdx.x, (int) blockIdx.y, (int) blockDim.x, (int) threadIdx.x }, g, fg, FW, flag, u, y, b );
-}
-
 template < typename T, int NR >
 __global__ void kFaceSor( LV g, const FaceGeom* __restrict__ fg, const T* __restrict__ FW, int flag, T* __restrict__ u,
                           const T* __restrict__ b, T omega, int colour )
@@ -1843,14 +1836,6 @@ __device__ __forceinline__ void kEdgeApply_b( const Blk& B, LV g, const EdgeGeom
       y[o] = b[o] - acc;
 }
 
-template < typename T, int NR, int MODE >
-__global__ void kEdgeApply( LV g, const EdgeGeom* __restrict__ eg, const EdgeTerm* __restrict__ terms,
-                            const T* __restrict__ EW, int flag, const T* __restrict__ u, T* __restrict__ y,
-                            const T* __restrict__ b )
-{
-   kEdgeApply_b< T, NR, MODE >( Blk{ (int) blockIdx.x, (int) blockIdx.y, (int) blockDim.x, (int) threadIdx.x }, g, eg, terms, EW, flag, u, y, b );
-}
-
 template < typename T, int NR >
 __global__ void kEdgeSor( LV g, const EdgeGeom* __restrict__ eg, const EdgeTerm* __restrict__ terms,
                           const T* __restrict__ EW, int flag, T* __restrict__ u, const T* __restrict__ b, T omega,
@@ -1908,14 +1893,6 @@ __device__ __forceinline__ void kVertexApply_b( const Blk& B, LV g, const Vertex
       y[o] += acc;
    else
       y[o] = b[o] - acc;
-}
-
-template < typename T, int NR, int MODE >
-__global__ void kVertexApply( LV g, const VertexGeom* __restrict__ vg, const int64_t* __restrict__ terms, int nV,
-                              const T* __restrict__ VW, int flag, const T* __restrict__ u, T* __restrict__ y,
-                              const T* __restrict__ b )
-{
-   kVertexApply_b< T, NR, MODE >( Blk{ (int) blockIdx.x, (int) blockIdx.y, (int) blockDim.x, (int) threadIdx.x }, g, vg, terms, nV, VW, flag, u, y, b );
 }
 
 // vertex, edge and face parts of an apply / residual in one launch (LowGrid)
@@ -2360,16 +2337,8 @@ __device__ __forceinline__ void kRestrictFlat_b( const Blk& B, LV gc, LV gf, con
    }
 }
 
-template < typename T >
-__global__ __launch_bounds__( 256 ) void kRestrictFlat( LV gc, LV gf, const MarchUnit* __restrict__ units,
-                                                        int64_t nUnits, const int* __restrict__ ctype, int flag,
-                                                        const T* __restrict__ uf, T* __restrict__ uc )
-{
-   kRestrictFlat_b< T >( Blk{ (int) blockIdx.x, 0, 256, (int) threadIdx.x }, gc, gf, units, nUnits, ctype, flag, uf, uc );
-}
-
 // Residual and restriction of the macro-cell interiors in one pass (one right-hand side): the fine residual
-// r = b - A u (the terms and summation order of kCellColumn) is restricted with kRestrictFlat's weights and
+// r = b - A u (the terms and summation order of kCellColumn) is restricted with kRestrictFlat_b's weights and
 // summation order straight into the coarse array.  A workgroup of kResBandY * 2 + 1 waves takes kResBandX x
 // kResBandY coarse points (X, Y) of one cell and a run of coarse slabs Za .. Zb (units from
 // Mesh::residualRestrictUnits of the coarse level, longest runs first).  Wave k marches fine row y = 2 Y0 - 1 + k
@@ -2574,13 +2543,6 @@ __device__ __forceinline__ void kRestrictFace_b( const Blk& B, LV gc, LV gf, con
 }
 
 template < typename T, int NR >
-__global__ void kRestrictFace( LV gc, LV gf, const FaceGeom* __restrict__ fgFine, int flag, const T* __restrict__ uf,
-                               T* __restrict__ uc )
-{
-   kRestrictFace_b< T, NR >( Blk{ (int) blockIdx.x, (int) blockIdx.y, (int) blockDim.x, (int) threadIdx.x }, gc, gf, fgFine, flag, uf, uc );
-}
-
-template < typename T, int NR >
 __device__ __forceinline__ void kRestrictEdge_b( const Blk& B, LV gc, LV gf, const EdgeGeom* __restrict__ egFine, const EdgeTerm* __restrict__ terms,
                                int flag, const T* __restrict__ uf, T* __restrict__ uc )
 {
@@ -2614,13 +2576,6 @@ __device__ __forceinline__ void kRestrictEdge_b( const Blk& B, LV gc, LV gf, con
 }
 
 template < typename T, int NR >
-__global__ void kRestrictEdge( LV gc, LV gf, const EdgeGeom* __restrict__ egFine, const EdgeTerm* __restrict__ terms,
-                               int flag, const T* __restrict__ uf, T* __restrict__ uc )
-{
-   kRestrictEdge_b< T, NR >( Blk{ (int) blockIdx.x, (int) blockIdx.y, (int) blockDim.x, (int) threadIdx.x }, gc, gf, egFine, terms, flag, uf, uc );
-}
-
-template < typename T, int NR >
 __device__ __forceinline__ void kRestrictVertex_b( const Blk& B, const VertexGeom* __restrict__ vgFine, const int64_t* __restrict__ termsFine, int nV,
                                  int flag, const T* __restrict__ uf, T* __restrict__ uc )
 {
@@ -2644,13 +2599,6 @@ __device__ __forceinline__ void kRestrictVertex_b( const Blk& B, const VertexGeo
    uc[(int64_t) G.vertex * NR + r] = uf[(int64_t) G.vertex * NR + r] + T( 0.5 ) * acc;
 }
 
-template < typename T, int NR >
-__global__ void kRestrictVertex( const VertexGeom* __restrict__ vgFine, const int64_t* __restrict__ termsFine, int nV,
-                                 int flag, const T* __restrict__ uf, T* __restrict__ uc )
-{
-   kRestrictVertex_b< T, NR >( Blk{ (int) blockIdx.x, (int) blockIdx.y, (int) blockDim.x, (int) threadIdx.x }, vgFine, termsFine, nV, flag, uf, uc );
-}
-
 // zeroes the ranges of dst (one range per logical block)
 template < typename T, int NR >
 __device__ __forceinline__ void kZeroRanges_b( const Blk& B, const Range* __restrict__ ranges, T* __restrict__ dst )
@@ -2662,7 +2610,7 @@ __device__ __forceinline__ void kZeroRanges_b( const Blk& B, const Range* __rest
 }
 
 // a restriction in one launch (LowGrid): vertex, edge and face parts, the macro-cell part (one right-hand side:
-// kRestrictFlat's units of the coarse level) and the zeroing of the coarse level of another function (ranges zr)
+// kRestrictFlat_b over the coarse level's transfer units) and the zeroing of the coarse level of another function (ranges zr)
 template < typename T, int NR >
 __global__ __launch_bounds__( 256 ) void kLowRestrict( LowGrid L, LV gc, LV gf, const VertexGeom* __restrict__ vgFine,
                                                        const int64_t* __restrict__ vtermsFine, int nV,
@@ -2812,14 +2760,6 @@ __device__ __forceinline__ void kProlongFlat_b( const Blk& B, LV gc, LV gf, cons
       fu.store( fo[j], fv[j] + T( 0.5 ) * ( ca[j] + cb[j] ), ok[j] );
 }
 
-template < typename T >
-__global__ __launch_bounds__( 256 ) void kProlongFlat( LV gc, LV gf, const MarchUnit* __restrict__ units,
-                                                       int64_t nUnits, const int* __restrict__ ctype, int flag,
-                                                       const T* __restrict__ uc, T* __restrict__ uf )
-{
-   kProlongFlat_b< T >( Blk{ (int) blockIdx.x, 0, 256, (int) threadIdx.x }, gc, gf, units, nUnits, ctype, flag, uc, uf );
-}
-
 template < typename T, int NR >
 __device__ __forceinline__ void kProlongFace_b( const Blk& B, LV gc, LV gf, const FaceGeom* __restrict__ fg, int flag, const T* __restrict__ uc,
                               T* __restrict__ uf )
@@ -2851,13 +2791,6 @@ __device__ __forceinline__ void kProlongFace_b( const Blk& B, LV gc, LV gf, cons
 }
 
 template < typename T, int NR >
-__global__ void kProlongFace( LV gc, LV gf, const FaceGeom* __restrict__ fg, int flag, const T* __restrict__ uc,
-                              T* __restrict__ uf )
-{
-   kProlongFace_b< T, NR >( Blk{ (int) blockIdx.x, (int) blockIdx.y, (int) blockDim.x, (int) threadIdx.x }, gc, gf, fg, flag, uc, uf );
-}
-
-template < typename T, int NR >
 __device__ __forceinline__ void kProlongEdge_b( const Blk& B, LV gc, LV gf, const EdgeGeom* __restrict__ eg, int flag, const T* __restrict__ uc,
                               T* __restrict__ uf )
 {
@@ -2874,13 +2807,6 @@ __device__ __forceinline__ void kProlongEdge_b( const Blk& B, LV gc, LV gf, cons
 }
 
 template < typename T, int NR >
-__global__ void kProlongEdge( LV gc, LV gf, const EdgeGeom* __restrict__ eg, int flag, const T* __restrict__ uc,
-                              T* __restrict__ uf )
-{
-   kProlongEdge_b< T, NR >( Blk{ (int) blockIdx.x, (int) blockIdx.y, (int) blockDim.x, (int) threadIdx.x }, gc, gf, eg, flag, uc, uf );
-}
-
-template < typename T, int NR >
 __device__ __forceinline__ void kProlongVertex_b( const Blk& B, const VertexGeom* __restrict__ vg, int nV, int flag, const T* __restrict__ uc,
                                 T* __restrict__ uf )
 {
@@ -2892,15 +2818,8 @@ __device__ __forceinline__ void kProlongVertex_b( const Blk& B, const VertexGeom
    uf[v * NR + r] += uc[v * NR + r];
 }
 
-template < typename T, int NR >
-__global__ void kProlongVertex( const VertexGeom* __restrict__ vg, int nV, int flag, const T* __restrict__ uc,
-                                T* __restrict__ uf )
-{
-   kProlongVertex_b< T, NR >( Blk{ (int) blockIdx.x, (int) blockIdx.y, (int) blockDim.x, (int) threadIdx.x }, vg, nV, flag, uc, uf );
-}
-
 // a prolongation in one launch (LowGrid): vertex, edge and face parts, and the macro-cell part (one right-hand
-// side: kProlongFlat's units of the fine level)
+// side: kProlongFlat_b over the fine level's transfer units)
 template < typename T, int NR >
 __global__ __launch_bounds__( 256 ) void kLowProlong( LowGrid L, LV gc, LV gf, const VertexGeom* __restrict__ vgFine,
                                                       int nV, const EdgeGeom* __restrict__ egFine,
