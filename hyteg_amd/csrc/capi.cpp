@@ -439,7 +439,8 @@ struct hymi_gmg_s
       const int flag = kInner | kNeumann;
       if ( L == minL )
       {
-         cgSolve( *A, x, b, L, cgIters, cgTol, cg );
+         // one launch on level 0 (every DoF a vertex); the prolongation after it refreshes x's closure copies
+         if ( !launchCgVertexOnly( *A, x, b, L, cgIters, cgTol ) ) cgSolve( *A, x, b, L, cgIters, cgTol, cg );
          return;
       }
       // ghost copies of x are current after a sweep and after the coarse-level reset; only the caller's x on

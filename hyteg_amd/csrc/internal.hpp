@@ -108,6 +108,7 @@ struct LevelPlan
    DevBuf< VertexGeom > dVertexGeom;
    DevBuf< int64_t >    dVertexTerms;
    DevBuf< int64_t >    dVertexEnds;
+   DevBuf< int32_t >    dVertexNbr; // level 0 only: per vertex term, the vertex it reads (kCgVertexOnly)
    // the vertex terms again, each read from a face's closure copy of the same edge point (a copy no kernel of
    // the vertex and edge phases writes), and per edge {vertex 0, vertex 1, 1 if the edge writes vertex 0's own
    // value, the same for vertex 1, the two vertices' term offsets, term counts and types}: the forward sweep's

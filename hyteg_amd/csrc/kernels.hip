@@ -2931,6 +2931,77 @@ __global__ void kXpbyScalar( const Range* __restrict__ ranges, T* __restrict__ p
 }
 
 // --------------------------------------------------------------------------------------------------------
+// Coarse-grid CG on level 0, where every DoF is a vertex (edges, faces and cells of width 2 have no interior
+// points): one workgroup holds the vertices, one per thread, and runs all iterations of the launched form
+// (capi.cpp cgSolve: r = b - A x, p = r, then per iteration A p, the gated ratios, x += alpha p, r -= alpha A p,
+// p = r + beta p) with workgroup sums instead of about fifteen launches per iteration.  The arithmetic is that of
+// the launched kernels (stencil terms in the vertex apply's order, FP64 dot products, the scalars rounded to T as
+// the axpy kernels round them); only the order of the dot products' sums differs.  Only the owned values of x are
+// written: its closure copies are refreshed by the prolongation that follows the coarse solve in a V-cycle.
+// nbr[k]: the vertex that term k reads (the other end of the term's edge).
+__device__ __forceinline__ double blockSumCg( double v, double* red )
+{
+   v = waveSum( v );
+   const int wave = (int) ( threadIdx.x >> 6 ), lane = (int) ( threadIdx.x & 63 );
+   __syncthreads(); // red is free (its last readers have passed this point)
+   if ( lane == 0 ) red[wave] = v;
+   __syncthreads();
+   double s = 0.0;
+   for ( int k = 0; k < (int) ( blockDim.x >> 6 ); k++ )
+      s += red[k];
+   return s;
+}
+
+template < typename T >
+__global__ __launch_bounds__( kCgVertexMax ) void kCgVertexOnly( const VertexGeom* __restrict__ vg,
+                                                                 const int32_t* __restrict__ nbr, int nV,
+                                                                 const T* __restrict__ VW, int flag, T* __restrict__ x,
+                                                                 const T* __restrict__ b, int maxIter, double tol2 )
+{
+   __shared__ T      sx[kCgVertexMax], sp[kCgVertexMax];
+   __shared__ double red[kCgVertexMax / 64];
+   const int         v  = (int) threadIdx.x;
+   const bool        in = v < nV;
+   VertexGeom        G{};
+   if ( in ) G = vg[v];
+   const bool act = in && ( G.type & flag );
+   const T*   vw  = VW + ( act ? G.vertex + G.termOff : 0 );
+   // (A q)(v) for q in LDS
+   auto applyAt = [&]( const T* q ) {
+      T acc = vw[0] * q[v];
+      for ( int k = 0; k < G.nTerms; k++ )
+         acc += vw[1 + k] * q[nbr[G.termOff + k]];
+      return acc;
+   };
+   sx[v] = in ? x[v] : T( 0 );
+   sp[v] = T( 0 );
+   __syncthreads();
+   T xv = sx[v];
+   T r  = act ? b[v] - applyAt( sx ) : T( 0 );
+   T pv = r;
+   sp[v] = pv;
+   double rsOld = blockSumCg( act ? (double) r * (double) r : 0.0, red ); // its barriers also publish sp
+   for ( int it = 0; it < maxIter; it++ )
+   {
+      const T      ap    = act ? applyAt( sp ) : T( 0 );
+      const double pAp   = blockSumCg( act ? (double) pv * (double) ap : 0.0, red );
+      const double alpha = ( pAp != 0.0 && rsOld > tol2 ) ? rsOld / pAp : 0.0;
+      if ( act )
+      {
+         xv += (T) alpha * pv;
+         r += (T) ( -alpha ) * ap;
+      }
+      const double rsNew = blockSumCg( act ? (double) r * (double) r : 0.0, red );
+      const double beta  = ( rsOld != 0.0 && rsNew > tol2 ) ? rsNew / rsOld : 0.0;
+      rsOld              = rsNew;
+      if ( act ) pv = r + (T) beta * pv;
+      sp[v] = act ? pv : T( 0 ); // every reader of sp (applyAt above) has passed blockSumCg's barriers
+      __syncthreads();
+   }
+   if ( act ) x[v] = xv;
+}
+
+// --------------------------------------------------------------------------------------------------------
 // dot products: block partials (per rhs) then one fixed-order reduction
 
 template < int NR >
@@ -4003,6 +4074,29 @@ struct XpbyF
 };
 
 } // namespace
+
+bool launchCgVertexOnly( Operator& op, Function& x, const Function& b, int level, int maxIter, double tol )
+{
+   Mesh&      M = *op.mesh;
+   LevelPlan& P = M.plan( level );
+   const int  nV = M.T.nV();
+   if ( level != 0 || x.nrhs != 1 || b.nrhs != 1 || x.prec != b.prec || nV > kCgVertexMax || P.dVertexNbr.p == nullptr )
+      return false;
+   if ( maxIter <= 0 ) return true;
+   auto&       Wt   = op.weights( level );
+   const int   flag = kInner | kNeumann | kFreeslip;
+   const dim3  block( (unsigned) ( ( nV + 63 ) / 64 * 64 ) );
+   hipStream_t s = M.ctx->stream;
+   if ( x.prec == kF64 )
+      hipLaunchKernelGGL( ( kCgVertexOnly< double > ), dim3( 1 ), block, 0, s, P.dVertexGeom.p, P.dVertexNbr.p, nV,
+                          Wt.vertex64.p, flag, (double*) x.ptr( level ), (const double*) b.ptr( level ), maxIter,
+                          tol * tol );
+   else
+      hipLaunchKernelGGL( ( kCgVertexOnly< float > ), dim3( 1 ), block, 0, s, P.dVertexGeom.p, P.dVertexNbr.p, nV,
+                          Wt.vertex32.p, flag, (float*) x.ptr( level ), (const float*) b.ptr( level ), maxIter,
+                          tol * tol );
+   return true;
+}
 
 void launchRefresh( Function& f, int level, int mask ) { dispatch< RefreshF >( f.prec, f.nrhs, f, level, mask ); }
 

@@ -37,6 +37,10 @@ void launchRestrict( Function& f, Function& dst, int srcLevel, int flag, bool ba
 void launchResidualRestrict( Operator& op, Function& x, Function& r, Function& b, int level, int flag,
                              bool refreshSrc, Function* zeroCoarse = nullptr );
 void launchProlongateAdd( Function& f, int srcLevel, int flag, bool refreshCoarse = true );
+// the coarse-grid CG of a V-cycle in one launch where it applies (level 0, one right-hand side, at most
+// kCgVertexMax vertices): x's owned values are updated, its closure copies are not; returns false (nothing
+// launched) where it does not apply
+bool launchCgVertexOnly( Operator& op, Function& x, const Function& b, int level, int maxIter, double tol );
 // dst = sum_k c[k] * src[k] (+ dst if accumulate) + constant, on the closures of primitives matching flag
 void launchLincomb( Function& dst, int level, int flag, int n, const double* c, Function* const* src, bool accumulate,
                     double constant );

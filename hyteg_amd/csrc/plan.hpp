@@ -96,6 +96,8 @@ constexpr int kEdgeMaxW = 1025;
 // buffer has a fixed size (it is allocated once, with the context: captured graphs hold its address)
 constexpr int     kDotBlocks     = 2048;
 constexpr int64_t kPartialsCap   = ( kDotBlocks + 64 ) * 32;
+// coarse-grid CG at level 0 in one workgroup (one vertex per thread) for meshes of at most this many vertices
+constexpr int kCgVertexMax = 1024;
 // levels whose cell SOR phase runs as one workgroup per macro-cell for all four colours
 constexpr int kCellSmallMaxLevel = 5;
 // fused macro-cell SOR sweep (one launch for the four colours): square xy tiles of 24 points per side, a halo of kSorTileHalo rings loaded around each tile and recomputed, kSorTileRing slab slots in LDS

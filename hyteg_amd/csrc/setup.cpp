@@ -436,6 +436,7 @@ void buildPlan( Mesh& M, LevelPlan& P, int level )
    P.hVertexGeom.clear();
    P.hVertexTerms.clear();
    P.hVertexEnds.clear();
+   std::vector< int32_t > nbr; // level 0: the vertex each term reads (the edge's other end, whose copy it is)
    for ( int v = 0; v < T.nV(); v++ )
    {
       VertexGeom g{};
@@ -447,6 +448,7 @@ void buildPlan( Mesh& M, LevelPlan& P, int level )
          const bool first = T.ev[e][0] == v;
          P.hVertexTerms.push_back( P.edgeOff( e ) + ( first ? 1 : w - 2 ) );
          P.hVertexEnds.push_back( P.edgeOff( e ) + ( first ? 0 : w - 1 ) );
+         nbr.push_back( first ? T.ev[e][1] : T.ev[e][0] );
       }
       g.nTerms = (int) P.hVertexTerms.size() - g.termOff;
       P.hVertexGeom.push_back( g );
@@ -454,6 +456,7 @@ void buildPlan( Mesh& M, LevelPlan& P, int level )
    if ( dev ) P.dVertexGeom.upload( P.hVertexGeom, s );
    if ( dev ) P.dVertexTerms.upload( P.hVertexTerms, s );
    if ( dev ) P.dVertexEnds.upload( P.hVertexEnds, s );
+   if ( dev && w == 2 ) P.dVertexNbr.upload( nbr, s );
 
    // the same vertex terms read from the closure copy of the first face of each edge (E2F entry of that face
    // and edge), and each edge's end vertices with the edge that writes the vertex's own value (its first edge)
