@@ -185,7 +185,7 @@ class PrimitiveStorage:
 
     def set_residual_restrict(self, level, mode):
         """Residual and restriction of a V-cycle from fine `level`: 1 = the macro-cell interiors in one pass, 0 = two
-        passes, -1 = default (one pass from level 8 up).  Results agree to rounding."""
+        passes, -1 = default (one pass from level 8 up).  Results are the same bit for bit."""
         _lib.call("hymi_mesh_set_residual_restrict", self.h, level, int(mode))
 
     def get_residual_restrict(self, level):

@@ -257,8 +257,8 @@ int hymi_mesh_set_cell_sor( hymi_mesh mesh, int level, int tile, int precisions 
 int hymi_mesh_get_cell_sor( hymi_mesh mesh, int level, int out[2] );
 /* Residual and restriction of a V-cycle step from fine `level` to level - 1: mode 1 computes the macro-cell
  * interiors' residual and restriction in one pass (one right-hand side, level >= 3), 0 runs them as two passes,
- * -1 selects the default (one pass from level 8 up).  The cycle's results agree to rounding (same terms in the
- * same order).  get returns 1 where the level runs the one pass.  Waits for the context stream; captured V-cycle
+ * -1 selects the default (one pass from level 8 up).  The cycle's results are the same bit for bit (same terms in
+ * the same order).  get returns 1 where the level runs the one pass.  Waits for the context stream; captured V-cycle
  * graphs are captured again.  No reference counterpart: the reference computes the residual
  * (src/hyteg/solvers/GeometricMultigridSolver.hpp:184-190) and restricts it (P1toP1LinearRestriction.cpp) as separate
  * passes */
