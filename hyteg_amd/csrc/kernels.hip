@@ -1016,7 +1016,7 @@ __global__ __launch_bounds__( 1024 ) void kCellSorSmall( LV g, const int* __rest
 // reach 3 - k rings beyond the tile, so colour k is recomputed on the tile widened by 3 - k rings and the tile
 // widened by kSorTileHalo = 4 rings is loaded; these recomputed halo values equal the owning tile's, because
 // they are computed with the same expression from the same inputs.  Slabs t-7 .. t+2 are in LDS (a ring of
-// kSorTileRing slots); loads run DP steps ahead in registers (slab s is loaded in step s - 2 - DP and stored
+// kSorTileRing = 16 slots, a power of two: slot arithmetic by masks); loads run DP steps ahead in registers (slab s is loaded in step s - 2 - DP and stored
 // into LDS at the end of step s - 2, the right-hand side of step t is loaded in step t - DP).  Slab t-7 is
 // final after step t-1 and is written back in step t.
 //
@@ -1060,6 +1060,7 @@ __global__ __launch_bounds__( 1024, sizeof( T ) == 4 ? 8 : 4 ) void kCellSorTile
 {
    constexpr int TS = kSorTile, H = kSorTileHalo, R = TS + 2 * H, RP = R + PAD, RS = R * RP, NT = 1024;
    constexpr int NL = R * R / NT, RING = kSorTileRing;
+   static_assert( ( RING & ( RING - 1 ) ) == 0 && RING >= 10, "the ring holds slabs t-7 .. t+2; slots by masks" );
    static_assert( R == 32 && NL == 1, "the lane mapping assumes a 32 x 32 region and 1024 threads" );
    __shared__ T lds[RING * RS];
    __shared__ int sTab[kEdgeMaxW + 1];
@@ -1227,7 +1228,7 @@ __global__ __launch_bounds__( 1024, sizeof( T ) == 4 ? 8 : 4 ) void kCellSorTile
       loadSlab( t + 2 + DP, xbuf[( P + 2 + DP ) % ( DP + 1 )] );
       loadRhs( t + DP, kTab[( P + DP ) & 3], bbuf[( P + DP ) % ( DP + 1 )] );
       const int sw     = t - 7; // final since step t - 1
-      const int swSlot = tm + 3 >= RING ? tm + 3 - RING : tm + 3;
+      const int swSlot = ( tm + RING - 7 ) & ( RING - 1 );
       const int swBase = sw >= 1 ? slabStart( sw ) : 0;
 #pragma unroll
       for ( int jc = 0; jc < NC; jc++ )
@@ -1240,9 +1241,7 @@ __global__ __launch_bounds__( 1024, sizeof( T ) == 4 ? 8 : 4 ) void kCellSorTile
       }
       // the wave's update: sweep position k on slab s, slots of s and its neighbours (uniform)
       const int k = kTab[P & 3], s = t - 2 * k;
-      int       sl = tm - 2 * k;
-      sl += sl < 0 ? RING : 0;
-      const int slT = sl == RING - 1 ? 0 : sl + 1, slB = sl == 0 ? RING - 1 : sl - 1;
+      const int sl = ( tm - 2 * k ) & ( RING - 1 ), slT = ( sl + 1 ) & ( RING - 1 ), slB = ( sl + RING - 1 ) & ( RING - 1 );
 #pragma unroll
       for ( int j = 0; j < NL; j++ )
       {
@@ -1262,7 +1261,7 @@ __global__ __launch_bounds__( 1024, sizeof( T ) == 4 ? 8 : 4 ) void kCellSorTile
          acc += cBN * Bp + cBNW * Bpl;
          lds[sl * RS + uq[j]] = ( T( 1 ) - omega ) * Z0 + relax * ( bbuf[P][j] - acc );
       }
-      storeSlab( tm + 2 >= RING ? tm + 2 - RING : tm + 2, xbuf[( P + 2 ) % ( DP + 1 )] );
+      storeSlab( ( tm + 2 ) & ( RING - 1 ), xbuf[( P + 2 ) % ( DP + 1 )] );
       __syncthreads();
    };
 
@@ -1302,7 +1301,7 @@ __global__ __launch_bounds__( 1024, sizeof( T ) == 4 ? 8 : 4 ) void kCellSorTile
       static_for< 0, DP + 1 >( [&]( auto pc ) {
          constexpr int p = decltype( pc )::value;
          step( t + p, tm, std::integral_constant< int, ( 1 + p ) % ( DP + 1 ) >{} );
-         tm = tm + 1 == RING ? 0 : tm + 1;
+         tm = ( tm + 1 ) & ( RING - 1 );
       } );
    }
 }

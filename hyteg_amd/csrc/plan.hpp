@@ -102,7 +102,7 @@ constexpr int kCgVertexMax = 1024;
 constexpr int kCellSmallMaxLevel = 5;
 // fused macro-cell SOR sweep (one launch for the four colours): square xy tiles of 24 points per side, a halo of kSorTileHalo rings loaded around each tile and recomputed, kSorTileRing slab slots in LDS
 constexpr int kSorTileHalo = 4;
-constexpr int kSorTileRing = 10;
+constexpr int kSorTileRing = 16; // slabs t-7 .. t+2 need 10; a power of two makes the slot arithmetic masks
 constexpr int kSorTileSide = 24;
 inline bool   sorTileSizeOk( int t ) { return t == kSorTileSide; }
 
