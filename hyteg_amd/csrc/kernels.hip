@@ -2696,20 +2696,27 @@ __global__ __launch_bounds__( 256 ) void kProlongCell( LV gc, LV gf, const CellS
    }
 }
 
-// Flat form for one right-hand side: a wave takes kTransferWindows windows of 64 consecutive elements of one
-// fine slab z (units from Mesh::transferUnits); each lane finds its row y, and computes the two coarse
-// indices with 32-bit arithmetic.  The coarse slabs (z -/+ pz) / 2 are wave-uniform because the
-// z-component of the direction always equals the parity of z.
+// the windows of one unit of the flat prolongation: fine and coarse values loaded, fine offsets, and whether
+// the lane's element is an interior point (a lane without one reads index 0 and stores nothing)
 template < typename T >
-__device__ __forceinline__ void kProlongFlat_b( const Blk& B, LV gc, LV gf, const MarchUnit* __restrict__ units,
-                                                       int64_t nUnits, const int* __restrict__ ctype, int flag,
-                                                       const T* __restrict__ uc, T* __restrict__ uf )
+struct ProlongWindows
 {
-   const int64_t ui = xcdBlockOf( B.x, ( nUnits + 3 ) / 4 ) * 4 + __builtin_amdgcn_readfirstlane( B.tid >> 6 );
-   if ( ui >= nUnits ) return;
-   const MarchUnit U = units[ui];
-   if ( !( ctype[U.cell] & flag ) ) return;
-   const int lane = B.tid & 63;
+   T    fv[kTransferWindows], ca[kTransferWindows], cb[kTransferWindows];
+   int  fo[kTransferWindows];
+   bool ok[kTransferWindows];
+};
+
+template < typename T >
+__device__ __forceinline__ void prolongFlatLoad( const MarchUnit& U, bool active, int lane, LV gc, LV gf,
+                                                 const T* __restrict__ uc, T* __restrict__ uf, ProlongWindows< T >& P )
+{
+#pragma unroll
+   for ( int j = 0; j < kTransferWindows; j++ )
+   {
+      P.ok[j] = false;
+      P.fo[j] = 0;
+   }
+   if ( !active ) return;
    const int w = gf.w, z = U.z, Wd = w - z, wc = gc.w;
    const int tf = tet32( w ), tc = tet32( wc );
    const int nEl = Wd * ( Wd + 1 ) / 2;
@@ -2723,40 +2730,101 @@ __device__ __forceinline__ void kProlongFlat_b( const Blk& B, LV gc, LV gf, cons
    const CellBuf< T > fu( uf + gf.offC + (int64_t) U.cell * gf.cS, gf.cS );
    auto      rs = [&]( int WW, int yy ) { return yy * ( WW + 1 ) - ( yy * ( yy + 1 ) >> 1 ); };
    int       e0 = U.x0;
-   // all windows' loads are issued before the first sum (twelve loads in flight per lane); lanes without
-   // an interior point read index 0 and store nothing
-   T    fv[kTransferWindows], ca[kTransferWindows], cb[kTransferWindows];
-   int  fo[kTransferWindows];
-   bool ok[kTransferWindows];
+   // the lane's row yy, its start rowS and the next row's start rowE
+   int yy = U.y0, rowS = rs( Wd, U.y0 ), rowE = rowS + Wd - U.y0;
 #pragma unroll
    for ( int j = 0; j < kTransferWindows; j++, e0 += 64 )
    {
-      ok[j] = false;
-      fo[j] = 0;
       int im = 0, ip = 0;
       if ( e0 < nEl )
       {
-         // the lane's row from the closed form (no scalar search over the rows of the window)
-         const int e  = e0 + lane;
-         const int yy = triRow32( Wd, min( e, nEl - 1 ) );
-         const int x  = e - rs( Wd, yy );
-         ok[j]       = e < nEl && x >= 1 && yy >= 1 && x + yy <= Wd - 2;
-         if ( ok[j] )
+         const int e = e0 + lane;
+         while ( e >= rowE && yy < Wd - 1 )
          {
-            int dx, dy, dz;
-            prolongDir( x & 1, yy & 1, pz, dx, dy, dz );
-            im    = Cm + rs( Wm, ( yy - dy ) / 2 ) + ( x - dx ) / 2;
-            ip    = Cp + rs( Wp, ( yy + dy ) / 2 ) + ( x + dx ) / 2;
-            fo[j] = S0 + e;
+            yy++;
+            rowS = rowE;
+            rowE += Wd - yy;
+         }
+         const int x = e - rowS;
+         P.ok[j]     = e < nEl && x >= 1 && yy >= 1 && x + yy <= Wd - 2;
+         if ( P.ok[j] )
+         {
+            // prolongDir's direction: dy = -1 for odd y unless x and z are even (then +1), dx = -1 for odd x
+            // with odd z and even y, else the parity of x
+            const int  px = x & 1, py = yy & 1, h = yy >> 1, hx = x >> 1;
+            const int  rm = rs( Wm, h ), rp = rm - ( pz ? h : 0 ); // row h of the coarse slabs Zm, Zp
+            const bool dyNeg = py && ( px || pz ), dyPos = py && !px && !pz;
+            const bool dxNeg = px && pz && !py, dxPos = px && !dxNeg;
+            im      = Cm + ( dyNeg ? rm + Wm - h : rm ) + hx + ( dxNeg ? 1 : 0 );
+            ip      = Cp + ( dyPos ? rp + Wp - h : rp ) + hx + ( dxPos ? 1 : 0 );
+            P.fo[j] = S0 + e;
          }
       }
-      fv[j] = fu[fo[j]];
-      ca[j] = cu[im];
-      cb[j] = cu[ip];
+      P.fv[j] = fu[P.fo[j]];
+      P.ca[j] = cu[im];
+      P.cb[j] = cu[ip];
    }
+}
+
+template < typename T >
+__device__ __forceinline__ void prolongFlatStore( const MarchUnit& U, bool active, LV gf, T* __restrict__ uf,
+                                                  const ProlongWindows< T >& P )
+{
+   if ( !active ) return;
+   const CellBuf< T > fu( uf + gf.offC + (int64_t) U.cell * gf.cS, gf.cS );
 #pragma unroll
    for ( int j = 0; j < kTransferWindows; j++ )
-      fu.store( fo[j], fv[j] + T( 0.5 ) * ( ca[j] + cb[j] ), ok[j] );
+      fu.store( P.fo[j], P.fv[j] + T( 0.5 ) * ( P.ca[j] + P.cb[j] ), P.ok[j] );
+}
+
+// Flat form for one right-hand side: a wave takes prolongUnitsPerWave consecutive units (Mesh::transferUnits)
+// of kTransferWindows windows of 64 consecutive elements of one fine slab z each, and issues the loads of all
+// before the first store.  In FP32 it takes two (24 loads in flight per lane: a wave that took one unit spent
+// most of its short life waiting for the unit's and the cell type's scalar loads and for one round of vector
+// loads; level 8, 376 -> 310 us); FP64 moves twice the bytes per load and runs near the HBM rate with one
+// (two were 2 % slower).  The coarse
+// indices use 32-bit arithmetic.  The coarse slabs (z -/+ pz) / 2 are wave-uniform because the z-component of
+// the direction always equals the parity of z.  A lane's row starts at the unit's first row and advances over
+// the row ends its elements pass (at most one per window where rows hold 64 or more elements).  With h = y / 2,
+// the coarse rows of (p - d) / 2 and (p + d) / 2 are both h for even y and h, h + 1 in the order the sign of
+// dy gives for odd y, and the start of row h in coarse slab Zp is that in slab Zm less pz h (the slab is pz
+// narrower), so one triangular term serves both; likewise in x with hx = x / 2.
+template < typename T >
+constexpr int prolongUnitsPerWave()
+{
+   return sizeof( T ) == 4 ? 2 : 1;
+}
+template < typename T >
+__device__ __forceinline__ void kProlongFlat_b( const Blk& B, LV gc, LV gf, const MarchUnit* __restrict__ units,
+                                                int64_t nUnits, const int* __restrict__ ctype, int flag,
+                                                const T* __restrict__ uc, T* __restrict__ uf )
+{
+   constexpr int UPW     = prolongUnitsPerWave< T >();
+   const int64_t nGroups = ( nUnits + UPW - 1 ) / UPW;
+   const int64_t gi = xcdBlockOf( B.x, ( nGroups + 3 ) / 4 ) * 4 + __builtin_amdgcn_readfirstlane( B.tid >> 6 );
+   if ( gi >= nGroups ) return;
+   const int64_t   u0   = UPW * gi;
+   const int       lane = B.tid & 63;
+   const MarchUnit U0   = units[u0];
+   const bool      a0   = ( ctype[U0.cell] & flag ) != 0;
+   ProlongWindows< T > P0;
+   if constexpr ( UPW == 2 )
+   {
+      const int64_t   u1 = u0 + 1;
+      const MarchUnit U1 = u1 < nUnits ? units[u1] : U0;
+      const bool      a1 = u1 < nUnits && ( ctype[U1.cell] & flag ) != 0;
+      ProlongWindows< T > P1;
+      prolongFlatLoad( U0, a0, lane, gc, gf, uc, uf, P0 );
+      prolongFlatLoad( U1, a1, lane, gc, gf, uc, uf, P1 );
+      prolongFlatStore( U0, a0, gf, uf, P0 );
+      prolongFlatStore( U1, a1, gf, uf, P1 );
+   }
+   else
+   {
+      static_assert( UPW == 1, "one or two units per wave" );
+      prolongFlatLoad( U0, a0, lane, gc, gf, uc, uf, P0 );
+      prolongFlatStore( U0, a0, gf, uf, P0 );
+   }
 }
 
 template < typename T, int NR >
@@ -3841,7 +3909,8 @@ void prolongT( Function& f, int srcLevel, int flag, bool refreshCoarse )
    if ( NR == 1 )
    {
       tu     = M.transferUnits( Pf, &nu );
-      lg.nbC = nu > 0 ? (int) xcdGrid( ( nu + 3 ) / 4 ) : 0;
+      constexpr int upw = prolongUnitsPerWave< T >();
+      lg.nbC            = nu > 0 ? (int) xcdGrid( ( ( nu + upw - 1 ) / upw + 3 ) / 4 ) : 0;
    }
    hipLaunchKernelGGL( ( kLowProlong< T, NR > ), dim3( lowBlocks( lg ) ), dim3( 256 ), 0, s, lg, gc, gf, Pf.dVertexGeom.p, nV,
                        Pf.dEdgeGeom.p, Pf.dFaceGeom.p, flag, uc, uf, tu, nu, Pf.dCellType.p );
