@@ -121,6 +121,14 @@ __device__ __forceinline__ int64_t crow( int64_t w, int64_t t0, int64_t y, int64
 }
 // 32-bit forms for cell-relative offsets (a cell of width w <= 1025 has fewer than 2^31 points)
 __device__ __forceinline__ int tet32( int w ) { return w * ( w + 1 ) / 2 * ( w + 2 ) / 3; }
+// a copy of v the compiler cannot see through: expressions of it are formed where they are used instead of being
+// hoisted out of unrolled loops into registers of their own
+__device__ __forceinline__ int opaqueInt( int v )
+{
+   int o;
+   asm volatile( "v_mov_b32 %0, %1" : "=v"( o ) : "v"( v ) );
+   return o;
+}
 __device__ __forceinline__ int crow32( int w, int t0, int y, int z )
 {
    const int wz = w - z;
@@ -1100,8 +1108,10 @@ __global__ __launch_bounds__( 1024, sizeof( T ) == 4 ? 8 : 4 ) void kCellSorTile
    int lq[NL], lsx[NL], ly_[NL], lbo[NL], lsnap[NL];
    // ping-pong: face-array offset of the position's point on the plane x + y + z = w - 1 (fdiag); for a position
    // on x = 0 or y = 0 (fcol), the point of slab s is (i, j) = (fcA + fcS s, fcB + fcT s) of its face, whose
-   // offset is folded into fcA (the slopes fcS, fcT in {-1, 0, 1} packed into fcST as (fcS + 1) | (fcT + 1) << 2)
-   int  fdiag[NL], fcA[NL], fcB[NL], fcST[NL];
+   // offset is folded into fcA; fcB and the slopes fcS, fcT in {-1, 0, 1} share one register, fcBST =
+   // fcB << 4 | (fcS + 1) | (fcT + 1) << 2 (with one more register the FP32 kernel spilled fcB, and the reload in
+   // the loop waited for every load in flight)
+   int  fdiag[NL], fcA[NL], fcBST[NL];
    bool fcol[NL];
    const CellFaceRead* fr = pingpong ? cfr + un.cell : nullptr;
    const int           fS = (int) g.fS;
@@ -1126,8 +1136,7 @@ __global__ __launch_bounds__( 1024, sizeof( T ) == 4 ? 8 : 4 ) void kCellSorTile
          const int  pl = x == 0 ? 2 : 1, pu = x == 0 ? y : x;
          const int* c  = fr->c[pl];
          fcA[j]        = fr->face[pl] * fS + c[0] + c[1] * pu;
-         fcB[j]        = c[3] + c[4] * pu;
-         fcST[j]       = ( c[2] + 1 ) | ( ( c[5] + 1 ) << 2 );
+         fcBST[j]      = ( ( c[3] + c[4] * pu ) << 4 ) | ( c[2] + 1 ) | ( ( c[5] + 1 ) << 2 );
       }
    }
    // update positions of colour class cls: LDS place, largest sweep position k whose ring holds the point,
@@ -1171,8 +1180,9 @@ __global__ __launch_bounds__( 1024, sizeof( T ) == 4 ? 8 : 4 ) void kCellSorTile
             const bool onFace = ok && ( fcol[j] || s == lsx[j] );
             if ( onFace )
             {
-               const int fj = fcB[j] + ( ( fcST[j] >> 2 ) - 1 ) * s;
-               v[j] = cf.load( fcol[j] ? fcA[j] + ( ( fcST[j] & 3 ) - 1 ) * s + faceRel( w, 0, fj ) : fdiag[j], true );
+               const int bst = opaqueInt( fcBST[j] ); // unpacked here, not hoisted into registers of its own
+               const int fj  = ( bst >> 4 ) + ( ( ( bst >> 2 ) & 3 ) - 1 ) * s;
+               v[j] = cf.load( fcol[j] ? fcA[j] + ( ( bst & 3 ) - 1 ) * s + faceRel( w, 0, fj ) : fdiag[j], true );
             }
             else
                v[j] = cu.load( o, ok );
@@ -2352,14 +2362,6 @@ __device__ __forceinline__ void kRestrictFlat_b( const Blk& B, LV gc, LV gf, con
 // columns and one fine row, written by the tile below only; the first slab of a run (2 Za - 1) is written by the
 // run below.  The few such points no tile reaches (listed after the tiles, nExtra of them per cell) are computed
 // from global memory by the run at (1, 1, 1).
-// a copy of v the compiler cannot see through: expressions of it are formed where they are used instead of being
-// hoisted out of unrolled loops into registers of their own
-__device__ __forceinline__ int opaqueInt( int v )
-{
-   int o;
-   asm volatile( "v_mov_b32 %0, %1" : "=v"( o ) : "v"( v ) );
-   return o;
-}
 
 template < typename T, int D >
 __global__ __launch_bounds__( 64 * ( 2 * kResBandY + 1 ), 8 ) void kResidualRestrict(
